@@ -1,0 +1,77 @@
+"""CPU checks of the C ABI: the library loads, exports every entry point that
+include/mordred_hip.h declares, and the ctypes structs match the C layout."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from mordred_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mordred_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mordred_\w+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert len(names) >= 22
+    assert set(abi.SIGNATURES) | {"mordred_version"} >= set(names) - {"mordred_version"}
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    assert lib.mordred_version().decode().startswith("mordred-hip")
+
+
+def test_bindings_cover_header():
+    assert set(declared_functions()) == set(abi.SIGNATURES) | {"mordred_version"}
+
+
+STRUCTS = {
+    "probeArgsGPU": abi.ProbeArgsGPU,
+    "buildArgsGPU": abi.BuildArgsGPU,
+    "groupbyArgsGPU": abi.GroupbyArgsGPU,
+    "filterArgsGPU": abi.FilterArgsGPU,
+    "offsetGPU": abi.OffsetGPU,
+}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void) {"]
+    for s, cls in STRUCTS.items():
+        lines.append(f'printf("{s} size %zu\\n", sizeof({s}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{s} {fname} %zu\\n", offsetof({s}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-std=c11", str(src), "-o", str(exe)])
+    out = subprocess.check_output([str(exe)], text=True).split("\n")
+    got = {}
+    for line in out:
+        if line:
+            s, f, v = line.split()
+            got[(s, f)] = int(v)
+    for s, cls in STRUCTS.items():
+        assert got[(s, "size")] == C.sizeof(cls), s
+        for fname, _ in cls._fields_:
+            assert got[(s, fname)] == getattr(cls, fname).offset, (s, fname)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(abi, "_lib", None)
+    monkeypatch.setattr(abi, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(ImportError):
+        abi.load()
