@@ -80,6 +80,19 @@ def cpu_baseline(q, data, budget_s: float):
                            f"median of {len(times)} runs, {med * 1e3:.1f} ms each"}
 
 
+def copy_probe():
+    """A device copy of exactly 1,920,000,000 bytes read + written, repeated 5
+    times: the reference kernel for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = 480_000_000
+    a = torch.ones(n, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": 4 * n, "calls": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,7 +102,12 @@ def main():
     ap.add_argument("--rows", type=int, default=None, help="override fact rows")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity check")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1.92 GB device-to-device copy (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this path runs on one GPU; multi-GPU runs are not part of this "
                                    "build (see DESIGN.md for how it would shard)"}))
@@ -134,6 +152,11 @@ def main():
     achieved = alg_bytes / (k_avg * 1e-3) / 1e9
     gpu_res = gq.result()
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath) and (args.rows is None):
+        with open(tpath) as f:
+            traffic = json.load(f).get(args.workload, {}).get("bytes_per_launch")
     out = {
         "metric": METRIC,
         "value": data.n / (ms_step * 1e-3),
@@ -150,7 +173,8 @@ def main():
         "config": dict(desc, form="plain per-query GPU (builds + fused fact kernel per step)"),
         "kernel_rows_per_s": data.n / (k_avg * 1e-3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE of this kernel)",
                      "kernel": "pipeline_kernel (fused fact probe+group-by)" if q.kind == "groupby"
                      else "pipeline_kernel (fused fact filter+probe+sum)",
                      "algorithmic_bytes": alg_bytes, "kernel_ms_avg": k_avg},

@@ -30,333 +30,10 @@
 #include <mutex>
 #include <unordered_map>
 
-#include "../../include/mordred_hip.h"
-#include "tile.hpp"
+#define MORDRED_DEFINE_PREP_KERNELS
+#include "pipeline.hpp"
 
 namespace mordred {
-
-enum Mode { SEG = 0, OFF = 1, PLAIN = 2 };
-enum Sink { GROUPBY_GLOBAL = 0, GROUPBY_LDS = 1, AGGR = 2, COMPACT = 3 };
-enum FilterKind {
-  F_NONE = 0,
-  F_BETWEEN = 1,       // fused kernels: filter1 in [c1,c2], filter2 in [c3,c4]
-  F_BETWEEN2ONLY = 2,  // filter_probe_aggr_GPU3 re-checks filter2 only (GPUProcessing.h:1805)
-  F_MODE = 3           // filter kernels: mode 0 EQ, 1 BETWEEN, 2 EQ-or-EQ
-};
-
-struct Pipe {
-  const int* cache;
-  const short* seg_group;
-  int start_offset;
-  int num_tuples;
-  int row_base;   // PLAIN: row id of position 0
-  int aligned;    // OFF/PLAIN: all contiguous inputs are 16-byte aligned
-  const int* in_lo;
-  const int* in_dim[4];
-  int fkind;
-  const int* fcol[2];
-  int cmp[4];
-  int fmode[2];
-  int pmask;      // bit k: table k+1 takes part in probing / dim-offset output
-  const int* key[4];
-  const int* ht[4];
-  int len[4];
-  int minkey[4];
-  const int* aggr[2];
-  const int* grp[4];
-  int minv[4];
-  int uniq[4];
-  int total_val;
-  int* res;
-  int* total;
-  int* out[5];
-};
-
-struct Build {
-  const int* cache;
-  const short* seg_group;
-  int start_offset;
-  int num_tuples;
-  int row_base;
-  int aligned;
-  const int* in_rows;
-  const int* fcol;
-  int fmode;
-  int c1, c2;
-  const int* key;
-  const int* val;
-  int* ht;
-  int num_slots;
-  int key_min;
-};
-
-// Contiguous tile pointer of a column: SEG -> cached segment run, PLAIN -> column.
-template <int MODE>
-__device__ __forceinline__ const int* tile_ptr(const int* cache, const int* col, int seg,
-                                               int start) {
-  if (MODE == SEG) return cache + ((size_t)col[seg] << kSegLog) + (start & kSegMask);
-  return col + start;
-}
-
-// Column values of this thread's items.  OFF mode gathers only items in `m`.
-template <int BLOCK, int VPT, int MODE>
-__device__ __forceinline__ void load_col(const int* cache, const int* col, int seg, int start,
-                                         bool vec, int n, const int (&lo)[4 * VPT], uint32_t m,
-                                         int (&x)[4 * VPT]) {
-  if (MODE == OFF) {
-#pragma unroll
-    for (int j = 0; j < 4 * VPT; j++) x[j] = (m >> j & 1u) ? gather(cache, col, lo[j]) : 0;
-  } else {
-    load_tile<BLOCK, VPT>(tile_ptr<MODE>(cache, col, seg, start), vec, n, x);
-  }
-}
-
-template <int VPT>
-__device__ __forceinline__ uint32_t pred_between(const int (&x)[4 * VPT], int lo, int hi) {
-  uint32_t m = 0;
-#pragma unroll
-  for (int j = 0; j < 4 * VPT; j++) m |= (uint32_t)(x[j] >= lo && x[j] <= hi) << j;
-  return m;
-}
-
-template <int VPT>
-__device__ __forceinline__ uint32_t pred_eq(const int (&x)[4 * VPT], int c) {
-  uint32_t m = 0;
-#pragma unroll
-  for (int j = 0; j < 4 * VPT; j++) m |= (uint32_t)(x[j] == c) << j;
-  return m;
-}
-
-// Mode predicate of filter_GPU2 / build_GPU2 (GPUProcessing.h:1231-1239, 972-980).
-template <int VPT>
-__device__ __forceinline__ uint32_t pred_mode(const int (&x)[4 * VPT], int mode, int c1, int c2,
-                                              uint32_t all) {
-  if (mode == 0) return pred_eq<VPT>(x, c1);
-  if (mode == 1) return pred_between<VPT>(x, c1, c2);
-  if (mode == 2) return pred_eq<VPT>(x, c1) | pred_eq<VPT>(x, c2);
-  return all;
-}
-
-template <int BLOCK, int VPT, int MODE, int SINK>
-__global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
-  constexpr int IPT = 4 * VPT;
-  constexpr int TILE = BLOCK * IPT;
-  __shared__ long long red[BLOCK / kWave];
-  __shared__ int scan[BLOCK / kWave + 1];
-  extern __shared__ unsigned long long gsum[];  // GROUPBY_LDS: sums, then touch bitmap
-  uint32_t* touched = reinterpret_cast<uint32_t*>(gsum + (SINK == GROUPBY_LDS ? P.total_val : 0));
-
-  if (SINK == GROUPBY_LDS) {
-    const int words = P.total_val + (P.total_val + 63) / 64;  // sums + bitmap (as u64 words)
-    for (int i = threadIdx.x; i < words; i += BLOCK) gsum[i] = 0;
-    __syncthreads();
-  }
-  long long acc = 0;
-  const int ntiles = (P.num_tuples + TILE - 1) / TILE;
-
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int start = tile * TILE;
-    const int n = min(TILE, P.num_tuples - start);
-    const bool full = n == TILE;
-    const uint32_t rmask = full ? (IPT == 32 ? 0xffffffffu : (1u << IPT) - 1u)
-                                : range_mask<BLOCK, VPT>(n);
-    const bool vec = full && P.aligned;
-    uint32_t sel = rmask;
-
-    int seg = 0, base_row = 0;
-    int lo[IPT];
-    if (MODE == SEG) {
-      seg = P.seg_group ? (int)P.seg_group[start >> kSegLog] : (P.start_offset + start) >> kSegLog;
-      base_row = seg * kSegSize + (start & kSegMask);
-    } else if (MODE == PLAIN) {
-      base_row = P.row_base + start;
-    }
-    if (MODE == OFF) {
-      load_tile<BLOCK, VPT>(P.in_lo + start, vec, n, lo);
-    } else {
-#pragma unroll
-      for (int j = 0; j < IPT; j++) lo[j] = base_row + item_row<BLOCK>(j);
-    }
-
-    int x[IPT];
-    // ---------------------------------------------------------- filters
-    if (P.fkind == F_BETWEEN || P.fkind == F_BETWEEN2ONLY) {
-      if (P.fkind == F_BETWEEN && P.fcol[0]) {
-        load_col<BLOCK, VPT, MODE>(P.cache, P.fcol[0], seg, start, vec, n, lo, sel, x);
-        sel &= pred_between<VPT>(x, P.cmp[0], P.cmp[1]);
-      }
-      if (P.fcol[1] && wave_any(sel)) {
-        load_col<BLOCK, VPT, MODE>(P.cache, P.fcol[1], seg, start, vec, n, lo, sel, x);
-        sel &= pred_between<VPT>(x, P.cmp[2], P.cmp[3]);
-      }
-    } else if (P.fkind == F_MODE) {
-      if (P.fcol[0]) {
-        load_col<BLOCK, VPT, MODE>(P.cache, P.fcol[0], seg, start, vec, n, lo, sel, x);
-        sel &= pred_mode<VPT>(x, P.fmode[0], P.cmp[0], P.cmp[1], rmask);
-      }
-      if (P.fcol[1]) {
-        if (P.fmode[1] == 2) {
-          // crystal AndEQ then OrEQ: (sel && x==c3) || x==c4 (GPUProcessing.h:1252-1253)
-          load_col<BLOCK, VPT, MODE>(P.cache, P.fcol[1], seg, start, vec, n, lo, rmask, x);
-          sel = ((sel & pred_eq<VPT>(x, P.cmp[2])) | pred_eq<VPT>(x, P.cmp[3])) & rmask;
-        } else if (wave_any(sel)) {
-          load_col<BLOCK, VPT, MODE>(P.cache, P.fcol[1], seg, start, vec, n, lo, sel, x);
-          if (P.fmode[1] == 0) sel &= pred_eq<VPT>(x, P.cmp[2]);
-          else if (P.fmode[1] == 1) sel &= pred_between<VPT>(x, P.cmp[2], P.cmp[3]);
-        }
-      }
-    }
-
-    // ---------------------------------------------------------- probes
-    int dv[4][IPT];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int dflt = SINK == COMPACT ? 1 : 0;  // BlockSetValue(...,1) / (...,0)
-#pragma unroll
-      for (int j = 0; j < IPT; j++) dv[k][j] = dflt;
-      if (!(P.pmask >> k & 1)) continue;
-      if (P.key[k] && P.ht[k]) {
-        if (!wave_any(sel)) continue;
-        load_col<BLOCK, VPT, MODE>(P.cache, P.key[k], seg, start, vec, n, lo, sel, x);
-        const int* ht = P.ht[k];
-        const int len = P.len[k], mk = P.minkey[k];
-#pragma unroll
-        for (int j = 0; j < IPT; j++) {
-          if (!(sel >> j & 1u)) continue;
-          const int h = phash(x[j], len, mk);
-          if (SINK == COMPACT) {
-            // BlockProbeGPU: hit iff ht[2h+1] != 0, dimension row = slot-1
-            const int s = ht[2 * (size_t)h + 1];
-            if (s != 0) dv[k][j] = s - 1;
-            else sel &= ~(1u << j);
-          } else {
-            // BlockProbeGroupByGPU: hit iff the 64-bit slot != 0, value = low word
-            const unsigned long long s =
-                *reinterpret_cast<const unsigned long long*>(ht + 2 * (size_t)h);
-            if (s != 0ull) dv[k][j] = (int)(unsigned)s;
-            else sel &= ~(1u << j);
-          }
-        }
-      } else if (MODE == OFF && SINK == COMPACT && P.in_dim[k]) {
-        // pass an earlier probe's dimension row through (GPUProcessing.h:859-861)
-        load_tile<BLOCK, VPT>(P.in_dim[k] + start, vec, n, dv[k]);
-      } else if (MODE == OFF && (SINK == GROUPBY_LDS || SINK == GROUPBY_GLOBAL) && P.grp[k]) {
-        // group value from an earlier probe's dimension row (GPUProcessing.h:203-206)
-        int d[IPT];
-        load_tile<BLOCK, VPT>(P.in_dim[k] + start, vec, n, d);
-#pragma unroll
-        for (int j = 0; j < IPT; j++)
-          if (sel >> j & 1u) dv[k][j] = gather(P.cache, P.grp[k], d[j]);
-      }
-    }
-
-    // ---------------------------------------------------------- sinks
-    if (SINK == COMPACT) {
-      int pos = block_reserve<BLOCK>(__popc(sel), P.total, scan);
-#pragma unroll
-      for (int j = 0; j < IPT; j++) {
-        if (!(sel >> j & 1u)) continue;
-        P.out[0][pos] = lo[j];
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-          if ((P.pmask >> k & 1) && P.out[k + 1]) P.out[k + 1][pos] = dv[k][j];
-        pos++;
-      }
-      continue;
-    }
-    if (!wave_any(sel)) continue;
-    int a1[IPT], a2[IPT];
-    if (P.aggr[0]) load_col<BLOCK, VPT, MODE>(P.cache, P.aggr[0], seg, start, vec, n, lo, sel, a1);
-    else for (int j = 0; j < IPT; j++) a1[j] = 0;
-    if (P.aggr[1]) load_col<BLOCK, VPT, MODE>(P.cache, P.aggr[1], seg, start, vec, n, lo, sel, a2);
-    else for (int j = 0; j < IPT; j++) a2[j] = 0;
-
-    if (SINK == AGGR) {
-#pragma unroll
-      for (int j = 0; j < IPT; j++)
-        if (sel >> j & 1u) acc += (long long)(a1[j] * a2[j]);  // int32 product (GPUProcessing.h:1763)
-      continue;
-    }
-#pragma unroll
-    for (int j = 0; j < IPT; j++) {
-      if (!(sel >> j & 1u)) continue;
-      const int h = ((dv[0][j] - P.minv[0]) * P.uniq[0] + (dv[1][j] - P.minv[1]) * P.uniq[1] +
-                     (dv[2][j] - P.minv[2]) * P.uniq[2] + (dv[3][j] - P.minv[3]) * P.uniq[3]) %
-                    P.total_val;
-      const unsigned long long v = (unsigned long long)(long long)(a1[j] - a2[j]);
-      int2* key = reinterpret_cast<int2*>(P.res + 6 * (size_t)h);
-      if (SINK == GROUPBY_GLOBAL) {
-        key[0] = make_int2(dv[0][j], dv[1][j]);
-        key[1] = make_int2(dv[2][j], dv[3][j]);
-        atomicAdd(reinterpret_cast<unsigned long long*>(key + 2), v);
-      } else {
-        const uint32_t bit = 1u << (h & 31);
-        if (!(atomicOr(&touched[h >> 5], bit) & bit)) {
-          key[0] = make_int2(dv[0][j], dv[1][j]);
-          key[1] = make_int2(dv[2][j], dv[3][j]);
-        }
-        atomicAdd(&gsum[h], v);
-      }
-    }
-  }
-
-  if (SINK == GROUPBY_LDS) {
-    __syncthreads();
-    for (int h = threadIdx.x; h < P.total_val; h += BLOCK)
-      if (touched[h >> 5] >> (h & 31) & 1u)
-        atomicAdd(reinterpret_cast<unsigned long long*>(P.res + 6 * (size_t)h + 4), gsum[h]);
-  } else if (SINK == AGGR) {
-    const long long s = block_sum<BLOCK>(acc, red);
-    if (threadIdx.x == 0 && s != 0)
-      atomicAdd(reinterpret_cast<unsigned long long*>(P.res + 4), (unsigned long long)s);
-  }
-}
-
-// Perfect-hash build: BlockBuildValueGPU / BlockBuildOffsetGPU
-// (BlockLibrary.cuh:594-735) and their offset-list twins (:737-890).
-template <int BLOCK, int VPT, int MODE>
-__global__ __launch_bounds__(BLOCK) void build_kernel(Build B) {
-  constexpr int IPT = 4 * VPT;
-  constexpr int TILE = BLOCK * IPT;
-  const int ntiles = (B.num_tuples + TILE - 1) / TILE;
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int start = tile * TILE;
-    const int n = min(TILE, B.num_tuples - start);
-    const bool full = n == TILE;
-    const uint32_t rmask = full ? (IPT == 32 ? 0xffffffffu : (1u << IPT) - 1u)
-                                : range_mask<BLOCK, VPT>(n);
-    const bool vec = full && B.aligned;
-    uint32_t sel = rmask;
-    int seg = 0, base_row = 0, row[IPT];
-    if (MODE == SEG) {
-      seg = B.seg_group ? (int)B.seg_group[start >> kSegLog] : (B.start_offset + start) >> kSegLog;
-      base_row = seg * kSegSize + (start & kSegMask);
-    } else if (MODE == PLAIN) {
-      base_row = B.row_base + start;
-    }
-    if (MODE == OFF) {
-      load_tile<BLOCK, VPT>(B.in_rows + start, vec, n, row);
-    } else {
-#pragma unroll
-      for (int j = 0; j < IPT; j++) row[j] = base_row + item_row<BLOCK>(j);
-    }
-    int x[IPT], v[IPT];
-    if (MODE != OFF && B.fcol) {
-      load_col<BLOCK, VPT, MODE>(B.cache, B.fcol, seg, start, vec, n, row, sel, x);
-      sel &= pred_mode<VPT>(x, B.fmode, B.c1, B.c2, rmask);
-    }
-    if (!wave_any(sel)) continue;
-    load_col<BLOCK, VPT, MODE>(B.cache, B.key, seg, start, vec, n, row, sel, x);
-    if (B.val) load_col<BLOCK, VPT, MODE>(B.cache, B.val, seg, start, vec, n, row, sel, v);
-#pragma unroll
-    for (int j = 0; j < IPT; j++) {
-      if (!(sel >> j & 1u)) continue;
-      const size_t h = (size_t)phash(x[j], B.num_slots, B.key_min);
-      if (B.val) atomicCAS(&B.ht[2 * h], 0, v[j]);
-      B.ht[2 * h + 1] = row[j] + 1;
-    }
-  }
-}
 
 // ----------------------------------------------------------------- host side
 
@@ -373,7 +50,7 @@ struct OccHash {
 };
 
 // Number of workgroups the device keeps resident for kernel f (cached).
-static int resident_groups(const void* f, int block, size_t lds) {
+int resident_groups(const void* f, int block, size_t lds) {
   static std::mutex mu;
   static std::unordered_map<OccKey, int, OccHash> cache;
   int dev = 0;
@@ -395,45 +72,166 @@ static int resident_groups(const void* f, int block, size_t lds) {
   return g;
 }
 
-constexpr int kBlock = 256;
-constexpr int kVpt = 2;
-constexpr int kLdsBlock = 512;
-constexpr size_t kLdsGroupBudget = 96 * 1024;
-
-static size_t lds_bytes_for(int total_val) {
-  return ((size_t)total_val + ((size_t)total_val + 63) / 64) * sizeof(unsigned long long);
+// Stream-ordered scratch for the compact probe tables: one memory pool per
+// device, created on first use and kept (freed blocks stay cached for reuse).
+static hipMemPool_t device_pool(int dev) {
+  static std::mutex mu;
+  static std::unordered_map<int, hipMemPool_t> pools;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = pools.find(dev);
+  if (it != pools.end()) return it->second;
+  hipMemPoolProps props{};
+  props.allocType = hipMemAllocationTypePinned;
+  props.handleTypes = hipMemHandleTypeNone;
+  props.location.type = hipMemLocationTypeDevice;
+  props.location.id = dev;
+  hipMemPool_t pool = nullptr;
+  if (hipMemPoolCreate(&pool, &props) != hipSuccess) pool = nullptr;
+  if (pool) {
+    uint64_t keep = ~0ull;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
+  pools[dev] = pool;
+  return pool;
 }
 
-template <int BLOCK, int VPT, int MODE, int SINK>
-static int launch_pipe(const Pipe& P, hipStream_t stream, size_t lds) {
-  constexpr int TILE = BLOCK * 4 * VPT;
-  const void* f = reinterpret_cast<const void*>(&pipeline_kernel<BLOCK, VPT, MODE, SINK>);
-  if (lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
+static bool probed(const Pipe& P, int k) { return (P.pmask >> k & 1) && P.key[k] && P.ht[k]; }
+
+struct ProbePlan {
+  int compact;   // bit k: build a compact table for table k
+  int lds_k;
+  int lds_max_w;
+};
+
+// Which tables get a compact probe table, and which one may live in LDS.
+// A table is compacted when the fact scan is long next to it (>= 16 rows per
+// slot, so the two pre-pass launches are paid back) and either it is probed by
+// every row (the first probe with no filter before it) or it is too large to
+// stay in L2.  The first probe may be copied to LDS when its entries are at
+// most lds_max_w bits wide.
+static ProbePlan plan_probes(const Pipe& P, int sink) {
+  ProbePlan plan{0, -1, 0};
+  const bool filtered = P.fkind != F_NONE && (P.fcol[0] || P.fcol[1]);
+  int first = -1;
+  for (int k = 0; k < 4; k++) {
+    if (!probed(P, k)) continue;
+    const bool is_first = first < 0;
+    if (is_first) first = k;
+    const int len = P.len[k];
+    if (len <= 0 || (long long)P.num_tuples < 16LL * len) continue;
+    if ((is_first && !filtered) || len >= kCompactMinLen) plan.compact |= 1 << k;
   }
-  const int ntiles = (P.num_tuples + TILE - 1) / TILE;
-  const int grid = std::max(1, std::min(ntiles, resident_groups(f, BLOCK, lds)));
-  pipeline_kernel<BLOCK, VPT, MODE, SINK><<<grid, BLOCK, lds, stream>>>(P);
-  return hipGetLastError();
+  if (first >= 0 && !filtered && (plan.compact >> first & 1)) {
+    const size_t g = group_lds_bytes(sink, P.total_val);
+    for (int w = 8; w >= 1; w >>= 1) {
+      if (g + staged_lds_bytes(P.len[first], w) <= kLdsBudget) {
+        plan.lds_k = first;
+        plan.lds_max_w = w;
+        break;
+      }
+    }
+  }
+  return plan;
+}
+
+// Compile-time shape of a segment-mode fused group-by or aggregate (see
+// pipeline_kernel), or kShapeGeneric.
+static int shape_of(const Pipe& P, int mode, int sink) {
+  if (mode != SEG || (sink != GROUPBY_LDS && sink != AGGR)) return kShapeGeneric;
+  if (P.fkind != F_NONE && P.fkind != F_BETWEEN) return kShapeGeneric;
+  int s = 0;
+  for (int k = 0; k < 4; k++)
+    if (probed(P, k)) s |= 1 << k;
+  if (P.fkind == F_BETWEEN) {
+    if (P.fcol[0]) s |= 0x10;
+    if (P.fcol[1]) s |= 0x20;
+  }
+  return s;
 }
 
 template <int MODE>
 static int run_pipe(const Pipe& P, int sink, hipStream_t stream) {
+  const int shape = shape_of(P, MODE, sink);
+  if (shape != kShapeGeneric) {
+    const int rc = sink == AGGR ? launch_aggr_spec(P, shape, stream) : launch_groupby_lds_spec(P, shape, stream);
+    if (rc != -1) return rc;
+  }
   switch (sink) {
-    case GROUPBY_LDS:
-      return launch_pipe<kLdsBlock, kVpt, MODE, GROUPBY_LDS>(P, stream, lds_bytes_for(P.total_val));
-    case GROUPBY_GLOBAL: return launch_pipe<kBlock, kVpt, MODE, GROUPBY_GLOBAL>(P, stream, 0);
-    case AGGR: return launch_pipe<kBlock, kVpt, MODE, AGGR>(P, stream, 0);
-    default: return launch_pipe<kBlock, kVpt, MODE, COMPACT>(P, stream, 0);
+    case GROUPBY_LDS: return launch_pipe<kLdsBlock, kVpt, MODE, GROUPBY_LDS>(P, stream);
+    case GROUPBY_GLOBAL: return launch_pipe<kBlock, kVpt, MODE, GROUPBY_GLOBAL>(P, stream);
+    case AGGR: return launch_pipe<kAggrBlock, kVpt, MODE, AGGR>(P, stream);
+    default: return launch_pipe<kBlock, kVpt, MODE, COMPACT>(P, stream);
   }
 }
 
-static int dispatch(const Pipe& P, int mode, int sink, hipStream_t stream) {
+// Build the compact probe tables of `plan` into pool scratch (stream-ordered).
+// Returns the scratch block (to free after the fact kernel) or nullptr, in
+// which case the fact kernel probes the hash tables directly.
+static void* build_compact(Pipe& P, const ProbePlan& plan, int sink, hipStream_t stream) {
+  if (!plan.compact) return nullptr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  hipMemPool_t pool = device_pool(dev);
+  if (!pool) return nullptr;
+  size_t off[4] = {0, 0, 0, 0}, bytes = 0;
+  for (int k = 0; k < 4; k++) {
+    if (!(plan.compact >> k & 1)) continue;
+    off[k] = bytes;
+    bytes += 256 + kStatsGroups * sizeof(int4) + ((compact_words_bytes(P.len[k]) + 255) & ~(size_t)255);
+  }
+  void* scratch = nullptr;
+  if (hipMallocFromPoolAsync(&scratch, bytes, pool, stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  Prep prep{};
+  prep.row_probe = sink == COMPACT;
+  int nt = 0, tab_of[4];
+  for (int k = 0; k < 4; k++) {
+    if (!(plan.compact >> k & 1)) continue;
+    char* base = static_cast<char*>(scratch) + off[k];
+    prep.ht[nt] = P.ht[k];
+    prep.len[nt] = P.len[k];
+    prep.hdr[nt] = reinterpret_cast<CompactHdr*>(base);
+    prep.partial[nt] = reinterpret_cast<int4*>(base + 256);
+    prep.words[nt] = reinterpret_cast<uint32_t*>(base + 256 + kStatsGroups * sizeof(int4));
+    P.chdr[k] = prep.hdr[nt];
+    P.cwords[k] = prep.words[nt];
+    tab_of[nt++] = k;
+  }
+  (void)tab_of;
+  prep_stats_kernel<<<dim3(kStatsGroups, nt), kPrepBlock, 0, stream>>>(prep);
+  int maxlen = 0;
+  for (int t = 0; t < nt; t++) maxlen = std::max(maxlen, prep.len[t]);
+  const int pack_groups = std::max(1, std::min(256, (maxlen / 2 + kPrepBlock - 1) / kPrepBlock));
+  prep_pack_kernel<<<dim3(pack_groups, nt), kPrepBlock, 0, stream>>>(prep);
+  if (hipGetLastError() != hipSuccess) {
+    (void)hipFreeAsync(scratch, stream);
+    for (int k = 0; k < 4; k++) { P.chdr[k] = nullptr; P.cwords[k] = nullptr; }
+    return nullptr;
+  }
+  return scratch;
+}
+
+// sink < 0: group-by, placed in LDS when the dense array fits.
+static int dispatch(Pipe P, int mode, int sink, hipStream_t stream) {
   if (P.num_tuples == 0) return hipSuccess;
-  if (mode == SEG) return run_pipe<SEG>(P, sink, stream);
-  if (mode == OFF) return run_pipe<OFF>(P, sink, stream);
-  return run_pipe<PLAIN>(P, sink, stream);
+  if (sink < 0) sink = group_lds_bytes(GROUPBY_LDS, P.total_val) <= kLdsBudget ? GROUPBY_LDS : GROUPBY_GLOBAL;
+  const ProbePlan plan = plan_probes(P, sink);
+  void* scratch = build_compact(P, plan, sink, stream);
+  if (scratch && plan.lds_k >= 0) {
+    P.lds_k = plan.lds_k;
+    P.lds_max_w = plan.lds_max_w;
+  }
+  int rc;
+  if (mode == SEG) rc = run_pipe<SEG>(P, sink, stream);
+  else if (mode == OFF) rc = run_pipe<OFF>(P, sink, stream);
+  else rc = run_pipe<PLAIN>(P, sink, stream);
+  if (scratch) {
+    const hipError_t e = hipFreeAsync(scratch, stream);
+    if (rc == hipSuccess) rc = e;
+  }
+  return rc;
 }
 
 static bool aligned16(const void* p) { return p == nullptr || ((uintptr_t)p & 15u) == 0; }
@@ -443,6 +241,7 @@ static Pipe make_pipe(const int* cache, int num_tuples) {
   P.cache = cache;
   P.num_tuples = num_tuples;
   P.aligned = aligned16(cache);  // SEG mode streams from the cache
+  P.lds_k = -1;
   return P;
 }
 
@@ -502,9 +301,7 @@ static bool groupby_inputs_ok(const Pipe& P) {
   return true;
 }
 
-static int groupby_sink(const Pipe& P) {
-  return lds_bytes_for(P.total_val) <= kLdsGroupBudget ? GROUPBY_LDS : GROUPBY_GLOBAL;
-}
+static constexpr int kGroupBySink = -1;
 
 template <int MODE>
 static int run_build(const Build& B, hipStream_t stream) {
@@ -512,7 +309,7 @@ static int run_build(const Build& B, hipStream_t stream) {
   constexpr int TILE = kBlock * 4 * kVpt;
   const void* f = reinterpret_cast<const void*>(&build_kernel<kBlock, kVpt, MODE>);
   const int ntiles = (B.num_tuples + TILE - 1) / TILE;
-  const int grid = std::max(1, std::min(ntiles, resident_groups(f, kBlock, 0)));
+  const int grid = balanced_grid(ntiles, resident_groups(f, kBlock, 0));
   build_kernel<kBlock, kVpt, MODE><<<grid, kBlock, 0, stream>>>(B);
   return hipGetLastError();
 }
@@ -540,7 +337,7 @@ int mordred_filter_probe_group_by_GPU2(const int* gpuCache, const filterArgsGPU*
   put_filter_between(P, fargs, F_BETWEEN);
   put_probe(P, pargs, 0xF);
   put_groupby(P, gargs);
-  return dispatch(P, SEG, groupby_sink(P), (hipStream_t)stream);
+  return dispatch(P, SEG, kGroupBySink, (hipStream_t)stream);
 }
 
 int mordred_filter_probe_group_by_GPU3(const int* gpuCache, const offsetGPU* offset,
@@ -554,7 +351,7 @@ int mordred_filter_probe_group_by_GPU3(const int* gpuCache, const offsetGPU* off
   put_probe(P, pargs, 0xF);
   put_groupby(P, gargs);
   INVALID_IF(!groupby_inputs_ok(P));
-  return dispatch(P, OFF, groupby_sink(P), (hipStream_t)stream);
+  return dispatch(P, OFF, kGroupBySink, (hipStream_t)stream);
 }
 
 int mordred_probe_group_by_GPU2(const int* gpuCache, const probeArgsGPU* pargs,
@@ -782,7 +579,7 @@ int mordred_probe_group_by_GPU(const int* key1, const int* key2, const int* key3
   P.total_val = total_val;
   P.aligned = aligned16(key1) && aligned16(key2) && aligned16(key3) && aligned16(key4) &&
               aligned16(aggr1) && aligned16(aggr2);
-  return dispatch(P, PLAIN, groupby_sink(P), (hipStream_t)stream);
+  return dispatch(P, PLAIN, kGroupBySink, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -1,0 +1,550 @@
+// Device side of the query pipeline: argument blocks, the generic tile
+// pipeline kernel and the perfect-hash build kernel.  Included by every
+// translation unit that instantiates kernels (kernels.hip for the generic
+// shapes, spec_*.hip for the compile-time specialised query shapes).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/mordred_hip.h"
+#include "probe_tables.hpp"
+#include "tile.hpp"
+
+namespace mordred {
+
+enum Mode { SEG = 0, OFF = 1, PLAIN = 2 };
+enum Sink { GROUPBY_GLOBAL = 0, GROUPBY_LDS = 1, AGGR = 2, COMPACT = 3 };
+enum FilterKind {
+  F_NONE = 0,
+  F_BETWEEN = 1,       // fused kernels: filter1 in [c1,c2], filter2 in [c3,c4]
+  F_BETWEEN2ONLY = 2,  // filter_probe_aggr_GPU3 re-checks filter2 only (GPUProcessing.h:1805)
+  F_MODE = 3           // filter kernels: mode 0 EQ, 1 BETWEEN, 2 EQ-or-EQ
+};
+
+struct Pipe {
+  const int* cache;
+  const short* seg_group;
+  int start_offset;
+  int num_tuples;
+  int row_base;   // PLAIN: row id of position 0
+  int aligned;    // OFF/PLAIN: all contiguous inputs are 16-byte aligned
+  const int* in_lo;
+  const int* in_dim[4];
+  int fkind;
+  const int* fcol[2];
+  int cmp[4];
+  int fmode[2];
+  int pmask;      // bit k: table k+1 takes part in probing / dim-offset output
+  const int* key[4];
+  const int* ht[4];
+  int len[4];
+  int minkey[4];
+  const int* aggr[2];
+  const int* grp[4];
+  int minv[4];
+  int uniq[4];
+  int total_val;
+  int* res;
+  int* total;
+  int* out[5];
+  // compact probe tables (probe_tables.hpp): header + packed words per table,
+  // nullptr = probe the hash table itself
+  const CompactHdr* chdr[4];
+  const uint32_t* cwords[4];
+  int lds_k;      // table whose compact words are copied to LDS, or -1
+  int lds_max_w;  // ... when its entry width is at most this
+};
+
+struct Build {
+  const int* cache;
+  const short* seg_group;
+  int start_offset;
+  int num_tuples;
+  int row_base;
+  int aligned;
+  const int* in_rows;
+  const int* fcol;
+  int fmode;
+  int c1, c2;
+  const int* key;
+  const int* val;
+  int* ht;
+  int num_slots;
+  int key_min;
+};
+
+// Contiguous tile pointer of a column: SEG -> cached segment run, PLAIN -> column.
+template <int MODE>
+__device__ __forceinline__ const int* tile_ptr(const int* cache, const int* col, int seg,
+                                               int start) {
+  if (MODE == SEG) return cache + ((size_t)col[seg] << kSegLog) + (start & kSegMask);
+  return col + start;
+}
+
+// Column values of this thread's items.  OFF mode gathers only items in `m`.
+template <int BLOCK, int VPT, int MODE>
+__device__ __forceinline__ void load_col(const int* cache, const int* col, int seg, int start,
+                                         bool vec, int n, const int (&lo)[4 * VPT], uint32_t m,
+                                         int (&x)[4 * VPT]) {
+  if (MODE == OFF) {
+#pragma unroll
+    for (int j = 0; j < 4 * VPT; j++) x[j] = (m >> j & 1u) ? gather(cache, col, lo[j]) : 0;
+  } else {
+    load_tile<BLOCK, VPT>(tile_ptr<MODE>(cache, col, seg, start), vec, n, x);
+  }
+}
+
+template <int VPT>
+__device__ __forceinline__ uint32_t pred_between(const int (&x)[4 * VPT], int lo, int hi) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int j = 0; j < 4 * VPT; j++) m |= (uint32_t)(x[j] >= lo && x[j] <= hi) << j;
+  return m;
+}
+
+template <int VPT>
+__device__ __forceinline__ uint32_t pred_eq(const int (&x)[4 * VPT], int c) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int j = 0; j < 4 * VPT; j++) m |= (uint32_t)(x[j] == c) << j;
+  return m;
+}
+
+// Mode predicate of filter_GPU2 / build_GPU2 (GPUProcessing.h:1231-1239, 972-980).
+template <int VPT>
+__device__ __forceinline__ uint32_t pred_mode(const int (&x)[4 * VPT], int mode, int c1, int c2,
+                                              uint32_t all) {
+  if (mode == 0) return pred_eq<VPT>(x, c1);
+  if (mode == 1) return pred_between<VPT>(x, c1, c2);
+  if (mode == 2) return pred_eq<VPT>(x, c1) | pred_eq<VPT>(x, c2);
+  return all;
+}
+
+// SHAPE < 0: everything decided at run time from the argument pointers.
+// SHAPE >= 0 (segment mode, fact BETWEEN filters, group-by/aggregate sinks):
+// bits 0-3 = tables probed, bit 4 / bit 5 = filter column 1 / 2 present; the
+// code for absent tables and filters is compiled out.
+constexpr int kShapeGeneric = -1;
+
+template <int BLOCK, int VPT, int MODE, int SINK, int SHAPE = kShapeGeneric>
+__global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
+  constexpr bool kGen = SHAPE < 0;
+  static_assert(kGen || (MODE != OFF && SINK != COMPACT), "specialised shapes: SEG/PLAIN group-by or aggregate");
+  constexpr int IPT = 4 * VPT;
+  constexpr int TILE = BLOCK * IPT;
+  constexpr bool kGroupBy = SINK == GROUPBY_LDS || SINK == GROUPBY_GLOBAL;
+  // SEG/PLAIN tiles are contiguous runs: every column load of a tile is issued
+  // before any of them is used.  OFF tiles gather only rows still selected.
+  constexpr bool kEager = MODE != OFF;
+  __shared__ long long red[BLOCK / kWave];
+  __shared__ int scan[BLOCK / kWave + 1];
+  // dynamic LDS: [group sums (u64) | group touch bitmap] for GROUPBY_LDS, then
+  // the packed words of the LDS-staged compact probe table
+  extern __shared__ unsigned long long lds[];
+  const int gwords = SINK == GROUPBY_LDS ? P.total_val + (P.total_val + 63) / 64 : 0;
+  unsigned long long* gsum = lds;
+  uint32_t* touched = reinterpret_cast<uint32_t*>(lds + (SINK == GROUPBY_LDS ? P.total_val : 0));
+  uint32_t* ldsw = reinterpret_cast<uint32_t*>(lds + gwords);
+
+  // per-table compact probe parameters (workgroup-uniform)
+  int cw[4], csh[4], cvmin[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    cw[k] = 0; csh[k] = 0; cvmin[k] = 0;
+    if ((kGen || (SHAPE >> k & 1)) && P.chdr[k]) {
+      const CompactHdr h = *P.chdr[k];
+      cw[k] = h.w; csh[k] = h.shift; cvmin[k] = h.vmin;
+    }
+  }
+
+  if (SINK == GROUPBY_LDS) {
+    for (int i = threadIdx.x; i < gwords; i += BLOCK) lds[i] = 0;
+  }
+  bool staged = false;
+  if (P.lds_k >= 0) {
+    const int k = P.lds_k;
+    const int w = k == 0 ? cw[0] : k == 1 ? cw[1] : k == 2 ? cw[2] : cw[3];
+    const int sh = k == 0 ? csh[0] : k == 1 ? csh[1] : k == 2 ? csh[2] : csh[3];
+    if (w > 0 && w <= P.lds_max_w) {
+      staged = true;
+      const int nwords = (P.len[k] + (1 << sh) - 1) >> sh;
+      const uint32_t* src = P.cwords[k];
+      for (int i = threadIdx.x; i < nwords; i += BLOCK) ldsw[i] = src[i];
+    }
+  }
+  if (SINK == GROUPBY_LDS || staged) __syncthreads();
+
+  long long acc = 0;
+  const int ntiles = (P.num_tuples + TILE - 1) / TILE;
+
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int start = tile * TILE;
+    const int n = min(TILE, P.num_tuples - start);
+    const bool full = n == TILE;
+    const uint32_t rmask = full ? (IPT == 32 ? 0xffffffffu : (1u << IPT) - 1u)
+                                : range_mask<BLOCK, VPT>(n);
+    const bool vec = full && P.aligned;
+    uint32_t sel = rmask;
+
+    int seg = 0, base_row = 0;
+    int lo[IPT];
+    if (MODE == SEG) {
+      seg = P.seg_group ? (int)P.seg_group[start >> kSegLog] : (P.start_offset + start) >> kSegLog;
+      base_row = seg * kSegSize + (start & kSegMask);
+    } else if (MODE == PLAIN) {
+      base_row = P.row_base + start;
+    }
+    if (MODE == OFF) {
+      load_tile<BLOCK, VPT>(P.in_lo + start, vec, n, lo);
+    } else {
+#pragma unroll
+      for (int j = 0; j < IPT; j++) lo[j] = base_row + item_row<BLOCK>(j);
+    }
+
+    // column values: fv filters, kv probe keys (replaced in place by the probe
+    // result: group value or dimension row), av aggregates
+    int fv[2][IPT], kv[4][IPT], av[2][IPT];
+    if (kEager) {
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+        if (kGen ? P.fcol[c] != nullptr : (SHAPE >> (4 + c) & 1))
+          load_col<BLOCK, VPT, MODE>(P.cache, P.fcol[c], seg, start, vec, n, lo, sel, fv[c]);
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (kGen ? ((P.pmask >> k & 1) && P.key[k] && P.ht[k]) : (SHAPE >> k & 1))
+          load_col<BLOCK, VPT, MODE>(P.cache, P.key[k], seg, start, vec, n, lo, sel, kv[k]);
+      if (SINK != COMPACT) {
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+          if (P.aggr[c]) load_col<BLOCK, VPT, MODE>(P.cache, P.aggr[c], seg, start, vec, n, lo, sel, av[c]);
+      }
+    }
+
+    // ---------------------------------------------------------- filters
+    if (!kGen) {
+      if (SHAPE >> 4 & 1) sel &= pred_between<VPT>(fv[0], P.cmp[0], P.cmp[1]);
+      if (SHAPE >> 5 & 1) sel &= pred_between<VPT>(fv[1], P.cmp[2], P.cmp[3]);
+    } else if (P.fkind == F_BETWEEN || P.fkind == F_BETWEEN2ONLY) {
+      if (P.fkind == F_BETWEEN && P.fcol[0]) {
+        if (!kEager) load_col<BLOCK, VPT, MODE>(P.cache, P.fcol[0], seg, start, vec, n, lo, sel, fv[0]);
+        sel &= pred_between<VPT>(fv[0], P.cmp[0], P.cmp[1]);
+      }
+      if (P.fcol[1] && (kEager || wave_any(sel))) {
+        if (!kEager) load_col<BLOCK, VPT, MODE>(P.cache, P.fcol[1], seg, start, vec, n, lo, sel, fv[1]);
+        sel &= pred_between<VPT>(fv[1], P.cmp[2], P.cmp[3]);
+      }
+    } else if (P.fkind == F_MODE) {
+      if (P.fcol[0]) {
+        if (!kEager) load_col<BLOCK, VPT, MODE>(P.cache, P.fcol[0], seg, start, vec, n, lo, sel, fv[0]);
+        sel &= pred_mode<VPT>(fv[0], P.fmode[0], P.cmp[0], P.cmp[1], rmask);
+      }
+      if (P.fcol[1]) {
+        if (P.fmode[1] == 2) {
+          // crystal AndEQ then OrEQ: (sel && x==c3) || x==c4 (GPUProcessing.h:1252-1253)
+          if (!kEager) load_col<BLOCK, VPT, MODE>(P.cache, P.fcol[1], seg, start, vec, n, lo, rmask, fv[1]);
+          sel = ((sel & pred_eq<VPT>(fv[1], P.cmp[2])) | pred_eq<VPT>(fv[1], P.cmp[3])) & rmask;
+        } else if (kEager || wave_any(sel)) {
+          if (!kEager) load_col<BLOCK, VPT, MODE>(P.cache, P.fcol[1], seg, start, vec, n, lo, sel, fv[1]);
+          if (P.fmode[1] == 0) sel &= pred_eq<VPT>(fv[1], P.cmp[2]);
+          else if (P.fmode[1] == 1) sel &= pred_between<VPT>(fv[1], P.cmp[2], P.cmp[3]);
+        }
+      }
+    }
+
+    // ---------------------------------------------------------- probes
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const bool probed = kGen ? ((P.pmask >> k & 1) && P.key[k] && P.ht[k]) : (SHAPE >> k & 1);
+      if (!probed) {
+        const int dflt = SINK == COMPACT ? 1 : 0;  // BlockSetValue(...,1) / (...,0)
+#pragma unroll
+        for (int j = 0; j < IPT; j++) kv[k][j] = dflt;
+        if (!kGen || !(P.pmask >> k & 1)) continue;
+        if (MODE == OFF && SINK == COMPACT && P.in_dim[k]) {
+          // pass an earlier probe's dimension row through (GPUProcessing.h:859-861)
+          load_tile<BLOCK, VPT>(P.in_dim[k] + start, vec, n, kv[k]);
+        } else if (MODE == OFF && kGroupBy && P.grp[k]) {
+          // group value from an earlier probe's dimension row (GPUProcessing.h:203-206)
+          int d[IPT];
+          load_tile<BLOCK, VPT>(P.in_dim[k] + start, vec, n, d);
+#pragma unroll
+          for (int j = 0; j < IPT; j++)
+            if (sel >> j & 1u) kv[k][j] = gather(P.cache, P.grp[k], d[j]);
+        }
+        continue;
+      }
+      if (!kEager) {
+        if (!wave_any(sel)) continue;
+        load_col<BLOCK, VPT, MODE>(P.cache, P.key[k], seg, start, vec, n, lo, sel, kv[k]);
+      }
+      const int* ht = P.ht[k];
+      const int len = P.len[k], mk = P.minkey[k];
+      // All slot reads of the tile are issued before any is used, so the
+      // 4*VPT probes of a thread overlap instead of paying one L2/MALL round
+      // trip each.  Items that are not probed read slot 0 (always valid), which
+      // the address coalescer folds into a single request per wave.
+      int hs[IPT];
+      uint32_t act = sel;
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        const int h = phash(kv[k][j], len, mk);
+        const bool a = sel >> j & 1u;
+        hs[j] = a ? h : 0;
+      }
+      if (cw[k] > 0) {
+        // compact probe table: entry e = 0 on a miss, else result - vmin + 1
+        const int w = cw[k], sh = csh[k], vmin = cvmin[k];
+        const uint32_t emask = (1u << w) - 1u;
+        uint32_t wv[IPT];
+        if (staged && k == P.lds_k) {
+#pragma unroll
+          for (int j = 0; j < IPT; j++) wv[j] = ldsw[hs[j] >> sh];
+        } else {
+          const uint32_t* cwk = P.cwords[k];
+#pragma unroll
+          for (int j = 0; j < IPT; j++) wv[j] = cwk[hs[j] >> sh];
+        }
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+          const uint32_t e = (wv[j] >> ((hs[j] & ((1 << sh) - 1)) * w)) & emask;
+          const bool hit = (act >> j & 1u) && e != 0u;
+          kv[k][j] = hit ? vmin + (int)e - 1 : kv[k][j];
+          if (!hit) act &= ~(1u << j);
+        }
+      } else if (SINK == COMPACT) {
+        // BlockProbeGPU: hit iff ht[2h+1] != 0, dimension row = slot-1
+        int sv[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; j++) sv[j] = ht[2 * (size_t)hs[j] + 1];
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+          const bool hit = (act >> j & 1u) && sv[j] != 0;
+          kv[k][j] = hit ? sv[j] - 1 : kv[k][j];
+          if (!hit) act &= ~(1u << j);
+        }
+      } else {
+        // BlockProbeGroupByGPU: hit iff the 64-bit slot != 0, value = low word
+        unsigned long long sv[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; j++)
+          sv[j] = *reinterpret_cast<const unsigned long long*>(ht + 2 * (size_t)hs[j]);
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+          const bool hit = (act >> j & 1u) && sv[j] != 0ull;
+          kv[k][j] = hit ? (int)(unsigned)sv[j] : kv[k][j];
+          if (!hit) act &= ~(1u << j);
+        }
+      }
+      sel = act;
+    }
+
+    // ---------------------------------------------------------- sinks
+    if (SINK == COMPACT) {
+      int pos = block_reserve<BLOCK>(__popc(sel), P.total, scan);
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        if (!(sel >> j & 1u)) continue;
+        P.out[0][pos] = lo[j];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if ((P.pmask >> k & 1) && P.out[k + 1]) P.out[k + 1][pos] = kv[k][j];
+        pos++;
+      }
+      continue;
+    }
+    if (!wave_any(sel)) continue;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      if (!P.aggr[c]) {
+#pragma unroll
+        for (int j = 0; j < IPT; j++) av[c][j] = 0;
+      } else if (!kEager) {
+        load_col<BLOCK, VPT, MODE>(P.cache, P.aggr[c], seg, start, vec, n, lo, sel, av[c]);
+      }
+    }
+
+    if (SINK == AGGR) {
+#pragma unroll
+      for (int j = 0; j < IPT; j++)
+        if (sel >> j & 1u) acc += (long long)(av[0][j] * av[1][j]);  // int32 product (GPUProcessing.h:1763)
+      continue;
+    }
+    // dense group index (GPUProcessing.h:142) of every selected item first,
+    // then the updates, so LDS round trips overlap
+    int gh[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++)
+      gh[j] = (sel >> j & 1u)
+                  ? mod_pos((kv[0][j] - P.minv[0]) * P.uniq[0] + (kv[1][j] - P.minv[1]) * P.uniq[1] +
+                                (kv[2][j] - P.minv[2]) * P.uniq[2] + (kv[3][j] - P.minv[3]) * P.uniq[3],
+                            P.total_val)
+                  : 0;
+    if (SINK == GROUPBY_GLOBAL) {
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        if (!(sel >> j & 1u)) continue;
+        int2* key = reinterpret_cast<int2*>(P.res + 6 * (size_t)gh[j]);
+        key[0] = make_int2(kv[0][j], kv[1][j]);
+        key[1] = make_int2(kv[2][j], kv[3][j]);
+        atomicAdd(reinterpret_cast<unsigned long long*>(key + 2),
+                  (unsigned long long)(long long)(av[0][j] - av[1][j]));
+      }
+    } else {
+      uint32_t first = 0;
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        if (!(sel >> j & 1u)) continue;
+        const uint32_t bit = 1u << (gh[j] & 31);
+        if (!(atomicOr(&touched[gh[j] >> 5], bit) & bit)) first |= 1u << j;
+      }
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        if (!(sel >> j & 1u)) continue;
+        atomicAdd(&gsum[gh[j]], (unsigned long long)(long long)(av[0][j] - av[1][j]));
+        if (first >> j & 1u) {
+          // keys are written by the first item of this workgroup that reaches
+          // the group; the reference stores them for every item, with the
+          // same values
+          int2* key = reinterpret_cast<int2*>(P.res + 6 * (size_t)gh[j]);
+          key[0] = make_int2(kv[0][j], kv[1][j]);
+          key[1] = make_int2(kv[2][j], kv[3][j]);
+        }
+      }
+    }
+  }
+
+  if (SINK == GROUPBY_LDS) {
+    __syncthreads();
+    for (int h = threadIdx.x; h < P.total_val; h += BLOCK)
+      if (touched[h >> 5] >> (h & 31) & 1u)
+        atomicAdd(reinterpret_cast<unsigned long long*>(P.res + 6 * (size_t)h + 4), gsum[h]);
+  } else if (SINK == AGGR) {
+    const long long s = block_sum<BLOCK>(acc, red);
+    if (threadIdx.x == 0 && s != 0)
+      atomicAdd(reinterpret_cast<unsigned long long*>(P.res + 4), (unsigned long long)s);
+  }
+}
+
+// Perfect-hash build: BlockBuildValueGPU / BlockBuildOffsetGPU
+// (BlockLibrary.cuh:594-735) and their offset-list twins (:737-890).
+template <int BLOCK, int VPT, int MODE>
+__global__ __launch_bounds__(BLOCK) void build_kernel(Build B) {
+  constexpr int IPT = 4 * VPT;
+  constexpr int TILE = BLOCK * IPT;
+  const int ntiles = (B.num_tuples + TILE - 1) / TILE;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int start = tile * TILE;
+    const int n = min(TILE, B.num_tuples - start);
+    const bool full = n == TILE;
+    const uint32_t rmask = full ? (IPT == 32 ? 0xffffffffu : (1u << IPT) - 1u)
+                                : range_mask<BLOCK, VPT>(n);
+    const bool vec = full && B.aligned;
+    uint32_t sel = rmask;
+    int seg = 0, base_row = 0, row[IPT];
+    if (MODE == SEG) {
+      seg = B.seg_group ? (int)B.seg_group[start >> kSegLog] : (B.start_offset + start) >> kSegLog;
+      base_row = seg * kSegSize + (start & kSegMask);
+    } else if (MODE == PLAIN) {
+      base_row = B.row_base + start;
+    }
+    if (MODE == OFF) {
+      load_tile<BLOCK, VPT>(B.in_rows + start, vec, n, row);
+    } else {
+#pragma unroll
+      for (int j = 0; j < IPT; j++) row[j] = base_row + item_row<BLOCK>(j);
+    }
+    int x[IPT], v[IPT];
+    if (MODE != OFF && B.fcol) {
+      load_col<BLOCK, VPT, MODE>(B.cache, B.fcol, seg, start, vec, n, row, sel, x);
+      sel &= pred_mode<VPT>(x, B.fmode, B.c1, B.c2, rmask);
+    }
+    if (!wave_any(sel)) continue;
+    load_col<BLOCK, VPT, MODE>(B.cache, B.key, seg, start, vec, n, row, sel, x);
+    if (B.val) load_col<BLOCK, VPT, MODE>(B.cache, B.val, seg, start, vec, n, row, sel, v);
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      if (!(sel >> j & 1u)) continue;
+      const size_t h = (size_t)phash(x[j], B.num_slots, B.key_min);
+      if (B.val) atomicCAS(&B.ht[2 * h], 0, v[j]);
+      B.ht[2 * h + 1] = row[j] + 1;
+    }
+  }
+}
+
+
+// ------------------------------------------------------------ host helpers
+
+
+#ifndef MORDRED_BLOCK
+#define MORDRED_BLOCK 256
+#endif
+#ifndef MORDRED_VPT
+#define MORDRED_VPT 2
+#endif
+#ifndef MORDRED_AGGR_BLOCK
+#define MORDRED_AGGR_BLOCK 512
+#endif
+#ifndef MORDRED_LDS_BLOCK
+#define MORDRED_LDS_BLOCK 1024
+#endif
+constexpr int kBlock = MORDRED_BLOCK;
+constexpr int kVpt = MORDRED_VPT;
+constexpr int kLdsBlock = MORDRED_LDS_BLOCK;
+// scalar aggregates end with one same-address atomic per workgroup, so they
+// use fewer, larger workgroups
+constexpr int kAggrBlock = MORDRED_AGGR_BLOCK;
+constexpr size_t kLdsBudget = 80 * 1024;   // dynamic LDS of one workgroup (2 per CU)
+constexpr int kCompactMinLen = 1 << 17;    // hash tables above 1 MB get a compact table
+
+// Workgroups the device keeps resident for kernel f (kernels.hip, cached).
+int resident_groups(const void* f, int block, size_t lds);
+
+// Persistent grid for ntiles tiles: every workgroup walks the same number of
+// tiles (ceil(ntiles / resident)), so a short input does not end with half the
+// workgroups running one extra tile.
+inline int balanced_grid(int ntiles, int resident) {
+  if (ntiles <= 0) return 1;
+  resident = std::max(1, resident);
+  const int per = (ntiles + resident - 1) / resident;
+  return (ntiles + per - 1) / per;
+}
+
+inline size_t group_lds_bytes(int sink, int total_val) {
+  if (sink != GROUPBY_LDS) return 0;
+  return ((size_t)total_val + ((size_t)total_val + 63) / 64) * sizeof(unsigned long long);
+}
+
+inline size_t staged_lds_bytes(int len, int max_w) {
+  const size_t words = ((size_t)len * max_w + 31) / 32;
+  return ((words + 1) / 2) * sizeof(unsigned long long);
+}
+
+inline size_t pipe_lds_bytes(const Pipe& P, int sink) {
+  size_t b = group_lds_bytes(sink, P.total_val);
+  if (P.lds_k >= 0) b += staged_lds_bytes(P.len[P.lds_k], P.lds_max_w);
+  return b;
+}
+
+template <int BLOCK, int VPT, int MODE, int SINK, int SHAPE = kShapeGeneric>
+int launch_pipe(const Pipe& P, hipStream_t stream) {
+  constexpr int TILE = BLOCK * 4 * VPT;
+  const size_t lds = pipe_lds_bytes(P, SINK);
+  const void* f = reinterpret_cast<const void*>(&pipeline_kernel<BLOCK, VPT, MODE, SINK, SHAPE>);
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  const int grid = balanced_grid((P.num_tuples + TILE - 1) / TILE, resident_groups(f, BLOCK, lds));
+  pipeline_kernel<BLOCK, VPT, MODE, SINK, SHAPE><<<grid, BLOCK, lds, stream>>>(P);
+  return hipGetLastError();
+}
+
+// Launchers of the compile-time specialised shapes (spec_*.hip).  They
+// return -1 when `shape` has no specialisation, so the caller uses the
+// generic kernel.
+int launch_groupby_lds_spec(const Pipe& P, int shape, hipStream_t stream);
+int launch_aggr_spec(const Pipe& P, int shape, hipStream_t stream);
+
+}  // namespace mordred

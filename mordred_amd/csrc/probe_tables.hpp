@@ -1,0 +1,170 @@
+// Compact probe tables: an exact, smaller stand-in for a perfect-hash table,
+// derived on the device right before a fact kernel probes it.
+//
+// A probe of slot h of a reference hash table ([value, row+1] int32 pairs,
+// BlockLibrary.cuh:613-614) yields (hit, result):
+//   value probe (BlockProbeGroupByGPU):  hit = 64-bit slot != 0, result = value word
+//   row probe   (BlockProbeGPU):         hit = row word != 0,    result = row word - 1
+// With vmin the smallest result over hit slots, the entry
+//   e(h) = hit ? result - vmin + 1 : 0
+// carries the same information in ceil(log2(range+2)) bits.  Entries are packed
+// at a power-of-two width w in {1,2,4,8,16} (32/w per 32-bit word, never split
+// across words); a table whose range needs more than 16 bits is not compacted
+// (w = 0) and the kernel probes the hash table itself.  A table built with no
+// value column (e.g. a filtered supplier table of a query that does not group
+// by it) has a constant result and compacts to one bit per slot.
+//
+// The caller's hash tables are only read.  Two launches per query: `stats`
+// writes a (min, max, any) partial per workgroup, `pack` reduces the partials
+// (every workgroup redundantly, so no third launch or pre-zeroed memory is
+// needed), writes the header and the packed words.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tile.hpp"
+
+namespace mordred {
+
+struct CompactHdr {
+  int w;      // entry width in bits; 0 = not compacted
+  int shift;  // log2(entries per 32-bit word)
+  int vmin;
+  int any;    // number of hit slots > 0
+};
+
+constexpr int kPrepMaxTables = 4;
+constexpr int kStatsGroups = 256;   // workgroups of the stats pass per table
+constexpr int kPrepBlock = 256;
+
+struct Prep {
+  const int* ht[kPrepMaxTables];
+  int len[kPrepMaxTables];
+  int row_probe;                    // 1: row probes, 0: value probes
+  int4* partial[kPrepMaxTables];    // kStatsGroups x (min, max, any, -)
+  CompactHdr* hdr[kPrepMaxTables];
+  uint32_t* words[kPrepMaxTables];
+};
+
+__device__ __forceinline__ bool slot_result(const int* ht, int s, int row_probe, int& r) {
+  const int2 v = *reinterpret_cast<const int2*>(ht + 2 * (size_t)s);
+  if (row_probe) {
+    r = v.y - 1;
+    return v.y != 0;
+  }
+  r = v.x;
+  return (v.x | v.y) != 0;
+}
+
+// The two pre-pass kernels are defined in exactly one translation unit
+// (kernels.hip, which defines MORDRED_DEFINE_PREP_KERNELS).
+__global__ void prep_stats_kernel(Prep P);
+__global__ void prep_pack_kernel(Prep P);
+
+#ifdef MORDRED_DEFINE_PREP_KERNELS
+// blockIdx.y = table, blockIdx.x < kStatsGroups
+__global__ __launch_bounds__(kPrepBlock) void prep_stats_kernel(Prep P) {
+  const int t = blockIdx.y;
+  const int* ht = P.ht[t];
+  const int len = P.len[t];
+  int mn = 0x7fffffff, mx = (int)0x80000000, any = 0;
+  for (int s = blockIdx.x * kPrepBlock + threadIdx.x; s < len; s += kStatsGroups * kPrepBlock) {
+    int r;
+    if (slot_result(ht, s, P.row_probe, r)) {
+      mn = min(mn, r);
+      mx = max(mx, r);
+      any = 1;
+    }
+  }
+  __shared__ int smn[kPrepBlock / kWave], smx[kPrepBlock / kWave], sany[kPrepBlock / kWave];
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    mn = min(mn, __shfl_xor(mn, d, kWave));
+    mx = max(mx, __shfl_xor(mx, d, kWave));
+    any |= __shfl_xor(any, d, kWave);
+  }
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  if (lane == 0) { smn[wave] = mn; smx[wave] = mx; sany[wave] = any; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kPrepBlock / kWave; w++) {
+      mn = min(mn, smn[w]);
+      mx = max(mx, smx[w]);
+      any |= sany[w];
+    }
+    P.partial[t][blockIdx.x] = make_int4(mn, mx, any, 0);
+  }
+}
+
+// Entry width for results in [vmin, vmax]: entries take values 0..range+1.
+__device__ __forceinline__ void choose_width(int vmin, int vmax, int any, int& w, int& shift) {
+  if (!any) { w = 1; shift = 5; return; }
+  const unsigned long long top = (unsigned long long)((long long)vmax - (long long)vmin) + 1ull;
+  int bits = 0;
+  while (bits < 64 && (top >> bits) != 0ull) bits++;
+  if (bits <= 1) { w = 1; shift = 5; }
+  else if (bits <= 2) { w = 2; shift = 4; }
+  else if (bits <= 4) { w = 4; shift = 3; }
+  else if (bits <= 8) { w = 8; shift = 2; }
+  else if (bits <= 16) { w = 16; shift = 1; }
+  else { w = 0; shift = 0; }
+}
+
+// blockIdx.y = table; grid-stride over packed words
+__global__ __launch_bounds__(kPrepBlock) void prep_pack_kernel(Prep P) {
+  const int t = blockIdx.y;
+  const int len = P.len[t];
+  __shared__ int4 red[kPrepBlock / kWave];
+  int mn = 0x7fffffff, mx = (int)0x80000000, any = 0;
+  for (int i = threadIdx.x; i < kStatsGroups; i += kPrepBlock) {
+    const int4 p = P.partial[t][i];
+    mn = min(mn, p.x);
+    mx = max(mx, p.y);
+    any |= p.z;
+  }
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    mn = min(mn, __shfl_xor(mn, d, kWave));
+    mx = max(mx, __shfl_xor(mx, d, kWave));
+    any |= __shfl_xor(any, d, kWave);
+  }
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  if (lane == 0) red[wave] = make_int4(mn, mx, any, 0);
+  __syncthreads();
+  for (int i = 0; i < kPrepBlock / kWave; i++) {
+    mn = min(mn, red[i].x);
+    mx = max(mx, red[i].y);
+    any |= red[i].z;
+  }
+  int w, shift;
+  choose_width(mn, mx, any, w, shift);
+  const int vmin = any ? mn : 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CompactHdr h;
+    h.w = w; h.shift = shift; h.vmin = vmin; h.any = any;
+    *P.hdr[t] = h;
+  }
+  if (w == 0) return;
+  const int epw = 1 << shift;
+  const int nwords = (len + epw - 1) >> shift;
+  const int* ht = P.ht[t];
+  for (int wi = blockIdx.x * kPrepBlock + threadIdx.x; wi < nwords; wi += gridDim.x * kPrepBlock) {
+    uint32_t word = 0;
+    const int s0 = wi << shift;
+    for (int e = 0; e < epw; e++) {
+      const int s = s0 + e;
+      int r;
+      if (s < len && slot_result(ht, s, P.row_probe, r))
+        word |= (uint32_t)(r - vmin + 1) << (e * w);
+    }
+    P.words[t][wi] = word;
+  }
+}
+
+#endif  // MORDRED_DEFINE_PREP_KERNELS
+
+// Bytes of the packed words at the widest compact width (16 bits per slot).
+inline size_t compact_words_bytes(int len) { return (((size_t)len + 1) / 2) * sizeof(uint32_t); }
+
+}  // namespace mordred

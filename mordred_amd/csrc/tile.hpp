@@ -24,8 +24,17 @@ constexpr int kWave = 64;
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
+// x % m (C semantics, m > 0) without an integer division when 0 <= x < m,
+// the common case for perfect-hash slots and dense group indexes; only lanes
+// outside that range take the division.
+__device__ __forceinline__ int mod_pos(int x, int m) {
+  int r = x;
+  if ((unsigned)x >= (unsigned)m) r = x % m;
+  return r;
+}
+
 // HASH(X,Y,Z) = ((X-Z) % Y)  (crystal/join.cuh:3)
-__device__ __forceinline__ int phash(int key, int len, int min_key) { return (key - min_key) % len; }
+__device__ __forceinline__ int phash(int key, int len, int min_key) { return mod_pos(key - min_key, len); }
 
 // Value of a cached column at fact/dimension row `row`: the segment table maps
 // row>>20 to a cache slot (BlockLibrary.cuh:554).  64-bit slot arithmetic.

@@ -75,3 +75,36 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(abi, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(ImportError):
         abi.load()
+
+
+def test_library_has_no_unresolved_symbols():
+    """Every symbol the library needs resolves (dlopen with RTLD_NOW)."""
+    C.CDLL(abi.LIB_PATH, mode=os.RTLD_NOW | os.RTLD_LOCAL)
+    out = subprocess.check_output(["nm", "-D", "--undefined-only", abi.LIB_PATH], text=True)
+    missing = [ln.split()[-1] for ln in out.splitlines()
+               if ln.split() and ln.split()[0] == "U" and "@" not in ln.split()[-1]]
+    assert not missing, missing
+
+
+def test_invalid_arguments_are_rejected_before_launch():
+    """Argument checks return hipErrorInvalidValue (1) without touching the
+    device, instead of the reference's device-side assert-and-return."""
+    lib = abi.load()
+    g = abi.GroupbyArgsGPU()
+    g.total_val = 0
+    assert lib.mordred_probe_group_by_GPU2(None, None, C.pointer(g), 10, None, 0, None, None) == 1
+    assert lib.mordred_filter_probe_GPU3(None, None, None, None, None, 5, None, None) == 1
+    # NULL segment group needs a segment-aligned start_offset
+    g.total_val = 7
+    assert lib.mordred_probe_group_by_GPU2(C.c_void_p(16), None, C.pointer(g), 10, C.c_void_p(16), 5, None,
+                                           None) == 1
+    b = abi.BuildArgsGPU(None, None, 0, 0, 0)
+    assert lib.mordred_build_GPU2(C.c_void_p(16), None, C.pointer(b), 10, C.c_void_p(16), 0, None, None) == 1
+    # build_GPU3 refuses a dimension filter (GPUProcessing.h:1017)
+    b = abi.BuildArgsGPU(C.c_void_p(16), None, 8, 0, 0)
+    f = abi.FilterArgsGPU()
+    f.filter_idx1 = 16
+    assert lib.mordred_build_GPU3(C.c_void_p(16), C.c_void_p(16), C.pointer(f), C.pointer(b), 10,
+                                  C.c_void_p(16), None) == 1
+    # an empty input is a successful no-op
+    assert lib.mordred_probe_group_by_GPU2(None, None, C.pointer(g), 0, C.c_void_p(16), 0, None, None) == 0

@@ -459,3 +459,49 @@ def test_on_demand_fact_kernels(gstar):
         abi.call("mordred_probe_group_by_GPU", *args)
     sync()
     np.testing.assert_array_equal(res.cpu().numpy(), oracle_res(q, data, segments=[s0], n=SEG))
+
+
+@pytest.fixture(scope="module")
+def big_part(cuda):
+    """Part table above the compaction threshold (300,000 slots) with enough
+    fact rows (>= 16 per slot) that the fused kernel probes compact tables:
+    supplier 1 bit in LDS, part 16-bit brand entries in L2; row probes of the
+    part table need 19 bits and fall back to the hash table itself."""
+    from mordred_amd import query as Q
+    q = Q.q21(5_000, 300_000)
+    data = Q.make_data(q, 5 * SEG + 4321, seed=77, S=5_000, P=300_000)
+    cache = Q.load_cache(data, q, shuffle_seed=11)
+    return q, data, cache
+
+
+def test_compact_tables_query(big_part):
+    from mordred_amd import query as Q
+    q, data, cache = big_part
+    print("[gpu] compact tables: fused group-by", flush=True)
+    gq = Q.GpuQuery(q, data, cache)
+    for _ in range(2):
+        gq.run()
+        sync()
+        np.testing.assert_array_equal(gq.result(), oracle_res(q, data))
+
+
+def test_compact_tables_probe_rows(big_part):
+    from mordred_amd import abi
+    q, data, cache = big_part
+    print("[gpu] compact tables: probe_GPU2 rows", flush=True)
+    gq = _gpu_query(q, data, cache)
+    n = data.n
+    outs = [_alloc(n) for _ in range(5)]
+    total = _alloc(1)
+    o = abi.OffsetGPU(*[t.data_ptr() for t in outs])
+    abi.call("mordred_probe_GPU2", cache.ptr, C.pointer(gq.pargs), C.pointer(o), n, total.data_ptr(), 0,
+             gq.fact_sg.data_ptr(), stream())
+    sync()
+    m = int(total.item())
+    hts = OQ.build_tables(q, data, CONV_GPU)
+    f, p, g, keep = OQ.fact_args(q, data, hts)
+    ro = [np.zeros(n, dtype=np.int32) for _ in range(5)]
+    rt = np.zeros(1, dtype=np.int32)
+    lib.oracle_probe_CPU(p, Offset(*[iptr(a) for a in ro]), n, iptr(rt), 0, sptr(OQ.segment_group(n)), CONV_GPU)
+    assert m == int(rt[0]) > 0
+    np.testing.assert_array_equal(_sorted_rows([t.cpu().numpy() for t in outs], m), _sorted_rows(ro, m))
