@@ -1,0 +1,119 @@
+/*
+ * mordred_engine.h -- host engine around the MI355X kernels (C ABI).
+ *
+ * The segment cache, the CPU/GPU split and the merge of the Mordred engine:
+ *   - a column store in pinned host memory, cut into 2^20-row segments, with
+ *     an HBM segment cache holding a subset of the segments
+ *     (src/gpudb/CacheManager.cu:34-96, 306-361);
+ *   - per query, fact segments are grouped by the residency of the query's
+ *     columns (QueryOptimizer.cu:1674-1716); a group whose columns are all
+ *     cached runs the fused GPU kernel, the others run on the engine's CPU
+ *     operators (the role of src/gpudb/CPUProcessing.cu); the two partial
+ *     results are merged (CPUProcessing.cu:1786-1819);
+ *   - residency and placement follow the analytic cost model of
+ *     src/gpudb/CostModel.cu:283-440 (bytes / bandwidth, GPU operators free).
+ * On top of the reference's placement, MORDRED_PLACE_COST_SPLIT lets the cost
+ * model send part of the non-resident segments over PCIe to the GPU (streamed
+ * through staging slots of the cache, as the reference's on-demand path does,
+ * QueryProcessing.cu:1001-1187) so that CPU work and transfers finish together.
+ *
+ * All entry points return 0 on success, a positive hipError_t from the HIP
+ * runtime, or -1 for an invalid argument (with a message from
+ * mordred_engine_last_error).  One engine is used by one host thread at a time.
+ */
+#ifndef MORDRED_ENGINE_H
+#define MORDRED_ENGINE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mordred_engine mordred_engine;
+
+/* One joined dimension (the per-table entries of QueryParams, KernelArgs.h:103-144). */
+typedef struct mordred_join {
+  int slot;                /* 1 supplier, 2 customer, 3 part, 4 date (table_id) */
+  const char* fact_key;    /* fact foreign-key column */
+  const char* dim_key;     /* dimension key column */
+  int dim_len;             /* hash-table slots */
+  int min_key;             /* HASH minimum */
+  const char* filter_col;  /* dimension filter column or NULL */
+  int filter_mode;         /* 0 EQ, 1 BETWEEN, 2 EQ-or-EQ */
+  int c1, c2;
+  const char* group_col;   /* value stored in the hash table, or NULL */
+  int min_val, unique_val; /* dense group-index stride of this table */
+} mordred_join;
+
+typedef struct mordred_fact_filter {
+  const char* col;         /* fact column, BETWEEN lo..hi */
+  int lo, hi;
+} mordred_fact_filter;
+
+enum { MORDRED_KIND_AGGR = 0, MORDRED_KIND_GROUPBY = 1 };
+
+typedef struct mordred_star_query {
+  int njoins;
+  mordred_join joins[4];
+  int nfilters;
+  mordred_fact_filter filters[2];
+  const char* aggr1;
+  const char* aggr2;       /* NULL allowed */
+  int kind;                /* AGGR: SUM(a1*a2); GROUPBY: per-group SUM(a1-a2) */
+  int total_val;           /* groups (1 for AGGR) */
+} mordred_star_query;
+
+enum {
+  MORDRED_PLACE_RESIDENCY = 0,  /* reference: cached groups on GPU, the rest on CPU */
+  MORDRED_PLACE_COST_SPLIT = 1, /* cost model splits non-resident segments CPU / PCIe+GPU */
+  MORDRED_PLACE_GPU_ONLY = 2,   /* every non-resident segment streamed to the GPU */
+  MORDRED_PLACE_CPU_ONLY = 3    /* every segment on the CPU operators */
+};
+
+typedef struct mordred_run_stats {
+  double total_ms;         /* whole query, builds to merged result */
+  double gpu_ms;           /* GPU part: builds, cached groups, streamed groups */
+  double cpu_ms;           /* CPU part: builds and CPU groups */
+  double merge_ms;
+  long long gpu_rows, cpu_rows, stream_rows;
+  int gpu_segments, cpu_segments, stream_segments;
+  double est_cpu_ms_per_segment;    /* analytic model */
+  double est_stream_ms_per_segment; /* analytic model */
+} mordred_run_stats;
+
+/* Engine on HIP device `device` with an HBM segment cache of cache_segments
+ * segments (4 MiB each) and `threads` CPU worker threads (0 = hardware). */
+mordred_engine* mordred_engine_create(int device, int cache_segments, int threads);
+void mordred_engine_destroy(mordred_engine* e);
+const char* mordred_engine_last_error(const mordred_engine* e);
+
+/* Copy a column (n int32 values) into the engine's pinned host store.  With
+ * gpu_resident set, every segment is also cached in HBM (dimension tables). */
+int mordred_engine_add_column(mordred_engine* e, const char* name, const int* values, long long n,
+                              int gpu_resident);
+int mordred_engine_column_segments(const mordred_engine* e, const char* name);
+
+/* Cache / evict one segment of a column (CacheManager.cu:306-361). */
+int mordred_engine_cache_segment(mordred_engine* e, const char* name, int segment);
+int mordred_engine_evict_segment(mordred_engine* e, const char* name, int segment);
+/* 1 if the segment of the column is in HBM. */
+int mordred_engine_is_cached(const mordred_engine* e, const char* name, int segment);
+
+/* Analytic-model bandwidths in bytes per second (CostModel.h:5-6 defaults
+ * 42e9 and 12e9). */
+int mordred_engine_set_bandwidth(mordred_engine* e, double bw_cpu, double bw_pci);
+
+/* Cost-model residency: rank the fact segments of q by the cost the GPU saves
+ * on them (CostModel::permute_cost weights) and cache the query's fact
+ * columns of the top `fraction` of segments, evicting the rest. */
+int mordred_engine_place(mordred_engine* e, const mordred_star_query* q, double fraction);
+
+/* Run q.  res (total_val*6 ints, or 6 for AGGR) receives the merged result
+ * (overwritten).  stats may be NULL. */
+int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int placement, int* res,
+                       mordred_run_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MORDRED_ENGINE_H */

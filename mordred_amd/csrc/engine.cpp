@@ -1,0 +1,927 @@
+// Host engine: segment store, HBM segment cache, CPU/GPU split, merge and the
+// analytic cost model (include/mordred_engine.h).
+//
+// Reference anchors: CacheManager.cu:34-96 (store + cache), 306-361 (cache /
+// evict a segment); QueryOptimizer.cu:1674-1747 (segment groups by residency,
+// operator device = GPU iff its columns are cached); QueryProcessing.cu:
+// 1346-1469 (runQuery: dims, fact groups on their own streams / threads, merge);
+// CPUProcessing.cu:439-1104 (the CPU operators); CostModel.cu:283-440 (cost).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/mordred_engine.h"
+#include "../../include/mordred_hip.h"
+
+namespace mordred_host {
+
+constexpr int kSegLog = 20;
+constexpr int kSeg = 1 << kSegLog;
+constexpr int kCpuChunk = 1 << 16;   // rows per CPU task
+constexpr int kStreamBatch = 4;      // segments per streamed batch
+constexpr double kCacheLine = 64.0;  // CostModel.h:4
+
+using Clock = std::chrono::steady_clock;
+static double ms_since(Clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+// ------------------------------------------------------------ worker pool
+// Persistent threads; run() hands out task indexes through an atomic counter
+// and returns when every task has finished.
+class Pool {
+ public:
+  explicit Pool(int n) {
+    if (n <= 0) n = std::max(1u, std::thread::hardware_concurrency());
+    for (int i = 0; i < n; i++) threads_.emplace_back([this, i] { loop(i); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : threads_) t.join();
+  }
+  int size() const { return (int)threads_.size(); }
+  void run(int ntasks, const std::function<void(int, int)>& fn) {
+    if (ntasks <= 0) return;
+    std::unique_lock<std::mutex> lk(mu_);
+    job_ = &fn;
+    ntasks_ = ntasks;
+    next_.store(0);
+    active_ = (int)threads_.size();
+    gen_++;
+    cv_.notify_all();
+    done_.wait(lk, [this] { return active_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void loop(int id) {
+    long seen = 0;
+    for (;;) {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      const auto* fn = job_;
+      const int nt = ntasks_;
+      lk.unlock();
+      for (int t = next_.fetch_add(1); t < nt; t = next_.fetch_add(1)) (*fn)(t, id);
+      lk.lock();
+      if (--active_ == 0) done_.notify_all();
+    }
+  }
+  std::vector<std::thread> threads_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int, int)>* job_ = nullptr;
+  int ntasks_ = 0, active_ = 0;
+  long gen_ = 0;
+  bool stop_ = false;
+  std::atomic<int> next_{0};
+};
+
+// ------------------------------------------------------------ column store
+struct Column {
+  std::string name;
+  long long n = 0;
+  int nseg = 0;
+  int* host = nullptr;          // pinned host copy of every value
+  std::vector<int> slot;        // cache slot per segment, -1 = not cached
+  int* d_idx = nullptr;         // device segment table (the reference col_idx)
+  bool pinned_gpu = false;
+};
+
+// Device buffer that only grows.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t reserve(size_t b) {
+    if (b <= bytes) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t reserve(size_t b) {
+    if (b <= bytes) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+}  // namespace mordred_host
+
+using namespace mordred_host;
+
+struct mordred_engine {
+  int device = 0;
+  int cache_segments = 0;
+  int* cache = nullptr;
+  std::vector<int> free_slots;
+  std::unordered_map<std::string, std::unique_ptr<Column>> cols;
+  std::unique_ptr<Pool> pool;
+  hipStream_t s_main = nullptr;
+  hipStream_t s_copy[2] = {nullptr, nullptr};
+  short* d_identity = nullptr;  // device segment group 0, 1, 2, ...
+  double bw_cpu = 42e9, bw_pci = 12e9;  // CostModel.h:5-6 (bytes per second)
+  std::unordered_map<long long, double> seg_weight;  // (column id << 32 | segment) -> weight
+  DevBuf d_res, d_sg, d_ht[4];
+  HostBuf h_stage;  // pinned scratch for segment groups / index updates
+  std::string err;
+};
+
+namespace {
+
+constexpr int kMaxIdentity = 32767;
+
+int fail(mordred_engine* e, const std::string& msg) {
+  e->err = msg;
+  return -1;
+}
+
+int hip_fail(mordred_engine* e, hipError_t rc, const char* what) {
+  e->err = std::string(what) + ": " + hipGetErrorString(rc);
+  return (int)rc;
+}
+
+#define HIP_OK(call)                                  \
+  do {                                                \
+    hipError_t rc_ = (call);                          \
+    if (rc_ != hipSuccess) return hip_fail(e, rc_, #call); \
+  } while (0)
+
+Column* col(mordred_engine* e, const char* name) {
+  if (!name) return nullptr;
+  auto it = e->cols.find(name);
+  return it == e->cols.end() ? nullptr : it->second.get();
+}
+
+int set_slot(mordred_engine* e, Column* c, int seg, int slot) {
+  c->slot[seg] = slot;
+  HIP_OK(hipMemcpy(c->d_idx + seg, &slot, sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int cache_seg(mordred_engine* e, Column* c, int seg) {
+  if (seg < 0 || seg >= c->nseg) return fail(e, "segment out of range");
+  if (c->slot[seg] >= 0) return 0;
+  if (e->free_slots.empty()) return fail(e, "segment cache full");
+  const int slot = e->free_slots.back();
+  e->free_slots.pop_back();
+  const long long rows = std::min<long long>(kSeg, c->n - (long long)seg * kSeg);
+  HIP_OK(hipMemcpy(e->cache + (size_t)slot * kSeg, c->host + (size_t)seg * kSeg, rows * sizeof(int),
+                   hipMemcpyHostToDevice));
+  return set_slot(e, c, seg, slot);
+}
+
+int evict_seg(mordred_engine* e, Column* c, int seg) {
+  if (seg < 0 || seg >= c->nseg) return fail(e, "segment out of range");
+  if (c->slot[seg] < 0 || c->pinned_gpu) return 0;
+  e->free_slots.push_back(c->slot[seg]);
+  return set_slot(e, c, seg, -1);
+}
+
+// ----------------------------------------------------------- resolved query
+struct Resolved {
+  const mordred_star_query* q = nullptr;
+  std::vector<Column*> fact_cols;   // distinct fact columns the query reads
+  Column* filt[2] = {nullptr, nullptr};
+  Column* key[4] = {nullptr, nullptr, nullptr, nullptr};
+  Column* dkey[4] = {nullptr, nullptr, nullptr, nullptr};
+  Column* dfilt[4] = {nullptr, nullptr, nullptr, nullptr};
+  Column* dval[4] = {nullptr, nullptr, nullptr, nullptr};
+  const mordred_join* join[4] = {nullptr, nullptr, nullptr, nullptr};
+  Column* a1 = nullptr;
+  Column* a2 = nullptr;
+  long long n = 0;
+  int nseg = 0;
+};
+
+int resolve(mordred_engine* e, const mordred_star_query* q, Resolved& r) {
+  if (!q || q->njoins < 0 || q->njoins > 4 || q->nfilters < 0 || q->nfilters > 2)
+    return fail(e, "bad query shape");
+  if (q->kind == MORDRED_KIND_GROUPBY && q->total_val <= 0) return fail(e, "total_val must be > 0");
+  r.q = q;
+  auto add_fact = [&](Column* c) {
+    if (std::find(r.fact_cols.begin(), r.fact_cols.end(), c) == r.fact_cols.end()) r.fact_cols.push_back(c);
+  };
+  for (int i = 0; i < q->nfilters; i++) {
+    if (!(r.filt[i] = col(e, q->filters[i].col))) return fail(e, "unknown fact filter column");
+    add_fact(r.filt[i]);
+  }
+  for (int i = 0; i < q->njoins; i++) {
+    const mordred_join& j = q->joins[i];
+    if (j.slot < 1 || j.slot > 4 || r.join[j.slot - 1]) return fail(e, "join slots must be distinct in 1..4");
+    const int k = j.slot - 1;
+    r.join[k] = &j;
+    if (!(r.key[k] = col(e, j.fact_key)) || !(r.dkey[k] = col(e, j.dim_key)))
+      return fail(e, "unknown join column");
+    if (j.filter_col && !(r.dfilt[k] = col(e, j.filter_col))) return fail(e, "unknown dimension filter column");
+    if (j.group_col && !(r.dval[k] = col(e, j.group_col))) return fail(e, "unknown group column");
+    if (j.dim_len <= 0) return fail(e, "dim_len must be > 0");
+    add_fact(r.key[k]);
+  }
+  if (!(r.a1 = col(e, q->aggr1))) return fail(e, "unknown aggregate column");
+  add_fact(r.a1);
+  if (q->aggr2) {
+    if (!(r.a2 = col(e, q->aggr2))) return fail(e, "unknown aggregate column");
+    add_fact(r.a2);
+  }
+  r.n = r.fact_cols[0]->n;
+  for (Column* c : r.fact_cols)
+    if (c->n != r.n) return fail(e, "fact columns differ in length");
+  r.nseg = r.fact_cols[0]->nseg;
+  return 0;
+}
+
+bool segment_cached(const Resolved& r, int seg) {
+  for (Column* c : r.fact_cols)
+    if (c->slot[seg] < 0) return false;
+  return true;
+}
+
+long long segment_rows(const Resolved& r, int seg) {
+  return std::min<long long>(kSeg, r.n - (long long)seg * kSeg);
+}
+
+// ------------------------------------------------------------ cost model
+// CostModel::calculate_cost with every operator of the fact pipeline on the
+// CPU (the GPU side costs 0): filters scan 4 B per row, probes scan 4 B and
+// touch one cache line per row, the group-by reads n_aggr cache lines and one
+// group line per row; each operator shrinks L by its selectivity
+// (CostModel.cu:365-440).  Returns seconds.
+struct Selectivity {
+  double filt[2] = {1, 1};
+  double join[4] = {1, 1, 1, 1};
+};
+
+double cpu_cost(const Resolved& r, const Selectivity& sel, double L, double bw_cpu) {
+  double cost = 0;
+  for (int i = 0; i < r.q->nfilters; i++) {
+    cost += L * 4 / bw_cpu;
+    L *= sel.filt[i];
+  }
+  for (int k = 0; k < 4; k++) {
+    if (!r.join[k]) continue;
+    cost += L * 4 / bw_cpu + L * kCacheLine / bw_cpu;
+    L *= sel.join[k];
+  }
+  const int n_aggr = r.a2 ? 2 : 1;
+  cost += L * kCacheLine * n_aggr / bw_cpu + L * kCacheLine / bw_cpu;
+  return cost;
+}
+
+// CostModel::transfer_cost(M): every fact column of the segment over PCIe.
+double transfer_cost(const Resolved& r, double L, double bw_pci) {
+  return L * 4 * (double)r.fact_cols.size() / bw_pci;
+}
+
+// ------------------------------------------------------------ CPU operators
+// The engine's CPU half (not the test oracle): same results as
+// CPUProcessing.cu build_CPU (:1034-1104), filter_probe_group_by_CPU
+// (:723-866) and filter_probe_aggr_CPU (:1628-1710), with per-thread dense
+// accumulators instead of one atomic per row.
+struct CpuTables {
+  std::vector<int> ht[4];
+};
+
+void cpu_build(mordred_engine* e, const Resolved& r, CpuTables& t) {
+  for (int k = 0; k < 4; k++) {
+    if (!r.join[k]) continue;
+    const mordred_join& j = *r.join[k];
+    t.ht[k].assign(2 * (size_t)j.dim_len, 0);
+    int* ht = t.ht[k].data();
+    const Column* key = r.dkey[k];
+    const Column* fc = r.dfilt[k];
+    const Column* vc = r.dval[k];
+    const long long n = key->n;
+    const int tasks = (int)((n + kCpuChunk - 1) / kCpuChunk);
+    e->pool->run(tasks, [&](int task, int) {
+      const long long b = (long long)task * kCpuChunk, en = std::min(n, b + kCpuChunk);
+      for (long long row = b; row < en; row++) {
+        if (fc) {
+          const int x = fc->host[row];
+          bool pass = true;
+          if (j.filter_mode == 1) pass = x >= j.c1 && x <= j.c2;
+          else if (j.filter_mode == 2) pass = x == j.c1 || x == j.c2;
+          else if (j.filter_mode == 0) pass = x == j.c1;
+          if (!pass) continue;
+        }
+        const int h = (key->host[row] - j.min_key) % j.dim_len;
+        if (vc) ht[2 * (size_t)h] = vc->host[row];
+        ht[2 * (size_t)h + 1] = (int)row + 1;
+      }
+    });
+  }
+}
+
+struct CpuAcc {
+  std::vector<long long> sum;
+  std::vector<int> key;        // 4 per group, 0 = not seen with a non-zero key
+  std::vector<uint8_t> touched;
+  long long scalar = 0;
+};
+
+void cpu_fact(mordred_engine* e, const Resolved& r, const CpuTables& t, const std::vector<int>& segs,
+              int* res) {
+  const mordred_star_query& q = *r.q;
+  const int G = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
+  const int nw = e->pool->size();
+  std::vector<CpuAcc> acc(nw);
+  std::vector<std::pair<int, long long>> tasks;  // (segment, first row of chunk)
+  for (int s : segs)
+    for (long long b = (long long)s * kSeg; b < (long long)s * kSeg + segment_rows(r, s); b += kCpuChunk)
+      tasks.emplace_back(s, b);
+  const int* fcol[2] = {r.filt[0] ? r.filt[0]->host : nullptr, r.filt[1] ? r.filt[1]->host : nullptr};
+  const int* kcol[4];
+  const int* hts[4];
+  int len[4], mk[4], mv[4], uq[4];
+  for (int k = 0; k < 4; k++) {
+    kcol[k] = r.join[k] ? r.key[k]->host : nullptr;
+    hts[k] = r.join[k] ? t.ht[k].data() : nullptr;
+    len[k] = r.join[k] ? r.join[k]->dim_len : 1;
+    mk[k] = r.join[k] ? r.join[k]->min_key : 0;
+    mv[k] = r.join[k] && r.join[k]->group_col ? r.join[k]->min_val : 0;
+    uq[k] = r.join[k] && r.join[k]->group_col ? r.join[k]->unique_val : 0;
+  }
+  const int* a1 = r.a1->host;
+  const int* a2 = r.a2 ? r.a2->host : nullptr;
+  e->pool->run((int)tasks.size(), [&](int ti, int w) {
+    CpuAcc& A = acc[w];
+    if (q.kind == MORDRED_KIND_GROUPBY && A.sum.empty()) {
+      A.sum.assign(G, 0);
+      A.key.assign(4 * (size_t)G, 0);
+      A.touched.assign(G, 0);
+    }
+    const long long b = tasks[ti].second;
+    const long long en = (long long)tasks[ti].first * kSeg + segment_rows(r, tasks[ti].first);
+    const long long stop = std::min(en, b + kCpuChunk);
+    for (long long row = b; row < stop; row++) {
+      bool ok = true;
+      for (int i = 0; i < q.nfilters && ok; i++) {
+        const int x = fcol[i][row];
+        ok = x >= q.filters[i].lo && x <= q.filters[i].hi;
+      }
+      if (!ok) continue;
+      int v[4] = {0, 0, 0, 0};
+      for (int k = 0; k < 4 && ok; k++) {
+        if (!kcol[k]) continue;
+        const int h = (kcol[k][row] - mk[k]) % len[k];
+        const int lo = hts[k][2 * (size_t)h], hi = hts[k][2 * (size_t)h + 1];
+        if ((lo | hi) == 0) ok = false;
+        else v[k] = lo;
+      }
+      if (!ok) continue;
+      const int x1 = a1[row], x2 = a2 ? a2[row] : 0;
+      if (q.kind == MORDRED_KIND_AGGR) {
+        A.scalar += (long long)(int)((unsigned)x1 * (unsigned)x2);
+        continue;
+      }
+      const int h = ((v[0] - mv[0]) * uq[0] + (v[1] - mv[1]) * uq[1] + (v[2] - mv[2]) * uq[2] +
+                     (v[3] - mv[3]) * uq[3]) % G;
+      A.sum[h] += (long long)(int)((unsigned)x1 - (unsigned)x2);
+      A.touched[h] = 1;
+      for (int k = 0; k < 4; k++)
+        if (v[k] != 0) A.key[4 * (size_t)h + k] = v[k];
+    }
+  });
+  // combine: keys as the reference CPU path leaves them (non-zero keys only)
+  if (q.kind == MORDRED_KIND_AGGR) {
+    long long s = 0;
+    for (auto& A : acc) s += A.scalar;
+    reinterpret_cast<long long*>(res)[2] += s;
+    return;
+  }
+  for (auto& A : acc) {
+    if (A.sum.empty()) continue;
+    for (int h = 0; h < G; h++) {
+      if (!A.touched[h]) continue;
+      for (int k = 0; k < 4; k++)
+        if (A.key[4 * (size_t)h + k] != 0) res[6 * (size_t)h + k] = A.key[4 * (size_t)h + k];
+      reinterpret_cast<long long*>(res + 6 * (size_t)h + 4)[0] += A.sum[h];
+    }
+  }
+}
+
+// merge (CPUProcessing.cu:1786-1819): a CPU key word stays unless it is 0,
+// then the GPU word is taken; sums add.
+void merge(int* cpu, const int* gpu, int groups) {
+  for (int i = 0; i < groups; i++) {
+    for (int k = 0; k < 4; k++)
+      if (cpu[i * 6 + k] == 0) cpu[i * 6 + k] = gpu[i * 6 + k];
+    reinterpret_cast<unsigned long long*>(cpu)[i * 3 + 2] += reinterpret_cast<const unsigned long long*>(gpu)[i * 3 + 2];
+  }
+}
+
+// ------------------------------------------------------------ GPU side
+struct GpuArgs {
+  filterArgsGPU f{};
+  probeArgsGPU p{};
+  groupbyArgsGPU g{};
+  bool has_filter = false;
+};
+
+void gpu_args(const Resolved& r, int* const ht[4], GpuArgs& a) {
+  const mordred_star_query& q = *r.q;
+  if (q.nfilters > 0) {
+    a.has_filter = true;
+    a.f.filter_idx1 = r.filt[0]->d_idx;
+    a.f.compare1 = q.filters[0].lo;
+    a.f.compare2 = q.filters[0].hi;
+    a.f.mode1 = 1;
+    if (q.nfilters > 1) {
+      a.f.filter_idx2 = r.filt[1]->d_idx;
+      a.f.compare3 = q.filters[1].lo;
+      a.f.compare4 = q.filters[1].hi;
+      a.f.mode2 = 1;
+    }
+  }
+  int** keys[4] = {&a.p.key_idx1, &a.p.key_idx2, &a.p.key_idx3, &a.p.key_idx4};
+  int** hts[4] = {&a.p.ht1, &a.p.ht2, &a.p.ht3, &a.p.ht4};
+  int* lens[4] = {&a.p.dim_len1, &a.p.dim_len2, &a.p.dim_len3, &a.p.dim_len4};
+  int* mks[4] = {&a.p.min_key1, &a.p.min_key2, &a.p.min_key3, &a.p.min_key4};
+  int** grp[4] = {&a.g.group_idx1, &a.g.group_idx2, &a.g.group_idx3, &a.g.group_idx4};
+  int* mvs[4] = {&a.g.min_val1, &a.g.min_val2, &a.g.min_val3, &a.g.min_val4};
+  int* uqs[4] = {&a.g.unique_val1, &a.g.unique_val2, &a.g.unique_val3, &a.g.unique_val4};
+  for (int k = 0; k < 4; k++) {
+    if (!r.join[k]) continue;
+    *keys[k] = r.key[k]->d_idx;
+    *hts[k] = ht[k];
+    *lens[k] = r.join[k]->dim_len;
+    *mks[k] = r.join[k]->min_key;
+    if (r.dval[k]) {
+      *grp[k] = r.dval[k]->d_idx;
+      *mvs[k] = r.join[k]->min_val;
+      *uqs[k] = r.join[k]->unique_val;
+    }
+  }
+  a.g.aggr_idx1 = r.a1->d_idx;
+  a.g.aggr_idx2 = r.a2 ? r.a2->d_idx : nullptr;
+  a.g.total_val = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
+}
+
+int launch_fact(mordred_engine* e, const Resolved& r, const GpuArgs& a, int n, int* d_res, const short* d_sg,
+                hipStream_t s) {
+  int rc;
+  if (r.q->kind == MORDRED_KIND_AGGR)
+    rc = mordred_filter_probe_aggr_GPU2(e->cache, a.has_filter ? &a.f : nullptr, &a.p, &a.g, n, d_res, 0, d_sg,
+                                        (mordred_stream_t)s);
+  else
+    rc = mordred_filter_probe_group_by_GPU2(e->cache, a.has_filter ? &a.f : nullptr, &a.p, &a.g, n, d_res, 0,
+                                            d_sg, (mordred_stream_t)s);
+  if (rc != 0) return hip_fail(e, (hipError_t)rc, "fused fact kernel");
+  return 0;
+}
+
+// Rows of a segment group; the partial last segment of the table, if present,
+// must be the group's last entry (QueryOptimizer.cu:1708-1714).
+long long group_rows(const Resolved& r, const std::vector<int>& segs) {
+  long long n = 0;
+  for (int s : segs) n += segment_rows(r, s);
+  return n;
+}
+
+void order_last_partial(const Resolved& r, std::vector<int>& segs) {
+  const int last = r.nseg - 1;
+  if (segment_rows(r, last) == kSeg) return;
+  auto it = std::find(segs.begin(), segs.end(), last);
+  if (it != segs.end()) {
+    segs.erase(it);
+    segs.push_back(last);
+  }
+}
+
+Selectivity estimate_selectivity(const Resolved& r, const CpuTables& t) {
+  Selectivity s;
+  // dimension filters: fraction of dimension rows the build kept
+  for (int k = 0; k < 4; k++) {
+    if (!r.join[k]) continue;
+    long long hits = 0;
+    const std::vector<int>& ht = t.ht[k];
+    for (size_t i = 1; i < ht.size(); i += 2) hits += ht[i] != 0;
+    s.join[k] = r.dkey[k]->n > 0 ? (double)hits / (double)r.dkey[k]->n : 1.0;
+  }
+  // fact filters: a sample of the first segment
+  const long long m = std::min<long long>(r.n, 1 << 14);
+  for (int i = 0; i < r.q->nfilters; i++) {
+    long long pass = 0;
+    for (long long row = 0; row < m; row++) {
+      const int x = r.filt[i]->host[row];
+      pass += x >= r.q->filters[i].lo && x <= r.q->filters[i].hi;
+    }
+    s.filt[i] = m > 0 ? (double)pass / (double)m : 1.0;
+  }
+  return s;
+}
+
+}  // namespace
+
+// ================================================================= C ABI
+extern "C" {
+
+mordred_engine* mordred_engine_create(int device, int cache_segments, int threads) {
+  if (cache_segments < 0) return nullptr;
+  auto* e = new mordred_engine();
+  e->device = device;
+  e->cache_segments = cache_segments;
+  if (device < 0) {  // host-only engine: CPU operators, no HIP calls
+    e->cache_segments = 0;
+    e->pool = std::make_unique<Pool>(threads);
+    return e;
+  }
+  bool ok = hipSetDevice(device) == hipSuccess;
+  ok = ok && (cache_segments == 0 ||
+              hipMalloc(&e->cache, (size_t)cache_segments * kSeg * sizeof(int)) == hipSuccess);
+  ok = ok && hipStreamCreate(&e->s_main) == hipSuccess && hipStreamCreate(&e->s_copy[0]) == hipSuccess &&
+       hipStreamCreate(&e->s_copy[1]) == hipSuccess;
+  if (ok) {
+    std::vector<short> ident(kMaxIdentity);
+    for (int i = 0; i < kMaxIdentity; i++) ident[i] = (short)i;
+    ok = hipMalloc(&e->d_identity, kMaxIdentity * sizeof(short)) == hipSuccess &&
+         hipMemcpy(e->d_identity, ident.data(), kMaxIdentity * sizeof(short), hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (!ok) {
+    mordred_engine_destroy(e);
+    return nullptr;
+  }
+  for (int s = cache_segments - 1; s >= 0; s--) e->free_slots.push_back(s);
+  e->pool = std::make_unique<Pool>(threads);
+  return e;
+}
+
+void mordred_engine_destroy(mordred_engine* e) {
+  if (!e) return;
+  if (e->device < 0) {
+    for (auto& kv : e->cols) std::free(kv.second->host);
+    delete e;
+    return;
+  }
+  (void)hipSetDevice(e->device);
+  for (auto& kv : e->cols) {
+    if (kv.second->host) (void)hipHostFree(kv.second->host);
+    if (kv.second->d_idx) (void)hipFree(kv.second->d_idx);
+  }
+  e->cols.clear();
+  for (hipStream_t s : {e->s_main, e->s_copy[0], e->s_copy[1]})
+    if (s) (void)hipStreamDestroy(s);
+  if (e->cache) (void)hipFree(e->cache);
+  if (e->d_identity) (void)hipFree(e->d_identity);
+  delete e;
+}
+
+const char* mordred_engine_last_error(const mordred_engine* e) { return e ? e->err.c_str() : "no engine"; }
+
+int mordred_engine_add_column(mordred_engine* e, const char* name, const int* values, long long n,
+                              int gpu_resident) {
+  if (!e || !name || (!values && n > 0) || n < 0) return e ? fail(e, "bad column") : -1;
+  if (e->cols.count(name)) return fail(e, std::string("duplicate column ") + name);
+  auto c = std::make_unique<Column>();
+  c->name = name;
+  c->n = n;
+  c->nseg = (int)((n + kSeg - 1) / kSeg);
+  if (c->nseg > kMaxIdentity) return fail(e, "column longer than 32767 segments");
+  const size_t bytes = std::max<size_t>(1, (size_t)n * sizeof(int));
+  if (e->device < 0) {
+    c->host = static_cast<int*>(std::malloc(bytes));
+    if (!c->host) return fail(e, "out of host memory");
+    if (n > 0) std::memcpy(c->host, values, (size_t)n * sizeof(int));
+    c->slot.assign(c->nseg, -1);
+    e->cols[name] = std::move(c);
+    return 0;
+  }
+  HIP_OK(hipSetDevice(e->device));
+  HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&c->host), bytes, hipHostMallocDefault));
+  if (n > 0) std::memcpy(c->host, values, (size_t)n * sizeof(int));
+  c->slot.assign(c->nseg, -1);
+  HIP_OK(hipMalloc(&c->d_idx, std::max(1, c->nseg) * sizeof(int)));
+  HIP_OK(hipMemcpy(c->d_idx, c->slot.data(), c->nseg * sizeof(int), hipMemcpyHostToDevice));
+  Column* raw = c.get();
+  e->cols[name] = std::move(c);
+  if (gpu_resident) {
+    for (int s = 0; s < raw->nseg; s++) {
+      const int rc = cache_seg(e, raw, s);
+      if (rc) return rc;
+    }
+    raw->pinned_gpu = true;
+  }
+  return 0;
+}
+
+int mordred_engine_column_segments(const mordred_engine* e, const char* name) {
+  if (!e || !name) return -1;
+  auto it = e->cols.find(name);
+  return it == e->cols.end() ? -1 : it->second->nseg;
+}
+
+int mordred_engine_cache_segment(mordred_engine* e, const char* name, int segment) {
+  if (!e) return -1;
+  if (e->device < 0) return fail(e, "host-only engine has no segment cache");
+  Column* c = col(e, name);
+  if (!c) return fail(e, "unknown column");
+  HIP_OK(hipSetDevice(e->device));
+  return cache_seg(e, c, segment);
+}
+
+int mordred_engine_evict_segment(mordred_engine* e, const char* name, int segment) {
+  if (!e) return -1;
+  if (e->device < 0) return fail(e, "host-only engine has no segment cache");
+  Column* c = col(e, name);
+  if (!c) return fail(e, "unknown column");
+  HIP_OK(hipSetDevice(e->device));
+  return evict_seg(e, c, segment);
+}
+
+int mordred_engine_is_cached(const mordred_engine* e, const char* name, int segment) {
+  if (!e || !name) return -1;
+  auto it = e->cols.find(name);
+  if (it == e->cols.end() || segment < 0 || segment >= it->second->nseg) return -1;
+  return it->second->slot[segment] >= 0;
+}
+
+int mordred_engine_set_bandwidth(mordred_engine* e, double bw_cpu, double bw_pci) {
+  if (!e || !(bw_cpu > 0) || !(bw_pci > 0)) return e ? fail(e, "bandwidths must be > 0") : -1;
+  e->bw_cpu = bw_cpu;
+  e->bw_pci = bw_pci;
+  return 0;
+}
+
+int mordred_engine_place(mordred_engine* e, const mordred_star_query* q, double fraction) {
+  if (!e) return -1;
+  if (e->device < 0) return fail(e, "host-only engine has no segment cache");
+  Resolved r;
+  int rc = resolve(e, q, r);
+  if (rc) return rc;
+  HIP_OK(hipSetDevice(e->device));
+  // permute_cost weight of a segment: what moving its pipeline from the CPU to
+  // the GPU saves (CostModel.cu:147-181), accumulated over calls like the
+  // reference's segment weights (CacheManager.cu:533-587).
+  CpuTables t;
+  cpu_build(e, r, t);
+  const Selectivity sel = estimate_selectivity(r, t);
+  std::vector<std::pair<double, int>> rank;
+  for (int s = 0; s < r.nseg; s++) {
+    double w = cpu_cost(r, sel, (double)segment_rows(r, s), e->bw_cpu);
+    for (Column* c : r.fact_cols) {
+      const long long id = ((long long)(intptr_t)c << 16) ^ s;
+      w += (e->seg_weight[id] += cpu_cost(r, sel, (double)segment_rows(r, s), e->bw_cpu));
+    }
+    rank.emplace_back(-w, s);
+  }
+  std::sort(rank.begin(), rank.end());
+  const int want = (int)std::min<double>(r.nseg, std::max(0.0, fraction) * r.nseg + 0.5);
+  std::vector<char> keep(r.nseg, 0);
+  for (int i = 0; i < want; i++) keep[rank[i].second] = 1;
+  for (int s = 0; s < r.nseg; s++)
+    if (!keep[s])
+      for (Column* c : r.fact_cols)
+        if ((rc = evict_seg(e, c, s))) return rc;
+  for (int s = 0; s < r.nseg; s++)
+    if (keep[s])
+      for (Column* c : r.fact_cols)
+        if ((rc = cache_seg(e, c, s))) return rc;
+  return 0;
+}
+
+int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int placement, int* res,
+                       mordred_run_stats* stats) {
+  if (!e) return -1;
+  if (!res) return fail(e, "res is NULL");
+  if (placement < MORDRED_PLACE_RESIDENCY || placement > MORDRED_PLACE_CPU_ONLY) return fail(e, "bad placement");
+  if (e->device < 0 && placement != MORDRED_PLACE_CPU_ONLY) return fail(e, "host-only engine runs CPU_ONLY");
+  Resolved r;
+  int rc = resolve(e, q, r);
+  if (rc) return rc;
+  for (int k = 0; k < 4; k++) {
+    if (!r.join[k]) continue;
+    for (Column* c : {r.dkey[k], r.dfilt[k], r.dval[k]})
+      if (c && !c->pinned_gpu && placement != MORDRED_PLACE_CPU_ONLY)
+        return fail(e, "dimension columns must be GPU-resident (add_column gpu_resident=1)");
+  }
+  if (e->device >= 0) HIP_OK(hipSetDevice(e->device));
+  const auto t0 = Clock::now();
+  const int G = q->kind == MORDRED_KIND_GROUPBY ? q->total_val : 1;
+  const size_t res_ints = 6 * (size_t)G;
+  std::memset(res, 0, res_ints * sizeof(int));
+  mordred_run_stats st{};
+
+  // ---- segment groups by residency (QueryOptimizer.cu:1680-1716)
+  std::vector<int> gpu_segs, other;
+  for (int s = 0; s < r.nseg; s++) {
+    if (placement != MORDRED_PLACE_CPU_ONLY && segment_cached(r, s)) gpu_segs.push_back(s);
+    else other.push_back(s);
+  }
+
+  // ---- CPU hash tables (needed for the CPU operators and the cost model)
+  const auto tc0 = Clock::now();
+  CpuTables t;
+  const bool need_cpu_tables = placement != MORDRED_PLACE_GPU_ONLY && !other.empty();
+  Selectivity sel;
+  if (need_cpu_tables || placement == MORDRED_PLACE_COST_SPLIT) {
+    cpu_build(e, r, t);
+    sel = estimate_selectivity(r, t);
+  }
+  double cpu_build_ms = ms_since(tc0);
+
+  // ---- split the non-cached segments between the CPU and the PCIe stream
+  std::vector<int> cpu_segs, stream_segs;
+  const double t_cpu = cpu_cost(r, sel, kSeg, e->bw_cpu) * 1e3;
+  const double t_str = transfer_cost(r, kSeg, e->bw_pci) * 1e3;
+  st.est_cpu_ms_per_segment = t_cpu;
+  st.est_stream_ms_per_segment = t_str;
+  if (placement == MORDRED_PLACE_RESIDENCY || placement == MORDRED_PLACE_CPU_ONLY) {
+    cpu_segs = other;
+  } else if (placement == MORDRED_PLACE_GPU_ONLY) {
+    stream_segs = other;
+  } else {
+    // both sides run concurrently: stream k segments so that k*t_str and
+    // (n-k)*t_cpu finish together
+    const int n = (int)other.size();
+    const int k = (int)std::min<double>(n, std::max(0.0, n * t_cpu / (t_cpu + t_str) + 0.5));
+    stream_segs.assign(other.begin(), other.begin() + k);
+    cpu_segs.assign(other.begin() + k, other.end());
+  }
+  if (!stream_segs.empty() && e->free_slots.size() < 2 * kStreamBatch * r.fact_cols.size()) {
+    // not enough free cache slots for two staging batches: CPU instead
+    cpu_segs.insert(cpu_segs.end(), stream_segs.begin(), stream_segs.end());
+    stream_segs.clear();
+  }
+  std::sort(cpu_segs.begin(), cpu_segs.end());
+
+  // ---- GPU: hash tables, cached segment group, streamed batches (async)
+  const auto tg0 = Clock::now();
+  const bool use_gpu = !gpu_segs.empty() || !stream_segs.empty();
+  std::vector<int> borrowed;  // staging slots taken from the free list
+  hipEvent_t ev_built = nullptr;
+  int* d_res = nullptr;
+  if (use_gpu) {
+    HIP_OK(e->d_res.reserve(res_ints * sizeof(int)));
+    d_res = static_cast<int*>(e->d_res.p);
+    HIP_OK(hipMemsetAsync(d_res, 0, res_ints * sizeof(int), e->s_main));
+    int* ht[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int k = 0; k < 4; k++) {
+      if (!r.join[k]) continue;
+      const mordred_join& j = *r.join[k];
+      HIP_OK(e->d_ht[k].reserve(2 * (size_t)j.dim_len * sizeof(int)));
+      ht[k] = static_cast<int*>(e->d_ht[k].p);
+      HIP_OK(hipMemsetAsync(ht[k], 0, 2 * (size_t)j.dim_len * sizeof(int), e->s_main));
+      filterArgsGPU bf{};
+      if (r.dfilt[k]) {
+        bf.filter_idx1 = r.dfilt[k]->d_idx;
+        bf.mode1 = j.filter_mode;
+        bf.compare1 = j.c1;
+        bf.compare2 = j.c2;
+      }
+      buildArgsGPU ba{r.dkey[k]->d_idx, r.dval[k] ? r.dval[k]->d_idx : nullptr, j.dim_len, j.min_key, 0};
+      const int brc = mordred_build_GPU2(e->cache, r.dfilt[k] ? &bf : nullptr, &ba, (int)r.dkey[k]->n, ht[k], 0,
+                                         e->d_identity, (mordred_stream_t)e->s_main);
+      if (brc) return hip_fail(e, (hipError_t)brc, "mordred_build_GPU2");
+    }
+    GpuArgs a;
+    gpu_args(r, ht, a);
+    // Streamed segments: two staging sets of kStreamBatch segments x fact
+    // columns are borrowed from the free slots.  Batch i uses set i % 2 on copy
+    // stream i % 2, so a set is reused only after the same stream has run the
+    // kernel of the batch before it.  Each streamed segment's table entry
+    // points at the slot of its batch for the whole run (one upload per
+    // column); entries of uncached segments are never read otherwise.
+    std::vector<int> seg_order = stream_segs;
+    order_last_partial(r, seg_order);
+    const size_t nc = r.fact_cols.size();
+    if (!stream_segs.empty()) {
+      for (size_t i = 0; i < 2 * kStreamBatch * nc; i++) {
+        borrowed.push_back(e->free_slots.back());
+        e->free_slots.pop_back();
+      }
+      for (size_t c = 0; c < nc; c++) {
+        Column* col_c = r.fact_cols[c];
+        std::vector<int>& mirror = col_c->slot;
+        std::vector<int> table = mirror;
+        for (size_t i = 0; i < seg_order.size(); i++) {
+          const size_t bi = i / kStreamBatch, within = i % kStreamBatch;
+          table[seg_order[i]] = borrowed[((bi & 1) * kStreamBatch + within) * nc + c];
+        }
+        HIP_OK(hipMemcpy(col_c->d_idx, table.data(), table.size() * sizeof(int), hipMemcpyHostToDevice));
+      }
+      HIP_OK(hipEventCreateWithFlags(&ev_built, hipEventDisableTiming));
+      HIP_OK(hipEventRecord(ev_built, e->s_main));
+      for (int b = 0; b < 2; b++) HIP_OK(hipStreamWaitEvent(e->s_copy[b], ev_built, 0));
+    }
+    // pinned scratch: segment groups (shorts) and staging index updates (ints)
+    const size_t nstream_batches = (stream_segs.size() + kStreamBatch - 1) / kStreamBatch;
+    const size_t stage_bytes = (size_t)r.nseg * sizeof(short) + 64 + nstream_batches * (kStreamBatch * sizeof(short) + 64);
+    HIP_OK(e->h_stage.reserve(stage_bytes));
+    HIP_OK(e->d_sg.reserve((size_t)r.nseg * sizeof(short) + nstream_batches * (kStreamBatch * sizeof(short) + 64) + 64));
+    char* hs = static_cast<char*>(e->h_stage.p);
+    char* ds = static_cast<char*>(e->d_sg.p);
+    if (!gpu_segs.empty()) {
+      order_last_partial(r, gpu_segs);
+      short* h_sg = reinterpret_cast<short*>(hs);
+      for (size_t i = 0; i < gpu_segs.size(); i++) h_sg[i] = (short)gpu_segs[i];
+      HIP_OK(hipMemcpyAsync(ds, h_sg, gpu_segs.size() * sizeof(short), hipMemcpyHostToDevice, e->s_main));
+      const int frc = launch_fact(e, r, a, (int)group_rows(r, gpu_segs), d_res, reinterpret_cast<short*>(ds),
+                                  e->s_main);
+      if (frc) return frc;
+      st.gpu_rows = group_rows(r, gpu_segs);
+      st.gpu_segments = (int)gpu_segs.size();
+    }
+    hs += (size_t)r.nseg * sizeof(short) + 64;
+    ds += (size_t)r.nseg * sizeof(short) + 64;
+    if (!stream_segs.empty()) {
+      for (size_t bi = 0; bi < nstream_batches; bi++) {
+        const int set = (int)(bi & 1);
+        hipStream_t s = e->s_copy[set];
+        const size_t first = bi * kStreamBatch;
+        const size_t cnt = std::min<size_t>(kStreamBatch, seg_order.size() - first);
+        short* h_bsg = reinterpret_cast<short*>(hs);
+        for (size_t i = 0; i < cnt; i++) {
+          const int seg = seg_order[first + i];
+          h_bsg[i] = (short)seg;
+          for (size_t c = 0; c < nc; c++) {
+            const int slot = borrowed[(set * kStreamBatch + i) * nc + c];
+            HIP_OK(hipMemcpyAsync(e->cache + (size_t)slot * kSeg, r.fact_cols[c]->host + (size_t)seg * kSeg,
+                                  segment_rows(r, seg) * sizeof(int), hipMemcpyHostToDevice, s));
+          }
+        }
+        HIP_OK(hipMemcpyAsync(ds, h_bsg, cnt * sizeof(short), hipMemcpyHostToDevice, s));
+        std::vector<int> bsegs(seg_order.begin() + first, seg_order.begin() + first + cnt);
+        const int frc = launch_fact(e, r, a, (int)group_rows(r, bsegs), d_res, reinterpret_cast<short*>(ds), s);
+        if (frc) return frc;
+        hs += kStreamBatch * sizeof(short) + 64;
+        ds += kStreamBatch * sizeof(short) + 64;
+        st.stream_rows += group_rows(r, bsegs);
+      }
+      st.stream_segments = (int)stream_segs.size();
+    }
+  }
+
+  // ---- CPU: the CPU segment group, concurrently with the GPU
+  const auto tc1 = Clock::now();
+  std::vector<int> cpu_res;
+  if (!cpu_segs.empty()) {
+    cpu_res.assign(res_ints, 0);
+    cpu_fact(e, r, t, cpu_segs, cpu_res.data());
+    st.cpu_rows = group_rows(r, cpu_segs);
+    st.cpu_segments = (int)cpu_segs.size();
+  }
+  st.cpu_ms = cpu_build_ms + ms_since(tc1);
+
+  // ---- wait for the GPU, merge
+  std::vector<int> gpu_res;
+  if (use_gpu) {
+    gpu_res.assign(res_ints, 0);
+    for (hipStream_t s : {e->s_main, e->s_copy[0], e->s_copy[1]}) HIP_OK(hipStreamSynchronize(s));
+    st.gpu_ms = ms_since(tg0);
+    HIP_OK(hipMemcpy(gpu_res.data(), d_res, res_ints * sizeof(int), hipMemcpyDeviceToHost));
+    for (int s : borrowed) e->free_slots.push_back(s);
+    if (ev_built) (void)hipEventDestroy(ev_built);
+  }
+  const auto tm0 = Clock::now();
+  if (!cpu_res.empty() && !gpu_res.empty()) {
+    if (q->kind == MORDRED_KIND_GROUPBY) merge(cpu_res.data(), gpu_res.data(), G);
+    else reinterpret_cast<long long*>(cpu_res.data())[2] += reinterpret_cast<long long*>(gpu_res.data())[2];
+    std::memcpy(res, cpu_res.data(), res_ints * sizeof(int));
+  } else if (!cpu_res.empty()) {
+    std::memcpy(res, cpu_res.data(), res_ints * sizeof(int));
+  } else if (!gpu_res.empty()) {
+    std::memcpy(res, gpu_res.data(), res_ints * sizeof(int));
+  }
+  st.merge_ms = ms_since(tm0);
+  st.total_ms = ms_since(t0);
+  if (stats) *stats = st;
+  return 0;
+}
+
+}  // extern "C"

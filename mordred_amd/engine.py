@@ -1,0 +1,159 @@
+"""ctypes binding of the host engine (include/mordred_engine.h).
+
+:class:`Engine` owns a column store (pinned host memory), an HBM segment cache
+and the CPU worker pool; :func:`star_query` converts a :class:`query.StarQuery`
+into the C struct.  ``Engine(device=-1)`` is a host-only engine that runs the
+CPU operators without any HIP call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+PLACE_RESIDENCY, PLACE_COST_SPLIT, PLACE_GPU_ONLY, PLACE_CPU_ONLY = 0, 1, 2, 3
+KIND_AGGR, KIND_GROUPBY = 0, 1
+
+
+class Join(C.Structure):
+    _fields_ = [("slot", C.c_int), ("fact_key", C.c_char_p), ("dim_key", C.c_char_p), ("dim_len", C.c_int),
+                ("min_key", C.c_int), ("filter_col", C.c_char_p), ("filter_mode", C.c_int), ("c1", C.c_int),
+                ("c2", C.c_int), ("group_col", C.c_char_p), ("min_val", C.c_int), ("unique_val", C.c_int)]
+
+
+class FactFilter(C.Structure):
+    _fields_ = [("col", C.c_char_p), ("lo", C.c_int), ("hi", C.c_int)]
+
+
+class StarQueryC(C.Structure):
+    _fields_ = [("njoins", C.c_int), ("joins", Join * 4), ("nfilters", C.c_int), ("filters", FactFilter * 2),
+                ("aggr1", C.c_char_p), ("aggr2", C.c_char_p), ("kind", C.c_int), ("total_val", C.c_int)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("gpu_ms", C.c_double), ("cpu_ms", C.c_double), ("merge_ms", C.c_double),
+                ("gpu_rows", C.c_longlong), ("cpu_rows", C.c_longlong), ("stream_rows", C.c_longlong),
+                ("gpu_segments", C.c_int), ("cpu_segments", C.c_int), ("stream_segments", C.c_int),
+                ("est_cpu_ms_per_segment", C.c_double), ("est_stream_ms_per_segment", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def _b(s):
+    return None if s is None else s.encode()
+
+
+def star_query(q) -> StarQueryC:
+    """C struct of a query.StarQuery (strings are encoded; keep the result alive)."""
+    c = StarQueryC()
+    c.njoins = len(q.joins)
+    for i, j in enumerate(q.joins):
+        c.joins[i] = Join(j.slot, _b(j.fact_key), _b(j.dim_key), j.dim_len, j.min_key, _b(j.filter_col),
+                          j.filter_mode, j.c1, j.c2, _b(j.group_col), j.min_val, j.unique_val)
+    c.nfilters = len(q.fact_filters)
+    for i, (col, lo, hi) in enumerate(q.fact_filters):
+        c.filters[i] = FactFilter(_b(col), lo, hi)
+    c.aggr1, c.aggr2 = _b(q.aggr[0]), _b(q.aggr[1])
+    c.kind = KIND_AGGR if q.kind == "aggr" else KIND_GROUPBY
+    c.total_val = max(1, q.total_val)
+    return c
+
+
+_SIGS = {
+    "mordred_engine_create": ([C.c_int, C.c_int, C.c_int], C.c_void_p),
+    "mordred_engine_destroy": ([C.c_void_p], None),
+    "mordred_engine_last_error": ([C.c_void_p], C.c_char_p),
+    "mordred_engine_add_column": ([C.c_void_p, C.c_char_p, C.c_void_p, C.c_longlong, C.c_int], C.c_int),
+    "mordred_engine_column_segments": ([C.c_void_p, C.c_char_p], C.c_int),
+    "mordred_engine_cache_segment": ([C.c_void_p, C.c_char_p, C.c_int], C.c_int),
+    "mordred_engine_evict_segment": ([C.c_void_p, C.c_char_p, C.c_int], C.c_int),
+    "mordred_engine_is_cached": ([C.c_void_p, C.c_char_p, C.c_int], C.c_int),
+    "mordred_engine_set_bandwidth": ([C.c_void_p, C.c_double, C.c_double], C.c_int),
+    "mordred_engine_place": ([C.c_void_p, C.POINTER(StarQueryC), C.c_double], C.c_int),
+    "mordred_engine_run": ([C.c_void_p, C.POINTER(StarQueryC), C.c_int, C.c_void_p, C.POINTER(RunStats)], C.c_int),
+}
+
+
+def _lib():
+    lib = abi.load()
+    if not getattr(lib, "_engine_bound", False):
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        lib._engine_bound = True
+    return lib
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+class Engine:
+    def __init__(self, device: int = 0, cache_segments: int = 0, threads: int = 0):
+        self.lib = _lib()
+        self.h = self.lib.mordred_engine_create(device, cache_segments, threads)
+        if not self.h:
+            raise EngineError(f"mordred_engine_create(device={device}, cache_segments={cache_segments}) failed")
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.lib.mordred_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise EngineError(f"{what}: rc={rc} {self.lib.mordred_engine_last_error(self.h).decode()}")
+
+    def add_column(self, name: str, values: np.ndarray, gpu_resident: bool = False):
+        v = np.ascontiguousarray(values, dtype=np.int32)
+        self._check(self.lib.mordred_engine_add_column(self.h, name.encode(), v.ctypes.data, len(v),
+                                                       int(gpu_resident)), f"add_column {name}")
+
+    def add_data(self, data, query):
+        """Every column a StarQuery reads: fact columns in the host store,
+        dimension columns cached in HBM (all of them) unless host-only."""
+        for c in query.fact_columns():
+            self.add_column(c, data.fact[c])
+        seen = set()
+        for j in query.joins:
+            for c in (j.dim_key, j.filter_col, j.group_col):
+                if c and c not in seen:
+                    seen.add(c)
+                    self.add_column(c, data.tables[j.table][c], gpu_resident=self.device >= 0)
+
+    def segments(self, name: str) -> int:
+        return self.lib.mordred_engine_column_segments(self.h, name.encode())
+
+    def cache_segment(self, name: str, seg: int):
+        self._check(self.lib.mordred_engine_cache_segment(self.h, name.encode(), seg), "cache_segment")
+
+    def evict_segment(self, name: str, seg: int):
+        self._check(self.lib.mordred_engine_evict_segment(self.h, name.encode(), seg), "evict_segment")
+
+    def is_cached(self, name: str, seg: int) -> bool:
+        return self.lib.mordred_engine_is_cached(self.h, name.encode(), seg) == 1
+
+    def set_bandwidth(self, bw_cpu: float, bw_pci: float):
+        self._check(self.lib.mordred_engine_set_bandwidth(self.h, bw_cpu, bw_pci), "set_bandwidth")
+
+    def place(self, query, fraction: float):
+        qc = star_query(query)
+        self._check(self.lib.mordred_engine_place(self.h, C.byref(qc), fraction), "place")
+
+    def run(self, query, placement: int = PLACE_RESIDENCY):
+        qc = star_query(query)
+        res = np.zeros(max(1, query.total_val) * 6, dtype=np.int32)
+        st = RunStats()
+        self._check(self.lib.mordred_engine_run(self.h, C.byref(qc), placement, res.ctypes.data, C.byref(st)), "run")
+        return res, st.as_dict()

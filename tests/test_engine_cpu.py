@@ -1,0 +1,69 @@
+"""The engine's CPU half (host-only engine, no GPU) against the oracle.
+
+The engine's CPU operators are product code, separate from the oracle; they
+must give the oracle's answers bit-exactly (CPU convention: only non-zero key
+words are written, CPUProcessing.cu:498-501).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import queries as OQ
+from mordred_amd import engine as E
+from tests import cases, semantics
+
+SEG = oracle.SEGMENT_SIZE
+
+
+@pytest.fixture(scope="module", params=["q11", "q21", "q4way"])
+def host_engine(request):
+    q, data = cases.star(request.param, 2 * SEG + 54_321, seed=5)
+    eng = E.Engine(device=-1, threads=4)
+    eng.add_data(data, q)
+    yield q, data, eng
+    eng.close()
+
+
+def test_cpu_only_matches_oracle(host_engine):
+    q, data, eng = host_engine
+    res, st = eng.run(q, E.PLACE_CPU_ONLY)
+    ref = OQ.run_query(q, data, oracle.CONV_CPU)
+    np.testing.assert_array_equal(res, ref)
+    assert st["cpu_rows"] == data.n and st["gpu_rows"] == 0
+    if q.kind == "groupby":
+        assert semantics.decode_groups(res) == semantics.group_map(q, data)
+
+
+@pytest.mark.parametrize("name", cases.golden_names())
+def test_cpu_only_golden(name):
+    q, data, exp = cases.load_golden(name)
+    eng = E.Engine(device=-1, threads=3)
+    eng.add_data(data, q)
+    res, _ = eng.run(q, E.PLACE_CPU_ONLY)
+    if q.kind == "aggr":
+        assert oracle.res_scalar(res) == exp["scalar"]
+    else:
+        assert semantics.decode_groups(res) == exp["groups"]
+    eng.close()
+
+
+def test_host_engine_rejects_gpu_placements(host_engine):
+    q, data, eng = host_engine
+    with pytest.raises(E.EngineError):
+        eng.run(q, E.PLACE_RESIDENCY)
+    with pytest.raises(E.EngineError):
+        eng.cache_segment(q.fact_columns()[0], 0)
+
+
+def test_bad_queries_are_rejected():
+    eng = E.Engine(device=-1, threads=1)
+    eng.add_column("a", np.arange(10, dtype=np.int32))
+    from mordred_amd import query as Q
+    q = Q.StarQuery("bad", [], [], ("missing", None), "aggr", 1)
+    with pytest.raises(E.EngineError, match="unknown aggregate"):
+        eng.run(q, E.PLACE_CPU_ONLY)
+    with pytest.raises(E.EngineError, match="duplicate"):
+        eng.add_column("a", np.arange(3, dtype=np.int32))
+    eng.close()

@@ -1,0 +1,77 @@
+"""Engine on the GPU: segment cache, CPU/GPU split, PCIe streaming, merge.
+
+Every placement must return the oracle's result buffer bit-exactly: the
+cached segment group on the fused GPU kernel, the rest on the engine's CPU
+operators and/or streamed to the GPU, merged as CPUProcessing.cu:1786-1819.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import queries as OQ
+from mordred_amd import engine as E
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+SEG = oracle.SEGMENT_SIZE
+
+
+@pytest.fixture(scope="module", params=["q11", "q21", "q4way"])
+def gengine(request, cuda):
+    q, data = cases.star(request.param, 5 * SEG + 77_777, seed=9)
+    eng = E.Engine(device=0, cache_segments=128, threads=8)
+    eng.add_data(data, q)
+    ref = OQ.run_query(q, data, oracle.CONV_GPU)
+    yield q, data, eng, ref
+    eng.close()
+
+
+@pytest.mark.parametrize("placement", [E.PLACE_RESIDENCY, E.PLACE_COST_SPLIT, E.PLACE_GPU_ONLY, E.PLACE_CPU_ONLY])
+def test_placements_match_oracle(gengine, placement):
+    q, data, eng, ref = gengine
+    print(f"[gpu] engine {q.name} placement {placement}", flush=True)
+    eng.place(q, 0.5)
+    res, st = eng.run(q, placement)
+    np.testing.assert_array_equal(res, ref)
+    assert st["gpu_rows"] + st["cpu_rows"] + st["stream_rows"] == data.n
+    if placement == E.PLACE_RESIDENCY:
+        assert st["gpu_segments"] == 3 and st["cpu_segments"] == 3 and st["stream_segments"] == 0
+    if placement == E.PLACE_GPU_ONLY:
+        assert st["cpu_rows"] == 0 and st["stream_segments"] == 3
+    if placement == E.PLACE_CPU_ONLY:
+        assert st["gpu_rows"] == 0 and st["stream_rows"] == 0
+
+
+def test_residency_follows_cost_model(gengine):
+    """place(0.5) keeps the full segments with the largest cost weight; the
+    partial last segment weighs least and is never the one kept."""
+    q, data, eng, ref = gengine
+    eng.place(q, 0.5)
+    cols = q.fact_columns()
+    nseg = eng.segments(cols[0])
+    cached = [all(eng.is_cached(c, s) for c in cols) for s in range(nseg)]
+    assert sum(cached) == 3 and not cached[nseg - 1]
+    eng.place(q, 1.0)
+    res, st = eng.run(q, E.PLACE_RESIDENCY)
+    np.testing.assert_array_equal(res, ref)
+    assert st["gpu_segments"] == nseg and st["cpu_rows"] == 0
+
+
+def test_partial_last_segment_cached_and_streamed(gengine):
+    q, data, eng, ref = gengine
+    cols = q.fact_columns()
+    nseg = eng.segments(cols[0])
+    eng.place(q, 0.0)
+    for c in cols:                      # cache only the partial last segment and segment 1
+        eng.cache_segment(c, nseg - 1)
+        eng.cache_segment(c, 1)
+    res, st = eng.run(q, E.PLACE_RESIDENCY)
+    np.testing.assert_array_equal(res, ref)
+    assert st["gpu_segments"] == 2
+    eng.evict_segment(cols[0], 1)       # one column missing: segment 1 leaves the GPU group
+    res, st = eng.run(q, E.PLACE_GPU_ONLY)
+    np.testing.assert_array_equal(res, ref)
+    assert st["gpu_segments"] == 1 and st["stream_segments"] == nseg - 1
