@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <future>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -454,16 +455,26 @@ struct GpuArgs {
   bool has_filter = false;
 };
 
-void gpu_args(const Resolved& r, int* const ht[4], GpuArgs& a) {
+// Kernel arguments of the fused fact kernel.  With `staged` set, every fact
+// column's segment table is the staged one (fact column i at staged + i*nseg)
+// instead of the column's own col_idx; dimension columns keep their own.
+void gpu_args(const Resolved& r, int* const ht[4], int* staged, GpuArgs& a) {
   const mordred_star_query& q = *r.q;
+  auto idx = [&](Column* c) -> int* {
+    if (staged) {
+      auto it = std::find(r.fact_cols.begin(), r.fact_cols.end(), c);
+      if (it != r.fact_cols.end()) return staged + (size_t)(it - r.fact_cols.begin()) * r.nseg;
+    }
+    return c->d_idx;
+  };
   if (q.nfilters > 0) {
     a.has_filter = true;
-    a.f.filter_idx1 = r.filt[0]->d_idx;
+    a.f.filter_idx1 = idx(r.filt[0]);
     a.f.compare1 = q.filters[0].lo;
     a.f.compare2 = q.filters[0].hi;
     a.f.mode1 = 1;
     if (q.nfilters > 1) {
-      a.f.filter_idx2 = r.filt[1]->d_idx;
+      a.f.filter_idx2 = idx(r.filt[1]);
       a.f.compare3 = q.filters[1].lo;
       a.f.compare4 = q.filters[1].hi;
       a.f.mode2 = 1;
@@ -478,7 +489,7 @@ void gpu_args(const Resolved& r, int* const ht[4], GpuArgs& a) {
   int* uqs[4] = {&a.g.unique_val1, &a.g.unique_val2, &a.g.unique_val3, &a.g.unique_val4};
   for (int k = 0; k < 4; k++) {
     if (!r.join[k]) continue;
-    *keys[k] = r.key[k]->d_idx;
+    *keys[k] = idx(r.key[k]);
     *hts[k] = ht[k];
     *lens[k] = r.join[k]->dim_len;
     *mks[k] = r.join[k]->min_key;
@@ -488,8 +499,8 @@ void gpu_args(const Resolved& r, int* const ht[4], GpuArgs& a) {
       *uqs[k] = r.join[k]->unique_val;
     }
   }
-  a.g.aggr_idx1 = r.a1->d_idx;
-  a.g.aggr_idx2 = r.a2 ? r.a2->d_idx : nullptr;
+  a.g.aggr_idx1 = idx(r.a1);
+  a.g.aggr_idx2 = r.a2 ? idx(r.a2) : nullptr;
   a.g.total_val = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
 }
 
@@ -545,6 +556,163 @@ Selectivity estimate_selectivity(const Resolved& r, const CpuTables& t) {
     s.filt[i] = m > 0 ? (double)pass / (double)m : 1.0;
   }
   return s;
+}
+
+// ------------------------------------------------------------ GPU half of a run
+// Runs on its own host thread so that the CPU operators start at once: the
+// HIP runtime may block an enqueue (hipMemcpyAsync of a streamed segment)
+// until earlier commands drain, which on the calling thread would serialise
+// the PCIe stream and the CPU group.  The builds and the cached segment group
+// are issued immediately; the streamed segments once the split is known.
+struct GpuPart {
+  std::vector<int> gpu_segs;              // cached segment group
+  std::future<std::vector<int>> streamed; // streamed segments, from the CPU side
+  std::vector<int> res;                   // partial result (res_ints)
+  int rc = 0;
+  double ms = 0;
+  long long gpu_rows = 0, stream_rows = 0;
+  int gpu_segments = 0, stream_segments = 0;
+};
+
+// Staging slots borrowed from the free list for one run; on every exit the
+// engine's streams are drained before the slots go back.
+struct SlotLoan {
+  mordred_engine* e;
+  std::vector<int> slots;
+  ~SlotLoan() {
+    if (slots.empty()) return;
+    for (hipStream_t s : {e->s_main, e->s_copy[0], e->s_copy[1]}) (void)hipStreamSynchronize(s);
+    for (int s : slots) e->free_slots.push_back(s);
+  }
+};
+
+struct Events {
+  hipEvent_t built = nullptr, tables = nullptr;
+  ~Events() {
+    if (built) (void)hipEventDestroy(built);
+    if (tables) (void)hipEventDestroy(tables);
+  }
+};
+
+inline size_t align64(size_t b) { return (b + 63) & ~size_t(63); }
+
+int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) {
+  const auto t0 = Clock::now();
+  HIP_OK(hipSetDevice(e->device));
+  HIP_OK(e->d_res.reserve(res_ints * sizeof(int)));
+  int* d_res = static_cast<int*>(e->d_res.p);
+  HIP_OK(hipMemsetAsync(d_res, 0, res_ints * sizeof(int), e->s_main));
+  int* ht[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int k = 0; k < 4; k++) {
+    if (!r.join[k]) continue;
+    const mordred_join& j = *r.join[k];
+    HIP_OK(e->d_ht[k].reserve(2 * (size_t)j.dim_len * sizeof(int)));
+    ht[k] = static_cast<int*>(e->d_ht[k].p);
+    HIP_OK(hipMemsetAsync(ht[k], 0, 2 * (size_t)j.dim_len * sizeof(int), e->s_main));
+    filterArgsGPU bf{};
+    if (r.dfilt[k]) {
+      bf.filter_idx1 = r.dfilt[k]->d_idx;
+      bf.mode1 = j.filter_mode;
+      bf.compare1 = j.c1;
+      bf.compare2 = j.c2;
+    }
+    buildArgsGPU ba{r.dkey[k]->d_idx, r.dval[k] ? r.dval[k]->d_idx : nullptr, j.dim_len, j.min_key, 0};
+    const int brc = mordred_build_GPU2(e->cache, r.dfilt[k] ? &bf : nullptr, &ba, (int)r.dkey[k]->n, ht[k], 0,
+                                       e->d_identity, (mordred_stream_t)e->s_main);
+    if (brc) return hip_fail(e, (hipError_t)brc, "mordred_build_GPU2");
+  }
+  Events ev;
+  HIP_OK(hipEventCreateWithFlags(&ev.built, hipEventDisableTiming));
+  HIP_OK(hipEventRecord(ev.built, e->s_main));
+
+  // pinned + device scratch: [cached group: nseg shorts][staged segment tables:
+  // nc x nseg ints][one segment group per streamed batch]
+  const size_t nseg = (size_t)r.nseg, nc = r.fact_cols.size();
+  const size_t sg_bytes = align64(nseg * sizeof(short)), tab_bytes = align64(nc * nseg * sizeof(int));
+  const size_t batch_bytes = align64(kStreamBatch * sizeof(short));
+  const size_t max_batches = (nseg + kStreamBatch - 1) / kStreamBatch;
+  const size_t scratch = sg_bytes + tab_bytes + max_batches * batch_bytes;
+  HIP_OK(e->h_stage.reserve(scratch));
+  HIP_OK(e->d_sg.reserve(scratch));
+  char* hs = static_cast<char*>(e->h_stage.p);
+  char* ds = static_cast<char*>(e->d_sg.p);
+
+  GpuArgs a;
+  gpu_args(r, ht, nullptr, a);
+  if (!g.gpu_segs.empty()) {
+    order_last_partial(r, g.gpu_segs);
+    short* h_sg = reinterpret_cast<short*>(hs);
+    for (size_t i = 0; i < g.gpu_segs.size(); i++) h_sg[i] = (short)g.gpu_segs[i];
+    HIP_OK(hipMemcpyAsync(ds, h_sg, g.gpu_segs.size() * sizeof(short), hipMemcpyHostToDevice, e->s_main));
+    g.gpu_rows = group_rows(r, g.gpu_segs);
+    const int frc = launch_fact(e, r, a, (int)g.gpu_rows, d_res, reinterpret_cast<short*>(ds), e->s_main);
+    if (frc) return frc;
+    g.gpu_segments = (int)g.gpu_segs.size();
+  }
+
+  std::vector<int> seg_order = g.streamed.get();
+  SlotLoan loan{e, {}};
+  if (!seg_order.empty()) {
+    // Two staging sets of kStreamBatch segments x fact columns.  Batch i uses
+    // set i % 2 on copy stream i % 2, so a set is refilled only after the same
+    // stream has run the kernel of the batch before it.  The staged tables map
+    // every streamed segment to the slot of its batch; other entries keep the
+    // column's own slot (or 0) and are not read by the streamed launches.
+    order_last_partial(r, seg_order);
+    for (size_t i = 0; i < 2 * kStreamBatch * nc; i++) {
+      loan.slots.push_back(e->free_slots.back());
+      e->free_slots.pop_back();
+    }
+    int* h_tab = reinterpret_cast<int*>(hs + sg_bytes);
+    int* d_tab = reinterpret_cast<int*>(ds + sg_bytes);
+    for (size_t c = 0; c < nc; c++) {
+      const std::vector<int>& mirror = r.fact_cols[c]->slot;
+      for (size_t s = 0; s < nseg; s++) h_tab[c * nseg + s] = std::max(0, mirror[s]);
+      for (size_t i = 0; i < seg_order.size(); i++) {
+        const size_t bi = i / kStreamBatch, within = i % kStreamBatch;
+        h_tab[c * nseg + seg_order[i]] = loan.slots[((bi & 1) * kStreamBatch + within) * nc + c];
+      }
+    }
+    for (int b = 0; b < 2; b++) HIP_OK(hipStreamWaitEvent(e->s_copy[b], ev.built, 0));
+    HIP_OK(hipMemcpyAsync(d_tab, h_tab, nc * nseg * sizeof(int), hipMemcpyHostToDevice, e->s_copy[0]));
+    HIP_OK(hipEventCreateWithFlags(&ev.tables, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(ev.tables, e->s_copy[0]));
+    HIP_OK(hipStreamWaitEvent(e->s_copy[1], ev.tables, 0));
+    GpuArgs as;
+    gpu_args(r, ht, d_tab, as);
+    char* hb = hs + sg_bytes + tab_bytes;
+    char* db = ds + sg_bytes + tab_bytes;
+    const size_t nbatches = (seg_order.size() + kStreamBatch - 1) / kStreamBatch;
+    for (size_t bi = 0; bi < nbatches; bi++) {
+      const int set = (int)(bi & 1);
+      hipStream_t s = e->s_copy[set];
+      const size_t first = bi * kStreamBatch;
+      const size_t cnt = std::min<size_t>(kStreamBatch, seg_order.size() - first);
+      short* h_bsg = reinterpret_cast<short*>(hb + bi * batch_bytes);
+      short* d_bsg = reinterpret_cast<short*>(db + bi * batch_bytes);
+      long long rows = 0;
+      for (size_t i = 0; i < cnt; i++) {
+        const int seg = seg_order[first + i];
+        h_bsg[i] = (short)seg;
+        rows += segment_rows(r, seg);
+        for (size_t c = 0; c < nc; c++) {
+          const int slot = loan.slots[(set * kStreamBatch + i) * nc + c];
+          HIP_OK(hipMemcpyAsync(e->cache + (size_t)slot * kSeg, r.fact_cols[c]->host + (size_t)seg * kSeg,
+                                segment_rows(r, seg) * sizeof(int), hipMemcpyHostToDevice, s));
+        }
+      }
+      HIP_OK(hipMemcpyAsync(d_bsg, h_bsg, cnt * sizeof(short), hipMemcpyHostToDevice, s));
+      const int frc = launch_fact(e, r, as, (int)rows, d_res, d_bsg, s);
+      if (frc) return frc;
+      g.stream_rows += rows;
+    }
+    g.stream_segments = (int)seg_order.size();
+  }
+  for (hipStream_t s : {e->s_main, e->s_copy[0], e->s_copy[1]}) HIP_OK(hipStreamSynchronize(s));
+  g.res.assign(res_ints, 0);
+  HIP_OK(hipMemcpy(g.res.data(), d_res, res_ints * sizeof(int), hipMemcpyDeviceToHost));
+  g.ms = ms_since(t0);
+  return 0;
 }
 
 }  // namespace
@@ -730,7 +898,6 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
       if (c && !c->pinned_gpu && placement != MORDRED_PLACE_CPU_ONLY)
         return fail(e, "dimension columns must be GPU-resident (add_column gpu_resident=1)");
   }
-  if (e->device >= 0) HIP_OK(hipSetDevice(e->device));
   const auto t0 = Clock::now();
   const int G = q->kind == MORDRED_KIND_GROUPBY ? q->total_val : 1;
   const size_t res_ints = 6 * (size_t)G;
@@ -738,22 +905,33 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   mordred_run_stats st{};
 
   // ---- segment groups by residency (QueryOptimizer.cu:1680-1716)
-  std::vector<int> gpu_segs, other;
+  GpuPart g;
+  std::vector<int> other;
   for (int s = 0; s < r.nseg; s++) {
-    if (placement != MORDRED_PLACE_CPU_ONLY && segment_cached(r, s)) gpu_segs.push_back(s);
+    if (placement != MORDRED_PLACE_CPU_ONLY && segment_cached(r, s)) g.gpu_segs.push_back(s);
     else other.push_back(s);
   }
+  const bool may_stream = !other.empty() &&
+                          (placement == MORDRED_PLACE_GPU_ONLY || placement == MORDRED_PLACE_COST_SPLIT);
+  const bool use_gpu = !g.gpu_segs.empty() || may_stream;
+  const size_t staging = 2 * (size_t)kStreamBatch * r.fact_cols.size();
 
-  // ---- CPU hash tables (needed for the CPU operators and the cost model)
+  // ---- GPU half on its own thread (builds + cached group start now)
+  std::promise<std::vector<int>> streamed;
+  g.streamed = streamed.get_future();
+  std::thread feeder;
+  if (use_gpu) feeder = std::thread([&] { g.rc = gpu_part(e, r, g, res_ints); });
+
+  // ---- CPU hash tables (the CPU operators and the cost model need them)
   const auto tc0 = Clock::now();
   CpuTables t;
-  const bool need_cpu_tables = placement != MORDRED_PLACE_GPU_ONLY && !other.empty();
   Selectivity sel;
-  if (need_cpu_tables || placement == MORDRED_PLACE_COST_SPLIT) {
+  bool cpu_tables = false;
+  if ((placement != MORDRED_PLACE_GPU_ONLY && !other.empty()) || placement == MORDRED_PLACE_COST_SPLIT) {
     cpu_build(e, r, t);
     sel = estimate_selectivity(r, t);
+    cpu_tables = true;
   }
-  double cpu_build_ms = ms_since(tc0);
 
   // ---- split the non-cached segments between the CPU and the PCIe stream
   std::vector<int> cpu_segs, stream_segs;
@@ -761,134 +939,29 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   const double t_str = transfer_cost(r, kSeg, e->bw_pci) * 1e3;
   st.est_cpu_ms_per_segment = t_cpu;
   st.est_stream_ms_per_segment = t_str;
-  if (placement == MORDRED_PLACE_RESIDENCY || placement == MORDRED_PLACE_CPU_ONLY) {
-    cpu_segs = other;
-  } else if (placement == MORDRED_PLACE_GPU_ONLY) {
+  if (placement == MORDRED_PLACE_GPU_ONLY) {
     stream_segs = other;
-  } else {
+  } else if (placement == MORDRED_PLACE_COST_SPLIT) {
     // both sides run concurrently: stream k segments so that k*t_str and
     // (n-k)*t_cpu finish together
     const int n = (int)other.size();
     const int k = (int)std::min<double>(n, std::max(0.0, n * t_cpu / (t_cpu + t_str) + 0.5));
     stream_segs.assign(other.begin(), other.begin() + k);
     cpu_segs.assign(other.begin() + k, other.end());
+  } else {
+    cpu_segs = other;
   }
-  if (!stream_segs.empty() && e->free_slots.size() < 2 * kStreamBatch * r.fact_cols.size()) {
+  if (!stream_segs.empty() && e->free_slots.size() < staging) {
     // not enough free cache slots for two staging batches: CPU instead
+    if (!cpu_tables) cpu_build(e, r, t);
+    cpu_tables = true;
     cpu_segs.insert(cpu_segs.end(), stream_segs.begin(), stream_segs.end());
     stream_segs.clear();
   }
   std::sort(cpu_segs.begin(), cpu_segs.end());
-
-  // ---- GPU: hash tables, cached segment group, streamed batches (async)
-  const auto tg0 = Clock::now();
-  const bool use_gpu = !gpu_segs.empty() || !stream_segs.empty();
-  std::vector<int> borrowed;  // staging slots taken from the free list
-  hipEvent_t ev_built = nullptr;
-  int* d_res = nullptr;
-  if (use_gpu) {
-    HIP_OK(e->d_res.reserve(res_ints * sizeof(int)));
-    d_res = static_cast<int*>(e->d_res.p);
-    HIP_OK(hipMemsetAsync(d_res, 0, res_ints * sizeof(int), e->s_main));
-    int* ht[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (int k = 0; k < 4; k++) {
-      if (!r.join[k]) continue;
-      const mordred_join& j = *r.join[k];
-      HIP_OK(e->d_ht[k].reserve(2 * (size_t)j.dim_len * sizeof(int)));
-      ht[k] = static_cast<int*>(e->d_ht[k].p);
-      HIP_OK(hipMemsetAsync(ht[k], 0, 2 * (size_t)j.dim_len * sizeof(int), e->s_main));
-      filterArgsGPU bf{};
-      if (r.dfilt[k]) {
-        bf.filter_idx1 = r.dfilt[k]->d_idx;
-        bf.mode1 = j.filter_mode;
-        bf.compare1 = j.c1;
-        bf.compare2 = j.c2;
-      }
-      buildArgsGPU ba{r.dkey[k]->d_idx, r.dval[k] ? r.dval[k]->d_idx : nullptr, j.dim_len, j.min_key, 0};
-      const int brc = mordred_build_GPU2(e->cache, r.dfilt[k] ? &bf : nullptr, &ba, (int)r.dkey[k]->n, ht[k], 0,
-                                         e->d_identity, (mordred_stream_t)e->s_main);
-      if (brc) return hip_fail(e, (hipError_t)brc, "mordred_build_GPU2");
-    }
-    GpuArgs a;
-    gpu_args(r, ht, a);
-    // Streamed segments: two staging sets of kStreamBatch segments x fact
-    // columns are borrowed from the free slots.  Batch i uses set i % 2 on copy
-    // stream i % 2, so a set is reused only after the same stream has run the
-    // kernel of the batch before it.  Each streamed segment's table entry
-    // points at the slot of its batch for the whole run (one upload per
-    // column); entries of uncached segments are never read otherwise.
-    std::vector<int> seg_order = stream_segs;
-    order_last_partial(r, seg_order);
-    const size_t nc = r.fact_cols.size();
-    if (!stream_segs.empty()) {
-      for (size_t i = 0; i < 2 * kStreamBatch * nc; i++) {
-        borrowed.push_back(e->free_slots.back());
-        e->free_slots.pop_back();
-      }
-      for (size_t c = 0; c < nc; c++) {
-        Column* col_c = r.fact_cols[c];
-        std::vector<int>& mirror = col_c->slot;
-        std::vector<int> table = mirror;
-        for (size_t i = 0; i < seg_order.size(); i++) {
-          const size_t bi = i / kStreamBatch, within = i % kStreamBatch;
-          table[seg_order[i]] = borrowed[((bi & 1) * kStreamBatch + within) * nc + c];
-        }
-        HIP_OK(hipMemcpy(col_c->d_idx, table.data(), table.size() * sizeof(int), hipMemcpyHostToDevice));
-      }
-      HIP_OK(hipEventCreateWithFlags(&ev_built, hipEventDisableTiming));
-      HIP_OK(hipEventRecord(ev_built, e->s_main));
-      for (int b = 0; b < 2; b++) HIP_OK(hipStreamWaitEvent(e->s_copy[b], ev_built, 0));
-    }
-    // pinned scratch: segment groups (shorts) and staging index updates (ints)
-    const size_t nstream_batches = (stream_segs.size() + kStreamBatch - 1) / kStreamBatch;
-    const size_t stage_bytes = (size_t)r.nseg * sizeof(short) + 64 + nstream_batches * (kStreamBatch * sizeof(short) + 64);
-    HIP_OK(e->h_stage.reserve(stage_bytes));
-    HIP_OK(e->d_sg.reserve((size_t)r.nseg * sizeof(short) + nstream_batches * (kStreamBatch * sizeof(short) + 64) + 64));
-    char* hs = static_cast<char*>(e->h_stage.p);
-    char* ds = static_cast<char*>(e->d_sg.p);
-    if (!gpu_segs.empty()) {
-      order_last_partial(r, gpu_segs);
-      short* h_sg = reinterpret_cast<short*>(hs);
-      for (size_t i = 0; i < gpu_segs.size(); i++) h_sg[i] = (short)gpu_segs[i];
-      HIP_OK(hipMemcpyAsync(ds, h_sg, gpu_segs.size() * sizeof(short), hipMemcpyHostToDevice, e->s_main));
-      const int frc = launch_fact(e, r, a, (int)group_rows(r, gpu_segs), d_res, reinterpret_cast<short*>(ds),
-                                  e->s_main);
-      if (frc) return frc;
-      st.gpu_rows = group_rows(r, gpu_segs);
-      st.gpu_segments = (int)gpu_segs.size();
-    }
-    hs += (size_t)r.nseg * sizeof(short) + 64;
-    ds += (size_t)r.nseg * sizeof(short) + 64;
-    if (!stream_segs.empty()) {
-      for (size_t bi = 0; bi < nstream_batches; bi++) {
-        const int set = (int)(bi & 1);
-        hipStream_t s = e->s_copy[set];
-        const size_t first = bi * kStreamBatch;
-        const size_t cnt = std::min<size_t>(kStreamBatch, seg_order.size() - first);
-        short* h_bsg = reinterpret_cast<short*>(hs);
-        for (size_t i = 0; i < cnt; i++) {
-          const int seg = seg_order[first + i];
-          h_bsg[i] = (short)seg;
-          for (size_t c = 0; c < nc; c++) {
-            const int slot = borrowed[(set * kStreamBatch + i) * nc + c];
-            HIP_OK(hipMemcpyAsync(e->cache + (size_t)slot * kSeg, r.fact_cols[c]->host + (size_t)seg * kSeg,
-                                  segment_rows(r, seg) * sizeof(int), hipMemcpyHostToDevice, s));
-          }
-        }
-        HIP_OK(hipMemcpyAsync(ds, h_bsg, cnt * sizeof(short), hipMemcpyHostToDevice, s));
-        std::vector<int> bsegs(seg_order.begin() + first, seg_order.begin() + first + cnt);
-        const int frc = launch_fact(e, r, a, (int)group_rows(r, bsegs), d_res, reinterpret_cast<short*>(ds), s);
-        if (frc) return frc;
-        hs += kStreamBatch * sizeof(short) + 64;
-        ds += kStreamBatch * sizeof(short) + 64;
-        st.stream_rows += group_rows(r, bsegs);
-      }
-      st.stream_segments = (int)stream_segs.size();
-    }
-  }
+  streamed.set_value(stream_segs);
 
   // ---- CPU: the CPU segment group, concurrently with the GPU
-  const auto tc1 = Clock::now();
   std::vector<int> cpu_res;
   if (!cpu_segs.empty()) {
     cpu_res.assign(res_ints, 0);
@@ -896,27 +969,25 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     st.cpu_rows = group_rows(r, cpu_segs);
     st.cpu_segments = (int)cpu_segs.size();
   }
-  st.cpu_ms = cpu_build_ms + ms_since(tc1);
+  st.cpu_ms = cpu_tables ? ms_since(tc0) : 0.0;
 
   // ---- wait for the GPU, merge
-  std::vector<int> gpu_res;
-  if (use_gpu) {
-    gpu_res.assign(res_ints, 0);
-    for (hipStream_t s : {e->s_main, e->s_copy[0], e->s_copy[1]}) HIP_OK(hipStreamSynchronize(s));
-    st.gpu_ms = ms_since(tg0);
-    HIP_OK(hipMemcpy(gpu_res.data(), d_res, res_ints * sizeof(int), hipMemcpyDeviceToHost));
-    for (int s : borrowed) e->free_slots.push_back(s);
-    if (ev_built) (void)hipEventDestroy(ev_built);
-  }
+  if (feeder.joinable()) feeder.join();
+  if (g.rc) return g.rc;
+  st.gpu_ms = g.ms;
+  st.gpu_rows = g.gpu_rows;
+  st.gpu_segments = g.gpu_segments;
+  st.stream_rows = g.stream_rows;
+  st.stream_segments = g.stream_segments;
   const auto tm0 = Clock::now();
-  if (!cpu_res.empty() && !gpu_res.empty()) {
-    if (q->kind == MORDRED_KIND_GROUPBY) merge(cpu_res.data(), gpu_res.data(), G);
-    else reinterpret_cast<long long*>(cpu_res.data())[2] += reinterpret_cast<long long*>(gpu_res.data())[2];
+  if (!cpu_res.empty() && !g.res.empty()) {
+    if (q->kind == MORDRED_KIND_GROUPBY) merge(cpu_res.data(), g.res.data(), G);
+    else reinterpret_cast<long long*>(cpu_res.data())[2] += reinterpret_cast<long long*>(g.res.data())[2];
     std::memcpy(res, cpu_res.data(), res_ints * sizeof(int));
   } else if (!cpu_res.empty()) {
     std::memcpy(res, cpu_res.data(), res_ints * sizeof(int));
-  } else if (!gpu_res.empty()) {
-    std::memcpy(res, gpu_res.data(), res_ints * sizeof(int));
+  } else if (!g.res.empty()) {
+    std::memcpy(res, g.res.data(), res_ints * sizeof(int));
   }
   st.merge_ms = ms_since(tm0);
   st.total_ms = ms_since(t0);
