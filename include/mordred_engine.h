@@ -112,6 +112,58 @@ int mordred_engine_place(mordred_engine* e, const mordred_star_query* q, double 
 int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int placement, int* res,
                        mordred_run_stats* stats);
 
+/* ---- shared-scan batches ------------------------------------------------
+ * No counterpart in the reference, which runs one query at a time: up to 64
+ * star queries execute in one persistent kernel that reads the fact columns
+ * once.  Each dimension is probed once per fact row through a record table
+ * that holds, per hash slot, the mask of queries whose filter the row passes
+ * and its group attributes; every query's result equals its own
+ * mordred_engine_run (GPU convention) bit for bit.  The launch sequence
+ * (clear, record builds, scan) is captured in a hipGraph on first use and
+ * replayed for every later batch of the same dataset and configuration.
+ * Requirements: every fact column of the batch cached in HBM (all segments),
+ * dimension columns GPU-resident, queries that join one dimension agree on
+ * its key columns, dim_len and min_key, at most 2 distinct group columns per
+ * dimension with a value span below 65536, at most 12 distinct fact columns
+ * and 4 distinct aggregate expressions, collision-free dimension hashing. */
+#define MORDRED_BATCH_MAX_QUERIES 64
+
+typedef struct mordred_batch_config {
+  int block;      /* threads per workgroup: 256, 512 or 1024 */
+  int vpt;        /* 4-row groups per thread: 1 or 2 (1024 threads: 1) */
+  int lds_bytes;  /* dynamic LDS for descriptors + group arrays, <= 160 KiB */
+} mordred_batch_config;
+
+typedef struct mordred_batch_stats {
+  double total_ms;           /* host: upload, graph launch, results back */
+  double gpu_ms;             /* HIP events around the graph launch */
+  int lds_queries;           /* queries whose group array sits in LDS */
+  int global_queries;        /* queries using global atomics */
+  int dead_queries;          /* empty predicate range: result stays zero */
+  int graph_captured;        /* 1 if this call (re)captured the graph */
+  mordred_batch_config config;
+} mordred_batch_stats;
+
+typedef struct mordred_batch_tune_entry {
+  mordred_batch_config config;
+  double gpu_ms;             /* best of the timed runs, < 0 if the config failed */
+} mordred_batch_tune_entry;
+
+/* Run nq star queries as one batch.  res[i] receives query i's result
+ * (total_val*6 ints, or 6 for AGGR), overwritten.  cfg NULL = the engine's
+ * tuned configuration (or its default). */
+int mordred_engine_batch(mordred_engine* e, const mordred_star_query* qs, int nq, int* const* res,
+                         const mordred_batch_config* cfg, mordred_batch_stats* stats);
+
+/* Autotune the batch launch on this batch: every candidate (threads per
+ * workgroup x items per thread x LDS budget, which moves group arrays between
+ * LDS and HBM) runs `reps` timed times after one warm-up; all candidates must
+ * agree bit for bit.  The fastest becomes the engine's configuration and is
+ * written to *best.  table (may be NULL) receives up to max_entries entries. */
+int mordred_engine_batch_tune(mordred_engine* e, const mordred_star_query* qs, int nq, int reps,
+                              mordred_batch_config* best, mordred_batch_tune_entry* table, int max_entries,
+                              int* nentries);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,342 @@
+// Shared-scan batch kernels for gfx950 (see batch.hpp for the record table
+// and the batch descriptor).
+//
+//   batch_fill_kernel     clears the record tables of the used dimensions, the
+//                         result arena and the global touched bitmaps;
+//   batch_records_kernel  one pass over a dimension: slot, query mask, packed
+//                         group attributes (the 64 builds of the batch in one);
+//   batch_kernel          the persistent shared scan of the fact columns.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "batch.hpp"
+#include "pipeline.hpp"
+#include "tile.hpp"
+
+namespace mordred {
+
+namespace {
+
+__device__ __forceinline__ bool dim_filter_pass(int mode, int x, int c1, int c2) {
+  // build_GPU2 predicate modes (GPUProcessing.h:972-980)
+  if (mode == 0) return x == c1;
+  if (mode == 1) return x >= c1 && x <= c2;
+  if (mode == 2) return x == c1 || x == c2;
+  return true;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __restrict__ D, unsigned* status,
+                                                         uint4* r0, uint4* r1, uint4* r2, uint4* r3,
+                                                         int n0, int n1, int n2, int n3) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) status[0] = 0;
+  const uint4 z = make_uint4(0, 0, 0, 0);
+  uint4* recs[4] = {r0, r1, r2, r3};
+  const int lens[4] = {n0, n1, n2, n3};
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    if (!recs[d] || !D->dim[d].used) continue;
+    for (long long i = t; i < lens[d]; i += stride) recs[d][i] = z;
+  }
+  uint4* res = reinterpret_cast<uint4*>(D->res_arena);
+  const long long nres = D->res_ints / 4;
+  for (long long i = t; i < nres; i += stride) res[i] = z;
+  uint4* tw = reinterpret_cast<uint4*>(D->touch_arena);
+  const long long ntw = D->touch_words / 4;
+  for (long long i = t; i < ntw; i += stride) tw[i] = z;
+}
+
+// One thread per dimension row: its slot, the mask of queries whose filter it
+// passes, its packed group attributes.  Filters are sorted by column, so each
+// distinct filter column is read once per row.
+__global__ __launch_bounds__(256) void batch_records_kernel(BatchLaunch L, int d) {
+  const BatchDim& B = L.desc->dim[d];
+  if (!B.used) return;
+  const int nrows = B.nrows, len = B.len, mk = B.minkey, nf = B.nfilt;
+  const int stride = gridDim.x * blockDim.x;
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < nrows; row += stride) {
+    const int key = gather(L.cache, B.key, row);
+    const int h = phash(key, len, mk);
+    if ((unsigned)h >= (unsigned)len) continue;  // key below min: never probed in range
+    unsigned long long mask = 0;
+    const int* prev = nullptr;
+    int x = 0;
+    for (int f = 0; f < nf; f++) {
+      const BatchFilt F = B.filt[f];
+      bool pass = true;
+      if (F.col) {
+        if (F.col != prev) {
+          x = gather(L.cache, F.col, row);
+          prev = F.col;
+        }
+        pass = dim_filter_pass(F.mode, x, F.c1, F.c2);
+      }
+      if (pass) mask |= 1ull << F.q;
+    }
+    uint32_t packed = 0;
+    if (B.attr[0]) packed |= (uint32_t)(gather(L.cache, B.attr[0], row) - B.attr_base[0]) & 0xFFFFu;
+    if (B.attr[1]) packed |= ((uint32_t)(gather(L.cache, B.attr[1], row) - B.attr_base[1]) & 0xFFFFu) << 16;
+    uint4* r = B.rec + h;
+    if (atomicCAS(&r->w, 0u, 1u) != 0u) atomicAdd(L.status, 1u);
+    r->x = (uint32_t)mask;
+    r->y = (uint32_t)(mask >> 32);
+    r->z = packed;
+  }
+}
+
+template <int BLOCK, int VPT>
+__global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
+  constexpr int IPT = 4 * VPT;
+  constexpr int TILE = BLOCK * IPT;
+  const BatchDesc* __restrict__ D = L.desc;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int nq = D->nq;
+  BatchQuery* lq = reinterpret_cast<BatchQuery*>(lds_raw);
+  unsigned long long* lsum = reinterpret_cast<unsigned long long*>(lds_raw + nq * sizeof(BatchQuery));
+  const int nsum = D->lds_sum_words;
+  uint32_t* ltouch = reinterpret_cast<uint32_t*>(lsum + nsum);
+  const int ntouch = D->lds_touch_words;
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(D->q);
+    uint4* dst = reinterpret_cast<uint4*>(lq);
+    for (int i = threadIdx.x; i < nq * (int)(sizeof(BatchQuery) / 16); i += BLOCK) dst[i] = src[i];
+    for (int i = threadIdx.x; i < nsum; i += BLOCK) lsum[i] = 0;
+    for (int i = threadIdx.x; i < ntouch; i += BLOCK) ltouch[i] = 0;
+  }
+  __syncthreads();
+
+  const int ncols = D->ncols, nexpr = D->nexpr;
+  const unsigned long long all = D->all;
+  int ab[4][2];
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    ab[d][0] = D->dim[d].attr_base[0];
+    ab[d][1] = D->dim[d].attr_base[1];
+  }
+  const int ntiles = (L.num_tuples + TILE - 1) / TILE;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int pos = tile * TILE;
+    const int n = min(TILE, L.num_tuples - pos);
+    const int seg = L.seg_group[pos >> kSegLog];
+    const int off = pos & kSegMask;
+    const bool vec = n == TILE;
+
+    // every column of the tile, issued before any use
+    int x[kBatchMaxCols][IPT];
+#pragma unroll
+    for (int c = 0; c < kBatchMaxCols; c++) {
+      if (c < ncols && D->col[c]) {
+        const int* p = L.cache + ((size_t)D->col[c][seg] << kSegLog) + off;
+        load_tile<BLOCK, VPT>(p, vec, n, x[c]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < IPT; j++) x[c][j] = 0;
+      }
+    }
+    const uint32_t inr = vec ? (IPT == 32 ? 0xFFFFFFFFu : (1u << IPT) - 1u) : range_mask<BLOCK, VPT>(n);
+    unsigned long long alive[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) alive[j] = (inr >> j & 1u) ? all : 0ull;
+
+    // fact BETWEEN predicates, grouped by column (GPUProcessing.h:72-86)
+#pragma unroll
+    for (int c = 0; c < kBatchMaxCols; c++) {
+      const int pe = c < ncols ? D->pred_begin[c + 1] : 0;
+      for (int p = D->pred_begin[c]; p < pe; p++) {
+        const int lo = D->pred[p].lo;
+        const unsigned span = (unsigned)D->pred[p].hi - (unsigned)lo;
+        const unsigned long long keep = ~(1ull << D->pred[p].q);
+#pragma unroll
+        for (int j = 0; j < IPT; j++)
+          if ((unsigned)(x[c][j] - lo) > span) alive[j] &= keep;
+      }
+    }
+
+    // one probe per dimension for all queries: record gathers of the tile are
+    // issued before any is used; items no query needs read slot 0
+    uint32_t attrw[4][IPT];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+#pragma unroll
+      for (int j = 0; j < IPT; j++) attrw[d][j] = 0;
+      if (!D->dim[d].used) continue;
+      const uint4* __restrict__ rec = D->dim[d].rec;
+      const int len = D->dim[d].len, mk = D->dim[d].minkey;
+      const unsigned long long uses = D->dim[d].uses;
+      int hs[IPT];
+      bool inb[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        const int h = phash(x[d][j], len, mk);
+        inb[j] = (unsigned)h < (unsigned)len;
+        hs[j] = (inb[j] && (alive[j] & uses)) ? h : 0;
+      }
+      uint4 r[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; j++) r[j] = rec[hs[j]];
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        const unsigned long long m = inb[j] ? ((unsigned long long)r[j].y << 32 | r[j].x) : 0ull;
+        alive[j] &= m | ~uses;
+        attrw[d][j] = r[j].z;
+      }
+    }
+
+    // aggregate expressions of the batch (int32 arithmetic, GPUProcessing.h:149, 1763)
+    int ev[kBatchMaxExpr][IPT];
+#pragma unroll
+    for (int e = 0; e < kBatchMaxExpr; e++) {
+#pragma unroll
+      for (int j = 0; j < IPT; j++) ev[e][j] = 0;
+      if (e >= nexpr) continue;
+      const int kind = D->expr[e].kind, a1 = D->expr[e].a1, a2 = D->expr[e].a2;
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        int v1 = 0, v2 = 0;
+#pragma unroll
+        for (int c = 0; c < kBatchMaxCols; c++) {
+          if (c == a1) v1 = x[c][j];
+          if (c == a2) v2 = x[c][j];
+        }
+        ev[e][j] = kind == kExprProduct ? (int)((unsigned)v1 * (unsigned)v2) : (int)((unsigned)v1 - (unsigned)v2);
+      }
+    }
+
+    // every surviving (row, query) pair
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      unsigned long long m = alive[j];
+      while (m) {
+        const int q = __builtin_ctzll(m);
+        m &= m - 1;
+        const BatchQuery& Q = lq[q];
+        const int gv0 = Q.attr[0] < 0 ? 0 : (Q.attr[0] ? ab[0][1] : ab[0][0]) + (int)((attrw[0][j] >> (16 * Q.attr[0])) & 0xFFFFu);
+        const int gv1 = Q.attr[1] < 0 ? 0 : (Q.attr[1] ? ab[1][1] : ab[1][0]) + (int)((attrw[1][j] >> (16 * Q.attr[1])) & 0xFFFFu);
+        const int gv2 = Q.attr[2] < 0 ? 0 : (Q.attr[2] ? ab[2][1] : ab[2][0]) + (int)((attrw[2][j] >> (16 * Q.attr[2])) & 0xFFFFu);
+        const int gv3 = Q.attr[3] < 0 ? 0 : (Q.attr[3] ? ab[3][1] : ab[3][0]) + (int)((attrw[3][j] >> (16 * Q.attr[3])) & 0xFFFFu);
+        // dense group index (GPUProcessing.h:142)
+        const int h = mod_pos((gv0 - Q.minv[0]) * Q.uniq[0] + (gv1 - Q.minv[1]) * Q.uniq[1] +
+                                  (gv2 - Q.minv[2]) * Q.uniq[2] + (gv3 - Q.minv[3]) * Q.uniq[3],
+                              Q.total_val);
+        // an index outside [0, total_val) (group values below min_val) would
+        // write outside the query's result; the reference does, here the pair
+        // is dropped
+        if ((unsigned)h >= (unsigned)Q.total_val) continue;
+        int val = 0;
+#pragma unroll
+        for (int e = 0; e < kBatchMaxExpr; e++)
+          if (e == Q.expr) val = ev[e][j];
+        const unsigned long long add = (unsigned long long)(long long)val;
+        int* res = reinterpret_cast<int*>(Q.res) + 6 * (size_t)h;
+        const uint32_t bit = 1u << (h & 31);
+        bool first = false;
+        if (Q.lds >= 0) {
+          atomicAdd(&lsum[Q.lds + h], add);
+          if (!(Q.flags & kQueryAggr)) first = !(atomicOr(&ltouch[Q.touch + (h >> 5)], bit) & bit);
+        } else {
+          atomicAdd(reinterpret_cast<unsigned long long*>(res + 4), add);
+          if (!(Q.flags & kQueryAggr)) {
+            unsigned* tw = D->touch_arena + Q.touch + (h >> 5);
+            first = !(*tw & bit) && !(atomicOr(tw, bit) & bit);
+          }
+        }
+        if (first) {
+          // the reference stores the keys for every row of the group
+          // (GPUProcessing.h:144-147); they are equal, so one store suffices
+          int2* key = reinterpret_cast<int2*>(res);
+          key[0] = make_int2(gv0, gv1);
+          key[1] = make_int2(gv2, gv3);
+        }
+      }
+    }
+  }
+
+  // flush the LDS group arrays (adding a zero sum changes nothing)
+  __syncthreads();
+  for (int q = 0; q < nq; q++) {
+    const int base = lq[q].lds;
+    if (base < 0) continue;
+    const int G = lq[q].total_val;
+    int* res = reinterpret_cast<int*>(lq[q].res);
+    for (int h = threadIdx.x; h < G; h += BLOCK) {
+      const unsigned long long s = lsum[base + h];
+      if (s) atomicAdd(reinterpret_cast<unsigned long long*>(res + 6 * (size_t)h + 4), s);
+    }
+  }
+}
+
+namespace {
+
+template <int BLOCK, int VPT>
+const void* scan_fn() {
+  return reinterpret_cast<const void*>(&batch_kernel<BLOCK, VPT>);
+}
+
+const void* pick(const BatchConfig& c) {
+  if (c.vpt == 1) {
+    if (c.block == 256) return scan_fn<256, 1>();
+    if (c.block == 512) return scan_fn<512, 1>();
+    if (c.block == 1024) return scan_fn<1024, 1>();
+  } else if (c.vpt == 2) {
+    if (c.block == 256) return scan_fn<256, 2>();
+    if (c.block == 512) return scan_fn<512, 2>();
+  }
+  return nullptr;
+}
+
+template <int BLOCK, int VPT>
+hipError_t launch(const BatchLaunch& L, int lds, hipStream_t s) {
+  constexpr int TILE = BLOCK * 4 * VPT;
+  const auto f = &batch_kernel<BLOCK, VPT>;
+  const int grid = balanced_grid((L.num_tuples + TILE - 1) / TILE,
+                                 resident_groups(reinterpret_cast<const void*>(f), BLOCK, lds));
+  f<<<grid, BLOCK, lds, s>>>(L);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+bool batch_config_ok(const BatchConfig& c) {
+  return pick(c) != nullptr && c.lds_bytes >= 0 && c.lds_bytes <= 160 * 1024;
+}
+
+hipError_t batch_prepare(const BatchConfig& c) {
+  const void* f = pick(c);
+  if (!f) return hipErrorInvalidValue;
+  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, std::max(c.lds_bytes, 1));
+}
+
+int batch_resident(const BatchConfig& c) {
+  const void* f = pick(c);
+  return f ? resident_groups(f, c.block, c.lds_bytes) : 0;
+}
+
+hipError_t batch_fill(const BatchLaunch& L, const uint4* const* rec, const int* rec_len, hipStream_t s) {
+  batch_fill_kernel<<<2048, 256, 0, s>>>(L.desc, L.status, const_cast<uint4*>(rec[0]), const_cast<uint4*>(rec[1]),
+                                         const_cast<uint4*>(rec[2]), const_cast<uint4*>(rec[3]), rec_len[0],
+                                         rec_len[1], rec_len[2], rec_len[3]);
+  return hipGetLastError();
+}
+
+hipError_t batch_records(const BatchLaunch& L, int d, int nrows, hipStream_t s) {
+  const int grid = std::max(1, std::min(4096, (nrows + 255) / 256));
+  batch_records_kernel<<<grid, 256, 0, s>>>(L, d);
+  return hipGetLastError();
+}
+
+hipError_t batch_scan(const BatchLaunch& L, const BatchConfig& c, hipStream_t s) {
+  if (!batch_config_ok(c)) return hipErrorInvalidValue;
+  if (c.vpt == 1) {
+    if (c.block == 256) return launch<256, 1>(L, c.lds_bytes, s);
+    if (c.block == 512) return launch<512, 1>(L, c.lds_bytes, s);
+    return launch<1024, 1>(L, c.lds_bytes, s);
+  }
+  if (c.block == 256) return launch<256, 2>(L, c.lds_bytes, s);
+  return launch<512, 2>(L, c.lds_bytes, s);
+}
+
+}  // namespace mordred

@@ -26,6 +26,7 @@
 
 #include "../../include/mordred_engine.h"
 #include "../../include/mordred_hip.h"
+#include "batch.hpp"
 
 namespace mordred_host {
 
@@ -106,6 +107,8 @@ struct Column {
   std::vector<int> slot;        // cache slot per segment, -1 = not cached
   int* d_idx = nullptr;         // device segment table (the reference col_idx)
   bool pinned_gpu = false;
+  bool ranged = false;          // vmin / vmax computed
+  int vmin = 0, vmax = 0;
 };
 
 // Device buffer that only grows.
@@ -161,6 +164,15 @@ struct mordred_engine {
   std::unordered_map<long long, double> seg_weight;  // (column id << 32 | segment) -> weight
   DevBuf d_res, d_sg, d_ht[4];
   HostBuf h_stage;  // pinned scratch for segment groups / index updates
+  // shared-scan batches: record tables, descriptor, result arena, the graph
+  DevBuf d_rec[4], d_desc, d_arena, d_touch, d_status;
+  HostBuf h_desc, h_arena;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<char> gkey;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  mordred::BatchConfig batch_cfg{512, 1, 64 * 1024};
+  bool batch_tuned = false;
   std::string err;
 };
 
@@ -715,6 +727,307 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
   return 0;
 }
 
+// ------------------------------------------------------------ shared-scan batches
+// Host side of batch.hpp: resolve the queries, lay out the descriptor, the
+// LDS and the result arena, capture / replay the graph.
+struct BatchPlan {
+  mordred::BatchDesc d;
+  int rec_len[4] = {0, 0, 0, 0};
+  long long n = 0;
+  std::vector<long long> res_off;  // ints
+  std::vector<int> G;
+  int lds_queries = 0, global_queries = 0, dead = 0;
+};
+
+void column_range(Column* c) {
+  if (c->ranged) return;
+  int mn = 0, mx = 0;
+  if (c->n > 0) {
+    mn = mx = c->host[0];
+    for (long long i = 1; i < c->n; i++) {
+      mn = std::min(mn, c->host[i]);
+      mx = std::max(mx, c->host[i]);
+    }
+  }
+  c->vmin = mn;
+  c->vmax = mx;
+  c->ranged = true;
+}
+
+// Whether query i probes table k: every join of a group-by query; only the
+// date table (slot 4) for an aggregate query, like filter_probe_aggr_GPU2
+// (GPUProcessing.h:1675-1776), which probes ht4 alone.
+bool batch_probes(const mordred_star_query& q, const Resolved& r, int k) {
+  return r.join[k] && (q.kind == MORDRED_KIND_GROUPBY || k == 3);
+}
+
+int plan_batch(mordred_engine* e, const mordred_star_query* qs, int nq, const mordred::BatchConfig& cfg,
+               BatchPlan& b) {
+  using namespace mordred;
+  if (nq < 1 || nq > kBatchMaxQ) return fail(e, "a batch holds 1..64 queries");
+  if (!batch_config_ok(cfg)) return fail(e, "bad batch configuration");
+  if ((size_t)cfg.lds_bytes < nq * sizeof(BatchQuery)) return fail(e, "lds_bytes below 64 bytes per query");
+  std::vector<Resolved> rs(nq);
+  for (int i = 0; i < nq; i++) {
+    const int rc = resolve(e, &qs[i], rs[i]);
+    if (rc) return rc;
+  }
+  BatchDesc& D = b.d;
+  std::memset(&D, 0, sizeof D);
+  D.nq = nq;
+  b.n = rs[0].n;
+  b.res_off.assign(nq, 0);
+  b.G.assign(nq, 1);
+  Column* cols[kBatchMaxCols] = {};
+  int ncols = 4;
+  auto slot_of = [&](Column* c) -> int {
+    for (int i = 0; i < ncols; i++)
+      if (cols[i] == c) return i;
+    if (ncols == kBatchMaxCols) return -1;
+    cols[ncols] = c;
+    return ncols++;
+  };
+  for (int i = 0; i < nq; i++) {
+    const Resolved& r = rs[i];
+    if (r.n != b.n) return fail(e, "batch queries read fact tables of different lengths");
+    for (Column* c : r.fact_cols)
+      for (int s = 0; s < c->nseg; s++)
+        if (c->slot[s] < 0) return fail(e, "batch fact columns must be cached in HBM: " + c->name);
+    for (int k = 0; k < 4; k++) {
+      if (!batch_probes(qs[i], r, k)) continue;
+      if (cols[k] && cols[k] != r.key[k]) return fail(e, "batch queries join one table through different fact keys");
+      cols[k] = r.key[k];
+    }
+  }
+  // dimensions: one record table each
+  Column* attrs[4][2] = {};
+  for (int k = 0; k < 4; k++) {
+    BatchDim& B = D.dim[k];
+    Column* key = nullptr;
+    for (int i = 0; i < nq; i++) {
+      const Resolved& r = rs[i];
+      if (!batch_probes(qs[i], r, k)) continue;
+      const mordred_join& j = *r.join[k];
+      for (Column* c : {r.dkey[k], r.dfilt[k], r.dval[k]}) {
+        if (c && !c->pinned_gpu) return fail(e, "dimension columns must be GPU-resident");
+        if (c && c->n < r.dkey[k]->n) return fail(e, "dimension column shorter than its key column: " + c->name);
+      }
+      if (!key) {
+        key = r.dkey[k];
+        B.key = key->d_idx;
+        B.len = j.dim_len;
+        B.minkey = j.min_key;
+        B.nrows = (int)key->n;
+      } else if (key != r.dkey[k] || B.len != j.dim_len || B.minkey != j.min_key) {
+        return fail(e, "batch queries disagree on a dimension's key, dim_len or min_key");
+      }
+      if (r.dfilt[k] && (j.filter_mode < 0 || j.filter_mode > 2)) return fail(e, "dimension filter mode must be 0..2");
+      B.filt[B.nfilt++] = BatchFilt{r.dfilt[k] ? r.dfilt[k]->d_idx : nullptr, j.filter_mode, j.c1, j.c2, i};
+      B.uses |= 1ull << i;
+      if (qs[i].kind == MORDRED_KIND_GROUPBY && r.dval[k]) {
+        Column* v = r.dval[k];
+        int a = (attrs[k][0] == v) ? 0 : (attrs[k][1] == v) ? 1 : -1;
+        if (a < 0) {
+          a = !attrs[k][0] ? 0 : !attrs[k][1] ? 1 : -1;
+          if (a < 0) return fail(e, "more than 2 group columns on one dimension in a batch");
+          column_range(v);
+          if ((long long)v->vmax - (long long)v->vmin > 0xFFFF) return fail(e, "group column span exceeds 16 bits: " + v->name);
+          attrs[k][a] = v;
+          B.attr[a] = v->d_idx;
+          B.attr_base[a] = v->vmin;
+        }
+      }
+    }
+    if (!key) continue;
+    B.used = 1;
+    std::sort(B.filt, B.filt + B.nfilt, [](const BatchFilt& x, const BatchFilt& y) {
+      return x.col == y.col ? x.q < y.q : (uintptr_t)x.col < (uintptr_t)y.col;
+    });
+    HIP_OK(e->d_rec[k].reserve((size_t)B.len * sizeof(uint4)));
+    B.rec = static_cast<uint4*>(e->d_rec[k].p);
+    b.rec_len[k] = B.len;
+  }
+  // fact predicates, aggregate expressions, query descriptors
+  std::vector<BatchPred> preds;
+  for (int i = 0; i < nq; i++) {
+    const mordred_star_query& q = qs[i];
+    const Resolved& r = rs[i];
+    bool dead = false;
+    for (int f = 0; f < q.nfilters; f++) {
+      const int sl = slot_of(r.filt[f]);
+      if (sl < 0) return fail(e, "a batch reads more than 12 fact columns");
+      if (q.filters[f].lo > q.filters[f].hi) dead = true;
+      preds.push_back(BatchPred{sl, q.filters[f].lo, q.filters[f].hi, i});
+    }
+    const int kind = q.kind == MORDRED_KIND_GROUPBY ? kExprDiff : kExprProduct;
+    const int a1 = slot_of(r.a1), a2 = r.a2 ? slot_of(r.a2) : -1;
+    if (a1 < 0 || (r.a2 && a2 < 0)) return fail(e, "a batch reads more than 12 fact columns");
+    int ex = -1;
+    for (int x = 0; x < D.nexpr; x++)
+      if (D.expr[x].kind == kind && D.expr[x].a1 == a1 && D.expr[x].a2 == a2) ex = x;
+    if (ex < 0) {
+      if (D.nexpr == kBatchMaxExpr) return fail(e, "more than 4 distinct aggregate expressions in a batch");
+      ex = D.nexpr++;
+      D.expr[ex] = BatchExpr{kind, a1, a2, 0};
+    }
+    BatchQuery& Q = D.q[i];
+    Q.total_val = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
+    Q.expr = ex;
+    Q.flags = q.kind == MORDRED_KIND_GROUPBY ? 0 : kQueryAggr;
+    for (int k = 0; k < 4; k++) {
+      Q.attr[k] = -1;
+      if (q.kind != MORDRED_KIND_GROUPBY || !r.join[k] || !r.dval[k]) continue;
+      Q.attr[k] = (signed char)(attrs[k][0] == r.dval[k] ? 0 : 1);
+      Q.minv[k] = r.join[k]->min_val;
+      Q.uniq[k] = r.join[k]->unique_val;
+    }
+    b.G[i] = Q.total_val;
+    if (dead) b.dead++;
+    else D.all |= 1ull << i;
+  }
+  std::stable_sort(preds.begin(), preds.end(), [](const BatchPred& x, const BatchPred& y) { return x.col < y.col; });
+  D.npred = (int)preds.size();
+  for (int p = 0; p < D.npred; p++) D.pred[p] = preds[p];
+  for (int c = 0, p = 0; c <= kBatchMaxCols; c++) {
+    while (p < D.npred && D.pred[p].col < c) p++;
+    D.pred_begin[c] = p;
+  }
+  D.ncols = ncols;
+  for (int c = 0; c < ncols; c++) D.col[c] = cols[c] ? cols[c]->d_idx : nullptr;
+  // group arrays: smallest first into LDS while the budget lasts
+  std::vector<int> order(nq);
+  for (int i = 0; i < nq; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b.G[x] < b.G[y]; });
+  size_t lds_used = nq * sizeof(BatchQuery);
+  std::vector<char> in_lds(nq, 0);
+  for (int i : order) {
+    const size_t need = batch_group_lds(b.G[i]);
+    if (lds_used + need <= (size_t)cfg.lds_bytes) {
+      in_lds[i] = 1;
+      lds_used += need;
+    }
+  }
+  long long sum_words = 0, lds_touch = 0, g_touch = 0, res_ints = 0;
+  for (int i = 0; i < nq; i++) {
+    BatchQuery& Q = D.q[i];
+    const int tw = (b.G[i] + 31) / 32;
+    if (in_lds[i]) {
+      Q.lds = (int)sum_words;
+      Q.touch = (int)lds_touch;
+      sum_words += b.G[i];
+      lds_touch += tw;
+      b.lds_queries++;
+    } else {
+      Q.lds = -1;
+      Q.touch = (int)g_touch;
+      g_touch += tw;
+      b.global_queries++;
+    }
+    b.res_off[i] = res_ints;
+    res_ints += 6 * (long long)b.G[i];
+  }
+  D.lds_sum_words = (int)sum_words;
+  D.lds_touch_words = (int)lds_touch;
+  D.res_ints = (res_ints + 3) & ~3ll;
+  D.touch_words = (g_touch + 3) & ~3ll;
+  HIP_OK(e->d_arena.reserve(std::max<long long>(16, D.res_ints * 4)));
+  HIP_OK(e->d_touch.reserve(std::max<long long>(16, D.touch_words * 4)));
+  D.res_arena = static_cast<int*>(e->d_arena.p);
+  D.touch_arena = static_cast<unsigned*>(e->d_touch.p);
+  for (int i = 0; i < nq; i++) D.q[i].res = (long long)(uintptr_t)(D.res_arena + b.res_off[i]);
+  return 0;
+}
+
+// Launch arguments and record tables a graph was captured with.
+struct GraphKey {
+  mordred::BatchLaunch L;
+  mordred::BatchConfig c;
+  const uint4* rec[4];
+  int rec_len[4];   // record slots cleared by the fill
+  int nrows[4];     // dimension rows (record build grids)
+  int used[4];
+};
+
+int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig& cfg, int* const* res,
+              mordred_batch_stats* st) {
+  using namespace mordred;
+  const auto t0 = Clock::now();
+  HIP_OK(hipSetDevice(e->device));
+  if (!e->ev0) HIP_OK(hipEventCreate(&e->ev0));
+  if (!e->ev1) HIP_OK(hipEventCreate(&e->ev1));
+  HIP_OK(e->d_desc.reserve(sizeof(BatchDesc)));
+  HIP_OK(e->h_desc.reserve(sizeof(BatchDesc)));
+  HIP_OK(e->d_status.reserve(64));
+  GraphKey k;
+  std::memset(&k, 0, sizeof k);
+  k.L = BatchLaunch{static_cast<const BatchDesc*>(e->d_desc.p), e->cache, e->d_identity, (int)b.n,
+                    static_cast<unsigned*>(e->d_status.p)};
+  k.c = cfg;
+  for (int d = 0; d < 4; d++) {
+    k.used[d] = b.d.dim[d].used;
+    k.rec[d] = k.used[d] ? b.d.dim[d].rec : nullptr;
+    k.rec_len[d] = k.used[d] ? b.rec_len[d] : 0;
+    k.nrows[d] = k.used[d] ? b.d.dim[d].nrows : 0;
+  }
+  int captured = 0;
+  const char* kp = reinterpret_cast<const char*>(&k);
+  if (!e->gexec || e->gkey.size() != sizeof k || std::memcmp(e->gkey.data(), kp, sizeof k) != 0) {
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->gexec = nullptr;
+    e->graph = nullptr;
+    e->gkey.clear();
+    HIP_OK(batch_prepare(cfg));
+    (void)batch_resident(cfg);  // occupancy query outside the capture
+    HIP_OK(hipStreamBeginCapture(e->s_main, hipStreamCaptureModeRelaxed));
+    hipError_t r1 = batch_fill(k.L, k.rec, k.rec_len, e->s_main);
+    for (int d = 0; d < 4 && r1 == hipSuccess; d++)
+      if (k.used[d]) r1 = batch_records(k.L, d, k.nrows[d], e->s_main);
+    if (r1 == hipSuccess) r1 = batch_scan(k.L, cfg, e->s_main);
+    hipGraph_t g = nullptr;
+    const hipError_t r2 = hipStreamEndCapture(e->s_main, &g);
+    if (r1 != hipSuccess || r2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return hip_fail(e, r1 != hipSuccess ? r1 : r2, "batch graph capture");
+    }
+    e->graph = g;
+    HIP_OK(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
+    e->gkey.assign(kp, kp + sizeof k);
+    captured = 1;
+  }
+  std::memcpy(e->h_desc.p, &b.d, sizeof(BatchDesc));
+  HIP_OK(hipMemcpyAsync(e->d_desc.p, e->h_desc.p, sizeof(BatchDesc), hipMemcpyHostToDevice, e->s_main));
+  HIP_OK(hipEventRecord(e->ev0, e->s_main));
+  HIP_OK(hipGraphLaunch(e->gexec, e->s_main));
+  HIP_OK(hipEventRecord(e->ev1, e->s_main));
+  const size_t rbytes = (size_t)b.d.res_ints * sizeof(int);
+  HIP_OK(e->h_arena.reserve(rbytes + 64));
+  char* ha = static_cast<char*>(e->h_arena.p);
+  if (rbytes) HIP_OK(hipMemcpyAsync(ha, b.d.res_arena, rbytes, hipMemcpyDeviceToHost, e->s_main));
+  HIP_OK(hipMemcpyAsync(ha + rbytes, e->d_status.p, sizeof(unsigned), hipMemcpyDeviceToHost, e->s_main));
+  HIP_OK(hipStreamSynchronize(e->s_main));
+  unsigned collisions = 0;
+  std::memcpy(&collisions, ha + rbytes, sizeof collisions);
+  if (collisions)
+    return fail(e, "dimension hash collision in a batch record table (" + std::to_string(collisions) +
+                       " slots): batches need collision-free perfect hashing");
+  const int* arena = reinterpret_cast<const int*>(ha);
+  for (int i = 0; i < b.d.nq; i++)
+    std::memcpy(res[i], arena + b.res_off[i], 6 * (size_t)b.G[i] * sizeof(int));
+  if (st) {
+    float gms = 0;
+    HIP_OK(hipEventElapsedTime(&gms, e->ev0, e->ev1));
+    st->gpu_ms = gms;
+    st->total_ms = ms_since(t0);
+    st->lds_queries = b.lds_queries;
+    st->global_queries = b.global_queries;
+    st->dead_queries = b.dead;
+    st->graph_captured = captured;
+    st->config = mordred_batch_config{cfg.block, cfg.vpt, cfg.lds_bytes};
+  }
+  return 0;
+}
+
 }  // namespace
 
 // ================================================================= C ABI
@@ -763,6 +1076,10 @@ void mordred_engine_destroy(mordred_engine* e) {
     if (kv.second->d_idx) (void)hipFree(kv.second->d_idx);
   }
   e->cols.clear();
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  if (e->ev0) (void)hipEventDestroy(e->ev0);
+  if (e->ev1) (void)hipEventDestroy(e->ev1);
   for (hipStream_t s : {e->s_main, e->s_copy[0], e->s_copy[1]})
     if (s) (void)hipStreamDestroy(s);
   if (e->cache) (void)hipFree(e->cache);
@@ -992,6 +1309,85 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   st.merge_ms = ms_since(tm0);
   st.total_ms = ms_since(t0);
   if (stats) *stats = st;
+  return 0;
+}
+
+int mordred_engine_batch(mordred_engine* e, const mordred_star_query* qs, int nq, int* const* res,
+                         const mordred_batch_config* cfg, mordred_batch_stats* stats) {
+  if (!e) return -1;
+  if (e->device < 0) return fail(e, "host-only engine has no batch path");
+  if (!qs || !res) return fail(e, "queries / results are NULL");
+  for (int i = 0; i < nq; i++)
+    if (!res[i]) return fail(e, "a result pointer is NULL");
+  const mordred::BatchConfig c = cfg ? mordred::BatchConfig{cfg->block, cfg->vpt, cfg->lds_bytes} : e->batch_cfg;
+  HIP_OK(hipSetDevice(e->device));
+  BatchPlan b;
+  const int rc = plan_batch(e, qs, nq, c, b);
+  if (rc) return rc;
+  return run_batch(e, b, c, res, stats);
+}
+
+int mordred_engine_batch_tune(mordred_engine* e, const mordred_star_query* qs, int nq, int reps,
+                              mordred_batch_config* best, mordred_batch_tune_entry* table, int max_entries,
+                              int* nentries) {
+  if (!e) return -1;
+  if (e->device < 0) return fail(e, "host-only engine has no batch path");
+  if (!qs || nq < 1 || nq > mordred::kBatchMaxQ) return fail(e, "a batch holds 1..64 queries");
+  HIP_OK(hipSetDevice(e->device));
+  reps = std::max(1, reps);
+  // candidates: threads per workgroup x items per thread x LDS budget (which
+  // decides how many group arrays live in LDS rather than HBM)
+  std::vector<mordred::BatchConfig> cands;
+  for (int block : {256, 512, 1024})
+    for (int vpt : {1, 2})
+      for (int lds : {16 << 10, 48 << 10, 96 << 10, 160 << 10}) {
+        const mordred::BatchConfig c{block, vpt, lds};
+        if (mordred::batch_config_ok(c)) cands.push_back(c);
+      }
+  std::vector<std::vector<int>> ref, cur;
+  std::vector<int*> ptrs(nq);
+  double best_ms = -1;
+  mordred::BatchConfig best_c = e->batch_cfg;
+  int n_out = 0;
+  std::string first_err;
+  for (const auto& c : cands) {
+    BatchPlan b;
+    double ms = -1;
+    int rc = plan_batch(e, qs, nq, c, b);
+    if (!rc) {
+      cur.assign(nq, {});
+      for (int i = 0; i < nq; i++) {
+        cur[i].assign(6 * (size_t)b.G[i], 0);
+        ptrs[i] = cur[i].data();
+      }
+      mordred_batch_stats st{};
+      rc = run_batch(e, b, c, ptrs.data(), &st);  // warm-up (and capture)
+      for (int r = 0; r < reps && !rc; r++) {
+        rc = run_batch(e, b, c, ptrs.data(), &st);
+        if (!rc && (ms < 0 || st.gpu_ms < ms)) ms = st.gpu_ms;
+      }
+    }
+    if (rc) {
+      if (first_err.empty()) first_err = e->err;
+      ms = -1;
+      (void)hipGetLastError();
+    } else if (ref.empty()) {
+      ref = cur;
+    } else if (ref != cur) {
+      return fail(e, "batch autotuner: launch configurations disagree on a result");
+    }
+    if (table && n_out < max_entries)
+      table[n_out++] = mordred_batch_tune_entry{{c.block, c.vpt, c.lds_bytes}, ms};
+    if (ms >= 0 && (best_ms < 0 || ms < best_ms)) {
+      best_ms = ms;
+      best_c = c;
+    }
+  }
+  if (nentries) *nentries = n_out;
+  if (best_ms < 0) return fail(e, "batch autotuner: no configuration ran (" + first_err + ")");
+  e->batch_cfg = best_c;
+  e->batch_tuned = true;
+  if (best) *best = mordred_batch_config{best_c.block, best_c.vpt, best_c.lds_bytes};
   return 0;
 }
 

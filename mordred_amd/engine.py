@@ -42,6 +42,28 @@ class RunStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("block", C.c_int), ("vpt", C.c_int), ("lds_bytes", C.c_int)]
+
+    def as_tuple(self):
+        return (self.block, self.vpt, self.lds_bytes)
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("gpu_ms", C.c_double), ("lds_queries", C.c_int),
+                ("global_queries", C.c_int), ("dead_queries", C.c_int), ("graph_captured", C.c_int),
+                ("config", BatchConfig)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "config"}
+        d["config"] = self.config.as_tuple()
+        return d
+
+
+class BatchTuneEntry(C.Structure):
+    _fields_ = [("config", BatchConfig), ("gpu_ms", C.c_double)]
+
+
 def _b(s):
     return None if s is None else s.encode()
 
@@ -74,6 +96,10 @@ _SIGS = {
     "mordred_engine_set_bandwidth": ([C.c_void_p, C.c_double, C.c_double], C.c_int),
     "mordred_engine_place": ([C.c_void_p, C.POINTER(StarQueryC), C.c_double], C.c_int),
     "mordred_engine_run": ([C.c_void_p, C.POINTER(StarQueryC), C.c_int, C.c_void_p, C.POINTER(RunStats)], C.c_int),
+    "mordred_engine_batch": ([C.c_void_p, C.POINTER(StarQueryC), C.c_int, C.POINTER(C.c_void_p),
+                              C.POINTER(BatchConfig), C.POINTER(BatchStats)], C.c_int),
+    "mordred_engine_batch_tune": ([C.c_void_p, C.POINTER(StarQueryC), C.c_int, C.c_int, C.POINTER(BatchConfig),
+                                   C.POINTER(BatchTuneEntry), C.c_int, C.POINTER(C.c_int)], C.c_int),
 }
 
 
@@ -132,6 +158,22 @@ class Engine:
                     seen.add(c)
                     self.add_column(c, data.tables[j.table][c], gpu_resident=self.device >= 0)
 
+    def add_mix_data(self, data, queries, cache_fact: bool = True):
+        """Every column a list of StarQuery reads: fact columns (all segments
+        cached in HBM with cache_fact) and GPU-resident dimension columns."""
+        seen = set()
+        for q in queries:
+            for c in q.fact_columns():
+                if c not in seen:
+                    seen.add(c)
+                    self.add_column(c, data.fact[c], gpu_resident=cache_fact)
+        for q in queries:
+            for j in q.joins:
+                for c in (j.dim_key, j.filter_col, j.group_col):
+                    if c and c not in seen:
+                        seen.add(c)
+                        self.add_column(c, data.tables[j.table][c], gpu_resident=self.device >= 0)
+
     def segments(self, name: str) -> int:
         return self.lib.mordred_engine_column_segments(self.h, name.encode())
 
@@ -157,3 +199,30 @@ class Engine:
         st = RunStats()
         self._check(self.lib.mordred_engine_run(self.h, C.byref(qc), placement, res.ctypes.data, C.byref(st)), "run")
         return res, st.as_dict()
+
+    # ---- shared-scan batches
+    def _queries(self, queries):
+        arr = (StarQueryC * len(queries))()
+        for i, q in enumerate(queries):
+            arr[i] = star_query(q)
+        return arr
+
+    def batch(self, queries, config=None):
+        """Results (one int32 array per query) and stats of one batch."""
+        arr = self._queries(queries)
+        outs = [np.zeros(max(1, q.total_val) * 6, dtype=np.int32) for q in queries]
+        ptrs = (C.c_void_p * len(queries))(*[o.ctypes.data for o in outs])
+        st = BatchStats()
+        cfg = None if config is None else C.byref(BatchConfig(*config))
+        self._check(self.lib.mordred_engine_batch(self.h, arr, len(queries), ptrs, cfg, C.byref(st)), "batch")
+        return outs, st.as_dict()
+
+    def batch_tune(self, queries, reps: int = 2):
+        """(best config tuple, [(config tuple, gpu_ms)]) of the batch autotuner."""
+        arr = self._queries(queries)
+        best = BatchConfig()
+        table = (BatchTuneEntry * 64)()
+        n = C.c_int(0)
+        self._check(self.lib.mordred_engine_batch_tune(self.h, arr, len(queries), reps, C.byref(best), table, 64,
+                                                       C.byref(n)), "batch_tune")
+        return best.as_tuple(), [(table[i].config.as_tuple(), table[i].gpu_ms) for i in range(n.value)]

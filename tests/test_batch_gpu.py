@@ -1,0 +1,122 @@
+"""Shared-scan batches on the GPU: 64 mix queries in one persistent kernel.
+
+Every query's result must equal the oracle (GPU convention) and its own plain
+engine run bit for bit, alone or in the mix, under every launch configuration
+the autotuner tries, and across graph replays with different batches.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import queries as OQ
+from mordred_amd import engine as E
+from mordred_amd import mix
+
+pytestmark = pytest.mark.gpu
+
+SEG = oracle.SEGMENT_SIZE
+SMALL = {"S": 2000, "P": 30000, "C": 40000}
+
+
+@pytest.fixture(scope="module")
+def bengine(cuda):
+    qs = [m.query for m in mix.make_mix(64, 11, SMALL)]
+    other = [m.query for m in mix.make_mix(64, 12, SMALL)]
+    n = 2 * SEG + 77_777
+    data = mix.make_mix_data(qs + other, n, seed=11, S=SMALL["S"], P=SMALL["P"], Cn=SMALL["C"])
+    eng = E.Engine(device=0, cache_segments=96, threads=8)
+    eng.add_mix_data(data, qs + other)
+    refs = {q.name: OQ.run_query(q, data, oracle.CONV_GPU) for q in qs + other}
+    yield qs, other, data, eng, refs
+    eng.close()
+
+
+def check(qs, outs, refs):
+    for q, r in zip(qs, outs):
+        np.testing.assert_array_equal(r, refs[q.name], err_msg=q.name)
+
+
+def test_batch_of_64_matches_oracle(bengine):
+    qs, _, _, eng, refs = bengine
+    print("[gpu] batch of 64, default config", flush=True)
+    outs, st = eng.batch(qs)
+    check(qs, outs, refs)
+    assert st["lds_queries"] + st["global_queries"] == 64
+
+
+@pytest.mark.parametrize("config", [(256, 1, 16 << 10), (512, 2, 96 << 10), (1024, 1, 160 << 10),
+                                    (256, 2, 4096)])
+def test_batch_configs_match_oracle(bengine, config):
+    """LDS budgets from descriptors-only (every group array in HBM) to 160 KiB."""
+    qs, _, _, eng, refs = bengine
+    print(f"[gpu] batch config {config}", flush=True)
+    outs, st = eng.batch(qs, config)
+    check(qs, outs, refs)
+    if config[2] == 4096:
+        assert st["lds_queries"] == 0
+
+
+def test_each_query_alone_equals_mix(bengine):
+    qs, _, _, eng, refs = bengine
+    mixed, _ = eng.batch(qs)
+    for i, q in enumerate(qs):
+        alone, _ = eng.batch([q])
+        np.testing.assert_array_equal(alone[0], mixed[i], err_msg=q.name)
+
+
+def test_batch_equals_plain_engine_runs(bengine):
+    qs, _, _, eng, refs = bengine
+    outs, _ = eng.batch(qs[:16])
+    for q, r in zip(qs[:16], outs):
+        plain, _ = eng.run(q, E.PLACE_RESIDENCY)
+        np.testing.assert_array_equal(r, plain, err_msg=q.name)
+
+
+def test_graph_replays_across_batches(bengine):
+    """One captured graph serves batches with different queries: only the
+    device descriptor changes between them."""
+    qs, other, _, eng, refs = bengine
+    cfg = (512, 1, 64 << 10)
+    a1, s1 = eng.batch(qs, cfg)
+    b1, s2 = eng.batch(other, cfg)
+    a2, s3 = eng.batch(qs, cfg)
+    assert s2["graph_captured"] == 0 and s3["graph_captured"] == 0
+    check(qs, a1, refs)
+    check(other, b1, refs)
+    check(qs, a2, refs)
+
+
+def test_autotuner_configs_agree(bengine):
+    qs, _, _, eng, refs = bengine
+    print("[gpu] batch autotune", flush=True)
+    best, table = eng.batch_tune(qs, reps=1)
+    assert best in [c for c, ms in table if ms >= 0]
+    assert sum(ms >= 0 for _, ms in table) >= 8
+    outs, st = eng.batch(qs)
+    assert st["config"] == best
+    check(qs, outs, refs)
+
+
+def test_dead_query_and_rejections(bengine):
+    qs, _, _, eng, refs = bengine
+    q = dataclasses.replace(qs[0], fact_filters=[("lo_revenue", 900_000, 100_000)])
+    outs, st = eng.batch([q, qs[1]])
+    assert st["dead_queries"] == 1 and not outs[0].any()
+    np.testing.assert_array_equal(outs[1], refs[qs[1].name])
+    with pytest.raises(E.EngineError, match="1..64"):
+        eng.batch(qs + qs[:1])
+
+
+def test_hash_collision_is_reported(bengine):
+    """A dimension table shorter than its key range maps two rows to one slot:
+    the record tables cannot stand for it, and the batch says so."""
+    qs, _, _, eng, _ = bengine
+    q = next(q for q in qs if any(j.slot == 1 for j in q.joins))
+    joins = [dataclasses.replace(j, dim_len=j.dim_len // 2) if j.slot == 1 else j for j in q.joins]
+    bad = dataclasses.replace(q, joins=joins)
+    with pytest.raises(E.EngineError, match="collision"):
+        eng.batch([bad])
