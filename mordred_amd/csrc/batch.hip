@@ -88,12 +88,17 @@ __global__ __launch_bounds__(256) void batch_records_kernel(BatchLaunch L, int d
   }
 }
 
-template <int BLOCK, int VPT>
+// The shared scan.  A workgroup walks one contiguous range of tiles; the
+// columns of the next tile are loaded while the current one is processed.
+// A wave keeps the cache slots of the current segment's columns in one VGPR
+// (lane c = column c) and reloads them only when the segment changes.
+template <int BLOCK, int VPT, int MAXC>
 __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   constexpr int IPT = 4 * VPT;
   constexpr int TILE = BLOCK * IPT;
   const BatchDesc* __restrict__ D = L.desc;
   extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ BatchHot H;
   const int nq = D->nq;
   BatchQuery* lq = reinterpret_cast<BatchQuery*>(lds_raw);
   unsigned long long* lsum = reinterpret_cast<unsigned long long*>(lds_raw + nq * sizeof(BatchQuery));
@@ -106,67 +111,140 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     for (int i = threadIdx.x; i < nq * (int)(sizeof(BatchQuery) / 16); i += BLOCK) dst[i] = src[i];
     for (int i = threadIdx.x; i < nsum; i += BLOCK) lsum[i] = 0;
     for (int i = threadIdx.x; i < ntouch; i += BLOCK) ltouch[i] = 0;
+    const int np = D->npred;
+    for (int i = threadIdx.x; i < np; i += BLOCK) {
+      const BatchPred P = D->pred[i];
+      H.pred[i] = make_int4(P.lo, (int)((unsigned)P.hi - (unsigned)P.lo), P.q, P.col);
+    }
+    if (threadIdx.x < kBatchMaxCols) H.col[threadIdx.x] = D->col[threadIdx.x];
+    if (threadIdx.x <= kBatchMaxCols) H.pred_begin[threadIdx.x] = D->pred_begin[threadIdx.x];
+    if (threadIdx.x < kBatchMaxExpr) {
+      const BatchExpr E = D->expr[threadIdx.x];
+      H.expr[threadIdx.x] = make_int4(E.kind, E.a1, E.a2, 0);
+    }
+    if (threadIdx.x < 4) {
+      const int d = threadIdx.x;
+      H.rec[d] = D->dim[d].rec;
+      H.uses[d] = D->dim[d].uses;
+      H.len[d] = D->dim[d].len;
+      H.minkey[d] = D->dim[d].minkey;
+    }
+    if (threadIdx.x == 0) {
+      int cm = 0, um = 0;
+      for (int c = 0; c < D->ncols; c++)
+        if (D->col[c]) cm |= 1 << c;
+      for (int d = 0; d < 4; d++)
+        if (D->dim[d].used) um |= 1 << d;
+      H.ncols = D->ncols;
+      H.colmask = cm;
+      H.usedmask = um;
+      H.nexpr = D->nexpr;
+      H.npred = np;
+      H.nq = nq;
+    }
   }
   __syncthreads();
 
-  const int ncols = D->ncols, nexpr = D->nexpr;
+  const int colmask = __builtin_amdgcn_readfirstlane(H.colmask);
+  const int usedmask = __builtin_amdgcn_readfirstlane(H.usedmask);
+  const int nexpr = __builtin_amdgcn_readfirstlane(H.nexpr);
   const unsigned long long all = D->all;
+  unsigned* const touch_arena = D->touch_arena;
   int ab[4][2];
 #pragma unroll
   for (int d = 0; d < 4; d++) {
-    ab[d][0] = D->dim[d].attr_base[0];
-    ab[d][1] = D->dim[d].attr_base[1];
+    ab[d][0] = __builtin_amdgcn_readfirstlane(D->dim[d].attr_base[0]);
+    ab[d][1] = __builtin_amdgcn_readfirstlane(D->dim[d].attr_base[1]);
   }
-  const int ntiles = (L.num_tuples + TILE - 1) / TILE;
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int pos = tile * TILE;
-    const int n = min(TILE, L.num_tuples - pos);
-    const int seg = L.seg_group[pos >> kSegLog];
-    const int off = pos & kSegMask;
-    const bool vec = n == TILE;
+  const int lane = threadIdx.x & (kWave - 1);
 
-    // every column of the tile, issued before any use
-    int x[kBatchMaxCols][IPT];
+  const int ntiles = (L.num_tuples + TILE - 1) / TILE;
+  const int per = (ntiles + gridDim.x - 1) / gridDim.x;
+  const int t_begin = blockIdx.x * per;
+  const int t_end = min(ntiles, t_begin + per);
+
+  int cur_seg = -1;
+  int slotv = 0;  // lane c: cache slot of column c in segment cur_seg
+  auto load_cols = [&](int t, int (&x)[MAXC][IPT]) {
+    const int pos = t * TILE;
+    const int sidx = pos >> kSegLog;
+    if (sidx != cur_seg) {
+      const int seg = __builtin_amdgcn_readfirstlane((int)L.seg_group[sidx]);
+      const int* tab = lane < MAXC && (colmask >> lane & 1) ? H.col[lane] : nullptr;
+      slotv = tab ? tab[seg] : 0;
+      cur_seg = sidx;
+    }
+    const int off = pos & kSegMask;
+    const int n = min(TILE, L.num_tuples - pos);
+    // every slot into a scalar first, then the loads back to back
+    const int* base[MAXC];
 #pragma unroll
-    for (int c = 0; c < kBatchMaxCols; c++) {
-      if (c < ncols && D->col[c]) {
-        const int* p = L.cache + ((size_t)D->col[c][seg] << kSegLog) + off;
-        load_tile<BLOCK, VPT>(p, vec, n, x[c]);
-      } else {
+    for (int c = 0; c < MAXC; c++)
+      base[c] = L.cache + ((size_t)(unsigned)__builtin_amdgcn_readlane(slotv, c) << kSegLog) + off;
+    if (n == TILE) {
 #pragma unroll
-        for (int j = 0; j < IPT; j++) x[c][j] = 0;
+      for (int c = 0; c < MAXC; c++) {
+        if (colmask >> c & 1) {
+#pragma unroll
+          for (int v = 0; v < VPT; v++) {
+            const v4i t = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(base[c]) + v * BLOCK + threadIdx.x);
+            x[c][4 * v + 0] = t.x; x[c][4 * v + 1] = t.y; x[c][4 * v + 2] = t.z; x[c][4 * v + 3] = t.w;
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < IPT; j++) x[c][j] = 0;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < MAXC; c++) {
+        if (colmask >> c & 1) {
+          load_tile<BLOCK, VPT>(base[c], false, n, x[c]);
+        } else {
+#pragma unroll
+          for (int j = 0; j < IPT; j++) x[c][j] = 0;
+        }
       }
     }
-    const uint32_t inr = vec ? (IPT == 32 ? 0xFFFFFFFFu : (1u << IPT) - 1u) : range_mask<BLOCK, VPT>(n);
+  };
+
+  auto process = [&](int t, const int (&x)[MAXC][IPT]) {
+    const int pos = t * TILE;
+    const int n = min(TILE, L.num_tuples - pos);
+    const uint32_t inr = n == TILE ? (IPT == 32 ? 0xFFFFFFFFu : (1u << IPT) - 1u) : range_mask<BLOCK, VPT>(n);
     unsigned long long alive[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; j++) alive[j] = (inr >> j & 1u) ? all : 0ull;
 
     // fact BETWEEN predicates, grouped by column (GPUProcessing.h:72-86)
 #pragma unroll
-    for (int c = 0; c < kBatchMaxCols; c++) {
-      const int pe = c < ncols ? D->pred_begin[c + 1] : 0;
-      for (int p = D->pred_begin[c]; p < pe; p++) {
-        const int lo = D->pred[p].lo;
-        const unsigned span = (unsigned)D->pred[p].hi - (unsigned)lo;
-        const unsigned long long keep = ~(1ull << D->pred[p].q);
+    for (int c = 0; c < MAXC; c++) {
+      if (!(colmask >> c & 1)) continue;
+      const int pb = __builtin_amdgcn_readfirstlane(H.pred_begin[c]);
+      const int pe = __builtin_amdgcn_readfirstlane(H.pred_begin[c + 1]);
+      for (int p = pb; p < pe; p++) {
+        const int4 P = H.pred[p];
+        const int lo = __builtin_amdgcn_readfirstlane(P.x);
+        const unsigned span = (unsigned)__builtin_amdgcn_readfirstlane(P.y);
+        const unsigned long long keep = ~(1ull << __builtin_amdgcn_readfirstlane(P.z));
 #pragma unroll
         for (int j = 0; j < IPT; j++)
           if ((unsigned)(x[c][j] - lo) > span) alive[j] &= keep;
       }
     }
 
-    // one probe per dimension for all queries: record gathers of the tile are
-    // issued before any is used; items no query needs read slot 0
+    // one probe per dimension for all queries: the record gathers of the tile
+    // are issued before any is used; items no query needs read slot 0
     uint32_t attrw[4][IPT];
 #pragma unroll
     for (int d = 0; d < 4; d++) {
 #pragma unroll
       for (int j = 0; j < IPT; j++) attrw[d][j] = 0;
-      if (!D->dim[d].used) continue;
-      const uint4* __restrict__ rec = D->dim[d].rec;
-      const int len = D->dim[d].len, mk = D->dim[d].minkey;
-      const unsigned long long uses = D->dim[d].uses;
+      if (!(usedmask >> d & 1)) continue;
+      const uint4* __restrict__ rec = H.rec[d];
+      const int len = __builtin_amdgcn_readfirstlane(H.len[d]);
+      const int mk = __builtin_amdgcn_readfirstlane(H.minkey[d]);
+      const unsigned long long uses = H.uses[d];
       int hs[IPT];
       bool inb[IPT];
 #pragma unroll
@@ -193,12 +271,15 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
 #pragma unroll
       for (int j = 0; j < IPT; j++) ev[e][j] = 0;
       if (e >= nexpr) continue;
-      const int kind = D->expr[e].kind, a1 = D->expr[e].a1, a2 = D->expr[e].a2;
+      const int4 E = H.expr[e];
+      const int kind = __builtin_amdgcn_readfirstlane(E.x);
+      const int a1 = __builtin_amdgcn_readfirstlane(E.y);
+      const int a2 = __builtin_amdgcn_readfirstlane(E.z);
 #pragma unroll
       for (int j = 0; j < IPT; j++) {
         int v1 = 0, v2 = 0;
 #pragma unroll
-        for (int c = 0; c < kBatchMaxCols; c++) {
+        for (int c = 0; c < MAXC; c++) {
           if (c == a1) v1 = x[c][j];
           if (c == a2) v2 = x[c][j];
         }
@@ -240,7 +321,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         } else {
           atomicAdd(reinterpret_cast<unsigned long long*>(res + 4), add);
           if (!(Q.flags & kQueryAggr)) {
-            unsigned* tw = D->touch_arena + Q.touch + (h >> 5);
+            unsigned* tw = touch_arena + Q.touch + (h >> 5);
             first = !(*tw & bit) && !(atomicOr(tw, bit) & bit);
           }
         }
@@ -252,6 +333,18 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
           key[1] = make_int2(gv2, gv3);
         }
       }
+    }
+  };
+
+  if (t_begin < t_end) {
+    int xa[MAXC][IPT], xb[MAXC][IPT];
+    load_cols(t_begin, xa);
+    for (int t = t_begin; t < t_end; t += 2) {
+      if (t + 1 < t_end) load_cols(t + 1, xb);
+      process(t, xa);
+      if (t + 1 >= t_end) break;
+      if (t + 2 < t_end) load_cols(t + 2, xa);
+      process(t + 1, xb);
     }
   }
 
@@ -271,48 +364,63 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
 
 namespace {
 
-template <int BLOCK, int VPT>
+template <int BLOCK, int VPT, int MAXC>
 const void* scan_fn() {
-  return reinterpret_cast<const void*>(&batch_kernel<BLOCK, VPT>);
+  return reinterpret_cast<const void*>(&batch_kernel<BLOCK, VPT, MAXC>);
 }
 
-const void* pick(const BatchConfig& c) {
+// Scan kernel of a configuration: columns held in registers per tile = 8 when
+// the batch reads at most 8 fact columns, else 12.
+const void* pick(const BatchConfig& c, int ncols) {
+  const bool small = ncols <= 8;
   if (c.vpt == 1) {
-    if (c.block == 256) return scan_fn<256, 1>();
-    if (c.block == 512) return scan_fn<512, 1>();
-    if (c.block == 1024) return scan_fn<1024, 1>();
+    if (c.block == 256) return small ? scan_fn<256, 1, 8>() : scan_fn<256, 1, 12>();
+    if (c.block == 512) return small ? scan_fn<512, 1, 8>() : scan_fn<512, 1, 12>();
+    if (c.block == 1024) return small ? scan_fn<1024, 1, 8>() : scan_fn<1024, 1, 12>();
   } else if (c.vpt == 2) {
-    if (c.block == 256) return scan_fn<256, 2>();
-    if (c.block == 512) return scan_fn<512, 2>();
+    if (c.block == 256) return small ? scan_fn<256, 2, 8>() : scan_fn<256, 2, 12>();
+    if (c.block == 512) return small ? scan_fn<512, 2, 8>() : scan_fn<512, 2, 12>();
   }
   return nullptr;
 }
 
-template <int BLOCK, int VPT>
+template <int BLOCK, int VPT, int MAXC>
 hipError_t launch(const BatchLaunch& L, int lds, hipStream_t s) {
   constexpr int TILE = BLOCK * 4 * VPT;
-  const auto f = &batch_kernel<BLOCK, VPT>;
+  const auto f = &batch_kernel<BLOCK, VPT, MAXC>;
   const int grid = balanced_grid((L.num_tuples + TILE - 1) / TILE,
-                                 resident_groups(reinterpret_cast<const void*>(f), BLOCK, lds));
-  f<<<grid, BLOCK, lds, s>>>(L);
+                                 resident_groups(reinterpret_cast<const void*>(f), BLOCK, lds - sizeof(BatchHot)));
+  f<<<grid, BLOCK, lds - (int)sizeof(BatchHot), s>>>(L);
   return hipGetLastError();
+}
+
+template <int MAXC>
+hipError_t launch_c(const BatchLaunch& L, const BatchConfig& c, hipStream_t s) {
+  if (c.vpt == 1) {
+    if (c.block == 256) return launch<256, 1, MAXC>(L, c.lds_bytes, s);
+    if (c.block == 512) return launch<512, 1, MAXC>(L, c.lds_bytes, s);
+    return launch<1024, 1, MAXC>(L, c.lds_bytes, s);
+  }
+  if (c.block == 256) return launch<256, 2, MAXC>(L, c.lds_bytes, s);
+  return launch<512, 2, MAXC>(L, c.lds_bytes, s);
 }
 
 }  // namespace
 
 bool batch_config_ok(const BatchConfig& c) {
-  return pick(c) != nullptr && c.lds_bytes >= 0 && c.lds_bytes <= 160 * 1024;
+  return pick(c, 8) != nullptr && c.lds_bytes >= (int)sizeof(BatchHot) && c.lds_bytes <= 160 * 1024;
 }
 
-hipError_t batch_prepare(const BatchConfig& c) {
-  const void* f = pick(c);
+hipError_t batch_prepare(const BatchConfig& c, int ncols) {
+  const void* f = pick(c, ncols);
   if (!f) return hipErrorInvalidValue;
-  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, std::max(c.lds_bytes, 1));
+  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             std::max(c.lds_bytes - (int)sizeof(BatchHot), 1));
 }
 
-int batch_resident(const BatchConfig& c) {
-  const void* f = pick(c);
-  return f ? resident_groups(f, c.block, c.lds_bytes) : 0;
+int batch_resident(const BatchConfig& c, int ncols) {
+  const void* f = pick(c, ncols);
+  return f ? resident_groups(f, c.block, c.lds_bytes - sizeof(BatchHot)) : 0;
 }
 
 hipError_t batch_fill(const BatchLaunch& L, const uint4* const* rec, const int* rec_len, hipStream_t s) {
@@ -328,15 +436,9 @@ hipError_t batch_records(const BatchLaunch& L, int d, int nrows, hipStream_t s) 
   return hipGetLastError();
 }
 
-hipError_t batch_scan(const BatchLaunch& L, const BatchConfig& c, hipStream_t s) {
-  if (!batch_config_ok(c)) return hipErrorInvalidValue;
-  if (c.vpt == 1) {
-    if (c.block == 256) return launch<256, 1>(L, c.lds_bytes, s);
-    if (c.block == 512) return launch<512, 1>(L, c.lds_bytes, s);
-    return launch<1024, 1>(L, c.lds_bytes, s);
-  }
-  if (c.block == 256) return launch<256, 2>(L, c.lds_bytes, s);
-  return launch<512, 2>(L, c.lds_bytes, s);
+hipError_t batch_scan(const BatchLaunch& L, const BatchConfig& c, int ncols, hipStream_t s) {
+  if (!batch_config_ok(c) || ncols > kBatchMaxCols) return hipErrorInvalidValue;
+  return ncols <= 8 ? launch_c<8>(L, c, s) : launch_c<12>(L, c, s);
 }
 
 }  // namespace mordred

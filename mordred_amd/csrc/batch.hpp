@@ -97,10 +97,24 @@ struct BatchLaunch {           // kernel arguments, fixed for a captured graph
   unsigned* status;            // [0] = hash-slot collisions in the record builds
 };
 
+// Hot part of the descriptor, staged in LDS once per workgroup so the tile
+// loop reads it with broadcast LDS loads instead of dependent global loads.
+struct BatchHot {
+  const int* col[kBatchMaxCols];
+  const uint4* rec[4];
+  unsigned long long uses[4];
+  int len[4], minkey[4];
+  int ncols, colmask, usedmask, nexpr, npred, nq;
+  int pred_begin[kBatchMaxCols + 1];
+  int4 expr[kBatchMaxExpr];
+  int4 pred[kBatchMaxPred];    // lo, hi - lo (unsigned), query, column
+};
+
 struct BatchConfig {
   int block;                   // 256, 512, 1024
   int vpt;                     // 1, 2 (items per thread = 4*vpt)
-  int lds_bytes;               // dynamic LDS of the scan kernel
+  int lds_bytes;               // LDS per workgroup: staged descriptor, query
+                               // descriptors and LDS group arrays
 };
 
 // Bytes of LDS a query set needs beyond the descriptors.
@@ -111,10 +125,11 @@ inline size_t batch_group_lds(int total_val) {
 // Host launchers (batch.hip).  Every launch reads the batch from L.desc.
 hipError_t batch_fill(const BatchLaunch& L, const uint4* const* rec, const int* rec_len, hipStream_t s);
 hipError_t batch_records(const BatchLaunch& L, int d, int nrows, hipStream_t s);
-hipError_t batch_scan(const BatchLaunch& L, const BatchConfig& c, hipStream_t s);
+// ncols: fact columns the batch reads (selects the register layout).
+hipError_t batch_scan(const BatchLaunch& L, const BatchConfig& c, int ncols, hipStream_t s);
 bool batch_config_ok(const BatchConfig& c);
 // Allow c.lds_bytes of dynamic LDS for the scan kernel (call before capture).
-hipError_t batch_prepare(const BatchConfig& c);
-int batch_resident(const BatchConfig& c);
+hipError_t batch_prepare(const BatchConfig& c, int ncols);
+int batch_resident(const BatchConfig& c, int ncols);
 
 }  // namespace mordred

@@ -766,7 +766,8 @@ int plan_batch(mordred_engine* e, const mordred_star_query* qs, int nq, const mo
   using namespace mordred;
   if (nq < 1 || nq > kBatchMaxQ) return fail(e, "a batch holds 1..64 queries");
   if (!batch_config_ok(cfg)) return fail(e, "bad batch configuration");
-  if ((size_t)cfg.lds_bytes < nq * sizeof(BatchQuery)) return fail(e, "lds_bytes below 64 bytes per query");
+  if ((size_t)cfg.lds_bytes < sizeof(BatchHot) + nq * sizeof(BatchQuery))
+    return fail(e, "lds_bytes below the staged descriptor plus 64 bytes per query");
   std::vector<Resolved> rs(nq);
   for (int i = 0; i < nq; i++) {
     const int rc = resolve(e, &qs[i], rs[i]);
@@ -898,7 +899,7 @@ int plan_batch(mordred_engine* e, const mordred_star_query* qs, int nq, const mo
   std::vector<int> order(nq);
   for (int i = 0; i < nq; i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b.G[x] < b.G[y]; });
-  size_t lds_used = nq * sizeof(BatchQuery);
+  size_t lds_used = sizeof(BatchHot) + nq * sizeof(BatchQuery);
   std::vector<char> in_lds(nq, 0);
   for (int i : order) {
     const size_t need = batch_group_lds(b.G[i]);
@@ -946,6 +947,7 @@ struct GraphKey {
   int rec_len[4];   // record slots cleared by the fill
   int nrows[4];     // dimension rows (record build grids)
   int used[4];
+  int ncols;        // selects the scan kernel's register layout
 };
 
 int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig& cfg, int* const* res,
@@ -969,6 +971,7 @@ int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig&
     k.rec_len[d] = k.used[d] ? b.rec_len[d] : 0;
     k.nrows[d] = k.used[d] ? b.d.dim[d].nrows : 0;
   }
+  k.ncols = b.d.ncols <= 8 ? 8 : 12;
   int captured = 0;
   const char* kp = reinterpret_cast<const char*>(&k);
   if (!e->gexec || e->gkey.size() != sizeof k || std::memcmp(e->gkey.data(), kp, sizeof k) != 0) {
@@ -977,13 +980,13 @@ int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig&
     e->gexec = nullptr;
     e->graph = nullptr;
     e->gkey.clear();
-    HIP_OK(batch_prepare(cfg));
-    (void)batch_resident(cfg);  // occupancy query outside the capture
+    HIP_OK(batch_prepare(cfg, b.d.ncols));
+    (void)batch_resident(cfg, b.d.ncols);  // occupancy query outside the capture
     HIP_OK(hipStreamBeginCapture(e->s_main, hipStreamCaptureModeRelaxed));
     hipError_t r1 = batch_fill(k.L, k.rec, k.rec_len, e->s_main);
     for (int d = 0; d < 4 && r1 == hipSuccess; d++)
       if (k.used[d]) r1 = batch_records(k.L, d, k.nrows[d], e->s_main);
-    if (r1 == hipSuccess) r1 = batch_scan(k.L, cfg, e->s_main);
+    if (r1 == hipSuccess) r1 = batch_scan(k.L, cfg, b.d.ncols, e->s_main);
     hipGraph_t g = nullptr;
     const hipError_t r2 = hipStreamEndCapture(e->s_main, &g);
     if (r1 != hipSuccess || r2 != hipSuccess) {
