@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmordred_hip.so")
 ARCH = os.environ.get("MORDRED_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "spec_groupby.hip", "spec_aggr.hip", "batch.hip", "engine.cpp"]
+SOURCES = ["kernels.hip", "spec_groupby.hip", "spec_aggr.hip", "batch.hip", "costmodel.hip", "engine.cpp"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -28,6 +28,8 @@ __device__ __forceinline__ bool dim_filter_pass(int mode, int x, int c1, int c2)
 
 }  // namespace
 
+// Clears the record tables of the used dimensions when they are rebuilt, and
+// the result rows and global touched bits of the queries being (re)started.
 __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __restrict__ D, unsigned* status,
                                                          uint4* r0, uint4* r1, uint4* r2, uint4* r3,
                                                          int n0, int n1, int n2, int n3) {
@@ -35,19 +37,29 @@ __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __rest
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0) status[0] = 0;
   const uint4 z = make_uint4(0, 0, 0, 0);
-  uint4* recs[4] = {r0, r1, r2, r3};
-  const int lens[4] = {n0, n1, n2, n3};
+  if (D->rebuild) {
+    uint4* recs[4] = {r0, r1, r2, r3};
+    const int lens[4] = {n0, n1, n2, n3};
 #pragma unroll
-  for (int d = 0; d < 4; d++) {
-    if (!recs[d] || !D->dim[d].used) continue;
-    for (long long i = t; i < lens[d]; i += stride) recs[d][i] = z;
+    for (int d = 0; d < 4; d++) {
+      if (!recs[d] || !D->dim[d].used) continue;
+      for (long long i = t; i < lens[d]; i += stride) recs[d][i] = z;
+    }
   }
-  uint4* res = reinterpret_cast<uint4*>(D->res_arena);
-  const long long nres = D->res_ints / 4;
-  for (long long i = t; i < nres; i += stride) res[i] = z;
-  uint4* tw = reinterpret_cast<uint4*>(D->touch_arena);
-  const long long ntw = D->touch_words / 4;
-  for (long long i = t; i < ntw; i += stride) tw[i] = z;
+  const unsigned long long reset = D->reset;
+  const int nq = D->nq;
+  for (int q = 0; q < nq; q++) {
+    if (!(reset >> q & 1ull)) continue;
+    const BatchQuery& Q = D->q[q];
+    int* res = reinterpret_cast<int*>(Q.res);
+    const long long ni = 6 * (long long)Q.total_val;
+    for (long long i = t; i < ni; i += stride) res[i] = 0;
+    if (Q.lds < 0 && !(Q.flags & kQueryAggr)) {
+      unsigned* tw = D->touch_arena + Q.touch;
+      const int nw = (Q.total_val + 31) / 32;
+      for (long long i = t; i < nw; i += stride) tw[i] = 0;
+    }
+  }
 }
 
 // One thread per dimension row: its slot, the mask of queries whose filter it
@@ -55,7 +67,7 @@ __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __rest
 // distinct filter column is read once per row.
 __global__ __launch_bounds__(256) void batch_records_kernel(BatchLaunch L, int d) {
   const BatchDim& B = L.desc->dim[d];
-  if (!B.used) return;
+  if (!B.used || !L.desc->rebuild) return;
   const int nrows = B.nrows, len = B.len, mk = B.minkey, nf = B.nfilt;
   const int stride = gridDim.x * blockDim.x;
   for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < nrows; row += stride) {
@@ -158,7 +170,9 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   }
   const int lane = threadIdx.x & (kWave - 1);
 
-  const int ntiles = (L.num_tuples + TILE - 1) / TILE;
+  const int num_tuples = D->num_tuples;
+  const short* __restrict__ seg_group = L.seg_group + D->seg_begin;
+  const int ntiles = (num_tuples + TILE - 1) / TILE;
   const int per = (ntiles + gridDim.x - 1) / gridDim.x;
   const int t_begin = blockIdx.x * per;
   const int t_end = min(ntiles, t_begin + per);
@@ -169,13 +183,13 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     const int pos = t * TILE;
     const int sidx = pos >> kSegLog;
     if (sidx != cur_seg) {
-      const int seg = __builtin_amdgcn_readfirstlane((int)L.seg_group[sidx]);
+      const int seg = __builtin_amdgcn_readfirstlane((int)seg_group[sidx]);
       const int* tab = lane < MAXC && (colmask >> lane & 1) ? H.col[lane] : nullptr;
       slotv = tab ? tab[seg] : 0;
       cur_seg = sidx;
     }
     const int off = pos & kSegMask;
-    const int n = min(TILE, L.num_tuples - pos);
+    const int n = min(TILE, num_tuples - pos);
     // every slot into a scalar first, then the loads back to back
     const int* base[MAXC];
 #pragma unroll
@@ -210,7 +224,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
 
   auto process = [&](int t, const int (&x)[MAXC][IPT]) {
     const int pos = t * TILE;
-    const int n = min(TILE, L.num_tuples - pos);
+    const int n = min(TILE, num_tuples - pos);
     const uint32_t inr = n == TILE ? (IPT == 32 ? 0xFFFFFFFFu : (1u << IPT) - 1u) : range_mask<BLOCK, VPT>(n);
     unsigned long long alive[IPT];
 #pragma unroll
@@ -388,7 +402,7 @@ template <int BLOCK, int VPT, int MAXC>
 hipError_t launch(const BatchLaunch& L, int lds, hipStream_t s) {
   constexpr int TILE = BLOCK * 4 * VPT;
   const auto f = &batch_kernel<BLOCK, VPT, MAXC>;
-  const int grid = balanced_grid((L.num_tuples + TILE - 1) / TILE,
+  const int grid = balanced_grid((L.max_tuples + TILE - 1) / TILE,
                                  resident_groups(reinterpret_cast<const void*>(f), BLOCK, lds - sizeof(BatchHot)));
   f<<<grid, BLOCK, lds - (int)sizeof(BatchHot), s>>>(L);
   return hipGetLastError();

@@ -75,10 +75,16 @@ struct BatchDim {
 struct BatchDesc {
   int nq, ncols, npred, nexpr;
   unsigned long long all;      // queries of the batch (dead ones cleared)
+  // row range of this launch: positions [0, num_tuples) of the segment group
+  // starting at seg_group[seg_begin] (a chunk of a circular scan, or all)
+  int seg_begin, num_tuples;
+  int rebuild;                 // record tables must be rebuilt (query set changed)
+  int pad0;
+  unsigned long long reset;    // queries whose result / touched bits are cleared first
   int lds_sum_words;           // u64 words of LDS group arrays
   int lds_touch_words;         // u32 words of LDS touched bitmaps
-  long long res_ints;          // result arena ints to clear (multiple of 4)
-  long long touch_words;       // global touched words to clear (multiple of 4)
+  long long res_ints;          // result arena ints
+  long long touch_words;       // global touched words
   int* res_arena;
   unsigned* touch_arena;
   const int* col[kBatchMaxCols];
@@ -93,7 +99,7 @@ struct BatchLaunch {           // kernel arguments, fixed for a captured graph
   const BatchDesc* desc;
   const int* cache;
   const short* seg_group;
-  int num_tuples;
+  int max_tuples;              // largest num_tuples of any launch (sizes the grid)
   unsigned* status;            // [0] = hash-slot collisions in the record builds
 };
 

@@ -1,0 +1,416 @@
+// Learned cost model: an fp32 MLP 32 -> 64 -> 64 -> 1 (ReLU) that predicts the
+// natural log of a plan's measured milliseconds, with hand-written gfx950
+// kernels for inference / forward, backward and Adam.  The reference has an
+// analytic cost model only (src/gpudb/CostModel.cu); this one is trained on
+// (plan features, measured time) pairs the engine's own kernels produce, and
+// is used only where it has proven more accurate than the analytic model on
+// fresh measurements (include/mordred_costmodel.h).
+//
+// Parameter layout (one float array): W1[32][64], b1[64], W2[64][64], b2[64],
+// W3[64], b3.  Loss: mean over the minibatch of (y - t)^2 on log-ms.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/mordred_costmodel.h"
+
+namespace mordred_cm {
+
+constexpr int kIn = MORDRED_COST_FEATURES;  // 32
+constexpr int kH = 64;
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kIn * kH;
+constexpr int kOffW2 = kOffB1 + kH;
+constexpr int kOffB2 = kOffW2 + kH * kH;
+constexpr int kOffW3 = kOffB2 + kH;
+constexpr int kOffB3 = kOffW3 + kH;
+constexpr int kParams = kOffB3 + 1;  // 6337
+constexpr int kBlock = 256;
+
+// Forward pass, one thread per sample; parameters staged in LDS.  Writes the
+// prediction and, when h1/h2 are given, the post-ReLU activations.
+__global__ __launch_bounds__(kBlock) void mlp_forward_kernel(const float* __restrict__ P, const float* __restrict__ X,
+                                                             int n, float* __restrict__ Y, float* __restrict__ H1,
+                                                             float* __restrict__ H2) {
+  __shared__ float w[kParams];
+  for (int i = threadIdx.x; i < kParams; i += kBlock) w[i] = P[i];
+  __syncthreads();
+  const int b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= n) return;
+  float x[kIn];
+#pragma unroll
+  for (int i = 0; i < kIn; i++) x[i] = X[(size_t)b * kIn + i];
+  float h1[kH];
+#pragma unroll 4
+  for (int o = 0; o < kH; o++) {
+    float a = w[kOffB1 + o];
+#pragma unroll
+    for (int i = 0; i < kIn; i++) a = fmaf(x[i], w[kOffW1 + i * kH + o], a);
+    h1[o] = fmaxf(a, 0.f);
+  }
+  float y = w[kOffB3];
+  for (int o = 0; o < kH; o++) {
+    float a = w[kOffB2 + o];
+#pragma unroll
+    for (int i = 0; i < kH; i++) a = fmaf(h1[i], w[kOffW2 + i * kH + o], a);
+    a = fmaxf(a, 0.f);
+    if (H2) H2[(size_t)b * kH + o] = a;
+    y = fmaf(a, w[kOffW3 + o], y);
+  }
+  if (H1)
+#pragma unroll
+    for (int o = 0; o < kH; o++) H1[(size_t)b * kH + o] = h1[o];
+  Y[b] = y;
+}
+
+// Backward deltas, one thread per sample: d3 = 2(y-t)/n, D2 = d3 W3 * [h2>0],
+// D1 = (D2 W2^T) * [h1>0].
+__global__ __launch_bounds__(kBlock) void mlp_delta_kernel(const float* __restrict__ P, const float* __restrict__ Y,
+                                                           const float* __restrict__ T, const float* __restrict__ H1,
+                                                           const float* __restrict__ H2, int n, float* __restrict__ D3,
+                                                           float* __restrict__ D2, float* __restrict__ D1) {
+  __shared__ float w[kH * kH + kH];
+  for (int i = threadIdx.x; i < kH * kH; i += kBlock) w[i] = P[kOffW2 + i];
+  for (int i = threadIdx.x; i < kH; i += kBlock) w[kH * kH + i] = P[kOffW3 + i];
+  __syncthreads();
+  const int b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= n) return;
+  const float d3 = 2.f * (Y[b] - T[b]) / (float)n;
+  D3[b] = d3;
+  float d2[kH];
+#pragma unroll
+  for (int o = 0; o < kH; o++) {
+    d2[o] = H2[(size_t)b * kH + o] > 0.f ? d3 * w[kH * kH + o] : 0.f;
+    D2[(size_t)b * kH + o] = d2[o];
+  }
+  for (int i = 0; i < kH; i++) {
+    float a = 0.f;
+#pragma unroll
+    for (int o = 0; o < kH; o++) a = fmaf(d2[o], w[i * kH + o], a);
+    D1[(size_t)b * kH + i] = H1[(size_t)b * kH + i] > 0.f ? a : 0.f;
+  }
+}
+
+// Parameter gradients, one thread per parameter, summed over the samples.
+__global__ __launch_bounds__(kBlock) void mlp_grad_kernel(const float* __restrict__ X, const float* __restrict__ H1,
+                                                          const float* __restrict__ H2, const float* __restrict__ D1,
+                                                          const float* __restrict__ D2, const float* __restrict__ D3,
+                                                          int n, float* __restrict__ G) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= kParams) return;
+  float g = 0.f;
+  if (p < kOffB1) {
+    const int i = p / kH, o = p % kH;
+    for (int b = 0; b < n; b++) g = fmaf(X[(size_t)b * kIn + i], D1[(size_t)b * kH + o], g);
+  } else if (p < kOffW2) {
+    const int o = p - kOffB1;
+    for (int b = 0; b < n; b++) g += D1[(size_t)b * kH + o];
+  } else if (p < kOffB2) {
+    const int i = (p - kOffW2) / kH, o = (p - kOffW2) % kH;
+    for (int b = 0; b < n; b++) g = fmaf(H1[(size_t)b * kH + i], D2[(size_t)b * kH + o], g);
+  } else if (p < kOffW3) {
+    const int o = p - kOffB2;
+    for (int b = 0; b < n; b++) g += D2[(size_t)b * kH + o];
+  } else if (p < kOffB3) {
+    const int o = p - kOffW3;
+    for (int b = 0; b < n; b++) g = fmaf(H2[(size_t)b * kH + o], D3[b], g);
+  } else {
+    for (int b = 0; b < n; b++) g += D3[b];
+  }
+  G[p] = g;
+}
+
+// Adam (Kingma & Ba, 2015), bias-corrected, one thread per parameter.
+__global__ __launch_bounds__(kBlock) void mlp_adam_kernel(float* __restrict__ P, const float* __restrict__ G,
+                                                          float* __restrict__ M, float* __restrict__ V, float lr,
+                                                          float b1, float b2, float eps, float c1, float c2) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= kParams) return;
+  const float g = G[p];
+  const float m = b1 * M[p] + (1.f - b1) * g;
+  const float v = b2 * V[p] + (1.f - b2) * g * g;
+  M[p] = m;
+  V[p] = v;
+  P[p] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+}  // namespace mordred_cm
+
+using namespace mordred_cm;
+
+struct mordred_cost_model {
+  int device = 0;
+  hipStream_t s = nullptr;
+  float* d_p = nullptr;   // parameters
+  float* d_g = nullptr;   // gradients
+  float* d_m = nullptr;   // Adam first moments
+  float* d_v = nullptr;   // Adam second moments
+  float* d_work = nullptr;
+  size_t work_floats = 0;
+  float* h_stage = nullptr;  // pinned staging
+  size_t stage_floats = 0;
+  long long adam_t = 0;
+  std::mt19937_64 rng;
+  std::vector<float> xs, ts;  // replay buffer (features, log-ms)
+  int capacity = 4096;
+  long long seen = 0;
+  double ewma_learned = -1, ewma_analytic = -1;
+  int scored = 0;
+  std::string err;
+};
+
+namespace {
+
+int cm_fail(mordred_cost_model* m, hipError_t rc, const char* what) {
+  m->err = std::string(what) + ": " + hipGetErrorString(rc);
+  return (int)rc;
+}
+
+#define CM_OK(call)                                       \
+  do {                                                    \
+    hipError_t rc_ = (call);                              \
+    if (rc_ != hipSuccess) return cm_fail(m, rc_, #call); \
+  } while (0)
+
+int reserve_work(mordred_cost_model* m, size_t floats) {
+  if (floats <= m->work_floats) return 0;
+  if (m->d_work) (void)hipFree(m->d_work);
+  m->d_work = nullptr;
+  m->work_floats = 0;
+  CM_OK(hipMalloc(&m->d_work, floats * sizeof(float)));
+  m->work_floats = floats;
+  return 0;
+}
+
+int reserve_stage(mordred_cost_model* m, size_t floats) {
+  if (floats <= m->stage_floats) return 0;
+  if (m->h_stage) (void)hipHostFree(m->h_stage);
+  m->h_stage = nullptr;
+  m->stage_floats = 0;
+  CM_OK(hipHostMalloc(reinterpret_cast<void**>(&m->h_stage), floats * sizeof(float), hipHostMallocDefault));
+  m->stage_floats = floats;
+  return 0;
+}
+
+// forward of n samples staged at X (device); y to host
+int forward(mordred_cost_model* m, const float* x_host, int n, float* y_host) {
+  const size_t need = (size_t)n * kIn + n;
+  int rc = reserve_work(m, need);
+  if (rc) return rc;
+  if ((rc = reserve_stage(m, need))) return rc;
+  float* dX = m->d_work;
+  float* dY = dX + (size_t)n * kIn;
+  std::memcpy(m->h_stage, x_host, (size_t)n * kIn * sizeof(float));
+  CM_OK(hipMemcpyAsync(dX, m->h_stage, (size_t)n * kIn * sizeof(float), hipMemcpyHostToDevice, m->s));
+  mlp_forward_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, m->s>>>(m->d_p, dX, n, dY, nullptr, nullptr);
+  CM_OK(hipGetLastError());
+  CM_OK(hipMemcpyAsync(m->h_stage, dY, n * sizeof(float), hipMemcpyDeviceToHost, m->s));
+  CM_OK(hipStreamSynchronize(m->s));
+  std::memcpy(y_host, m->h_stage, n * sizeof(float));
+  return 0;
+}
+
+// gradients of the mean squared loss over n samples (X, T on host); leaves
+// them in d_g and the loss in *loss
+int gradients(mordred_cost_model* m, const float* x_host, const float* t_host, int n, float* loss) {
+  const size_t need = (size_t)n * (kIn + 2 + 5 * kH + 1);
+  int rc = reserve_work(m, need);
+  if (rc) return rc;
+  if ((rc = reserve_stage(m, (size_t)n * (kIn + 1)))) return rc;
+  float* dX = m->d_work;
+  float* dT = dX + (size_t)n * kIn;
+  float* dY = dT + n;
+  float* H1 = dY + n;
+  float* H2 = H1 + (size_t)n * kH;
+  float* D1 = H2 + (size_t)n * kH;
+  float* D2 = D1 + (size_t)n * kH;
+  float* D3 = D2 + (size_t)n * kH;
+  std::memcpy(m->h_stage, x_host, (size_t)n * kIn * sizeof(float));
+  std::memcpy(m->h_stage + (size_t)n * kIn, t_host, n * sizeof(float));
+  CM_OK(hipMemcpyAsync(dX, m->h_stage, (size_t)n * (kIn + 1) * sizeof(float), hipMemcpyHostToDevice, m->s));
+  const int grid = (n + kBlock - 1) / kBlock;
+  mlp_forward_kernel<<<grid, kBlock, 0, m->s>>>(m->d_p, dX, n, dY, H1, H2);
+  mlp_delta_kernel<<<grid, kBlock, 0, m->s>>>(m->d_p, dY, dT, H1, H2, n, D3, D2, D1);
+  mlp_grad_kernel<<<(kParams + kBlock - 1) / kBlock, kBlock, 0, m->s>>>(dX, H1, H2, D1, D2, D3, n, m->d_g);
+  CM_OK(hipGetLastError());
+  if (loss) {
+    std::vector<float> y(n);
+    CM_OK(hipMemcpyAsync(m->h_stage, dY, n * sizeof(float), hipMemcpyDeviceToHost, m->s));
+    CM_OK(hipStreamSynchronize(m->s));
+    double l = 0;
+    for (int b = 0; b < n; b++) {
+      const double d = (double)m->h_stage[b] - (double)t_host[b];
+      l += d * d;
+    }
+    *loss = (float)(l / n);
+  }
+  return 0;
+}
+
+int adam_step(mordred_cost_model* m, float lr) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  m->adam_t++;
+  const float c1 = 1.f - (float)std::pow((double)b1, (double)m->adam_t);
+  const float c2 = 1.f - (float)std::pow((double)b2, (double)m->adam_t);
+  mlp_adam_kernel<<<(kParams + kBlock - 1) / kBlock, kBlock, 0, m->s>>>(m->d_p, m->d_g, m->d_m, m->d_v, lr, b1, b2,
+                                                                       eps, c1, c2);
+  CM_OK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+mordred_cost_model* mordred_cost_model_create(int device, unsigned long long seed) {
+  auto* m = new mordred_cost_model();
+  m->device = device;
+  m->rng.seed(seed);
+  bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreate(&m->s) == hipSuccess;
+  for (float** p : {&m->d_p, &m->d_g, &m->d_m, &m->d_v})
+    ok = ok && hipMalloc(p, kParams * sizeof(float)) == hipSuccess;
+  if (ok) {
+    // He initialisation of the weights, zero biases
+    std::vector<float> h(kParams, 0.f);
+    std::normal_distribution<float> n1(0.f, std::sqrt(2.f / kIn)), n2(0.f, std::sqrt(2.f / kH));
+    for (int i = 0; i < kIn * kH; i++) h[kOffW1 + i] = n1(m->rng);
+    for (int i = 0; i < kH * kH; i++) h[kOffW2 + i] = n2(m->rng);
+    for (int i = 0; i < kH; i++) h[kOffW3 + i] = n2(m->rng) * 0.5f;
+    ok = hipMemcpy(m->d_p, h.data(), kParams * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemset(m->d_m, 0, kParams * sizeof(float)) == hipSuccess &&
+         hipMemset(m->d_v, 0, kParams * sizeof(float)) == hipSuccess;
+  }
+  if (!ok) {
+    mordred_cost_model_destroy(m);
+    return nullptr;
+  }
+  return m;
+}
+
+void mordred_cost_model_destroy(mordred_cost_model* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  for (float* p : {m->d_p, m->d_g, m->d_m, m->d_v, m->d_work})
+    if (p) (void)hipFree(p);
+  if (m->h_stage) (void)hipHostFree(m->h_stage);
+  if (m->s) (void)hipStreamDestroy(m->s);
+  delete m;
+}
+
+const char* mordred_cost_model_last_error(const mordred_cost_model* m) { return m ? m->err.c_str() : "no model"; }
+
+int mordred_cost_model_num_params(void) { return kParams; }
+
+int mordred_cost_model_get_params(mordred_cost_model* m, float* out) {
+  if (!m || !out) return -1;
+  CM_OK(hipSetDevice(m->device));
+  CM_OK(hipMemcpy(out, m->d_p, kParams * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int mordred_cost_model_set_params(mordred_cost_model* m, const float* in) {
+  if (!m || !in) return -1;
+  CM_OK(hipSetDevice(m->device));
+  CM_OK(hipMemcpy(m->d_p, in, kParams * sizeof(float), hipMemcpyHostToDevice));
+  CM_OK(hipMemset(m->d_m, 0, kParams * sizeof(float)));
+  CM_OK(hipMemset(m->d_v, 0, kParams * sizeof(float)));
+  m->adam_t = 0;
+  return 0;
+}
+
+int mordred_cost_model_predict(mordred_cost_model* m, const float* x, int n, float* y) {
+  if (!m || !x || !y || n < 0) return -1;
+  if (n == 0) return 0;
+  CM_OK(hipSetDevice(m->device));
+  return forward(m, x, n, y);
+}
+
+int mordred_cost_model_gradients(mordred_cost_model* m, const float* x, const float* t, int n, float* grads,
+                                 float* loss) {
+  if (!m || !x || !t || n <= 0) return -1;
+  CM_OK(hipSetDevice(m->device));
+  int rc = gradients(m, x, t, n, loss);
+  if (rc) return rc;
+  if (grads) {
+    CM_OK(hipMemcpyAsync(grads, m->d_g, kParams * sizeof(float), hipMemcpyDeviceToHost, m->s));
+    CM_OK(hipStreamSynchronize(m->s));
+  }
+  return 0;
+}
+
+int mordred_cost_model_step(mordred_cost_model* m, const float* x, const float* t, int n, float lr, float* loss) {
+  if (!m || !x || !t || n <= 0 || !(lr > 0)) return -1;
+  CM_OK(hipSetDevice(m->device));
+  int rc = gradients(m, x, t, n, loss);
+  if (rc) return rc;
+  if ((rc = adam_step(m, lr))) return rc;
+  CM_OK(hipStreamSynchronize(m->s));
+  return 0;
+}
+
+int mordred_cost_model_observe(mordred_cost_model* m, const float* x, double measured_ms, double analytic_ms) {
+  if (!m || !x || !(measured_ms > 0)) return -1;
+  const float t = (float)std::log(measured_ms);
+  // score the model on the fresh measurement before it is trained on it
+  if (m->seen > 0) {
+    float y = 0;
+    int rc = mordred_cost_model_predict(m, x, 1, &y);
+    if (rc) return rc;
+    const double el = std::fabs((double)y - t);
+    m->ewma_learned = m->ewma_learned < 0 ? el : 0.9 * m->ewma_learned + 0.1 * el;
+    m->scored++;
+  }
+  if (analytic_ms > 0) {
+    const double ea = std::fabs(std::log(analytic_ms) - t);
+    m->ewma_analytic = m->ewma_analytic < 0 ? ea : 0.9 * m->ewma_analytic + 0.1 * ea;
+  }
+  const int slot = (int)(m->seen % m->capacity);
+  if ((int)m->ts.size() < m->capacity) {
+    m->xs.insert(m->xs.end(), x, x + kIn);
+    m->ts.push_back(t);
+  } else {
+    std::memcpy(&m->xs[(size_t)slot * kIn], x, kIn * sizeof(float));
+    m->ts[slot] = t;
+  }
+  m->seen++;
+  return 0;
+}
+
+int mordred_cost_model_train(mordred_cost_model* m, int steps, int batch, float lr, float* last_loss) {
+  if (!m || steps < 0 || batch <= 0 || !(lr > 0)) return -1;
+  const int have = (int)m->ts.size();
+  if (have == 0) return 0;
+  const int n = std::min(batch, have);
+  std::vector<float> x((size_t)n * kIn), t(n);
+  std::uniform_int_distribution<int> pick(0, have - 1);
+  float loss = 0;
+  for (int s = 0; s < steps; s++) {
+    for (int b = 0; b < n; b++) {
+      const int i = n == have ? b : pick(m->rng);
+      std::memcpy(&x[(size_t)b * kIn], &m->xs[(size_t)i * kIn], kIn * sizeof(float));
+      t[b] = m->ts[i];
+    }
+    const int rc = mordred_cost_model_step(m, x.data(), t.data(), n, lr, s + 1 == steps ? &loss : nullptr);
+    if (rc) return rc;
+  }
+  if (last_loss) *last_loss = loss;
+  return 0;
+}
+
+int mordred_cost_model_get_stats(const mordred_cost_model* m, mordred_cost_model_stats* st) {
+  if (!m || !st) return -1;
+  st->samples = m->seen;
+  st->scored = m->scored;
+  st->ewma_abs_log_error = m->ewma_learned;
+  st->ewma_abs_log_error_analytic = m->ewma_analytic;
+  st->trusted = m->scored >= 16 && m->ewma_learned >= 0 &&
+                (m->ewma_analytic < 0 || m->ewma_learned < m->ewma_analytic) && m->ewma_learned < 0.5;
+  return 0;
+}
+
+}  // extern "C"

@@ -761,24 +761,40 @@ bool batch_probes(const mordred_star_query& q, const Resolved& r, int k) {
   return r.join[k] && (q.kind == MORDRED_KIND_GROUPBY || k == 3);
 }
 
-int plan_batch(mordred_engine* e, const mordred_star_query* qs, int nq, const mordred::BatchConfig& cfg,
-               BatchPlan& b) {
+// Slots of a batch: query i holds bit i of every mask.  An empty slot
+// (qp[i] == NULL) takes no part.  With `fixed` set, slot i's result and
+// global touched bits sit at fixed strides of the given arenas (a serving
+// loop keeps them across launches); otherwise they are packed into the
+// engine's batch arenas.
+struct SlotArenas {
+  int* res = nullptr;
+  unsigned* touch = nullptr;
+  long long res_stride = 0;    // ints per slot
+  long long touch_stride = 0;  // words per slot
+};
+
+int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, const mordred::BatchConfig& cfg,
+               BatchPlan& b, const SlotArenas* fixed = nullptr) {
   using namespace mordred;
   if (nq < 1 || nq > kBatchMaxQ) return fail(e, "a batch holds 1..64 queries");
   if (!batch_config_ok(cfg)) return fail(e, "bad batch configuration");
   if ((size_t)cfg.lds_bytes < sizeof(BatchHot) + nq * sizeof(BatchQuery))
     return fail(e, "lds_bytes below the staged descriptor plus 64 bytes per query");
   std::vector<Resolved> rs(nq);
+  int first = -1;
   for (int i = 0; i < nq; i++) {
-    const int rc = resolve(e, &qs[i], rs[i]);
+    if (!qp[i]) continue;
+    const int rc = resolve(e, qp[i], rs[i]);
     if (rc) return rc;
+    if (first < 0) first = i;
   }
+  if (first < 0) return fail(e, "a batch needs at least one query");
   BatchDesc& D = b.d;
   std::memset(&D, 0, sizeof D);
   D.nq = nq;
-  b.n = rs[0].n;
+  b.n = rs[first].n;
   b.res_off.assign(nq, 0);
-  b.G.assign(nq, 1);
+  b.G.assign(nq, 0);
   Column* cols[kBatchMaxCols] = {};
   int ncols = 4;
   auto slot_of = [&](Column* c) -> int {
@@ -789,13 +805,14 @@ int plan_batch(mordred_engine* e, const mordred_star_query* qs, int nq, const mo
     return ncols++;
   };
   for (int i = 0; i < nq; i++) {
+    if (!qp[i]) continue;
     const Resolved& r = rs[i];
     if (r.n != b.n) return fail(e, "batch queries read fact tables of different lengths");
     for (Column* c : r.fact_cols)
       for (int s = 0; s < c->nseg; s++)
         if (c->slot[s] < 0) return fail(e, "batch fact columns must be cached in HBM: " + c->name);
     for (int k = 0; k < 4; k++) {
-      if (!batch_probes(qs[i], r, k)) continue;
+      if (!batch_probes(*qp[i], r, k)) continue;
       if (cols[k] && cols[k] != r.key[k]) return fail(e, "batch queries join one table through different fact keys");
       cols[k] = r.key[k];
     }
@@ -806,8 +823,9 @@ int plan_batch(mordred_engine* e, const mordred_star_query* qs, int nq, const mo
     BatchDim& B = D.dim[k];
     Column* key = nullptr;
     for (int i = 0; i < nq; i++) {
+      if (!qp[i]) continue;
       const Resolved& r = rs[i];
-      if (!batch_probes(qs[i], r, k)) continue;
+      if (!batch_probes(*qp[i], r, k)) continue;
       const mordred_join& j = *r.join[k];
       for (Column* c : {r.dkey[k], r.dfilt[k], r.dval[k]}) {
         if (c && !c->pinned_gpu) return fail(e, "dimension columns must be GPU-resident");
@@ -825,7 +843,7 @@ int plan_batch(mordred_engine* e, const mordred_star_query* qs, int nq, const mo
       if (r.dfilt[k] && (j.filter_mode < 0 || j.filter_mode > 2)) return fail(e, "dimension filter mode must be 0..2");
       B.filt[B.nfilt++] = BatchFilt{r.dfilt[k] ? r.dfilt[k]->d_idx : nullptr, j.filter_mode, j.c1, j.c2, i};
       B.uses |= 1ull << i;
-      if (qs[i].kind == MORDRED_KIND_GROUPBY && r.dval[k]) {
+      if (qp[i]->kind == MORDRED_KIND_GROUPBY && r.dval[k]) {
         Column* v = r.dval[k];
         int a = (attrs[k][0] == v) ? 0 : (attrs[k][1] == v) ? 1 : -1;
         if (a < 0) {
@@ -851,7 +869,8 @@ int plan_batch(mordred_engine* e, const mordred_star_query* qs, int nq, const mo
   // fact predicates, aggregate expressions, query descriptors
   std::vector<BatchPred> preds;
   for (int i = 0; i < nq; i++) {
-    const mordred_star_query& q = qs[i];
+    if (!qp[i]) continue;
+    const mordred_star_query& q = *qp[i];
     const Resolved& r = rs[i];
     bool dead = false;
     for (int f = 0; f < q.nfilters; f++) {
@@ -902,6 +921,7 @@ int plan_batch(mordred_engine* e, const mordred_star_query* qs, int nq, const mo
   size_t lds_used = sizeof(BatchHot) + nq * sizeof(BatchQuery);
   std::vector<char> in_lds(nq, 0);
   for (int i : order) {
+    if (!qp[i]) continue;
     const size_t need = batch_group_lds(b.G[i]);
     if (lds_used + need <= (size_t)cfg.lds_bytes) {
       in_lds[i] = 1;
@@ -911,6 +931,10 @@ int plan_batch(mordred_engine* e, const mordred_star_query* qs, int nq, const mo
   long long sum_words = 0, lds_touch = 0, g_touch = 0, res_ints = 0;
   for (int i = 0; i < nq; i++) {
     BatchQuery& Q = D.q[i];
+    if (!qp[i]) {
+      Q.lds = -1;
+      continue;
+    }
     const int tw = (b.G[i] + 31) / 32;
     if (in_lds[i]) {
       Q.lds = (int)sum_words;
@@ -920,22 +944,38 @@ int plan_batch(mordred_engine* e, const mordred_star_query* qs, int nq, const mo
       b.lds_queries++;
     } else {
       Q.lds = -1;
-      Q.touch = (int)g_touch;
+      Q.touch = fixed ? (int)(i * fixed->touch_stride) : (int)g_touch;
       g_touch += tw;
       b.global_queries++;
     }
-    b.res_off[i] = res_ints;
+    if (fixed && (6 * (long long)b.G[i] > fixed->res_stride || tw > fixed->touch_stride))
+      return fail(e, "query result larger than its serving slot");
+    b.res_off[i] = fixed ? i * fixed->res_stride : res_ints;
     res_ints += 6 * (long long)b.G[i];
   }
   D.lds_sum_words = (int)sum_words;
   D.lds_touch_words = (int)lds_touch;
-  D.res_ints = (res_ints + 3) & ~3ll;
-  D.touch_words = (g_touch + 3) & ~3ll;
-  HIP_OK(e->d_arena.reserve(std::max<long long>(16, D.res_ints * 4)));
-  HIP_OK(e->d_touch.reserve(std::max<long long>(16, D.touch_words * 4)));
-  D.res_arena = static_cast<int*>(e->d_arena.p);
-  D.touch_arena = static_cast<unsigned*>(e->d_touch.p);
-  for (int i = 0; i < nq; i++) D.q[i].res = (long long)(uintptr_t)(D.res_arena + b.res_off[i]);
+  if (fixed) {
+    D.res_ints = nq * fixed->res_stride;
+    D.touch_words = nq * fixed->touch_stride;
+    D.res_arena = fixed->res;
+    D.touch_arena = fixed->touch;
+  } else {
+    D.res_ints = (res_ints + 3) & ~3ll;
+    D.touch_words = (g_touch + 3) & ~3ll;
+    HIP_OK(e->d_arena.reserve(std::max<long long>(16, D.res_ints * 4)));
+    HIP_OK(e->d_touch.reserve(std::max<long long>(16, D.touch_words * 4)));
+    D.res_arena = static_cast<int*>(e->d_arena.p);
+    D.touch_arena = static_cast<unsigned*>(e->d_touch.p);
+  }
+  for (int i = 0; i < nq; i++) {
+    D.q[i].res = (long long)(uintptr_t)(D.res_arena + b.res_off[i]);
+    if (qp[i]) D.reset |= 1ull << i;
+  }
+  // one launch over every row, record tables built
+  D.seg_begin = 0;
+  D.num_tuples = (int)b.n;
+  D.rebuild = 1;
   return 0;
 }
 
@@ -1324,8 +1364,10 @@ int mordred_engine_batch(mordred_engine* e, const mordred_star_query* qs, int nq
     if (!res[i]) return fail(e, "a result pointer is NULL");
   const mordred::BatchConfig c = cfg ? mordred::BatchConfig{cfg->block, cfg->vpt, cfg->lds_bytes} : e->batch_cfg;
   HIP_OK(hipSetDevice(e->device));
+  std::vector<const mordred_star_query*> qp(std::max(0, nq));
+  for (int i = 0; i < nq; i++) qp[i] = &qs[i];
   BatchPlan b;
-  const int rc = plan_batch(e, qs, nq, c, b);
+  const int rc = plan_batch(e, qp.data(), nq, c, b);
   if (rc) return rc;
   return run_batch(e, b, c, res, stats);
 }
@@ -1353,10 +1395,12 @@ int mordred_engine_batch_tune(mordred_engine* e, const mordred_star_query* qs, i
   mordred::BatchConfig best_c = e->batch_cfg;
   int n_out = 0;
   std::string first_err;
+  std::vector<const mordred_star_query*> qp(nq);
+  for (int i = 0; i < nq; i++) qp[i] = &qs[i];
   for (const auto& c : cands) {
     BatchPlan b;
     double ms = -1;
-    int rc = plan_batch(e, qs, nq, c, b);
+    int rc = plan_batch(e, qp.data(), nq, c, b);
     if (!rc) {
       cur.assign(nq, {});
       for (int i = 0; i < nq; i++) {

@@ -14,6 +14,7 @@ from mordred_amd import abi
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "mordred_hip.h")
 ENGINE_HEADER = os.path.join(ROOT, "include", "mordred_engine.h")
+COST_HEADER = os.path.join(ROOT, "include", "mordred_costmodel.h")
 
 
 def declared_functions(header=HEADER):
@@ -30,7 +31,7 @@ def test_header_declares_entry_points():
 
 def test_library_exports_every_declared_symbol():
     lib = abi.load()
-    for name in declared_functions() + declared_functions(ENGINE_HEADER):
+    for name in declared_functions() + declared_functions(ENGINE_HEADER) + declared_functions(COST_HEADER):
         assert hasattr(lib, name), name
     assert lib.mordred_version().decode().startswith("mordred-hip")
 
@@ -109,3 +110,9 @@ def test_invalid_arguments_are_rejected_before_launch():
                                   C.c_void_p(16), None) == 1
     # an empty input is a successful no-op
     assert lib.mordred_probe_group_by_GPU2(None, None, C.pointer(g), 0, C.c_void_p(16), 0, None, None) == 0
+
+
+def test_engine_and_cost_model_bindings_cover_headers():
+    from mordred_amd import costmodel, engine
+    assert set(declared_functions(ENGINE_HEADER)) == set(engine._SIGS)
+    assert set(declared_functions(COST_HEADER)) == set(costmodel._SIGS)
