@@ -274,6 +274,106 @@ def batch_bench(args):
         sys.exit(1)
 
 
+def serve_bench(args):
+    """Poisson request streams of the 64-query mix over the 1.2B-row fact table
+    (all columns in HBM, admission budget 200 GB), rate swept 10..2,000 q/s:
+    the plain form (first come first served, each query alone through its
+    builds and fused kernel) against continuous batching (the circular shared
+    scan, launch sizes ranked by the learned cost model once it is trusted).
+    Every request's answer must be identical in both forms.  One JSON line."""
+    from mordred_amd import engine as E
+    from mordred_amd import mix, serving
+    n = args.rows or 1_200_000_000
+    dims = {"S": 40_000, "P": 1_400_000, "C": 3_000_000}
+    mq = mix.make_mix(64, 2024, dims)
+    qs = [m.query for m in mq]
+    t0 = time.time()
+    data = mix.make_mix_data(qs, n, seed=2024, S=dims["S"], P=dims["P"], Cn=dims["C"])
+    cols = mix.fact_columns(qs)
+    log(f"generated {n} rows x {len(cols)} fact columns in {time.time() - t0:.1f}s")
+    nseg = (n + (1 << 20) - 1) >> 20
+    threads = min(16, os.cpu_count() or 1)
+    eng = E.Engine(device=0, cache_segments=len(cols) * nseg + 64, threads=threads)
+    t0 = time.time()
+    eng.add_mix_data(data, qs)
+    log(f"loaded into the segment cache in {time.time() - t0:.1f}s")
+    best, _ = eng.batch_tune(qs, reps=1)
+    log(f"batch configuration {best}")
+    budget = 200e9
+    target = args.target_chunk_ms
+    # warm-up stream: compiles the graph and trains the cost model online
+    warm = serving.request_stream(qs, 400, seed=1)
+    _, _, wst = eng.serve(warm, serving.poisson_arrivals(4000.0, len(warm), seed=1), E.SERVE_CONTINUOUS,
+                          target_chunk_ms=target, hbm_budget_bytes=budget, learned=True)
+    log(f"warm-up: {wst}")
+    rates = [float(r) for r in args.rates.split(",")]
+    rows, same_all = [], True
+    for i, rate in enumerate(rates):
+        nreq = int(min(600, max(60, rate * 2.0)))
+        reqs = serving.request_stream(qs, nreq, seed=100 + i)
+        arr = serving.poisson_arrivals(rate, nreq, seed=100 + i)
+        po, pt, pst = eng.serve(reqs, arr, E.SERVE_PLAIN)
+        co, ct, cst = eng.serve(reqs, arr, E.SERVE_CONTINUOUS, target_chunk_ms=target, hbm_budget_bytes=budget,
+                                learned=True)
+        same = all(bool((a == b).all()) for a, b in zip(po, co))
+        same_all = same_all and same
+        p, c = serving.latency_summary(pt), serving.latency_summary(ct)
+        rows.append({"rate_qps": rate, "requests": nreq, "plain": p, "served": c, "identical": same,
+                     "served_launches": cst["launches"], "served_max_active": cst["max_active_seen"],
+                     "learned_choices": cst["learned_choices"], "analytic_choices": cst["analytic_choices"],
+                     "learned_error": cst["learned_error"], "analytic_error": cst["analytic_error"],
+                     "learned_trusted": cst["learned_trusted"]})
+        log(f"rate {rate:7.0f} q/s: plain {p['throughput_qps']:7.1f} q/s p50 {p['p50_ms']:8.1f} p99 {p['p99_ms']:8.1f} | "
+            f"served {c['throughput_qps']:7.1f} q/s p50 {c['p50_ms']:8.1f} p99 {c['p99_ms']:8.1f} | "
+            f"launches {cst['launches']} learned {cst['learned_choices']}/{cst['launches']} identical {same}")
+    top = rows[-1]
+    out = {
+        "metric": METRIC + " (request loop, Poisson arrivals)",
+        "value": top["served"]["throughput_qps"],
+        "unit": "queries/s at the highest offered rate",
+        "n_gpus": 1,
+        "steps": len(rates),
+        "warmup": 1,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 columns, int64 sums",
+        "data": "synthetic, seeded SplitMix64 columns (mordred_amd/datagen.py); query mix mordred_amd/mix.py seed 2024",
+        "config": {"workload": "Poisson stream of the 64-query star mix, continuous batching vs plain per-query",
+                   "fact_rows": n, "fact_columns": len(cols), "hbm_budget_bytes": budget,
+                   "target_chunk_ms": target,
+                   "launch": {"block": best[0], "items_per_thread": 4 * best[1], "lds_bytes": best[2]}},
+        "throughput_vs_plain": top["served"]["throughput_qps"] / top["plain"]["throughput_qps"],
+        "p99_vs_plain": top["served"]["p99_ms"] / top["plain"]["p99_ms"],
+        "sweep": rows,
+        "parity_plain_vs_served": same_all,
+    }
+    if not args.no_cpu:
+        import oracle
+        from oracle import queries as OQ
+        oracle.set_threads(threads)
+        sample = [0, 21, 42]
+        ok = True
+        t0 = time.perf_counter()
+        for j in sample:
+            ref = OQ.run_query(qs[j], data, oracle.CONV_GPU)
+            got, _, _ = eng.serve([qs[j]], [0.0], E.SERVE_CONTINUOUS, target_chunk_ms=target)
+            ok = ok and bool((ref == got[0]).all())
+        cpu_s = (time.perf_counter() - t0) / len(sample)
+        out["cpu_baseline"] = {"value": 1.0 / cpu_s, "unit": "queries/s", "cores": threads, "kind": "port",
+                               "sample": f"oracle on mix queries {sample} over all {n} rows, {cpu_s * 1e3:.0f} ms per query"}
+        out["parity_oracle_sample"] = ok
+        out["parity"] = ok and same_all
+    else:
+        out["parity"] = same_all
+    print(json.dumps(out), flush=True)
+    eng.close()
+    if not out["parity"]:
+        log("PARITY FAILURE")
+        sys.exit(1)
+
+
 def copy_probe():
     """A device copy of exactly 1,920,000,000 bytes read + written, repeated 5
     times: the reference kernel for scaling FETCH_SIZE / WRITE_SIZE."""
@@ -292,7 +392,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["q21", "q11", "q4way-split", "batch64"], default="q21")
+    ap.add_argument("--workload", choices=["q21", "q11", "q4way-split", "batch64", "serve"], default="q21")
+    ap.add_argument("--rates", default="10,50,200,500,1000,2000", help="serve: offered rates (q/s)")
+    ap.add_argument("--target-chunk-ms", type=float, default=8.0, help="serve: launch-time target")
     ap.add_argument("--rows", type=int, default=None, help="override fact rows")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity check")
@@ -311,6 +413,9 @@ def main():
         return
     if args.workload == "batch64":
         batch_bench(args)
+        return
+    if args.workload == "serve":
+        serve_bench(args)
         return
 
     import torch

@@ -164,6 +164,59 @@ int mordred_engine_batch_tune(mordred_engine* e, const mordred_star_query* qs, i
                               mordred_batch_config* best, mordred_batch_tune_entry* table, int max_entries,
                               int* nentries);
 
+/* ---- request loop -------------------------------------------------------
+ * No counterpart in the reference.  Replays a stream of requests (query i
+ * arrives arrival_ms[i] after the loop starts, ascending) in real time on
+ * this thread and returns when every request is answered.
+ *   MORDRED_SERVE_PLAIN: first come first served, each query alone through
+ *     mordred_engine_run (its builds and its fused kernel).
+ *   MORDRED_SERVE_CONTINUOUS: a circular shared scan.  The GPU walks the fact
+ *     table in launches of consecutive segments; every launch runs the shared
+ *     scan for all active queries.  A request is admitted at the next launch
+ *     boundary when a query slot is free and its result and touched-bit memory
+ *     fits the HBM budget, and finishes once the scan has covered every
+ *     segment exactly once for it (launches never cross an active query's
+ *     starting segment), so its result equals its own run bit for bit.
+ *     Launch sizes (segments per launch) are chosen per launch so that the
+ *     predicted launch time stays within target_chunk_ms; predictions come
+ *     from the learned cost model once it is trusted (mordred_costmodel.h),
+ *     else from the analytic launch model.  The learned model is trained
+ *     online on the measured launch times and persists in the engine.
+ * The fact columns must be cached in HBM (all segments) for both modes. */
+enum { MORDRED_SERVE_PLAIN = 0, MORDRED_SERVE_CONTINUOUS = 1 };
+
+typedef struct mordred_serve_opts {
+  int mode;
+  int max_active;             /* query slots (<= 64) */
+  int chunk_segments;         /* > 0: fixed launch size; 0: chosen per launch */
+  double target_chunk_ms;     /* launch-time target for the chosen launch size */
+  double hbm_budget_bytes;    /* admission budget for per-query device state */
+  int learned;                /* 1: use the learned cost model when trusted */
+  mordred_batch_config config;  /* block 0: the engine's batch configuration */
+} mordred_serve_opts;
+
+typedef struct mordred_serve_times {
+  double arrival_ms, admit_ms, done_ms;  /* since the loop started */
+  int launches;                          /* launches the request took part in */
+} mordred_serve_times;
+
+typedef struct mordred_serve_stats {
+  double wall_ms;
+  long long launches;
+  double gpu_ms;                  /* sum of launch times (events) */
+  int max_active_seen;
+  long long learned_choices;      /* launch sizes chosen with the learned model */
+  long long analytic_choices;
+  double learned_error;           /* EWMA |log error| of each model on launches */
+  double analytic_error;
+  int learned_trusted;
+  double peak_state_bytes;        /* admitted per-query device state, at most */
+} mordred_serve_stats;
+
+int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n, const double* arrival_ms,
+                         int* const* res, const mordred_serve_opts* opts, mordred_serve_times* times,
+                         mordred_serve_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,8 +54,8 @@ __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __rest
     int* res = reinterpret_cast<int*>(Q.res);
     const long long ni = 6 * (long long)Q.total_val;
     for (long long i = t; i < ni; i += stride) res[i] = 0;
-    if (Q.lds < 0 && !(Q.flags & kQueryAggr)) {
-      unsigned* tw = D->touch_arena + Q.touch;
+    if (!(Q.flags & kQueryAggr)) {
+      unsigned* tw = D->touch_arena + Q.gtouch;
       const int nw = (Q.total_val + 31) / 32;
       for (long long i = t; i < nw; i += stride) tw[i] = 0;
     }
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         } else {
           atomicAdd(reinterpret_cast<unsigned long long*>(res + 4), add);
           if (!(Q.flags & kQueryAggr)) {
-            unsigned* tw = touch_arena + Q.touch + (h >> 5);
+            unsigned* tw = touch_arena + Q.gtouch + (h >> 5);
             first = !(*tw & bit) && !(atomicOr(tw, bit) & bit);
           }
         }

@@ -39,14 +39,18 @@ enum BatchQueryFlags { kQueryAggr = 1 };
 struct BatchQuery {            // 64 bytes; staged in LDS by the scan kernel
   long long res;               // device address of the query's 6*total_val ints
   int total_val;
-  int expr;                    // index into BatchDesc::expr
+  short expr;                  // index into BatchDesc::expr
+  short flags;                 // kQueryAggr: no key words
   int lds;                     // u64 offset of its group array in LDS, -1 = global atomics
-  int touch;                   // u32 offset of its touched bitmap (LDS or global arena)
-  int flags;                   // kQueryAggr: no key words
+  int touch;                   // u32 offset of its LDS touched bitmap (when lds >= 0)
+  int gtouch;                  // u32 offset of its touched bitmap in the global arena,
+                               // valid whatever the placement (a query of a request
+                               // loop may change placement between launches)
   signed char attr[4];         // per dimension slot: -1 no group value, else attribute 0 / 1
   int minv[4];
   int uniq[4];
 };
+static_assert(sizeof(BatchQuery) == 64, "BatchQuery is staged as 4 x 16 bytes");
 
 struct BatchPred {             // fact BETWEEN predicate of one query
   int col, lo, hi, q;

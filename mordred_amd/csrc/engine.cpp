@@ -24,6 +24,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "../../include/mordred_costmodel.h"
 #include "../../include/mordred_engine.h"
 #include "../../include/mordred_hip.h"
 #include "batch.hpp"
@@ -173,6 +174,9 @@ struct mordred_engine {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   mordred::BatchConfig batch_cfg{512, 1, 64 * 1024};
   bool batch_tuned = false;
+  // request loop: persistent slot arenas and the learned cost model
+  DevBuf d_serve_res, d_serve_touch;
+  mordred_cost_model* cm = nullptr;
   std::string err;
 };
 
@@ -892,8 +896,8 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     }
     BatchQuery& Q = D.q[i];
     Q.total_val = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
-    Q.expr = ex;
-    Q.flags = q.kind == MORDRED_KIND_GROUPBY ? 0 : kQueryAggr;
+    Q.expr = (short)ex;
+    Q.flags = (short)(q.kind == MORDRED_KIND_GROUPBY ? 0 : kQueryAggr);
     for (int k = 0; k < 4; k++) {
       Q.attr[k] = -1;
       if (q.kind != MORDRED_KIND_GROUPBY || !r.join[k] || !r.dval[k]) continue;
@@ -936,6 +940,8 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       continue;
     }
     const int tw = (b.G[i] + 31) / 32;
+    Q.gtouch = fixed ? (int)(i * fixed->touch_stride) : (int)g_touch;
+    g_touch += tw;
     if (in_lds[i]) {
       Q.lds = (int)sum_words;
       Q.touch = (int)lds_touch;
@@ -944,8 +950,7 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       b.lds_queries++;
     } else {
       Q.lds = -1;
-      Q.touch = fixed ? (int)(i * fixed->touch_stride) : (int)g_touch;
-      g_touch += tw;
+      Q.touch = Q.gtouch;
       b.global_queries++;
     }
     if (fixed && (6 * (long long)b.G[i] > fixed->res_stride || tw > fixed->touch_stride))
@@ -990,10 +995,10 @@ struct GraphKey {
   int ncols;        // selects the scan kernel's register layout
 };
 
-int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig& cfg, int* const* res,
-              mordred_batch_stats* st) {
+// Capture (first use / changed launch) and launch the batch graph of plan b on
+// the main stream, bracketed by the engine's events.  Does not wait.
+int launch_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig& cfg, int* captured_out) {
   using namespace mordred;
-  const auto t0 = Clock::now();
   HIP_OK(hipSetDevice(e->device));
   if (!e->ev0) HIP_OK(hipEventCreate(&e->ev0));
   if (!e->ev1) HIP_OK(hipEventCreate(&e->ev1));
@@ -1043,6 +1048,24 @@ int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig&
   HIP_OK(hipEventRecord(e->ev0, e->s_main));
   HIP_OK(hipGraphLaunch(e->gexec, e->s_main));
   HIP_OK(hipEventRecord(e->ev1, e->s_main));
+  if (captured_out) *captured_out = captured;
+  return 0;
+}
+
+int batch_status(mordred_engine* e, unsigned collisions) {
+  if (collisions)
+    return fail(e, "dimension hash collision in a batch record table (" + std::to_string(collisions) +
+                       " slots): batches need collision-free perfect hashing");
+  return 0;
+}
+
+int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig& cfg, int* const* res,
+              mordred_batch_stats* st) {
+  using namespace mordred;
+  const auto t0 = Clock::now();
+  int captured = 0;
+  int rc = launch_batch(e, b, cfg, &captured);
+  if (rc) return rc;
   const size_t rbytes = (size_t)b.d.res_ints * sizeof(int);
   HIP_OK(e->h_arena.reserve(rbytes + 64));
   char* ha = static_cast<char*>(e->h_arena.p);
@@ -1051,9 +1074,7 @@ int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig&
   HIP_OK(hipStreamSynchronize(e->s_main));
   unsigned collisions = 0;
   std::memcpy(&collisions, ha + rbytes, sizeof collisions);
-  if (collisions)
-    return fail(e, "dimension hash collision in a batch record table (" + std::to_string(collisions) +
-                       " slots): batches need collision-free perfect hashing");
+  if ((rc = batch_status(e, collisions))) return rc;
   const int* arena = reinterpret_cast<const int*>(ha);
   for (int i = 0; i < b.d.nq; i++)
     std::memcpy(res[i], arena + b.res_off[i], 6 * (size_t)b.G[i] * sizeof(int));
@@ -1069,6 +1090,122 @@ int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig&
     st->config = mordred_batch_config{cfg.block, cfg.vpt, cfg.lds_bytes};
   }
   return 0;
+}
+
+// ------------------------------------------------------------ request loop
+// Selectivity estimate of one query (the fraction of fact rows it keeps):
+// dimension filters on an evenly spaced sample of the dimension rows, fact
+// BETWEEN filters from the column's value range assuming uniform values.
+struct QueryEst {
+  double sel = 1.0;
+  int G = 1;
+  int njoins = 0;
+  int dims = 0;
+};
+
+double dim_pass_fraction(const Resolved& r, int k) {
+  const Column* f = r.dfilt[k];
+  if (!f) return 1.0;
+  const mordred_join& j = *r.join[k];
+  const long long n = r.dkey[k]->n;
+  const long long step = std::max<long long>(1, n / 4096);
+  long long seen = 0, pass = 0;
+  for (long long row = 0; row < n; row += step, seen++) {
+    const int x = f->host[row];
+    const bool ok = j.filter_mode == 0 ? x == j.c1 : j.filter_mode == 1 ? (x >= j.c1 && x <= j.c2)
+                                                                       : (x == j.c1 || x == j.c2);
+    pass += ok;
+  }
+  return seen ? (double)pass / (double)seen : 1.0;
+}
+
+int estimate_query(mordred_engine* e, const mordred_star_query& q, QueryEst& out) {
+  Resolved r;
+  const int rc = resolve(e, &q, r);
+  if (rc) return rc;
+  double sel = 1.0;
+  for (int k = 0; k < 4; k++) {
+    if (!batch_probes(q, r, k)) continue;
+    sel *= dim_pass_fraction(r, k);
+    out.njoins++;
+    out.dims |= 1 << k;
+  }
+  for (int i = 0; i < q.nfilters; i++) {
+    Column* c = r.filt[i];
+    column_range(c);
+    const double lo = std::max<double>(q.filters[i].lo, c->vmin), hi = std::min<double>(q.filters[i].hi, c->vmax);
+    sel *= hi >= lo ? (hi - lo + 1) / ((double)c->vmax - c->vmin + 1) : 0.0;
+  }
+  out.sel = sel;
+  out.G = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
+  return 0;
+}
+
+// Features of one launch of the shared scan (MORDRED_COST_FEATURES floats).
+void launch_features(const BatchPlan& b, const std::vector<QueryEst>& est, long long rows, int nreset,
+                     const mordred::BatchConfig& c, float* f) {
+  std::fill(f, f + MORDRED_COST_FEATURES, 0.f);
+  const mordred::BatchDesc& D = b.d;
+  double s_all = 0, s_glob = 0, s_lds = 0, g_all = 0, g_lds = 0, gmax = 0;
+  int nq = 0, nglob = 0, deep = 0;
+  int per_dim[4] = {0, 0, 0, 0};
+  for (int i = 0; i < D.nq; i++) {
+    if (!(D.all >> i & 1ull)) continue;
+    const QueryEst& q = est[i];
+    nq++;
+    s_all += q.sel;
+    g_all += q.G;
+    gmax = std::max<double>(gmax, q.G);
+    if (D.q[i].lds >= 0) {
+      s_lds += q.sel;
+      g_lds += q.G;
+    } else {
+      s_glob += q.sel;
+      nglob++;
+    }
+    deep += q.njoins >= 3;
+    for (int k = 0; k < 4; k++) per_dim[k] += q.dims >> k & 1;
+  }
+  f[0] = (float)(std::log2((double)rows + 1.0) / 32.0);
+  f[1] = nq / 64.f;
+  f[2] = (float)s_all;
+  f[3] = (float)s_glob;
+  f[4] = (float)s_lds;
+  f[5] = (float)(std::log2(1.0 + g_all) / 16.0);
+  f[6] = nglob / 64.f;
+  f[7] = D.ncols / 12.f;
+  for (int k = 0; k < 4; k++) {
+    f[8 + k] = (float)D.dim[k].used;
+    f[12 + k] = per_dim[k] / 64.f;
+  }
+  f[16] = D.npred / 128.f;
+  f[17] = D.nexpr / 4.f;
+  f[18] = c.block == 256;
+  f[19] = c.block == 512;
+  f[20] = c.block == 1024;
+  f[21] = (float)(c.vpt - 1);
+  f[22] = c.lds_bytes / (160.f * 1024.f);
+  f[23] = (float)D.rebuild;
+  f[24] = nreset / 64.f;
+  f[25] = (float)(std::log2(1.0 + g_lds) / 16.0);
+  f[26] = (float)(gmax / 17500.0);
+  f[27] = nq ? (float)deep / nq : 0.f;
+  f[28] = (float)(rows * s_all / 1e9);
+  f[29] = (float)(rows * s_glob / 1e9);
+  f[30] = (float)(rows / 1e9);
+  f[31] = 1.f;
+}
+
+// Analytic launch model: column bytes at the shared scan's streaming rate,
+// one LDS or global update per surviving (row, query) pair, a fixed launch
+// cost and the record rebuild.  Constants from MI355X measurements of the
+// shared scan (64-query batches over 600M rows).
+double analytic_launch_ms(const float* f, int ncols) {
+  const double rows = f[30] * 1e9;
+  const double bytes = rows * 4.0 * ncols;
+  const double pairs_lds = (double)f[28] * 1e9 - (double)f[29] * 1e9;
+  const double pairs_glob = (double)f[29] * 1e9;
+  return 1e3 * (bytes / 1.3e12 + pairs_lds * 8e-12 + pairs_glob * 60e-12) + 0.08 + (f[23] > 0 ? 0.3 : 0.0);
 }
 
 }  // namespace
@@ -1119,6 +1256,7 @@ void mordred_engine_destroy(mordred_engine* e) {
     if (kv.second->d_idx) (void)hipFree(kv.second->d_idx);
   }
   e->cols.clear();
+  if (e->cm) mordred_cost_model_destroy(e->cm);
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
@@ -1435,6 +1573,225 @@ int mordred_engine_batch_tune(mordred_engine* e, const mordred_star_query* qs, i
   e->batch_cfg = best_c;
   e->batch_tuned = true;
   if (best) *best = mordred_batch_config{best_c.block, best_c.vpt, best_c.lds_bytes};
+  return 0;
+}
+
+int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n, const double* arrival_ms,
+                         int* const* res, const mordred_serve_opts* o, mordred_serve_times* times,
+                         mordred_serve_stats* stats) {
+  using namespace mordred;
+  if (!e) return -1;
+  if (e->device < 0) return fail(e, "host-only engine has no request loop");
+  if (!qs || !arrival_ms || !res || !o || !times || n < 0) return fail(e, "serve: NULL argument");
+  for (int i = 0; i < n; i++) {
+    if (!res[i]) return fail(e, "serve: a result pointer is NULL");
+    if (i && arrival_ms[i] < arrival_ms[i - 1]) return fail(e, "serve: arrivals must be ascending");
+  }
+  HIP_OK(hipSetDevice(e->device));
+  mordred_serve_stats st{};
+  auto wall0 = Clock::now();
+  auto now_ms = [&] { return ms_since(wall0); };
+  auto wait_until = [&](double t) {
+    while (now_ms() < t) std::this_thread::sleep_for(std::chrono::microseconds(50));
+  };
+  for (int i = 0; i < n; i++) times[i] = mordred_serve_times{arrival_ms[i], -1, -1, 0};
+
+  if (o->mode == MORDRED_SERVE_PLAIN) {
+    for (int i = 0; i < n; i++) {
+      wait_until(arrival_ms[i]);
+      times[i].admit_ms = now_ms();
+      mordred_run_stats rs{};
+      const int rc = mordred_engine_run(e, &qs[i], MORDRED_PLACE_RESIDENCY, res[i], &rs);
+      if (rc) return rc;
+      if (rs.cpu_rows) return fail(e, "serve: fact columns must be cached in HBM");
+      times[i].done_ms = now_ms();
+      times[i].launches = 1;
+      st.launches++;
+      st.gpu_ms += rs.gpu_ms;
+      st.max_active_seen = 1;
+    }
+    st.wall_ms = now_ms();
+    if (stats) *stats = st;
+    return 0;
+  }
+  if (o->mode != MORDRED_SERVE_CONTINUOUS) return fail(e, "serve: unknown mode");
+  const int nslot = std::max(1, std::min(o->max_active > 0 ? o->max_active : kBatchMaxQ, kBatchMaxQ));
+  const BatchConfig cfg = o->config.block > 0 ? BatchConfig{o->config.block, o->config.vpt, o->config.lds_bytes}
+                                              : e->batch_cfg;
+  if (!batch_config_ok(cfg)) return fail(e, "serve: bad batch configuration");
+  // per-request estimates and the slot arenas (result rows and touched bits)
+  std::vector<QueryEst> est(n);
+  int gmax = 1;
+  for (int i = 0; i < n; i++) {
+    const int rc = estimate_query(e, qs[i], est[i]);
+    if (rc) return rc;
+    gmax = std::max(gmax, est[i].G);
+  }
+  SlotArenas ar;
+  ar.res_stride = ((6 * (long long)gmax) + 3) & ~3ll;
+  ar.touch_stride = (((gmax + 31) / 32) + 3) & ~3ll;
+  HIP_OK(e->d_serve_res.reserve((size_t)nslot * ar.res_stride * sizeof(int)));
+  HIP_OK(e->d_serve_touch.reserve((size_t)nslot * ar.touch_stride * sizeof(unsigned)));
+  ar.res = static_cast<int*>(e->d_serve_res.p);
+  ar.touch = static_cast<unsigned*>(e->d_serve_touch.p);
+  auto state_bytes = [&](int i) { return 24.0 * est[i].G + 4.0 * ((est[i].G + 31) / 32); };
+  if (o->learned && !e->cm) {
+    e->cm = mordred_cost_model_create(e->device, 12345);
+    if (!e->cm) return fail(e, "serve: cost model creation failed");
+  }
+  HIP_OK(hipSetDevice(e->device));
+  // the fact table: every request reads the same one
+  long long nrows = 0;
+  int nseg = 0;
+  if (n > 0) {
+    Resolved r0;
+    int rc = resolve(e, &qs[0], r0);
+    if (rc) return rc;
+    nrows = r0.n;
+    nseg = r0.nseg;
+  }
+  struct Slot {
+    int req = -1;
+    int start = 0;     // segment where its scan began
+    int covered = 0;   // segments scanned for it
+  };
+  std::vector<Slot> slots(nslot);
+  std::vector<const mordred_star_query*> qp(nslot, nullptr);
+  int next = 0, done = 0, cur = 0, active = 0;
+  double state = 0;
+  bool changed = true;
+  const int sizes[] = {8, 16, 32, 64, 128, 256, 512, 1024};
+  float feat[MORDRED_COST_FEATURES];
+  long long since_train = 0;
+  while (done < n) {
+    // admission: free slot, arrived request, budget
+    unsigned long long reset = 0;
+    for (int sidx = 0; sidx < nslot && next < n && arrival_ms[next] <= now_ms(); sidx++) {
+      if (slots[sidx].req >= 0) continue;
+      if (o->hbm_budget_bytes > 0 && state + state_bytes(next) > o->hbm_budget_bytes && active > 0) break;
+      slots[sidx] = Slot{next, cur, 0};
+      qp[sidx] = &qs[next];
+      times[next].admit_ms = now_ms();
+      state += state_bytes(next);
+      st.peak_state_bytes = std::max(st.peak_state_bytes, state);
+      reset |= 1ull << sidx;
+      active++;
+      next++;
+      changed = true;
+    }
+    if (active == 0) {
+      if (next < n) wait_until(arrival_ms[next]);
+      continue;
+    }
+    st.max_active_seen = std::max(st.max_active_seen, active);
+    // launch range: from cur, never past the table end or an active query's
+    // starting segment (each query sees every segment exactly once)
+    int limit = nseg;
+    for (const Slot& sl : slots)
+      if (sl.req >= 0 && sl.start > cur) limit = std::min(limit, sl.start);
+    BatchPlan b;
+    int rc = plan_batch(e, qp.data(), nslot, cfg, b, &ar);
+    if (rc) return rc;
+    std::vector<QueryEst> es(nslot);
+    for (int sidx = 0; sidx < nslot; sidx++)
+      if (slots[sidx].req >= 0) es[sidx] = est[slots[sidx].req];
+    b.d.reset = reset;
+    b.d.rebuild = changed ? 1 : 0;
+    int nreset = 0;
+    for (int sidx = 0; sidx < nslot; sidx++) nreset += reset >> sidx & 1ull;
+    auto rows_of = [&](int from, int to) {
+      return std::min<long long>((long long)to << kSegLog, nrows) - ((long long)from << kSegLog);
+    };
+    // launch size: the largest candidate predicted within the target
+    int span = limit - cur;
+    bool used_learned = false;
+    if (o->chunk_segments > 0) {
+      span = std::min(span, o->chunk_segments);
+    } else {
+      mordred_cost_model_stats cs{};
+      const bool trusted = o->learned && e->cm && mordred_cost_model_get_stats(e->cm, &cs) == 0 && cs.trusted;
+      std::vector<float> xs;
+      std::vector<int> cand;
+      for (int sz : sizes) {
+        const int sp = std::min(sz, limit - cur);
+        if (!cand.empty() && sp == cand.back()) break;
+        cand.push_back(sp);
+        launch_features(b, es, rows_of(cur, cur + sp), nreset, cfg, feat);
+        xs.insert(xs.end(), feat, feat + MORDRED_COST_FEATURES);
+      }
+      std::vector<float> pred(cand.size());
+      if (trusted) {
+        rc = mordred_cost_model_predict(e->cm, xs.data(), (int)cand.size(), pred.data());
+        if (rc) return fail(e, std::string("cost model: ") + mordred_cost_model_last_error(e->cm));
+        for (float& p : pred) p = std::exp(p);
+        used_learned = true;
+      } else {
+        for (size_t c = 0; c < cand.size(); c++)
+          pred[c] = (float)analytic_launch_ms(&xs[c * MORDRED_COST_FEATURES], b.d.ncols);
+      }
+      span = cand[0];
+      for (size_t c = 0; c < cand.size(); c++)
+        if (pred[c] <= o->target_chunk_ms) span = cand[c];
+      (used_learned ? st.learned_choices : st.analytic_choices)++;
+    }
+    const int end = cur + span;
+    b.d.seg_begin = cur;
+    b.d.num_tuples = (int)rows_of(cur, end);
+    rc = launch_batch(e, b, cfg, nullptr);
+    if (rc) return rc;
+    HIP_OK(e->h_arena.reserve(64));
+    HIP_OK(hipMemcpyAsync(e->h_arena.p, e->d_status.p, sizeof(unsigned), hipMemcpyDeviceToHost, e->s_main));
+    HIP_OK(hipStreamSynchronize(e->s_main));
+    unsigned collisions = 0;
+    std::memcpy(&collisions, e->h_arena.p, sizeof collisions);
+    if ((rc = batch_status(e, collisions))) return rc;
+    float gms = 0;
+    HIP_OK(hipEventElapsedTime(&gms, e->ev0, e->ev1));
+    st.gpu_ms += gms;
+    st.launches++;
+    changed = false;
+    // learn from the measured launch
+    if (e->cm) {
+      launch_features(b, es, b.d.num_tuples, nreset, cfg, feat);
+      rc = mordred_cost_model_observe(e->cm, feat, std::max(1e-3, (double)gms), analytic_launch_ms(feat, b.d.ncols));
+      if (rc) return fail(e, std::string("cost model: ") + mordred_cost_model_last_error(e->cm));
+      if (++since_train >= 16) {
+        since_train = 0;
+        rc = mordred_cost_model_train(e->cm, 24, 128, 2e-3f, nullptr);
+        if (rc) return fail(e, std::string("cost model: ") + mordred_cost_model_last_error(e->cm));
+      }
+    }
+    // account the launch; retire queries that have seen every segment
+    for (int sidx = 0; sidx < nslot; sidx++) {
+      Slot& sl = slots[sidx];
+      if (sl.req < 0) continue;
+      sl.covered += span;
+      times[sl.req].launches++;
+      if (sl.covered < nseg) continue;
+      const mordred_star_query& q = qs[sl.req];
+      const int G = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
+      HIP_OK(hipMemcpy(res[sl.req], ar.res + sidx * ar.res_stride, 6 * (size_t)G * sizeof(int),
+                       hipMemcpyDeviceToHost));
+      times[sl.req].done_ms = now_ms();
+      state -= state_bytes(sl.req);
+      sl = Slot{};
+      qp[sidx] = nullptr;
+      active--;
+      done++;
+      changed = true;
+    }
+    cur = end >= nseg ? 0 : end;
+  }
+  st.wall_ms = now_ms();
+  if (e->cm) {
+    mordred_cost_model_stats cs{};
+    if (mordred_cost_model_get_stats(e->cm, &cs) == 0) {
+      st.learned_error = cs.ewma_abs_log_error;
+      st.analytic_error = cs.ewma_abs_log_error_analytic;
+      st.learned_trusted = cs.trusted;
+    }
+  }
+  if (stats) *stats = st;
   return 0;
 }
 

@@ -64,6 +64,30 @@ class BatchTuneEntry(C.Structure):
     _fields_ = [("config", BatchConfig), ("gpu_ms", C.c_double)]
 
 
+SERVE_PLAIN, SERVE_CONTINUOUS = 0, 1
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [("mode", C.c_int), ("max_active", C.c_int), ("chunk_segments", C.c_int),
+                ("target_chunk_ms", C.c_double), ("hbm_budget_bytes", C.c_double), ("learned", C.c_int),
+                ("config", BatchConfig)]
+
+
+class ServeTimes(C.Structure):
+    _fields_ = [("arrival_ms", C.c_double), ("admit_ms", C.c_double), ("done_ms", C.c_double),
+                ("launches", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall_ms", C.c_double), ("launches", C.c_longlong), ("gpu_ms", C.c_double),
+                ("max_active_seen", C.c_int), ("learned_choices", C.c_longlong), ("analytic_choices", C.c_longlong),
+                ("learned_error", C.c_double), ("analytic_error", C.c_double), ("learned_trusted", C.c_int),
+                ("peak_state_bytes", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 def _b(s):
     return None if s is None else s.encode()
 
@@ -100,6 +124,9 @@ _SIGS = {
                               C.POINTER(BatchConfig), C.POINTER(BatchStats)], C.c_int),
     "mordred_engine_batch_tune": ([C.c_void_p, C.POINTER(StarQueryC), C.c_int, C.c_int, C.POINTER(BatchConfig),
                                    C.POINTER(BatchTuneEntry), C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "mordred_engine_serve": ([C.c_void_p, C.POINTER(StarQueryC), C.c_int, C.POINTER(C.c_double),
+                              C.POINTER(C.c_void_p), C.POINTER(ServeOpts), C.POINTER(ServeTimes),
+                              C.POINTER(ServeStats)], C.c_int),
 }
 
 
@@ -226,3 +253,23 @@ class Engine:
         self._check(self.lib.mordred_engine_batch_tune(self.h, arr, len(queries), reps, C.byref(best), table, 64,
                                                        C.byref(n)), "batch_tune")
         return best.as_tuple(), [(table[i].config.as_tuple(), table[i].gpu_ms) for i in range(n.value)]
+
+    # ---- request loop
+    def serve(self, queries, arrivals_ms, mode=SERVE_CONTINUOUS, max_active: int = 64, chunk_segments: int = 0,
+              target_chunk_ms: float = 8.0, hbm_budget_bytes: float = 200e9, learned: bool = True, config=None):
+        """Replay requests (query i arriving arrivals_ms[i] after the start) in
+        real time.  Returns (results, per-request times, stats)."""
+        n = len(queries)
+        arr = self._queries(queries) if n else (StarQueryC * 1)()
+        outs = [np.zeros(max(1, q.total_val) * 6, dtype=np.int32) for q in queries]
+        ptrs = (C.c_void_p * max(1, n))(*[o.ctypes.data for o in outs])
+        arrivals = (C.c_double * max(1, n))(*[float(a) for a in arrivals_ms])
+        o = ServeOpts(mode, max_active, chunk_segments, target_chunk_ms, hbm_budget_bytes, int(learned),
+                      BatchConfig(*(config or (0, 0, 0))))
+        times = (ServeTimes * max(1, n))()
+        st = ServeStats()
+        self._check(self.lib.mordred_engine_serve(self.h, arr, n, arrivals, ptrs, C.byref(o), times, C.byref(st)),
+                    "serve")
+        t = [{"arrival_ms": times[i].arrival_ms, "admit_ms": times[i].admit_ms, "done_ms": times[i].done_ms,
+              "launches": times[i].launches} for i in range(n)]
+        return outs, t, st.as_dict()
