@@ -174,10 +174,13 @@ static void* build_compact(Pipe& P, const ProbePlan& plan, int sink, hipStream_t
   hipMemPool_t pool = device_pool(dev);
   if (!pool) return nullptr;
   size_t off[4] = {0, 0, 0, 0}, bytes = 0;
+  bool with_bits[4] = {false, false, false, false};
   for (int k = 0; k < 4; k++) {
     if (!(plan.compact >> k & 1)) continue;
     off[k] = bytes;
+    with_bits[k] = P.len[k] >= kCompactMinLen;
     bytes += 256 + kStatsGroups * sizeof(int4) + ((compact_words_bytes(P.len[k]) + 255) & ~(size_t)255);
+    if (with_bits[k]) bytes += (compact_bits_bytes(P.len[k]) + 255) & ~(size_t)255;
   }
   void* scratch = nullptr;
   if (hipMallocFromPoolAsync(&scratch, bytes, pool, stream) != hipSuccess) {
@@ -195,8 +198,13 @@ static void* build_compact(Pipe& P, const ProbePlan& plan, int sink, hipStream_t
     prep.hdr[nt] = reinterpret_cast<CompactHdr*>(base);
     prep.partial[nt] = reinterpret_cast<int4*>(base + 256);
     prep.words[nt] = reinterpret_cast<uint32_t*>(base + 256 + kStatsGroups * sizeof(int4));
+    prep.bits[nt] = with_bits[k] ? reinterpret_cast<uint32_t*>(
+                                       base + 256 + kStatsGroups * sizeof(int4) +
+                                       ((compact_words_bytes(P.len[k]) + 255) & ~(size_t)255))
+                                 : nullptr;
     P.chdr[k] = prep.hdr[nt];
     P.cwords[k] = prep.words[nt];
+    P.cbits[k] = prep.bits[nt];
     tab_of[nt++] = k;
   }
   (void)tab_of;
@@ -207,7 +215,7 @@ static void* build_compact(Pipe& P, const ProbePlan& plan, int sink, hipStream_t
   prep_pack_kernel<<<dim3(pack_groups, nt), kPrepBlock, 0, stream>>>(prep);
   if (hipGetLastError() != hipSuccess) {
     (void)hipFreeAsync(scratch, stream);
-    for (int k = 0; k < 4; k++) { P.chdr[k] = nullptr; P.cwords[k] = nullptr; }
+    for (int k = 0; k < 4; k++) { P.chdr[k] = nullptr; P.cwords[k] = nullptr; P.cbits[k] = nullptr; }
     return nullptr;
   }
   return scratch;

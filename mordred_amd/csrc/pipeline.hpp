@@ -53,8 +53,10 @@ struct Pipe {
   // nullptr = probe the hash table itself
   const CompactHdr* chdr[4];
   const uint32_t* cwords[4];
+  const uint32_t* cbits[4];  // existence bitmaps of large compact tables, or nullptr
   int lds_k;      // table whose compact words are copied to LDS, or -1
   int lds_max_w;  // ... when its entry width is at most this
+  int ring_off;      // u64 offset of the survivor rings in dynamic LDS
 };
 
 struct Build {
@@ -128,6 +130,21 @@ __device__ __forceinline__ uint32_t pred_mode(const int (&x)[4 * VPT], int mode,
 // code for absent tables and filters is compiled out.
 constexpr int kShapeGeneric = -1;
 
+// Survivor compaction (specialised segment-mode group-by / aggregate shapes
+// that probe two or more tables): after the first probe, each wave appends
+// the surviving rows' remaining keys and aggregate values to a ring in LDS
+// (positions from ballot + mbcnt), and the later probes and the sink run on
+// full rounds of 64 dense entries instead of on every item slot.
+constexpr int kRing = 128;   // entries per wave (a round of 64 plus one item of carry)
+constexpr int kRingFields = 6;  // values of tables 0-3, aggregate 1, aggregate 2
+constexpr bool shape_compacts(int mode, int sink, int shape) {
+  return shape >= 0 && mode == SEG && (sink == GROUPBY_LDS || sink == AGGR) &&
+         ((shape & 1) + (shape >> 1 & 1) + (shape >> 2 & 1) + (shape >> 3 & 1)) >= 2;
+}
+constexpr int shape_first(int shape) {
+  return (shape & 1) ? 0 : (shape & 2) ? 1 : (shape & 4) ? 2 : (shape & 8) ? 3 : -1;
+}
+
 template <int BLOCK, int VPT, int MODE, int SINK, int SHAPE = kShapeGeneric>
 __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
   constexpr bool kGen = SHAPE < 0;
@@ -147,15 +164,23 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
   unsigned long long* gsum = lds;
   uint32_t* touched = reinterpret_cast<uint32_t*>(lds + (SINK == GROUPBY_LDS ? P.total_val : 0));
   uint32_t* ldsw = reinterpret_cast<uint32_t*>(lds + gwords);
+  constexpr bool kCompact = shape_compacts(MODE, SINK, SHAPE);
+  constexpr int kFirst = shape_first(SHAPE);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  int* ring = kCompact ? reinterpret_cast<int*>(lds + P.ring_off) + wave * (kRing * kRingFields) : nullptr;
+  int ring_head = 0, ring_tail = 0;  // wave-uniform
 
-  // per-table compact probe parameters (workgroup-uniform)
+  // per-table compact probe parameters (workgroup-uniform); cb: probe the
+  // existence bitmap first (a large table where fewer than half the slots hit)
   int cw[4], csh[4], cvmin[4];
+  bool cb[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    cw[k] = 0; csh[k] = 0; cvmin[k] = 0;
+    cw[k] = 0; csh[k] = 0; cvmin[k] = 0; cb[k] = false;
     if ((kGen || (SHAPE >> k & 1)) && P.chdr[k]) {
       const CompactHdr h = *P.chdr[k];
       cw[k] = h.w; csh[k] = h.shift; cvmin[k] = h.vmin;
+      cb[k] = P.cbits[k] != nullptr && h.w > 1 && h.any < P.len[k] / 2;
     }
   }
 
@@ -178,6 +203,54 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
 
   long long acc = 0;
   const int ntiles = (P.num_tuples + TILE - 1) / TILE;
+
+  // one round over `count` ring entries from ring_head, one entry per lane:
+  // the remaining probes, then the sink
+  auto ring_round = [&](int count) {
+    if constexpr (kCompact) {
+      __builtin_amdgcn_wave_barrier();
+      const int e = (ring_head + lane) & (kRing - 1);
+      bool hit = lane < count;
+      int v[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (SHAPE >> k & 1) v[k] = ring[k * kRing + e];
+      const int a0 = ring[4 * kRing + e], a1 = ring[5 * kRing + e];
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (!(SHAPE >> k & 1) || k == kFirst) continue;
+        const int hs = hit ? phash(v[k], P.len[k], P.minkey[k]) : 0;
+        if (cw[k] > 0) {
+          const int w = cw[k], sh = csh[k];
+          if (cb[k]) hit = hit && (P.cbits[k][hs >> 5] >> (hs & 31) & 1u);
+          uint32_t word = 0;
+          if (hit) word = (staged && k == P.lds_k) ? ldsw[hs >> sh] : P.cwords[k][hs >> sh];
+          const uint32_t ent = (word >> ((hs & ((1 << sh) - 1)) * w)) & ((1u << w) - 1u);
+          hit = hit && ent != 0u;
+          v[k] = cvmin[k] + (int)ent - 1;
+        } else {
+          const unsigned long long sv = *reinterpret_cast<const unsigned long long*>(P.ht[k] + 2 * (size_t)hs);
+          hit = hit && sv != 0ull;
+          v[k] = (int)(unsigned)sv;
+        }
+      }
+      if (SINK == AGGR) {
+        if (hit) acc += (long long)(a0 * a1);  // int32 product (GPUProcessing.h:1763)
+      } else if (hit) {
+        const int g = mod_pos((v[0] - P.minv[0]) * P.uniq[0] + (v[1] - P.minv[1]) * P.uniq[1] +
+                                  (v[2] - P.minv[2]) * P.uniq[2] + (v[3] - P.minv[3]) * P.uniq[3],
+                              P.total_val);
+        atomicAdd(&gsum[g], (unsigned long long)(long long)(a0 - a1));
+        const uint32_t bit = 1u << (g & 31);
+        if (!(atomicOr(&touched[g >> 5], bit) & bit)) {
+          int2* key = reinterpret_cast<int2*>(P.res + 6 * (size_t)g);
+          key[0] = make_int2(v[0], v[1]);
+          key[1] = make_int2(v[2], v[3]);
+        }
+      }
+    }
+  };
 
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int start = tile * TILE;
@@ -256,6 +329,7 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
     // ---------------------------------------------------------- probes
 #pragma unroll
     for (int k = 0; k < 4; k++) {
+      if (kCompact && k != kFirst) continue;
       const bool probed = kGen ? ((P.pmask >> k & 1) && P.key[k] && P.ht[k]) : (SHAPE >> k & 1);
       if (!probed) {
         const int dflt = SINK == COMPACT ? 1 : 0;  // BlockSetValue(...,1) / (...,0)
@@ -340,6 +414,30 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
       sel = act;
     }
 
+    if constexpr (kCompact) {
+      // append this tile's survivors to the wave's ring; run full rounds
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        const unsigned long long b = __ballot(sel >> j & 1u);
+        if (b == 0ull) continue;
+        if (sel >> j & 1u) {
+          const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+          const int e = (ring_tail + pos) & (kRing - 1);
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+            if (SHAPE >> k & 1) ring[k * kRing + e] = kv[k][j];
+          ring[4 * kRing + e] = P.aggr[0] ? av[0][j] : 0;
+          ring[5 * kRing + e] = P.aggr[1] ? av[1][j] : 0;
+        }
+        ring_tail += __popcll(b);
+        if (ring_tail - ring_head >= kWave) {
+          ring_round(kWave);
+          ring_head += kWave;
+        }
+      }
+      continue;
+    }
+
     // ---------------------------------------------------------- sinks
     if (SINK == COMPACT) {
       int pos = block_reserve<BLOCK>(__popc(sel), P.total, scan);
@@ -415,6 +513,9 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
     }
   }
 
+  if constexpr (kCompact) {
+    if (ring_tail > ring_head) ring_round(ring_tail - ring_head);
+  }
   if (SINK == GROUPBY_LDS) {
     __syncthreads();
     for (int h = threadIdx.x; h < P.total_val; h += BLOCK)
@@ -528,9 +629,14 @@ inline size_t pipe_lds_bytes(const Pipe& P, int sink) {
 }
 
 template <int BLOCK, int VPT, int MODE, int SINK, int SHAPE = kShapeGeneric>
-int launch_pipe(const Pipe& P, hipStream_t stream) {
+int launch_pipe(const Pipe& Pin, hipStream_t stream) {
   constexpr int TILE = BLOCK * 4 * VPT;
-  const size_t lds = pipe_lds_bytes(P, SINK);
+  Pipe P = Pin;
+  size_t lds = pipe_lds_bytes(P, SINK);
+  if (shape_compacts(MODE, SINK, SHAPE)) {
+    P.ring_off = (int)(lds / sizeof(unsigned long long));
+    lds += (size_t)(BLOCK / kWave) * kRing * kRingFields * sizeof(int);
+  }
   const void* f = reinterpret_cast<const void*>(&pipeline_kernel<BLOCK, VPT, MODE, SINK, SHAPE>);
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -15,9 +15,14 @@
 // by it) has a constant result and compacts to one bit per slot.
 //
 // The caller's hash tables are only read.  Two launches per query: `stats`
-// writes a (min, max, any) partial per workgroup, `pack` reduces the partials
+// writes a (min, max, hits) partial per workgroup, `pack` reduces the partials
 // (every workgroup redundantly, so no third launch or pre-zeroed memory is
 // needed), writes the header and the packed words.
+//
+// A large table also gets an existence bitmap (one bit per slot).  When few
+// slots hit, a probe reads the bitmap first (len/8 bytes, which stays in L2
+// while the fact columns stream through) and fetches the wider entry only on
+// a hit, instead of pulling a cache line of entries from MALL for every probe.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,7 +36,7 @@ struct CompactHdr {
   int w;      // entry width in bits; 0 = not compacted
   int shift;  // log2(entries per 32-bit word)
   int vmin;
-  int any;    // number of hit slots > 0
+  int any;    // number of hit slots
 };
 
 constexpr int kPrepMaxTables = 4;
@@ -45,6 +50,7 @@ struct Prep {
   int4* partial[kPrepMaxTables];    // kStatsGroups x (min, max, any, -)
   CompactHdr* hdr[kPrepMaxTables];
   uint32_t* words[kPrepMaxTables];
+  uint32_t* bits[kPrepMaxTables];   // existence bitmap, or nullptr
 };
 
 __device__ __forceinline__ bool slot_result(const int* ht, int s, int row_probe, int& r) {
@@ -74,7 +80,7 @@ __global__ __launch_bounds__(kPrepBlock) void prep_stats_kernel(Prep P) {
     if (slot_result(ht, s, P.row_probe, r)) {
       mn = min(mn, r);
       mx = max(mx, r);
-      any = 1;
+      any++;
     }
   }
   __shared__ int smn[kPrepBlock / kWave], smx[kPrepBlock / kWave], sany[kPrepBlock / kWave];
@@ -82,7 +88,7 @@ __global__ __launch_bounds__(kPrepBlock) void prep_stats_kernel(Prep P) {
   for (int d = kWave / 2; d > 0; d >>= 1) {
     mn = min(mn, __shfl_xor(mn, d, kWave));
     mx = max(mx, __shfl_xor(mx, d, kWave));
-    any |= __shfl_xor(any, d, kWave);
+    any += __shfl_xor(any, d, kWave);
   }
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   if (lane == 0) { smn[wave] = mn; smx[wave] = mx; sany[wave] = any; }
@@ -91,7 +97,7 @@ __global__ __launch_bounds__(kPrepBlock) void prep_stats_kernel(Prep P) {
     for (int w = 1; w < kPrepBlock / kWave; w++) {
       mn = min(mn, smn[w]);
       mx = max(mx, smx[w]);
-      any |= sany[w];
+      any += sany[w];
     }
     P.partial[t][blockIdx.x] = make_int4(mn, mx, any, 0);
   }
@@ -121,21 +127,24 @@ __global__ __launch_bounds__(kPrepBlock) void prep_pack_kernel(Prep P) {
     const int4 p = P.partial[t][i];
     mn = min(mn, p.x);
     mx = max(mx, p.y);
-    any |= p.z;
+    any += p.z;
   }
 #pragma unroll
   for (int d = kWave / 2; d > 0; d >>= 1) {
     mn = min(mn, __shfl_xor(mn, d, kWave));
     mx = max(mx, __shfl_xor(mx, d, kWave));
-    any |= __shfl_xor(any, d, kWave);
+    any += __shfl_xor(any, d, kWave);
   }
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   if (lane == 0) red[wave] = make_int4(mn, mx, any, 0);
   __syncthreads();
+  mn = 0x7fffffff;
+  mx = (int)0x80000000;
+  any = 0;
   for (int i = 0; i < kPrepBlock / kWave; i++) {
     mn = min(mn, red[i].x);
     mx = max(mx, red[i].y);
-    any |= red[i].z;
+    any += red[i].z;
   }
   int w, shift;
   choose_width(mn, mx, any, w, shift);
@@ -144,6 +153,18 @@ __global__ __launch_bounds__(kPrepBlock) void prep_pack_kernel(Prep P) {
     CompactHdr h;
     h.w = w; h.shift = shift; h.vmin = vmin; h.any = any;
     *P.hdr[t] = h;
+  }
+  if (P.bits[t]) {
+    const int nb = (len + 31) >> 5;
+    for (int bi = blockIdx.x * kPrepBlock + threadIdx.x; bi < nb; bi += gridDim.x * kPrepBlock) {
+      uint32_t word = 0;
+      for (int e = 0; e < 32; e++) {
+        const int s = (bi << 5) + e;
+        int r;
+        if (s < len && slot_result(P.ht[t], s, P.row_probe, r)) word |= 1u << e;
+      }
+      P.bits[t][bi] = word;
+    }
   }
   if (w == 0) return;
   const int epw = 1 << shift;
@@ -166,5 +187,7 @@ __global__ __launch_bounds__(kPrepBlock) void prep_pack_kernel(Prep P) {
 
 // Bytes of the packed words at the widest compact width (16 bits per slot).
 inline size_t compact_words_bytes(int len) { return (((size_t)len + 1) / 2) * sizeof(uint32_t); }
+// Bytes of an existence bitmap.
+inline size_t compact_bits_bytes(int len) { return (((size_t)len + 31) / 32) * sizeof(uint32_t); }
 
 }  // namespace mordred
