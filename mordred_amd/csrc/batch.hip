@@ -1,10 +1,11 @@
 // Shared-scan batch kernels for gfx950 (see batch.hpp for the record table
 // and the batch descriptor).
 //
-//   batch_fill_kernel     clears the record tables of the used dimensions, the
-//                         result arena and the global touched bitmaps;
-//   batch_records_kernel  one pass over a dimension: slot, query mask, packed
-//                         group attributes (the 64 builds of the batch in one);
+//   batch_fill_kernel     clears the packed tables of the used dimensions (when
+//                         rebuilt), and the result rows and global touched
+//                         bits of the queries being started;
+//   batch_records_kernel  one pass over a dimension: slot and packed entry
+//                         (the builds of all the batch's queries in one);
 //   batch_kernel          the persistent shared scan of the fact columns.
 #include <hip/hip_runtime.h>
 
@@ -16,19 +17,8 @@
 
 namespace mordred {
 
-namespace {
 
-__device__ __forceinline__ bool dim_filter_pass(int mode, int x, int c1, int c2) {
-  // build_GPU2 predicate modes (GPUProcessing.h:972-980)
-  if (mode == 0) return x == c1;
-  if (mode == 1) return x >= c1 && x <= c2;
-  if (mode == 2) return x == c1 || x == c2;
-  return true;
-}
-
-}  // namespace
-
-// Clears the record tables of the used dimensions when they are rebuilt, and
+// Clears the packed tables of the used dimensions when they are rebuilt, and
 // the result rows and global touched bits of the queries being (re)started.
 __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __restrict__ D, unsigned* status,
                                                          uint4* r0, uint4* r1, uint4* r2, uint4* r3,
@@ -62,41 +52,26 @@ __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __rest
   }
 }
 
-// One thread per dimension row: its slot, the mask of queries whose filter it
-// passes, its packed group attributes.  Filters are sorted by column, so each
-// distinct filter column is read once per row.
+// One thread per dimension row: its slot and its packed entry (present bit
+// plus the attributes the batch filters or groups on).  The host has checked
+// that no two rows share a slot.
 __global__ __launch_bounds__(256) void batch_records_kernel(BatchLaunch L, int d) {
   const BatchDim& B = L.desc->dim[d];
   if (!B.used || !L.desc->rebuild) return;
-  const int nrows = B.nrows, len = B.len, mk = B.minkey, nf = B.nfilt;
+  const int nrows = B.nrows, len = B.len, mk = B.minkey, pw = B.pw, na = B.nattr;
   const int stride = gridDim.x * blockDim.x;
   for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < nrows; row += stride) {
-    const int key = gather(L.cache, B.key, row);
-    const int h = phash(key, len, mk);
+    const int h = phash(gather(L.cache, B.key, row), len, mk);
     if ((unsigned)h >= (unsigned)len) continue;  // key below min: never probed in range
-    unsigned long long mask = 0;
-    const int* prev = nullptr;
-    int x = 0;
-    for (int f = 0; f < nf; f++) {
-      const BatchFilt F = B.filt[f];
-      bool pass = true;
-      if (F.col) {
-        if (F.col != prev) {
-          x = gather(L.cache, F.col, row);
-          prev = F.col;
-        }
-        pass = dim_filter_pass(F.mode, x, F.c1, F.c2);
-      }
-      if (pass) mask |= 1ull << F.q;
+    uint32_t w = 1u << (8 * pw - 1);
+    for (int a = 0; a < na; a++) {
+      const BatchAttr A = B.attr[a];
+      const uint32_t v = (uint32_t)(gather(L.cache, A.col, row) - A.base);
+      w |= (v & ((1u << A.bits) - 1u)) << A.off;
     }
-    uint32_t packed = 0;
-    if (B.attr[0]) packed |= (uint32_t)(gather(L.cache, B.attr[0], row) - B.attr_base[0]) & 0xFFFFu;
-    if (B.attr[1]) packed |= ((uint32_t)(gather(L.cache, B.attr[1], row) - B.attr_base[1]) & 0xFFFFu) << 16;
-    uint4* r = B.rec + h;
-    if (atomicCAS(&r->w, 0u, 1u) != 0u) atomicAdd(L.status, 1u);
-    r->x = (uint32_t)mask;
-    r->y = (uint32_t)(mask >> 32);
-    r->z = packed;
+    if (pw == 1) static_cast<uint8_t*>(B.pk)[h] = (uint8_t)w;
+    else if (pw == 2) static_cast<uint16_t*>(B.pk)[h] = (uint16_t)w;
+    else static_cast<uint32_t*>(B.pk)[h] = w;
   }
 }
 
@@ -113,7 +88,8 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   __shared__ BatchHot H;
   const int nq = D->nq;
   BatchQuery* lq = reinterpret_cast<BatchQuery*>(lds_raw);
-  unsigned long long* lsum = reinterpret_cast<unsigned long long*>(lds_raw + nq * sizeof(BatchQuery));
+  unsigned long long* llut = reinterpret_cast<unsigned long long*>(lds_raw + nq * sizeof(BatchQuery));
+  unsigned long long* lsum = llut + D->nlut;
   const int nsum = D->lds_sum_words;
   uint32_t* ltouch = reinterpret_cast<uint32_t*>(lsum + nsum);
   const int ntouch = D->lds_touch_words;
@@ -136,11 +112,28 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     }
     if (threadIdx.x < 4) {
       const int d = threadIdx.x;
-      H.rec[d] = D->dim[d].rec;
-      H.uses[d] = D->dim[d].uses;
-      H.len[d] = D->dim[d].len;
-      H.minkey[d] = D->dim[d].minkey;
+      const BatchDim& B = D->dim[d];
+      BatchHotDim& Hd = H.dim[d];
+      Hd.pk = B.pk;
+      Hd.uses = B.uses;
+      Hd.len = B.len;
+      Hd.minkey = B.minkey;
+      Hd.pw = B.pw;
+      Hd.nattr = B.nattr;
+      Hd.ncmp = B.ncmp;
+      for (int a = 0; a < kBatchMaxAttr; a++) {
+        Hd.off[a] = B.attr[a].off;
+        Hd.bits[a] = B.attr[a].bits;
+        Hd.lut[a] = B.attr[a].lut;
+        Hd.base[a] = B.attr[a].base;
+      }
     }
+    for (int i = threadIdx.x; i < 4 * kBatchMaxCmp; i += BLOCK) {
+      const BatchCmp C = D->dim[i / kBatchMaxCmp].cmp[i % kBatchMaxCmp];
+      H.cmp[i / kBatchMaxCmp][i % kBatchMaxCmp] = make_int4(C.src, C.lo, (int)((unsigned)C.hi - (unsigned)C.lo), C.q);
+    }
+    const int nl = D->nlut;
+    for (int i = threadIdx.x; i < nl; i += BLOCK) llut[i] = D->lut[i];
     if (threadIdx.x == 0) {
       int cm = 0, um = 0;
       for (int c = 0; c < D->ncols; c++)
@@ -153,6 +146,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       H.nexpr = D->nexpr;
       H.npred = np;
       H.nq = nq;
+      H.nlut = nl;
     }
   }
   __syncthreads();
@@ -162,12 +156,6 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   const int nexpr = __builtin_amdgcn_readfirstlane(H.nexpr);
   const unsigned long long all = D->all;
   unsigned* const touch_arena = D->touch_arena;
-  int ab[4][2];
-#pragma unroll
-  for (int d = 0; d < 4; d++) {
-    ab[d][0] = __builtin_amdgcn_readfirstlane(D->dim[d].attr_base[0]);
-    ab[d][1] = __builtin_amdgcn_readfirstlane(D->dim[d].attr_base[1]);
-  }
   const int lane = threadIdx.x & (kWave - 1);
 
   const int num_tuples = D->num_tuples;
@@ -226,11 +214,16 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     const int pos = t * TILE;
     const int n = min(TILE, num_tuples - pos);
     const uint32_t inr = n == TILE ? (IPT == 32 ? 0xFFFFFFFFu : (1u << IPT) - 1u) : range_mask<BLOCK, VPT>(n);
-    unsigned long long alive[IPT];
+    // alive queries of each item, as two 32-bit halves
+    uint32_t alo[IPT], ahi[IPT];
 #pragma unroll
-    for (int j = 0; j < IPT; j++) alive[j] = (inr >> j & 1u) ? all : 0ull;
+    for (int j = 0; j < IPT; j++) {
+      alo[j] = (inr >> j & 1u) ? (uint32_t)all : 0u;
+      ahi[j] = (inr >> j & 1u) ? (uint32_t)(all >> 32) : 0u;
+    }
 
-    // fact BETWEEN predicates, grouped by column (GPUProcessing.h:72-86)
+    // fact BETWEEN predicates, grouped by column (GPUProcessing.h:72-86); the
+    // half a predicate clears is chosen by a wave-uniform branch
 #pragma unroll
     for (int c = 0; c < MAXC; c++) {
       if (!(colmask >> c & 1)) continue;
@@ -240,25 +233,38 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         const int4 P = H.pred[p];
         const int lo = __builtin_amdgcn_readfirstlane(P.x);
         const unsigned span = (unsigned)__builtin_amdgcn_readfirstlane(P.y);
-        const unsigned long long keep = ~(1ull << __builtin_amdgcn_readfirstlane(P.z));
+        const int q = __builtin_amdgcn_readfirstlane(P.z);
+        const uint32_t keep = ~(1u << (q & 31));
+        if (q < 32) {
 #pragma unroll
-        for (int j = 0; j < IPT; j++)
-          if ((unsigned)(x[c][j] - lo) > span) alive[j] &= keep;
+          for (int j = 0; j < IPT; j++)
+            if ((unsigned)(x[c][j] - lo) > span) alo[j] &= keep;
+        } else {
+#pragma unroll
+          for (int j = 0; j < IPT; j++)
+            if ((unsigned)(x[c][j] - lo) > span) ahi[j] &= keep;
+        }
       }
     }
+    unsigned long long alive[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) alive[j] = (unsigned long long)ahi[j] << 32 | alo[j];
 
-    // one probe per dimension for all queries: the record gathers of the tile
-    // are issued before any is used; items no query needs read slot 0
+    // one probe per dimension for all queries: the packed entries of the tile
+    // are gathered before any is used (items no query needs read slot 0);
+    // the filter masks come from the attribute mask tables and range compares
     uint32_t attrw[4][IPT];
 #pragma unroll
     for (int d = 0; d < 4; d++) {
 #pragma unroll
       for (int j = 0; j < IPT; j++) attrw[d][j] = 0;
       if (!(usedmask >> d & 1)) continue;
-      const uint4* __restrict__ rec = H.rec[d];
-      const int len = __builtin_amdgcn_readfirstlane(H.len[d]);
-      const int mk = __builtin_amdgcn_readfirstlane(H.minkey[d]);
-      const unsigned long long uses = H.uses[d];
+      const BatchHotDim& Hd = H.dim[d];
+      const int len = __builtin_amdgcn_readfirstlane(Hd.len);
+      const int mk = __builtin_amdgcn_readfirstlane(Hd.minkey);
+      const int pw = __builtin_amdgcn_readfirstlane(Hd.pw);
+      const unsigned long long uses = Hd.uses;
+      const void* pk = Hd.pk;
       int hs[IPT];
       bool inb[IPT];
 #pragma unroll
@@ -267,14 +273,73 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         inb[j] = (unsigned)h < (unsigned)len;
         hs[j] = (inb[j] && (alive[j] & uses)) ? h : 0;
       }
-      uint4 r[IPT];
+      uint32_t w[IPT];
+      if (pw == 1) {
 #pragma unroll
-      for (int j = 0; j < IPT; j++) r[j] = rec[hs[j]];
+        for (int j = 0; j < IPT; j++) w[j] = static_cast<const uint8_t*>(pk)[hs[j]];
+      } else if (pw == 2) {
+#pragma unroll
+        for (int j = 0; j < IPT; j++) w[j] = static_cast<const uint16_t*>(pk)[hs[j]];
+      } else {
+#pragma unroll
+        for (int j = 0; j < IPT; j++) w[j] = static_cast<const uint32_t*>(pk)[hs[j]];
+      }
+      const int top = 8 * pw - 1;
+      uint32_t mlo[IPT], mhi[IPT];
 #pragma unroll
       for (int j = 0; j < IPT; j++) {
-        const unsigned long long m = inb[j] ? ((unsigned long long)r[j].y << 32 | r[j].x) : 0ull;
+        if (!inb[j]) w[j] = 0;
+        mlo[j] = 0xFFFFFFFFu;
+        mhi[j] = 0xFFFFFFFFu;
+      }
+      const int na = __builtin_amdgcn_readfirstlane(Hd.nattr);
+      for (int a = 0; a < na; a++) {
+        const int lut = __builtin_amdgcn_readfirstlane(Hd.lut[a]);
+        if (lut < 0) continue;
+        const int off = __builtin_amdgcn_readfirstlane(Hd.off[a]);
+        const uint32_t vm = (1u << __builtin_amdgcn_readfirstlane(Hd.bits[a])) - 1u;
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+          const unsigned long long m = llut[lut + ((w[j] >> off) & vm)];
+          mlo[j] &= (uint32_t)m;
+          mhi[j] &= (uint32_t)(m >> 32);
+        }
+      }
+      const int nc = __builtin_amdgcn_readfirstlane(Hd.ncmp);
+      for (int c = 0; c < nc; c++) {
+        const int4 C = H.cmp[d][c];
+        const int src = __builtin_amdgcn_readfirstlane(C.x);
+        const int lo = __builtin_amdgcn_readfirstlane(C.y);
+        const unsigned span = (unsigned)__builtin_amdgcn_readfirstlane(C.z);
+        const int q = __builtin_amdgcn_readfirstlane(C.w);
+        int v[IPT];
+        if (src < 0) {
+#pragma unroll
+          for (int j = 0; j < IPT; j++) v[j] = x[d][j];
+        } else {
+          const int off = __builtin_amdgcn_readfirstlane(Hd.off[src]);
+          const uint32_t vm = (1u << __builtin_amdgcn_readfirstlane(Hd.bits[src])) - 1u;
+          const int base = __builtin_amdgcn_readfirstlane(Hd.base[src]);
+#pragma unroll
+          for (int j = 0; j < IPT; j++) v[j] = base + (int)((w[j] >> off) & vm);
+        }
+        const uint32_t keep = ~(1u << (q & 31));
+        if (q < 32) {
+#pragma unroll
+          for (int j = 0; j < IPT; j++)
+            if ((unsigned)(v[j] - lo) > span) mlo[j] &= keep;
+        } else {
+#pragma unroll
+          for (int j = 0; j < IPT; j++)
+            if ((unsigned)(v[j] - lo) > span) mhi[j] &= keep;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        const bool present = (w[j] >> top) & 1u;
+        const unsigned long long m = present ? ((unsigned long long)mhi[j] << 32 | mlo[j]) : 0ull;
         alive[j] &= m | ~uses;
-        attrw[d][j] = r[j].z;
+        attrw[d][j] = w[j];
       }
     }
 
@@ -289,16 +354,24 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       const int kind = __builtin_amdgcn_readfirstlane(E.x);
       const int a1 = __builtin_amdgcn_readfirstlane(E.y);
       const int a2 = __builtin_amdgcn_readfirstlane(E.z);
+      int v1[IPT], v2[IPT];
 #pragma unroll
-      for (int j = 0; j < IPT; j++) {
-        int v1 = 0, v2 = 0;
+      for (int j = 0; j < IPT; j++) v1[j] = v2[j] = 0;
 #pragma unroll
-        for (int c = 0; c < MAXC; c++) {
-          if (c == a1) v1 = x[c][j];
-          if (c == a2) v2 = x[c][j];
+      for (int c = 0; c < MAXC; c++) {
+        if (c == a1) {
+#pragma unroll
+          for (int j = 0; j < IPT; j++) v1[j] = x[c][j];
         }
-        ev[e][j] = kind == kExprProduct ? (int)((unsigned)v1 * (unsigned)v2) : (int)((unsigned)v1 - (unsigned)v2);
+        if (c == a2) {
+#pragma unroll
+          for (int j = 0; j < IPT; j++) v2[j] = x[c][j];
+        }
       }
+#pragma unroll
+      for (int j = 0; j < IPT; j++)
+        ev[e][j] = kind == kExprProduct ? (int)((unsigned)v1[j] * (unsigned)v2[j])
+                                        : (int)((unsigned)v1[j] - (unsigned)v2[j]);
     }
 
     // every surviving (row, query) pair
@@ -309,13 +382,10 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         const int q = __builtin_ctzll(m);
         m &= m - 1;
         const BatchQuery& Q = lq[q];
-        const int gv0 = Q.attr[0] < 0 ? 0 : (Q.attr[0] ? ab[0][1] : ab[0][0]) + (int)((attrw[0][j] >> (16 * Q.attr[0])) & 0xFFFFu);
-        const int gv1 = Q.attr[1] < 0 ? 0 : (Q.attr[1] ? ab[1][1] : ab[1][0]) + (int)((attrw[1][j] >> (16 * Q.attr[1])) & 0xFFFFu);
-        const int gv2 = Q.attr[2] < 0 ? 0 : (Q.attr[2] ? ab[2][1] : ab[2][0]) + (int)((attrw[2][j] >> (16 * Q.attr[2])) & 0xFFFFu);
-        const int gv3 = Q.attr[3] < 0 ? 0 : (Q.attr[3] ? ab[3][1] : ab[3][0]) + (int)((attrw[3][j] >> (16 * Q.attr[3])) & 0xFFFFu);
-        // dense group index (GPUProcessing.h:142)
-        const int h = mod_pos((gv0 - Q.minv[0]) * Q.uniq[0] + (gv1 - Q.minv[1]) * Q.uniq[1] +
-                                  (gv2 - Q.minv[2]) * Q.uniq[2] + (gv3 - Q.minv[3]) * Q.uniq[3],
+        const int h = mod_pos(Q.hbase + (int)((attrw[0][j] >> Q.shift[0]) & ((1u << Q.bits[0]) - 1u)) * Q.uniq[0] +
+                                  (int)((attrw[1][j] >> Q.shift[1]) & ((1u << Q.bits[1]) - 1u)) * Q.uniq[1] +
+                                  (int)((attrw[2][j] >> Q.shift[2]) & ((1u << Q.bits[2]) - 1u)) * Q.uniq[2] +
+                                  (int)((attrw[3][j] >> Q.shift[3]) & ((1u << Q.bits[3]) - 1u)) * Q.uniq[3],
                               Q.total_val);
         // an index outside [0, total_val) (group values below min_val) would
         // write outside the query's result; the reference does, here the pair
@@ -342,9 +412,14 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         if (first) {
           // the reference stores the keys for every row of the group
           // (GPUProcessing.h:144-147); they are equal, so one store suffices
+          int gv[4];
+#pragma unroll
+          for (int d = 0; d < 4; d++)
+            gv[d] = Q.attr[d] < 0 ? 0
+                                  : H.dim[d].base[Q.attr[d]] + (int)((attrw[d][j] >> Q.shift[d]) & ((1u << Q.bits[d]) - 1u));
           int2* key = reinterpret_cast<int2*>(res);
-          key[0] = make_int2(gv0, gv1);
-          key[1] = make_int2(gv2, gv3);
+          key[0] = make_int2(gv[0], gv[1]);
+          key[1] = make_int2(gv[2], gv[3]);
         }
       }
     }

@@ -2,25 +2,30 @@
 //
 // One persistent kernel scans the fact columns once for the whole batch.  Per
 // fact row it evaluates every query's fact predicates as bits of a 64-bit
-// "alive" word, probes each joined dimension ONCE through a per-dimension
-// record table, and accumulates every (row, query) pair that survives.
+// "alive" word, probes each joined dimension ONCE through a packed attribute
+// table, and accumulates every (row, query) pair that survives.
 //
-// Record table of dimension d (one 16-byte record per hash slot, HASH =
-// (key-min) % len, the slot of the reference's perfect-hash table,
-// crystal/join.cuh:3):
-//   x, y  64-bit mask: bit q set iff query q joins d and the dimension row in
-//         this slot passes q's dimension filter (the row query q's own
-//         build_GPU2 would have inserted, GPUProcessing.h:927-997);
-//   z     two packed 16-bit group attributes of the row (value - base);
-//   w     1 when a row occupies the slot (collision check).
-// A fact row survives for query q iff every table q joins has bit q set in the
-// probed slot -- exactly the rows q's own fused kernel keeps -- and its group
-// value for table d is the attribute q groups on (the hash-table value word
-// of q's own build), so every query's result equals its plain run bit for bit.
+// Packed table of dimension d (one 1-, 2- or 4-byte entry per hash slot, HASH
+// = (key-min) % len, the slot of the reference's perfect-hash table,
+// crystal/join.cuh:3): the top bit says a dimension row occupies the slot,
+// the other bits hold the row's attributes that the batch filters or groups
+// on (value - base).  The mask of queries whose dimension filter the row
+// passes is rebuilt per fact row:
+//   * a filter on an attribute with a small value span: a 64-bit mask per
+//     attribute value (bit q = query q passes, or does not filter on it),
+//     staged in LDS and AND-ed over the attributes;
+//   * a filter on the dimension key or on a wide attribute: a range compare.
+// A fact row survives for query q iff every table q joins holds a row in the
+// probed slot that passes q's filter -- exactly the rows q's own build_GPU2
+// inserts (GPUProcessing.h:927-997) -- and its group value for table d is the
+// attribute q groups on (the value word of q's own hash table), so every
+// query's result equals its plain run bit for bit.  The host checks that keys
+// cannot collide (dimension and fact key ranges within dim_len), which makes
+// the dimension key of a probed row the fact row's key.
 //
 // Everything that changes between batches lives in a device BatchDesc, so
-// one captured hipGraph (fill, record builds, scan) serves every batch of the
-// same dataset and launch configuration.
+// one captured hipGraph (fill, packed-table builds, scan) serves every batch
+// of the same dataset and launch configuration.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -32,6 +37,10 @@ constexpr int kBatchMaxQ = 64;
 constexpr int kBatchMaxCols = 12;   // fact columns; slots 0..3 are the dimension keys
 constexpr int kBatchMaxPred = 2 * kBatchMaxQ;
 constexpr int kBatchMaxExpr = 4;
+constexpr int kBatchMaxAttr = 3;    // packed attributes per dimension
+constexpr int kBatchMaxCmp = 64;    // range-compare filters per dimension (one per query)
+constexpr int kBatchMaxLut = 4096;  // 64-bit mask entries staged in LDS (32 KiB)
+constexpr int kBatchLutSpan = 2048; // widest attribute span given a mask table
 
 enum BatchExprKind { kExprDiff = 0, kExprProduct = 1 };  // a1 - a2 (group-by), a1 * a2 (aggregate)
 enum BatchQueryFlags { kQueryAggr = 1 };
@@ -46,8 +55,16 @@ struct BatchQuery {            // 64 bytes; staged in LDS by the scan kernel
   int gtouch;                  // u32 offset of its touched bitmap in the global arena,
                                // valid whatever the placement (a query of a request
                                // loop may change placement between launches)
-  signed char attr[4];         // per dimension slot: -1 no group value, else attribute 0 / 1
-  int minv[4];
+  signed char attr[4];         // per dimension slot: -1 no group value, else its attribute
+  // dense group index (GPUProcessing.h:142) as hbase + sum_d field_d * uniq[d]
+  // with field_d = (packed entry >> shift[d]) & ((1 << bits[d]) - 1) and hbase
+  // = sum over grouped dimensions of (attribute base - min_val) * unique_val;
+  // bits[d] = uniq[d] = 0 for a dimension without group value (identical in
+  // 32-bit wrap-around)
+  int hbase;
+  unsigned char shift[4];
+  unsigned char bits[4];
+  int pad;
   int uniq[4];
 };
 static_assert(sizeof(BatchQuery) == 64, "BatchQuery is staged as 4 x 16 bytes");
@@ -60,20 +77,28 @@ struct BatchExpr {
   int kind, a1, a2, pad;       // a2 < 0: absent (0)
 };
 
-struct BatchFilt {             // dimension filter of one query (build_GPU2 modes)
-  const int* col;              // segment table of the filter column, NULL = pass
-  int mode, c1, c2, q;         // mode 0 EQ, 1 BETWEEN, 2 EQ-or-EQ
+struct BatchAttr {             // one packed attribute of a dimension
+  const int* col;              // segment table of the dimension column
+  int base;                    // smallest value
+  int off, bits;               // position in the packed entry
+  int lut;                     // offset of its mask table (one entry per value) or -1
+  int span;                    // values base .. base + span - 1
+};
+
+struct BatchCmp {              // range filter of one query, evaluated per row:
+  int src;                     // -1 = the key, else attribute index
+  int lo, hi, q;               // src value in [lo, hi]
 };
 
 struct BatchDim {
   const int* key;              // segment table of the dimension key column
-  const int* attr[2];          // segment tables of the packed attributes (or NULL)
-  int attr_base[2];
-  uint4* rec;                  // record table, len entries
+  void* pk;                    // packed table, len entries of pw bytes
   unsigned long long uses;     // queries that probe this dimension
   int used, len, minkey, nrows;
-  int nfilt, pad;
-  BatchFilt filt[kBatchMaxQ];  // sorted by column
+  int pw;                      // bytes per packed entry: 1, 2 or 4
+  int nattr, ncmp, pad;
+  BatchAttr attr[kBatchMaxAttr];
+  BatchCmp cmp[kBatchMaxCmp];
 };
 
 struct BatchDesc {
@@ -97,6 +122,8 @@ struct BatchDesc {
   BatchExpr expr[kBatchMaxExpr];
   BatchDim dim[4];
   BatchQuery q[kBatchMaxQ];
+  int nlut, pad1;
+  unsigned long long lut[kBatchMaxLut];  // attribute mask tables
 };
 
 struct BatchLaunch {           // kernel arguments, fixed for a captured graph
@@ -104,20 +131,26 @@ struct BatchLaunch {           // kernel arguments, fixed for a captured graph
   const int* cache;
   const short* seg_group;
   int max_tuples;              // largest num_tuples of any launch (sizes the grid)
-  unsigned* status;            // [0] = hash-slot collisions in the record builds
+  unsigned* status;            // reserved status words (cleared by the fill)
 };
 
 // Hot part of the descriptor, staged in LDS once per workgroup so the tile
 // loop reads it with broadcast LDS loads instead of dependent global loads.
+struct BatchHotDim {
+  const void* pk;
+  unsigned long long uses;
+  int len, minkey, pw, nattr, ncmp, pad;
+  int off[kBatchMaxAttr], bits[kBatchMaxAttr], lut[kBatchMaxAttr], base[kBatchMaxAttr];
+};
+
 struct BatchHot {
   const int* col[kBatchMaxCols];
-  const uint4* rec[4];
-  unsigned long long uses[4];
-  int len[4], minkey[4];
-  int ncols, colmask, usedmask, nexpr, npred, nq;
+  BatchHotDim dim[4];
+  int ncols, colmask, usedmask, nexpr, npred, nq, nlut, pad;
   int pred_begin[kBatchMaxCols + 1];
   int4 expr[kBatchMaxExpr];
   int4 pred[kBatchMaxPred];    // lo, hi - lo (unsigned), query, column
+  int4 cmp[4][kBatchMaxCmp];   // per dimension: src, lo, hi - lo (unsigned), query
 };
 
 struct BatchConfig {
@@ -127,12 +160,13 @@ struct BatchConfig {
                                // descriptors and LDS group arrays
 };
 
-// Bytes of LDS a query set needs beyond the descriptors.
+// Bytes of LDS a query's group array needs.
 inline size_t batch_group_lds(int total_val) {
   return (size_t)total_val * 8 + (size_t)((total_val + 31) / 32) * 4;
 }
 
 // Host launchers (batch.hip).  Every launch reads the batch from L.desc.
+// rec / rec_len: packed tables and their sizes in 16-byte units
 hipError_t batch_fill(const BatchLaunch& L, const uint4* const* rec, const int* rec_len, hipStream_t s);
 hipError_t batch_records(const BatchLaunch& L, int d, int nrows, hipStream_t s);
 // ncols: fact columns the batch reads (selects the register layout).

@@ -821,11 +821,29 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       cols[k] = r.key[k];
     }
   }
-  // dimensions: one record table each
-  Column* attrs[4][2] = {};
+  // dimensions: one packed attribute table each; mask tables only as far as
+  // the launch configuration's LDS holds them (the rest use range compares)
+  Column* attrs[4][kBatchMaxAttr] = {};
+  D.nlut = 0;
+  const long long lut_cap = std::min<long long>(
+      kBatchMaxLut, std::max<long long>(0, ((long long)cfg.lds_bytes - (long long)sizeof(BatchHot) -
+                                            (long long)nq * (long long)sizeof(BatchQuery)) / 8));
   for (int k = 0; k < 4; k++) {
     BatchDim& B = D.dim[k];
     Column* key = nullptr;
+    int na = 0;
+    auto attr_of = [&](Column* c) -> int {
+      for (int a = 0; a < na; a++)
+        if (attrs[k][a] == c) return a;
+      if (na == kBatchMaxAttr) return -1;
+      attrs[k][na] = c;
+      return na++;
+    };
+    struct Filt {
+      int q, mode, c1, c2;
+      Column* col;
+    };
+    std::vector<Filt> filts;
     for (int i = 0; i < nq; i++) {
       if (!qp[i]) continue;
       const Resolved& r = rs[i];
@@ -844,31 +862,73 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       } else if (key != r.dkey[k] || B.len != j.dim_len || B.minkey != j.min_key) {
         return fail(e, "batch queries disagree on a dimension's key, dim_len or min_key");
       }
-      if (r.dfilt[k] && (j.filter_mode < 0 || j.filter_mode > 2)) return fail(e, "dimension filter mode must be 0..2");
-      B.filt[B.nfilt++] = BatchFilt{r.dfilt[k] ? r.dfilt[k]->d_idx : nullptr, j.filter_mode, j.c1, j.c2, i};
       B.uses |= 1ull << i;
-      if (qp[i]->kind == MORDRED_KIND_GROUPBY && r.dval[k]) {
-        Column* v = r.dval[k];
-        int a = (attrs[k][0] == v) ? 0 : (attrs[k][1] == v) ? 1 : -1;
-        if (a < 0) {
-          a = !attrs[k][0] ? 0 : !attrs[k][1] ? 1 : -1;
-          if (a < 0) return fail(e, "more than 2 group columns on one dimension in a batch");
-          column_range(v);
-          if ((long long)v->vmax - (long long)v->vmin > 0xFFFF) return fail(e, "group column span exceeds 16 bits: " + v->name);
-          attrs[k][a] = v;
-          B.attr[a] = v->d_idx;
-          B.attr_base[a] = v->vmin;
-        }
+      if (r.dfilt[k]) {
+        if (j.filter_mode < 0 || j.filter_mode > 2) return fail(e, "dimension filter mode must be 0..2");
+        filts.push_back(Filt{i, j.filter_mode, j.c1, j.c2, r.dfilt[k]});
+        if (r.dfilt[k] != key && attr_of(r.dfilt[k]) < 0)
+          return fail(e, "more than 3 filter and group columns on one dimension in a batch");
       }
+      if (qp[i]->kind == MORDRED_KIND_GROUPBY && r.dval[k] && attr_of(r.dval[k]) < 0)
+        return fail(e, "more than 3 filter and group columns on one dimension in a batch");
     }
     if (!key) continue;
+    // no two keys may share a slot: the dimension's and the fact's key ranges
+    // lie within dim_len, so the row in a probed slot has the fact row's key
+    Column* fkey = cols[k];
+    column_range(key);
+    column_range(fkey);
+    const long long klo = std::min(key->vmin, fkey->vmin), khi = std::max(key->vmax, fkey->vmax);
+    if (khi - klo >= (long long)B.len)
+      return fail(e, "dimension key range exceeds dim_len (hash collisions possible): batches need perfect hashing");
+    // packed layout: attributes from bit 0, the present bit on top
+    int off = 0;
+    for (int a = 0; a < na; a++) {
+      Column* c = attrs[k][a];
+      column_range(c);
+      const long long span = (long long)c->vmax - c->vmin + 1;
+      int bits = 1;
+      while ((1ll << bits) < span) bits++;
+      B.attr[a] = BatchAttr{c->d_idx, c->vmin, off, bits, -1, (int)std::min<long long>(span, 1 << 30)};
+      off += bits;
+    }
+    if (off > 31) return fail(e, "packed attributes of one dimension exceed 31 bits");
+    B.nattr = na;
+    B.pw = off + 1 <= 8 ? 1 : off + 1 <= 16 ? 2 : 4;
+    // filters: a mask table per narrow filtered attribute, range compares for
+    // the key and wide attributes
+    auto pass = [](const Filt& f, int x) {
+      return f.mode == 0 ? x == f.c1 : f.mode == 1 ? (x >= f.c1 && x <= f.c2) : (x == f.c1 || x == f.c2);
+    };
+    for (int a = 0; a < na; a++) {
+      BatchAttr& A = B.attr[a];
+      bool filtered = false;
+      for (const Filt& f : filts) filtered |= f.col == attrs[k][a];
+      if (!filtered || A.span > kBatchLutSpan || D.nlut + A.span > lut_cap) continue;
+      A.lut = D.nlut;
+      for (int v = 0; v < A.span; v++) {
+        unsigned long long m = ~0ull;
+        for (const Filt& f : filts)
+          if (f.col == attrs[k][a] && !pass(f, A.base + v)) m &= ~(1ull << f.q);
+        D.lut[D.nlut + v] = m;
+      }
+      D.nlut += A.span;
+    }
+    for (const Filt& f : filts) {
+      int src = -1;
+      if (f.col != key) {
+        src = attr_of(f.col);
+        if (B.attr[src].lut >= 0) continue;
+      }
+      if (f.mode == 2) return fail(e, "EQ-or-EQ filter on a dimension key or wide column in a batch");
+      if (B.ncmp == kBatchMaxCmp) return fail(e, "too many range filters on one dimension in a batch");
+      B.cmp[B.ncmp++] = BatchCmp{src, f.c1, f.mode == 0 ? f.c1 : f.c2, f.q};
+    }
     B.used = 1;
-    std::sort(B.filt, B.filt + B.nfilt, [](const BatchFilt& x, const BatchFilt& y) {
-      return x.col == y.col ? x.q < y.q : (uintptr_t)x.col < (uintptr_t)y.col;
-    });
-    HIP_OK(e->d_rec[k].reserve((size_t)B.len * sizeof(uint4)));
-    B.rec = static_cast<uint4*>(e->d_rec[k].p);
-    b.rec_len[k] = B.len;
+    const size_t bytes = (((size_t)B.len * B.pw) + 15) & ~(size_t)15;
+    HIP_OK(e->d_rec[k].reserve(bytes));
+    B.pk = e->d_rec[k].p;
+    b.rec_len[k] = (int)(bytes / 16);
   }
   // fact predicates, aggregate expressions, query descriptors
   std::vector<BatchPred> preds;
@@ -898,13 +958,23 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     Q.total_val = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
     Q.expr = (short)ex;
     Q.flags = (short)(q.kind == MORDRED_KIND_GROUPBY ? 0 : kQueryAggr);
+    unsigned hbase = 0;
     for (int k = 0; k < 4; k++) {
       Q.attr[k] = -1;
+      Q.shift[k] = 0;
+      Q.bits[k] = 0;
+      Q.uniq[k] = 0;
       if (q.kind != MORDRED_KIND_GROUPBY || !r.join[k] || !r.dval[k]) continue;
-      Q.attr[k] = (signed char)(attrs[k][0] == r.dval[k] ? 0 : 1);
-      Q.minv[k] = r.join[k]->min_val;
+      int a = 0;
+      while (a < kBatchMaxAttr && attrs[k][a] != r.dval[k]) a++;
+      const BatchAttr& A = D.dim[k].attr[a];
+      Q.attr[k] = (signed char)a;
+      Q.shift[k] = (unsigned char)A.off;
+      Q.bits[k] = (unsigned char)A.bits;
       Q.uniq[k] = r.join[k]->unique_val;
+      hbase += ((unsigned)A.base - (unsigned)r.join[k]->min_val) * (unsigned)r.join[k]->unique_val;
     }
+    Q.hbase = (int)hbase;
     b.G[i] = Q.total_val;
     if (dead) b.dead++;
     else D.all |= 1ull << i;
@@ -922,7 +992,8 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
   std::vector<int> order(nq);
   for (int i = 0; i < nq; i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b.G[x] < b.G[y]; });
-  size_t lds_used = sizeof(BatchHot) + nq * sizeof(BatchQuery);
+  size_t lds_used = sizeof(BatchHot) + nq * sizeof(BatchQuery) + (size_t)D.nlut * sizeof(unsigned long long);
+  if (lds_used > (size_t)cfg.lds_bytes) return fail(e, "lds_bytes below the staged descriptor and mask tables");
   std::vector<char> in_lds(nq, 0);
   for (int i : order) {
     if (!qp[i]) continue;
@@ -1012,7 +1083,7 @@ int launch_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConf
   k.c = cfg;
   for (int d = 0; d < 4; d++) {
     k.used[d] = b.d.dim[d].used;
-    k.rec[d] = k.used[d] ? b.d.dim[d].rec : nullptr;
+    k.rec[d] = k.used[d] ? static_cast<const uint4*>(b.d.dim[d].pk) : nullptr;
     k.rec_len[d] = k.used[d] ? b.rec_len[d] : 0;
     k.nrows[d] = k.used[d] ? b.d.dim[d].nrows : 0;
   }
@@ -1052,10 +1123,8 @@ int launch_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConf
   return 0;
 }
 
-int batch_status(mordred_engine* e, unsigned collisions) {
-  if (collisions)
-    return fail(e, "dimension hash collision in a batch record table (" + std::to_string(collisions) +
-                       " slots): batches need collision-free perfect hashing");
+int batch_status(mordred_engine* e, unsigned status) {
+  if (status) return fail(e, "batch kernels reported status " + std::to_string(status));
   return 0;
 }
 
