@@ -374,6 +374,112 @@ def serve_bench(args):
         sys.exit(1)
 
 
+def skip_bench(args):
+    """SSB Q1.1, Q1.2 and Q1.3 over a 600M-row lineorder sorted by
+    lo_orderdate, read from on-disk column files and minmax files, every fact
+    segment cached in HBM; each query with segment skipping off and on.
+    Prints one JSON line (value: Q1.1 fact rows/s with skipping)."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+    from mordred_amd import columnar
+    from mordred_amd import engine as E
+    from mordred_amd import query as Q
+    n = args.rows or 600_000_000
+    qs = [Q.q11(), Q.q12(), Q.q13()]
+    cols = qs[0].fact_columns()
+    t0 = time.time()
+    data = Q.make_data(qs[0], n, seed=2024, sort_orderdate=True)
+    log(f"generated {n} rows x {len(cols)} columns (sorted by lo_orderdate) in {time.time() - t0:.1f}s")
+    tmp = tempfile.mkdtemp(prefix="mordred_columns_")
+    need = 4 * n * len(cols)
+    if shutil.disk_usage(tmp).free < need + (1 << 30):
+        shutil.rmtree(tmp, ignore_errors=True)
+        raise SystemExit(f"skip workload: {need / 1e9:.1f} GB of column files do not fit in {tmp}")
+    nseg = (n + (1 << 20) - 1) >> 20
+    threads = min(16, os.cpu_count() or 1)
+    eng = E.Engine(device=0, cache_segments=nseg * len(cols) + 64, threads=threads)
+    try:
+        t0 = time.time()
+        columnar.write_table(tmp, {c: data.fact[c] for c in cols}, sorted_table=True)
+        log(f"wrote {len(cols)} column files + minmax files in {time.time() - t0:.1f}s")
+        t0 = time.perf_counter()
+        columnar.load_table(eng, tmp, cols, sorted_table=True)
+        load_s = time.perf_counter() - t0
+        log(f"loaded the column files and minmax files in {load_s:.2f}s "
+            f"({4 * n * len(cols) / load_s / 1e9:.1f} GB/s, page cache)")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    for c in ("d_datekey", "d_year", "d_yearmonthnum"):
+        eng.add_column(c, data.tables["date"][c], gpu_resident=True)
+    for c in cols:
+        for sgm in range(nseg):
+            eng.cache_segment(c, sgm)
+    runs = {}
+    for q in qs:
+        for skipping in (False, True):
+            eng.set_skipping(skipping)
+            for _ in range(args.warmup):
+                eng.run(q, E.PLACE_RESIDENCY)
+            times, gpu, last = [], [], None
+            for _ in range(args.steps):
+                r, st = eng.run(q, E.PLACE_RESIDENCY)
+                times.append(st["total_ms"])
+                gpu.append(st["gpu_ms"])
+                last = (r, st)
+            key = f"{q.name}_{'skip' if skipping else 'scan'}"
+            runs[key] = {"ms": statistics.median(times), "gpu_ms": statistics.median(gpu), "res": last[0],
+                         "skipped_segments": last[1]["skipped_segments"], "gpu_rows": last[1]["gpu_rows"]}
+            log(f"{key}: {runs[key]['ms']:.3f} ms (GPU part {runs[key]['gpu_ms']:.3f} ms), "
+                f"skipped {runs[key]['skipped_segments']} of {nseg} segments")
+    head = runs["q11_skip"]
+    scanned_bytes = head["gpu_rows"] * 4 * len(cols)
+    out = {
+        "metric": METRIC + " (sorted lineorder, per-segment min/max skipping)",
+        "value": n / (head["ms"] * 1e-3),
+        "unit": "fact rows/s (table rows over the Q1.1 query time, skipping on)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 columns, int64 sums",
+        "data": "synthetic, seeded SplitMix64 columns (mordred_amd/datagen.py), lineorder sorted by lo_orderdate, "
+                "written to and read from column files",
+        "config": {"workload": "SSB Q1.1-Q1.3 on a lineorder sorted by lo_orderdate, all segments in HBM",
+                   "fact_rows": n, "segments": nseg, "fact_columns": cols},
+        "load_GBps": 4 * n * len(cols) / load_s / 1e9,
+        "queries": {k: {a: b for a, b in v.items() if a != "res"} for k, v in runs.items()},
+        "speedup_by_skipping": {q.name: runs[f"{q.name}_scan"]["ms"] / runs[f"{q.name}_skip"]["ms"] for q in qs},
+        "roofline": {"bound": "hbm", "achieved": scanned_bytes / (head["gpu_ms"] * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": scanned_bytes / (head["gpu_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "note": "bytes of the segments Q1.1 reads (4 columns x 4 B x rows) over the host-timed GPU "
+                             "part (date build + fused kernel + result copy)"},
+    }
+    parity = all(bool((runs[f"{q.name}_scan"]["res"] == runs[f"{q.name}_skip"]["res"]).all()) for q in qs)
+    if not args.no_cpu:
+        import oracle
+        from oracle import queries as OQ
+        oracle.set_threads(threads)
+        t0 = time.perf_counter()
+        refs = {q.name: OQ.run_query(q, data, oracle.CONV_GPU) for q in qs}
+        cpu_s = (time.perf_counter() - t0) / len(qs)
+        parity = parity and all(bool((refs[q.name] == runs[f"{q.name}_skip"]["res"]).all()) for q in qs)
+        out["cpu_baseline"] = {"value": n / cpu_s, "unit": "fact rows/s", "cores": threads, "kind": "port",
+                               "sample": f"oracle, Q1.1-Q1.3 each on all {n} rows without skipping, "
+                                         f"{cpu_s * 1e3:.0f} ms per query"}
+    out["parity"] = parity
+    print(json.dumps(out), flush=True)
+    eng.close()
+    if not parity:
+        log("PARITY FAILURE")
+        sys.exit(1)
+
+
 def copy_probe():
     """A device copy of exactly 1,920,000,000 bytes read + written, repeated 5
     times: the reference kernel for scaling FETCH_SIZE / WRITE_SIZE."""
@@ -392,7 +498,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["q21", "q11", "q4way-split", "batch64", "serve"], default="q21")
+    ap.add_argument("--workload", choices=["q21", "q11", "q4way-split", "batch64", "serve", "skip"],
+                    default="q21")
     ap.add_argument("--rates", default="10,50,200,500,1000,2000", help="serve: offered rates (q/s)")
     ap.add_argument("--target-chunk-ms", type=float, default=8.0, help="serve: launch-time target")
     ap.add_argument("--rows", type=int, default=None, help="override fact rows")
@@ -416,6 +523,9 @@ def main():
         return
     if args.workload == "serve":
         serve_bench(args)
+        return
+    if args.workload == "skip":
+        skip_bench(args)
         return
 
     import torch

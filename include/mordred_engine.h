@@ -78,6 +78,7 @@ typedef struct mordred_run_stats {
   int gpu_segments, cpu_segments, stream_segments;
   double est_cpu_ms_per_segment;    /* analytic model */
   double est_stream_ms_per_segment; /* analytic model */
+  int skipped_segments;             /* fact segments excluded by min/max */
 } mordred_run_stats;
 
 /* Engine on HIP device `device` with an HBM segment cache of cache_segments
@@ -97,6 +98,34 @@ int mordred_engine_cache_segment(mordred_engine* e, const char* name, int segmen
 int mordred_engine_evict_segment(mordred_engine* e, const char* name, int segment);
 /* 1 if the segment of the column is in HBM. */
 int mordred_engine_is_cached(const mordred_engine* e, const char* name, int segment);
+
+/* ---- on-disk columns, per-segment min/max, segment skipping -------------
+ * Column files are headerless little-endian int32 arrays, one file per
+ * column (common.h:137-201 loadColumn / storeColumn; the reference names them
+ * <TABLE><index>, e.g. LINEORDER5, and LINEORDERSORT5 for the table sorted by
+ * lo_orderdate, common.h:84-135).  Every column the engine holds carries a
+ * [min, max] per 2^20-row segment, computed from the values as they are
+ * copied or read (what minmax.cpp:28-45 computes offline).  A minmax file has
+ * one "<min> <max>" line per segment (CacheManager.cu:154-184).
+ * With skipping on (the default, as main.cu:21), mordred_engine_run and
+ * mordred_engine_place leave out the fact segments that a query's predicates
+ * exclude (QueryOptimizer.cu:1640-1662): the fact filters, and the
+ * foreign-key range a dimension filter implies (sorted or clustered key
+ * columns only).  Results do not change. */
+
+/* Load n values (n = -1: the whole file) of a column file into the store, as
+ * mordred_engine_add_column.  The file may hold more values; fewer is an error. */
+int mordred_engine_load_column_file(mordred_engine* e, const char* name, const char* path, long long n,
+                                    int gpu_resident);
+int mordred_engine_store_column_file(mordred_engine* e, const char* name, const char* path);
+/* Replace a column's skipping ranges with a minmax file's.  The file must have
+ * one line per segment, min <= max, and cover the segment's values (a range
+ * tighter than the data is refused, since skipping with it would drop rows). */
+int mordred_engine_load_segment_minmax(mordred_engine* e, const char* name, const char* path);
+int mordred_engine_store_segment_minmax(mordred_engine* e, const char* name, const char* path);
+/* mins / maxs receive one entry per segment (mordred_engine_column_segments). */
+int mordred_engine_segment_minmax(mordred_engine* e, const char* name, int* mins, int* maxs);
+int mordred_engine_set_skipping(mordred_engine* e, int on);
 
 /* Analytic-model bandwidths in bytes per second (CostModel.h:5-6 defaults
  * 42e9 and 12e9). */

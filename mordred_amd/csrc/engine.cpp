@@ -5,8 +5,13 @@
 // evict a segment); QueryOptimizer.cu:1674-1747 (segment groups by residency,
 // operator device = GPU iff its columns are cached); QueryProcessing.cu:
 // 1346-1469 (runQuery: dims, fact groups on their own streams / threads, merge);
-// CPUProcessing.cu:439-1104 (the CPU operators); CostModel.cu:283-440 (cost).
+// CPUProcessing.cu:439-1104 (the CPU operators); CostModel.cu:283-440 (cost);
+// common.h:137-201 (column files), CacheManager.cu:154-184 and
+// QueryOptimizer.cu:1640-1662, 1695-1706 (per-segment min/max, skipping).
+#include <fcntl.h>
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <atomic>
@@ -14,6 +19,8 @@
 #include <condition_variable>
 #include <cstdint>
 #include <cstdlib>
+#include <climits>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <future>
@@ -109,7 +116,13 @@ struct Column {
   int* d_idx = nullptr;         // device segment table (the reference col_idx)
   bool pinned_gpu = false;
   bool ranged = false;          // vmin / vmax computed
-  int vmin = 0, vmax = 0;
+  int vmin = 0, vmax = 0;       // exact value range of the column
+  // Per-segment [min, max] used for segment skipping: computed from the
+  // values when the column is filled, or read from a minmax file that covers
+  // them (CacheManager.cu:154-184).
+  std::vector<int> smin, smax;
+  std::vector<int> emin, emax;  // exact per-segment min / max of the values
+  int clustered = -1;           // -1 unknown, else 1 if segments cover narrow ranges
 };
 
 // Device buffer that only grows.
@@ -162,6 +175,7 @@ struct mordred_engine {
   hipStream_t s_copy[2] = {nullptr, nullptr};
   short* d_identity = nullptr;  // device segment group 0, 1, 2, ...
   double bw_cpu = 42e9, bw_pci = 12e9;  // CostModel.h:5-6 (bytes per second)
+  bool skipping = true;                 // segment skipping (main.cu:21)
   std::unordered_map<long long, double> seg_weight;  // (column id << 32 | segment) -> weight
   DevBuf d_res, d_sg, d_ht[4];
   HostBuf h_stage;  // pinned scratch for segment groups / index updates
@@ -231,6 +245,108 @@ int evict_seg(mordred_engine* e, Column* c, int seg) {
   return set_slot(e, c, seg, -1);
 }
 
+// ------------------------------------------------------------ column fill
+void free_column(mordred_engine* e, Column* c) {
+  if (e->device < 0) {
+    std::free(c->host);
+  } else {
+    if (c->host) (void)hipHostFree(c->host);
+    if (c->d_idx) (void)hipFree(c->d_idx);
+  }
+  c->host = nullptr;
+  c->d_idx = nullptr;
+}
+
+// Fill a column's host store segment by segment on the worker pool, from a
+// caller buffer or (values == NULL) from a headerless little-endian int32
+// file with pread (common.h:137-147 loadColumn; the host is little-endian, so
+// the bytes are the values), and record each segment's min and max on the way:
+// the statistics minmax.cpp:28-45 computes in an offline pass.
+int fill_column(mordred_engine* e, Column* c, const int* values, int fd) {
+  c->smin.assign(c->nseg, 0);
+  c->smax.assign(c->nseg, 0);
+  c->emin.assign(c->nseg, 0);
+  c->emax.assign(c->nseg, 0);
+  c->clustered = -1;
+  std::atomic<int> short_read{0};
+  e->pool->run(c->nseg, [&](int seg, int) {
+    const long long b = (long long)seg * kSeg;
+    const long long n = std::min<long long>(kSeg, c->n - b);
+    int* dst = c->host + b;
+    if (values) {
+      std::memcpy(dst, values + b, (size_t)n * sizeof(int));
+    } else {
+      const size_t want = (size_t)n * sizeof(int);
+      size_t done = 0;
+      while (done < want) {
+        const ssize_t got = pread(fd, reinterpret_cast<char*>(dst) + done, want - done,
+                                  (off_t)((size_t)b * sizeof(int) + done));
+        if (got <= 0) {
+          short_read.store(1);
+          return;
+        }
+        done += (size_t)got;
+      }
+    }
+    int mn = dst[0], mx = dst[0];
+    for (long long i = 1; i < n; i++) {
+      mn = std::min(mn, dst[i]);
+      mx = std::max(mx, dst[i]);
+    }
+    c->smin[seg] = c->emin[seg] = mn;
+    c->smax[seg] = c->emax[seg] = mx;
+  });
+  if (short_read.load()) return fail(e, "column file shorter than n values: " + c->name);
+  c->vmin = c->vmax = 0;
+  for (int seg = 0; seg < c->nseg; seg++) {
+    c->vmin = seg ? std::min(c->vmin, c->smin[seg]) : c->smin[seg];
+    c->vmax = seg ? std::max(c->vmax, c->smax[seg]) : c->smax[seg];
+  }
+  c->ranged = true;
+  return 0;
+}
+
+// New column of n values: host store, segment table, statistics, and with
+// gpu_resident every segment cached.  values == NULL reads file descriptor fd.
+int add_column(mordred_engine* e, const char* name, const int* values, int fd, long long n, int gpu_resident) {
+  if (e->cols.count(name)) return fail(e, std::string("duplicate column ") + name);
+  const int nseg = (int)((n + kSeg - 1) / kSeg);
+  if (nseg > kMaxIdentity) return fail(e, "column longer than 32767 segments");
+  auto owned = std::make_unique<Column>();
+  Column* c = owned.get();
+  c->name = name;
+  c->n = n;
+  c->nseg = nseg;
+  c->slot.assign(nseg, -1);
+  const size_t bytes = std::max<size_t>(1, (size_t)n * sizeof(int));
+  if (e->device < 0) {
+    c->host = static_cast<int*>(std::malloc(bytes));
+    if (!c->host) return fail(e, "out of host memory");
+  } else {
+    HIP_OK(hipSetDevice(e->device));
+    hipError_t rc = hipHostMalloc(reinterpret_cast<void**>(&c->host), bytes, hipHostMallocDefault);
+    if (rc == hipSuccess) rc = hipMalloc(&c->d_idx, std::max(1, nseg) * sizeof(int));
+    if (rc == hipSuccess)
+      rc = hipMemcpy(c->d_idx, c->slot.data(), nseg * sizeof(int), hipMemcpyHostToDevice);
+    if (rc != hipSuccess) {
+      free_column(e, c);
+      return hip_fail(e, rc, "column allocation");
+    }
+  }
+  int rc = fill_column(e, c, values, fd);
+  if (rc) {
+    free_column(e, c);
+    return rc;
+  }
+  e->cols[name] = std::move(owned);
+  if (gpu_resident && e->device >= 0) {
+    for (int seg = 0; seg < nseg; seg++)
+      if ((rc = cache_seg(e, c, seg))) return rc;
+    c->pinned_gpu = true;
+  }
+  return 0;
+}
+
 // ----------------------------------------------------------- resolved query
 struct Resolved {
   const mordred_star_query* q = nullptr;
@@ -292,6 +408,117 @@ bool segment_cached(const Resolved& r, int seg) {
 
 long long segment_rows(const Resolved& r, int seg) {
   return std::min<long long>(kSeg, r.n - (long long)seg * kSeg);
+}
+
+// ------------------------------------------------------------ segment skipping
+// QueryOptimizer::checkPredicate (QueryOptimizer.cu:1640-1662) as used by
+// groupBitmapSegmentTable (:1695-1706): a fact segment is skipped when, for one
+// predicate of the query, the segment's [min, max] of the predicate's column
+// misses the predicate's range.  The predicates are
+//   - the fact filters (BETWEEN lo..hi);
+//   - per probed join with a dimension filter, the foreign-key range that the
+//     filter implies.  The reference writes that range by hand next to the
+//     date filter (params->compare1/2[lo_orderdate], QueryOptimizer.cu:
+//     2580-2595); here it is derived from the slots the dimension build fills.
+// Why a skipped segment contributes nothing: a fact row with key x in
+// [min_key, min_key + dim_len) probes slot x - min_key, which is filled only
+// if a dimension row passing the filter has that slot; so if every key of the
+// segment lies in [min_key, min_key + dim_len) and none in
+// [min_key + first filled slot, min_key + last filled slot], no row of the
+// segment survives the join.  A segment with keys outside the hashed range is
+// never skipped on that join.
+struct SkipPred {
+  const Column* c;
+  long long lo, hi;        // predicate range (empty if lo > hi)
+  long long in_lo, in_hi;  // skip only segments whose range lies inside this
+};
+
+// A column whose segments each cover less than half of its value range on
+// average: sorted or clustered, so that min/max can exclude segments.
+bool clustered(Column* c) {
+  if (c->clustered < 0) {
+    const double span = (double)c->vmax - (double)c->vmin;
+    double sum = 0;
+    for (int seg = 0; seg < c->nseg; seg++) sum += (double)c->smax[seg] - (double)c->smin[seg];
+    c->clustered = c->nseg > 1 && sum <= 0.5 * span * c->nseg;
+  }
+  return c->clustered == 1;
+}
+
+// First and last hash slot that join k's build fills (cpu_build's filter
+// semantics, which equal the GPU build's); false if a key falls outside the
+// hashed range.  Empty: first > last.
+bool filled_slots(mordred_engine* e, const Resolved& r, int k, long long& first, long long& last) {
+  const mordred_join& j = *r.join[k];
+  const Column* key = r.dkey[k];
+  const Column* fc = r.dfilt[k];
+  const long long n = key->n;
+  const int tasks = (int)((n + kCpuChunk - 1) / kCpuChunk);
+  std::vector<long long> lo(tasks, LLONG_MAX), hi(tasks, LLONG_MIN);
+  std::atomic<int> outside{0};
+  e->pool->run(tasks, [&](int task, int) {
+    const long long b = (long long)task * kCpuChunk, en = std::min(n, b + kCpuChunk);
+    long long mn = LLONG_MAX, mx = LLONG_MIN;
+    for (long long row = b; row < en; row++) {
+      const int x = fc->host[row];
+      bool pass;
+      if (j.filter_mode == 1) pass = x >= j.c1 && x <= j.c2;
+      else if (j.filter_mode == 2) pass = x == j.c1 || x == j.c2;
+      else pass = x == j.c1;
+      if (!pass) continue;
+      const long long d = (long long)key->host[row] - j.min_key;
+      if (d < 0) {
+        outside.store(1);
+        continue;
+      }
+      const long long slot = d % j.dim_len;
+      mn = std::min(mn, slot);
+      mx = std::max(mx, slot);
+    }
+    lo[task] = mn;
+    hi[task] = mx;
+  });
+  if (outside.load()) return false;
+  first = LLONG_MAX;
+  last = LLONG_MIN;
+  for (int t = 0; t < tasks; t++) {
+    first = std::min(first, lo[t]);
+    last = std::max(last, hi[t]);
+  }
+  if (first > last) first = 1, last = 0;
+  return true;
+}
+
+// skip[s] = 1 for every fact segment the query's predicates exclude.
+int skip_segments(mordred_engine* e, const Resolved& r, std::vector<char>& skip) {
+  const mordred_star_query& q = *r.q;
+  std::vector<SkipPred> preds;
+  for (int i = 0; i < q.nfilters; i++)
+    preds.push_back({r.filt[i], q.filters[i].lo, q.filters[i].hi, LLONG_MIN, LLONG_MAX});
+  for (int k = 0; k < 4; k++) {
+    // an aggregate query's GPU kernel probes the date table only
+    // (filter_probe_aggr_GPU2), so only that join may exclude segments
+    if (!r.join[k] || !r.dfilt[k] || (q.kind != MORDRED_KIND_GROUPBY && k != 3)) continue;
+    if (!clustered(r.key[k])) continue;
+    long long first, last;
+    if (!filled_slots(e, r, k, first, last)) continue;
+    const long long mk = r.join[k]->min_key;
+    preds.push_back({r.key[k], mk + first, mk + last, mk, mk + r.join[k]->dim_len - 1});
+  }
+  skip.assign(r.nseg, 0);
+  int n = 0;
+  for (int seg = 0; seg < r.nseg; seg++) {
+    for (const SkipPred& p : preds) {
+      const long long mn = p.c->smin[seg], mx = p.c->smax[seg];
+      if (mn < p.in_lo || mx > p.in_hi) continue;
+      if (p.lo > p.hi || mx < p.lo || mn > p.hi) {
+        skip[seg] = 1;
+        n++;
+        break;
+      }
+    }
+  }
+  return n;
 }
 
 // ------------------------------------------------------------ cost model
@@ -1315,15 +1542,12 @@ mordred_engine* mordred_engine_create(int device, int cache_segments, int thread
 void mordred_engine_destroy(mordred_engine* e) {
   if (!e) return;
   if (e->device < 0) {
-    for (auto& kv : e->cols) std::free(kv.second->host);
+    for (auto& kv : e->cols) free_column(e, kv.second.get());
     delete e;
     return;
   }
   (void)hipSetDevice(e->device);
-  for (auto& kv : e->cols) {
-    if (kv.second->host) (void)hipHostFree(kv.second->host);
-    if (kv.second->d_idx) (void)hipFree(kv.second->d_idx);
-  }
+  for (auto& kv : e->cols) free_column(e, kv.second.get());
   e->cols.clear();
   if (e->cm) mordred_cost_model_destroy(e->cm);
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
@@ -1342,37 +1566,7 @@ const char* mordred_engine_last_error(const mordred_engine* e) { return e ? e->e
 int mordred_engine_add_column(mordred_engine* e, const char* name, const int* values, long long n,
                               int gpu_resident) {
   if (!e || !name || (!values && n > 0) || n < 0) return e ? fail(e, "bad column") : -1;
-  if (e->cols.count(name)) return fail(e, std::string("duplicate column ") + name);
-  auto c = std::make_unique<Column>();
-  c->name = name;
-  c->n = n;
-  c->nseg = (int)((n + kSeg - 1) / kSeg);
-  if (c->nseg > kMaxIdentity) return fail(e, "column longer than 32767 segments");
-  const size_t bytes = std::max<size_t>(1, (size_t)n * sizeof(int));
-  if (e->device < 0) {
-    c->host = static_cast<int*>(std::malloc(bytes));
-    if (!c->host) return fail(e, "out of host memory");
-    if (n > 0) std::memcpy(c->host, values, (size_t)n * sizeof(int));
-    c->slot.assign(c->nseg, -1);
-    e->cols[name] = std::move(c);
-    return 0;
-  }
-  HIP_OK(hipSetDevice(e->device));
-  HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&c->host), bytes, hipHostMallocDefault));
-  if (n > 0) std::memcpy(c->host, values, (size_t)n * sizeof(int));
-  c->slot.assign(c->nseg, -1);
-  HIP_OK(hipMalloc(&c->d_idx, std::max(1, c->nseg) * sizeof(int)));
-  HIP_OK(hipMemcpy(c->d_idx, c->slot.data(), c->nseg * sizeof(int), hipMemcpyHostToDevice));
-  Column* raw = c.get();
-  e->cols[name] = std::move(c);
-  if (gpu_resident) {
-    for (int s = 0; s < raw->nseg; s++) {
-      const int rc = cache_seg(e, raw, s);
-      if (rc) return rc;
-    }
-    raw->pinned_gpu = true;
-  }
-  return 0;
+  return add_column(e, name, values, -1, n, gpu_resident);
 }
 
 int mordred_engine_column_segments(const mordred_engine* e, const char* name) {
@@ -1413,6 +1607,114 @@ int mordred_engine_set_bandwidth(mordred_engine* e, double bw_cpu, double bw_pci
   return 0;
 }
 
+int mordred_engine_load_column_file(mordred_engine* e, const char* name, const char* path, long long n,
+                                    int gpu_resident) {
+  if (!e) return -1;
+  if (!name || !path || n < -1) return fail(e, "bad column file arguments");
+  const int fd = open(path, O_RDONLY);
+  if (fd < 0) return fail(e, std::string("cannot open column file ") + path);
+  struct stat sb;
+  if (fstat(fd, &sb) != 0) {
+    close(fd);
+    return fail(e, std::string("cannot stat column file ") + path);
+  }
+  if (n == -1) {
+    if (sb.st_size % (off_t)sizeof(int) != 0) {
+      close(fd);
+      return fail(e, std::string("column file size is not a multiple of 4 bytes: ") + path);
+    }
+    n = (long long)(sb.st_size / (off_t)sizeof(int));
+  } else if ((long long)sb.st_size < n * (long long)sizeof(int)) {
+    close(fd);
+    return fail(e, std::string("column file shorter than n values: ") + path);
+  }
+  const int rc = add_column(e, name, nullptr, fd, n, gpu_resident);
+  close(fd);
+  return rc;
+}
+
+int mordred_engine_store_column_file(mordred_engine* e, const char* name, const char* path) {
+  if (!e) return -1;
+  Column* c = col(e, name);
+  if (!c || !path) return fail(e, "unknown column or NULL path");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(e, std::string("cannot create column file ") + path);
+  const size_t w = c->n > 0 ? std::fwrite(c->host, sizeof(int), (size_t)c->n, f) : 0;
+  const bool ok = w == (size_t)c->n && std::fclose(f) == 0;
+  if (!ok) return fail(e, std::string("short write to column file ") + path);
+  return 0;
+}
+
+int mordred_engine_load_segment_minmax(mordred_engine* e, const char* name, const char* path) {
+  if (!e) return -1;
+  Column* c = col(e, name);
+  if (!c || !path) return fail(e, "unknown column or NULL path");
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(e, std::string("cannot open minmax file ") + path);
+  std::vector<int> mn, mx;
+  char line[128];
+  std::string bad;
+  while (bad.empty() && std::fgets(line, sizeof line, f)) {
+    if (line[0] == '\n' || line[0] == '\0') continue;
+    char* p = line;
+    char* q = nullptr;
+    const long a = std::strtol(p, &q, 10);
+    if (q == p) bad = "expected '<min> <max>'";
+    p = q;
+    const long b = bad.empty() ? std::strtol(p, &q, 10) : 0;
+    if (bad.empty() && q == p) bad = "expected '<min> <max>'";
+    if (bad.empty() && (a < INT_MIN || a > INT_MAX || b < INT_MIN || b > INT_MAX)) bad = "value out of int32 range";
+    if (bad.empty() && a > b) bad = "min > max";
+    if (bad.empty()) {
+      mn.push_back((int)a);
+      mx.push_back((int)b);
+    }
+  }
+  std::fclose(f);
+  if (!bad.empty())
+    return fail(e, std::string("minmax file ") + path + " line " + std::to_string(mn.size() + 1) + ": " + bad);
+  if ((int)mn.size() != c->nseg)
+    return fail(e, std::string("minmax file ") + path + " has " + std::to_string(mn.size()) + " lines for " +
+                       std::to_string(c->nseg) + " segments");
+  // skipping with ranges tighter than the data would drop rows: refuse them
+  for (int seg = 0; seg < c->nseg; seg++)
+    if (mn[seg] > c->emin[seg] || mx[seg] < c->emax[seg])
+      return fail(e, std::string("minmax file ") + path + " segment " + std::to_string(seg) +
+                         " does not cover the column's values");
+  c->smin = mn;
+  c->smax = mx;
+  c->clustered = -1;
+  return 0;
+}
+
+int mordred_engine_store_segment_minmax(mordred_engine* e, const char* name, const char* path) {
+  if (!e) return -1;
+  Column* c = col(e, name);
+  if (!c || !path) return fail(e, "unknown column or NULL path");
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(e, std::string("cannot create minmax file ") + path);
+  bool ok = true;
+  for (int seg = 0; seg < c->nseg && ok; seg++) ok = std::fprintf(f, "%d %d\n", c->smin[seg], c->smax[seg]) > 0;
+  ok = std::fclose(f) == 0 && ok;
+  if (!ok) return fail(e, std::string("short write to minmax file ") + path);
+  return 0;
+}
+
+int mordred_engine_segment_minmax(mordred_engine* e, const char* name, int* mins, int* maxs) {
+  if (!e) return -1;
+  Column* c = col(e, name);
+  if (!c || !mins || !maxs) return fail(e, "unknown column or NULL output");
+  std::copy(c->smin.begin(), c->smin.end(), mins);
+  std::copy(c->smax.begin(), c->smax.end(), maxs);
+  return 0;
+}
+
+int mordred_engine_set_skipping(mordred_engine* e, int on) {
+  if (!e) return -1;
+  e->skipping = on != 0;
+  return 0;
+}
+
 int mordred_engine_place(mordred_engine* e, const mordred_star_query* q, double fraction) {
   if (!e) return -1;
   if (e->device < 0) return fail(e, "host-only engine has no segment cache");
@@ -1426,8 +1728,17 @@ int mordred_engine_place(mordred_engine* e, const mordred_star_query* q, double 
   CpuTables t;
   cpu_build(e, r, t);
   const Selectivity sel = estimate_selectivity(r, t);
+  // segments the query skips weigh nothing and are ranked last (the
+  // reference updates segment statistics only for segments it processes,
+  // QueryOptimizer.cu:1696-1699)
+  std::vector<char> skip;
+  if (e->skipping) skip_segments(e, r, skip);
   std::vector<std::pair<double, int>> rank;
   for (int s = 0; s < r.nseg; s++) {
+    if (!skip.empty() && skip[s]) {
+      rank.emplace_back(1.0, s);
+      continue;
+    }
     double w = cpu_cost(r, sel, (double)segment_rows(r, s), e->bw_cpu);
     for (Column* c : r.fact_cols) {
       const long long id = ((long long)(intptr_t)c << 16) ^ s;
@@ -1471,10 +1782,14 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   std::memset(res, 0, res_ints * sizeof(int));
   mordred_run_stats st{};
 
-  // ---- segment groups by residency (QueryOptimizer.cu:1680-1716)
+  // ---- segment groups by residency (QueryOptimizer.cu:1680-1716), without
+  // the segments the predicates exclude
+  std::vector<char> skip;
+  if (e->skipping) st.skipped_segments = skip_segments(e, r, skip);
   GpuPart g;
   std::vector<int> other;
   for (int s = 0; s < r.nseg; s++) {
+    if (!skip.empty() && skip[s]) continue;
     if (placement != MORDRED_PLACE_CPU_ONLY && segment_cached(r, s)) g.gpu_segs.push_back(s);
     else other.push_back(s);
   }

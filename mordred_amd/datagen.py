@@ -112,9 +112,18 @@ def make_part(n: int, seed: int) -> dict:
             "p_brand1": uniform(seed, "p_brand1", 0, 999, n)}
 
 
-def lineorder_column(name: str, n: int, seed: int, dims: dict, start: int = 0) -> np.ndarray:
-    """Rows [start, start+n) of one fact column.  dims: {'S','P','C'} sizes."""
+def lineorder_column(name: str, n: int, seed: int, dims: dict, start: int = 0,
+                     sorted_rows: int = 0) -> np.ndarray:
+    """Rows [start, start+n) of one fact column.  dims: {'S','P','C'} sizes.
+
+    sorted_rows > 0 gives the layout of a lineorder sorted by lo_orderdate
+    (test/ssb/sort.c:123 sorts on column 5) of sorted_rows rows: row i holds
+    date key i * 2556 // sorted_rows, so every date covers an equal run of rows.
+    The other columns do not depend on the order."""
     if name == "lo_orderdate":
+        if sorted_rows > 0:
+            rows = np.arange(start, start + n, dtype=np.int64)
+            return date_keys()[(rows * 2556) // sorted_rows]
         return date_keys()[uniform(seed, name, 0, 2555, n, start)]
     if name == "lo_suppkey":
         return uniform(seed, name, 1, dims["S"], n, start)
@@ -126,10 +135,12 @@ def lineorder_column(name: str, n: int, seed: int, dims: dict, start: int = 0) -
     return uniform(seed, name, lo, hi, n, start)
 
 
-def make_lineorder(n: int, seed: int, dims: dict, columns, threads: int = 8) -> dict:
+def make_lineorder(n: int, seed: int, dims: dict, columns, threads: int = 8,
+                   sort_orderdate: bool = False) -> dict:
     """Fact columns, generated concurrently (numpy releases the GIL)."""
     from concurrent.futures import ThreadPoolExecutor
     columns = list(columns)
+    srt = n if sort_orderdate else 0
     with ThreadPoolExecutor(max(1, min(threads, len(columns)))) as ex:
-        arrays = list(ex.map(lambda c: lineorder_column(c, n, seed, dims), columns))
+        arrays = list(ex.map(lambda c: lineorder_column(c, n, seed, dims, sorted_rows=srt), columns))
     return dict(zip(columns, arrays))

@@ -36,7 +36,8 @@ class RunStats(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("gpu_ms", C.c_double), ("cpu_ms", C.c_double), ("merge_ms", C.c_double),
                 ("gpu_rows", C.c_longlong), ("cpu_rows", C.c_longlong), ("stream_rows", C.c_longlong),
                 ("gpu_segments", C.c_int), ("cpu_segments", C.c_int), ("stream_segments", C.c_int),
-                ("est_cpu_ms_per_segment", C.c_double), ("est_stream_ms_per_segment", C.c_double)]
+                ("est_cpu_ms_per_segment", C.c_double), ("est_stream_ms_per_segment", C.c_double),
+                ("skipped_segments", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -118,6 +119,12 @@ _SIGS = {
     "mordred_engine_evict_segment": ([C.c_void_p, C.c_char_p, C.c_int], C.c_int),
     "mordred_engine_is_cached": ([C.c_void_p, C.c_char_p, C.c_int], C.c_int),
     "mordred_engine_set_bandwidth": ([C.c_void_p, C.c_double, C.c_double], C.c_int),
+    "mordred_engine_load_column_file": ([C.c_void_p, C.c_char_p, C.c_char_p, C.c_longlong, C.c_int], C.c_int),
+    "mordred_engine_store_column_file": ([C.c_void_p, C.c_char_p, C.c_char_p], C.c_int),
+    "mordred_engine_load_segment_minmax": ([C.c_void_p, C.c_char_p, C.c_char_p], C.c_int),
+    "mordred_engine_store_segment_minmax": ([C.c_void_p, C.c_char_p, C.c_char_p], C.c_int),
+    "mordred_engine_segment_minmax": ([C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p], C.c_int),
+    "mordred_engine_set_skipping": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_place": ([C.c_void_p, C.POINTER(StarQueryC), C.c_double], C.c_int),
     "mordred_engine_run": ([C.c_void_p, C.POINTER(StarQueryC), C.c_int, C.c_void_p, C.POINTER(RunStats)], C.c_int),
     "mordred_engine_batch": ([C.c_void_p, C.POINTER(StarQueryC), C.c_int, C.POINTER(C.c_void_p),
@@ -215,6 +222,38 @@ class Engine:
 
     def set_bandwidth(self, bw_cpu: float, bw_pci: float):
         self._check(self.lib.mordred_engine_set_bandwidth(self.h, bw_cpu, bw_pci), "set_bandwidth")
+
+    # ---- on-disk columns, per-segment min/max, skipping
+    def load_column_file(self, name: str, path, n: int = -1, gpu_resident: bool = False):
+        """Headerless little-endian int32 column file (n = -1: the whole file)."""
+        self._check(self.lib.mordred_engine_load_column_file(self.h, name.encode(), str(path).encode(), n,
+                                                             int(gpu_resident)), f"load_column_file {name}")
+
+    def store_column_file(self, name: str, path):
+        self._check(self.lib.mordred_engine_store_column_file(self.h, name.encode(), str(path).encode()),
+                    f"store_column_file {name}")
+
+    def load_segment_minmax(self, name: str, path):
+        self._check(self.lib.mordred_engine_load_segment_minmax(self.h, name.encode(), str(path).encode()),
+                    f"load_segment_minmax {name}")
+
+    def store_segment_minmax(self, name: str, path):
+        self._check(self.lib.mordred_engine_store_segment_minmax(self.h, name.encode(), str(path).encode()),
+                    f"store_segment_minmax {name}")
+
+    def segment_minmax(self, name: str):
+        """(mins, maxs) per segment, the ranges skipping uses."""
+        n = self.segments(name)
+        if n < 0:
+            raise EngineError(f"unknown column {name}")
+        mins = np.zeros(max(1, n), dtype=np.int32)
+        maxs = np.zeros(max(1, n), dtype=np.int32)
+        self._check(self.lib.mordred_engine_segment_minmax(self.h, name.encode(), mins.ctypes.data, maxs.ctypes.data),
+                    f"segment_minmax {name}")
+        return mins[:n], maxs[:n]
+
+    def set_skipping(self, on: bool):
+        self._check(self.lib.mordred_engine_set_skipping(self.h, int(on)), "set_skipping")
 
     def place(self, query, fraction: float):
         qc = star_query(query)

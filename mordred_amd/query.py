@@ -67,15 +67,34 @@ class StarQuery:
 # ---------------------------------------------------------------- plans
 
 def q11(year: int = 1993) -> StarQuery:
-    """cfg 1 = SSB Q1.1 (QueryOptimizer.cu:176-236, 2558-2606, 2708-2750)."""
+    """SSB Q1.1 (QueryOptimizer.cu:176-236, 2558-2606, 2708-2750)."""
     date = DimJoin(4, "date", "lo_orderdate", "d_datekey", datagen.DATE_HT_LEN, datagen.DATE_MIN_KEY,
                    filter_col="d_year", filter_mode=1, c1=year, c2=year)
     return StarQuery("q11", [date], [("lo_discount", 1, 3), ("lo_quantity", 0, 24)],
                      ("lo_extendedprice", "lo_discount"), "aggr", 1)
 
 
+def q12(yearmonth: int = 199401) -> StarQuery:
+    """SSB Q1.2 (QueryOptimizer.cu:2607-2656): d_yearmonthnum = 199401,
+    lo_discount 4..6, lo_quantity 26..35."""
+    date = DimJoin(4, "date", "lo_orderdate", "d_datekey", datagen.DATE_HT_LEN, datagen.DATE_MIN_KEY,
+                   filter_col="d_yearmonthnum", filter_mode=1, c1=yearmonth, c2=yearmonth)
+    return StarQuery("q12", [date], [("lo_discount", 4, 6), ("lo_quantity", 26, 35)],
+                     ("lo_extendedprice", "lo_discount"), "aggr", 1)
+
+
+def q13(first: int = 19940204, last: int = 19940210) -> StarQuery:
+    """SSB Q1.3 (QueryOptimizer.cu:2657-2705): the reference expresses week 6
+    of 1994 as d_datekey BETWEEN 19940204 AND 19940210; lo_discount 5..7,
+    lo_quantity 26..35."""
+    date = DimJoin(4, "date", "lo_orderdate", "d_datekey", datagen.DATE_HT_LEN, datagen.DATE_MIN_KEY,
+                   filter_col="d_datekey", filter_mode=1, c1=first, c2=last)
+    return StarQuery("q13", [date], [("lo_discount", 5, 7), ("lo_quantity", 26, 35)],
+                     ("lo_extendedprice", "lo_discount"), "aggr", 1)
+
+
 def q21(S: int, P: int, region: int = 1, category: int = 1) -> StarQuery:
-    """cfg 2 = SSB Q2.1 (QueryOptimizer.cu:362-446, 2755-2800, 2906-2959) with
+    """SSB Q2.1 (QueryOptimizer.cu:362-446, 2755-2800, 2906-2959) with
     p_brand1 over 1,000 codes: group index brand*7 + (year-1992)."""
     supp = DimJoin(1, "supplier", "lo_suppkey", "s_suppkey", S, 0, filter_col="s_region", c1=region, c2=region)
     part = DimJoin(3, "part", "lo_partkey", "p_partkey", P, 0, filter_col="p_category", c1=category,
@@ -86,7 +105,7 @@ def q21(S: int, P: int, region: int = 1, category: int = 1) -> StarQuery:
 
 
 def q4way(S: int, P: int, Cn: int, region: int = 1, category: int = 1) -> StarQuery:
-    """cfg 3: 4-way join grouped by (c_nation, s_nation, year) = 25x25x7 with the
+    """4-way join grouped by (c_nation, s_nation, year) = 25x25x7 with the
     Q4.x aggregate revenue - supplycost (strides of Q3.1, QueryOptimizer.cu:3009-3014)."""
     supp = DimJoin(1, "supplier", "lo_suppkey", "s_suppkey", S, 0, filter_col="s_region", c1=region,
                    c2=region, group_col="s_nation", min_val=0, unique_val=25 * 7)
@@ -112,7 +131,7 @@ class StarData:
 
 
 def make_data(query: StarQuery, n: int, seed: int, S: int = 40_000, P: int = 1_400_000,
-              Cn: int = 3_000_000) -> StarData:
+              Cn: int = 3_000_000, sort_orderdate: bool = False) -> StarData:
     dims = {"S": S, "P": P, "C": Cn}
     tables = {"date": datagen.make_date()}
     names = {j.table for j in query.joins}
@@ -122,7 +141,7 @@ def make_data(query: StarQuery, n: int, seed: int, S: int = 40_000, P: int = 1_4
         tables["part"] = datagen.make_part(P, seed)
     if "customer" in names:
         tables["customer"] = datagen.make_customer(Cn, seed)
-    fact = datagen.make_lineorder(n, seed, dims, query.fact_columns())
+    fact = datagen.make_lineorder(n, seed, dims, query.fact_columns(), sort_orderdate=sort_orderdate)
     return StarData(n, seed, dims, fact, tables)
 
 

@@ -561,7 +561,7 @@ void oracle_probe_aggr_CPU2(oracle_offset offset, oracle_probeArgs pargs, oracle
   atomic_add_u64(res, 4, sum);
 }
 
-/* c17: filter_probe_aggr_CPU  CPUProcessing.cu:1628-1710 (cfg-1 CPU path;
+/* c17: filter_probe_aggr_CPU  CPUProcessing.cu:1628-1710 (the SSB Q1.1 CPU path;
  * GPU twin filter_probe_aggr_GPU2, GPUProcessing.h:1675-1776) */
 void oracle_filter_probe_aggr_CPU(oracle_filterArgs fargs, oracle_probeArgs pargs,
                                   oracle_groupbyArgs gargs, int num_tuples, int* res,

@@ -75,3 +75,50 @@ def test_partial_last_segment_cached_and_streamed(gengine):
     res, st = eng.run(q, E.PLACE_GPU_ONLY)
     np.testing.assert_array_equal(res, ref)
     assert st["gpu_segments"] == 1 and st["stream_segments"] == nseg - 1
+
+
+# ---------------------------------------------------------------- segment skipping
+def _date_filtered_groupby():
+    from mordred_amd import query as Q
+    q = Q.q21(2000, 30000)
+    q.joins[2].filter_col, q.joins[2].filter_mode, q.joins[2].c1, q.joins[2].c2 = "d_year", 1, 1995, 1996
+    return q
+
+
+@pytest.fixture(scope="module", params=["q11", "q21_dated"])
+def sorted_engine(request, cuda):
+    """A lineorder sorted by lo_orderdate (9 full segments + a partial one)."""
+    from mordred_amd import query as Q
+    if request.param == "q11":
+        q = Q.q11()
+        data = Q.make_data(q, 9 * SEG + 12_345, seed=21, sort_orderdate=True)
+    else:
+        q = _date_filtered_groupby()
+        data = Q.make_data(q, 9 * SEG + 12_345, seed=22, S=2000, P=30000, sort_orderdate=True)
+    eng = E.Engine(device=0, cache_segments=128, threads=8)
+    eng.add_data(data, q)
+    ref = OQ.run_query(q, data, oracle.CONV_GPU)
+    yield q, data, eng, ref
+    eng.close()
+
+
+@pytest.mark.parametrize("placement", [E.PLACE_RESIDENCY, E.PLACE_COST_SPLIT, E.PLACE_GPU_ONLY])
+def test_skipping_on_gpu_placements(sorted_engine, placement):
+    """Skipped segments leave every segment group (cached, CPU and streamed);
+    place() ranks them last, so the segments the query reads are cached."""
+    q, data, eng, ref = sorted_engine
+    print(f"[gpu] skipping {q.name} placement {placement}", flush=True)
+    eng.set_skipping(True)
+    eng.place(q, 0.4)                   # 4 segments: every segment q11 and q21_dated read
+    res, st = eng.run(q, placement)
+    np.testing.assert_array_equal(res, ref)
+    skipped = st["skipped_segments"]
+    assert skipped >= 6
+    assert st["gpu_segments"] + st["cpu_segments"] + st["stream_segments"] == 10 - skipped
+    if placement == E.PLACE_RESIDENCY:
+        assert st["cpu_segments"] == 0 and st["gpu_segments"] == 10 - skipped
+    eng.set_skipping(False)
+    res, st = eng.run(q, placement)
+    np.testing.assert_array_equal(res, ref)
+    assert st["skipped_segments"] == 0
+    eng.set_skipping(True)
