@@ -97,43 +97,6 @@ static hipMemPool_t device_pool(int dev) {
 
 static bool probed(const Pipe& P, int k) { return (P.pmask >> k & 1) && P.key[k] && P.ht[k]; }
 
-struct ProbePlan {
-  int compact;   // bit k: build a compact table for table k
-  int lds_k;
-  int lds_max_w;
-};
-
-// Which tables get a compact probe table, and which one may live in LDS.
-// A table is compacted when the fact scan is long next to it (>= 16 rows per
-// slot, so the two pre-pass launches are paid back) and either it is probed by
-// every row (the first probe with no filter before it) or it is too large to
-// stay in L2.  The first probe may be copied to LDS when its entries are at
-// most lds_max_w bits wide.
-static ProbePlan plan_probes(const Pipe& P, int sink) {
-  ProbePlan plan{0, -1, 0};
-  const bool filtered = P.fkind != F_NONE && (P.fcol[0] || P.fcol[1]);
-  int first = -1;
-  for (int k = 0; k < 4; k++) {
-    if (!probed(P, k)) continue;
-    const bool is_first = first < 0;
-    if (is_first) first = k;
-    const int len = P.len[k];
-    if (len <= 0 || (long long)P.num_tuples < 16LL * len) continue;
-    if ((is_first && !filtered) || len >= kCompactMinLen) plan.compact |= 1 << k;
-  }
-  if (first >= 0 && !filtered && (plan.compact >> first & 1)) {
-    const size_t g = group_lds_bytes(sink, P.total_val);
-    for (int w = 8; w >= 1; w >>= 1) {
-      if (g + staged_lds_bytes(P.len[first], w) <= kLdsBudget) {
-        plan.lds_k = first;
-        plan.lds_max_w = w;
-        break;
-      }
-    }
-  }
-  return plan;
-}
-
 // Compile-time shape of a segment-mode fused group-by or aggregate (see
 // pipeline_kernel), or kShapeGeneric.
 static int shape_of(const Pipe& P, int mode, int sink) {
@@ -147,6 +110,57 @@ static int shape_of(const Pipe& P, int mode, int sink) {
     if (P.fcol[1]) s |= 0x20;
   }
   return s;
+}
+
+struct ProbePlan {
+  int compact;    // bit k: build a compact table for table k
+  int stage;      // bit k: table k may be staged in LDS
+  int lds_words;  // LDS (32-bit words) reserved for staged tables
+};
+
+static int sink_block(int sink) {
+  return sink == GROUPBY_LDS ? kLdsBlock : sink == AGGR ? kAggrBlock : kBlock;
+}
+
+// Dynamic LDS a workgroup of `sink` may use.
+static size_t lds_cap(int sink) { return sink == GROUPBY_LDS ? kLdsBudgetOneGroup : kLdsBudget; }
+
+// Which tables get a compact probe table, and which may be staged in LDS.
+// A table is compacted when the fact scan is long next to it (>= 16 rows per
+// slot, so the two pre-pass launches are paid back) and either it is probed by
+// every row (the first probe with no filter before it), or it is too large to
+// stay in L2, or (>= 256 rows per slot) its packed form may fit in the LDS the
+// group array and the survivor rings leave.  A compacted table whose 1-bit form
+// fits may be staged; the kernel stages tables in probe order while their
+// actual packed words fit in lds_words.
+static ProbePlan plan_probes(const Pipe& P, int mode, int sink) {
+  ProbePlan plan{0, 0, 0};
+  const bool filtered = P.fkind != F_NONE && (P.fcol[0] || P.fcol[1]);
+  size_t fixed = group_lds_bytes(sink, P.total_val);
+  if (shape_compacts(mode, sink, shape_of(P, mode, sink))) fixed += ring_lds_bytes(sink_block(sink));
+  const size_t avail = lds_cap(sink) > fixed ? lds_cap(sink) - fixed : 0;
+  size_t want = 0;
+  int first = -1;
+  for (int k = 0; k < 4; k++) {
+    if (!probed(P, k)) continue;
+    const bool is_first = first < 0;
+    if (is_first) first = k;
+    const int len = P.len[k];
+    if (len <= 0 || (long long)P.num_tuples < 16LL * len) continue;
+    const bool stageable = staged_words(len, 1) * 4 <= avail;
+    const bool lds_worth = (long long)P.num_tuples >= 256LL * len && staged_words(len, 4) * 4 <= avail;
+    if ((is_first && !filtered) || len >= kCompactMinLen || lds_worth) {
+      plan.compact |= 1 << k;
+      if (stageable && (lds_worth || (is_first && !filtered))) {
+        plan.stage |= 1 << k;
+        // one workgroup per CU: take all that is left; otherwise reserve for
+        // entries of up to 4 bits, so that LDS does not cut occupancy
+        want += staged_words(len, sink == GROUPBY_LDS ? 16 : 4) * 4;
+      }
+    }
+  }
+  plan.lds_words = (int)(std::min(avail, want) / 4);
+  return plan;
 }
 
 template <int MODE>
@@ -224,12 +238,15 @@ static void* build_compact(Pipe& P, const ProbePlan& plan, int sink, hipStream_t
 // sink < 0: group-by, placed in LDS when the dense array fits.
 static int dispatch(Pipe P, int mode, int sink, hipStream_t stream) {
   if (P.num_tuples == 0) return hipSuccess;
-  if (sink < 0) sink = group_lds_bytes(GROUPBY_LDS, P.total_val) <= kLdsBudget ? GROUPBY_LDS : GROUPBY_GLOBAL;
-  const ProbePlan plan = plan_probes(P, sink);
+  if (sink < 0)
+    sink = group_lds_bytes(GROUPBY_LDS, P.total_val) + ring_lds_bytes(kLdsBlock) <= kLdsBudgetOneGroup
+               ? GROUPBY_LDS
+               : GROUPBY_GLOBAL;
+  const ProbePlan plan = plan_probes(P, mode, sink);
   void* scratch = build_compact(P, plan, sink, stream);
-  if (scratch && plan.lds_k >= 0) {
-    P.lds_k = plan.lds_k;
-    P.lds_max_w = plan.lds_max_w;
+  if (scratch && plan.stage) {
+    P.lds_stage = plan.stage;
+    P.lds_words = plan.lds_words;
   }
   int rc;
   if (mode == SEG) rc = run_pipe<SEG>(P, sink, stream);
@@ -249,7 +266,6 @@ static Pipe make_pipe(const int* cache, int num_tuples) {
   P.cache = cache;
   P.num_tuples = num_tuples;
   P.aligned = aligned16(cache);  // SEG mode streams from the cache
-  P.lds_k = -1;
   return P;
 }
 

@@ -54,8 +54,11 @@ struct Pipe {
   const CompactHdr* chdr[4];
   const uint32_t* cwords[4];
   const uint32_t* cbits[4];  // existence bitmaps of large compact tables, or nullptr
-  int lds_k;      // table whose compact words are copied to LDS, or -1
-  int lds_max_w;  // ... when its entry width is at most this
+  // Compact tables in `lds_stage` (bit k = table k) are copied to dynamic LDS
+  // at the start of the kernel, in probe order, while their packed words fit
+  // in `lds_words` 32-bit words (entry widths are known only on the device).
+  int lds_stage;
+  int lds_words;
   int ring_off;      // u64 offset of the survivor rings in dynamic LDS
 };
 
@@ -131,12 +134,15 @@ __device__ __forceinline__ uint32_t pred_mode(const int (&x)[4 * VPT], int mode,
 constexpr int kShapeGeneric = -1;
 
 // Survivor compaction (specialised segment-mode group-by / aggregate shapes
-// that probe two or more tables): after the first probe, each wave appends
-// the surviving rows' remaining keys and aggregate values to a ring in LDS
-// (positions from ballot + mbcnt), and the later probes and the sink run on
-// full rounds of 64 dense entries instead of on every item slot.
+// that probe two or more tables): after the first probe and the existence-
+// bitmap tests of later tables, each wave appends the surviving rows to a
+// ring in LDS (positions from ballot + mbcnt): the row id, the first probe's
+// result and the keys already loaded.  The later probes and the sink run on
+// full rounds of 64 dense entries instead of on every item slot, and gather
+// the remaining keys and the aggregates by row (late materialization: only
+// cache lines that hold a survivor are read).
 constexpr int kRing = 128;   // entries per wave (a round of 64 plus one item of carry)
-constexpr int kRingFields = 6;  // values of tables 0-3, aggregate 1, aggregate 2
+constexpr int kRingFields = 5;  // table 0-3 value or key, row id
 constexpr bool shape_compacts(int mode, int sink, int shape) {
   return shape >= 0 && mode == SEG && (sink == GROUPBY_LDS || sink == AGGR) &&
          ((shape & 1) + (shape >> 1 & 1) + (shape >> 2 & 1) + (shape >> 3 & 1)) >= 2;
@@ -187,19 +193,28 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
   if (SINK == GROUPBY_LDS) {
     for (int i = threadIdx.x; i < gwords; i += BLOCK) lds[i] = 0;
   }
-  bool staged = false;
-  if (P.lds_k >= 0) {
-    const int k = P.lds_k;
-    const int w = k == 0 ? cw[0] : k == 1 ? cw[1] : k == 2 ? cw[2] : cw[3];
-    const int sh = k == 0 ? csh[0] : k == 1 ? csh[1] : k == 2 ? csh[2] : csh[3];
-    if (w > 0 && w <= P.lds_max_w) {
-      staged = true;
-      const int nwords = (P.len[k] + (1 << sh) - 1) >> sh;
-      const uint32_t* src = P.cwords[k];
-      for (int i = threadIdx.x; i < nwords; i += BLOCK) ldsw[i] = src[i];
-    }
+  // stage compact tables in LDS (workgroup-uniform decisions): table k's
+  // packed words at ldsw + soff[k] when stg[k]; reads branch on stg[k] so
+  // that they stay LDS or global loads (not flat)
+  bool stg[4];
+  int soff[4];
+  bool any_staged = false;
+  int cursor = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    stg[k] = false;
+    soff[k] = 0;
+    if (!((kGen || (SHAPE >> k & 1)) && (P.lds_stage >> k & 1) && cw[k] > 0)) continue;
+    const int nwords = (P.len[k] + (1 << csh[k]) - 1) >> csh[k];
+    if (cursor + nwords > P.lds_words) continue;
+    stg[k] = true;
+    soff[k] = cursor;
+    any_staged = true;
+    const uint32_t* src = P.cwords[k];
+    for (int i = threadIdx.x; i < nwords; i += BLOCK) ldsw[cursor + i] = src[i];
+    cursor += nwords;
   }
-  if (SINK == GROUPBY_LDS || staged) __syncthreads();
+  if (SINK == GROUPBY_LDS || any_staged) __syncthreads();
 
   long long acc = 0;
   const int ntiles = (P.num_tuples + TILE - 1) / TILE;
@@ -214,18 +229,31 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
       int v[4] = {0, 0, 0, 0};
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        if (SHAPE >> k & 1) v[k] = ring[k * kRing + e];
-      const int a0 = ring[4 * kRing + e], a1 = ring[5 * kRing + e];
+        if ((SHAPE >> k & 1) && (k == kFirst || cb[k])) v[k] = ring[k * kRing + e];
+      const int row = ring[4 * kRing + e];
       __builtin_amdgcn_wave_barrier();
+      // late columns of the entry's row: segment slot, then value
+      const int rseg = hit ? row >> kSegLog : 0, roff = row & kSegMask;
+      int a0 = 0, a1 = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if ((SHAPE >> k & 1) && k != kFirst && !cb[k] && hit)
+          v[k] = P.cache[((size_t)P.key[k][rseg] << kSegLog) + roff];
+      if (P.aggr[0] && hit) a0 = P.cache[((size_t)P.aggr[0][rseg] << kSegLog) + roff];
+      if (P.aggr[1] && hit) a1 = P.cache[((size_t)P.aggr[1][rseg] << kSegLog) + roff];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         if (!(SHAPE >> k & 1) || k == kFirst) continue;
         const int hs = hit ? phash(v[k], P.len[k], P.minkey[k]) : 0;
         if (cw[k] > 0) {
+          // (an existence bitmap, if any, was tested before the ring)
           const int w = cw[k], sh = csh[k];
-          if (cb[k]) hit = hit && (P.cbits[k][hs >> 5] >> (hs & 31) & 1u);
           uint32_t word = 0;
-          if (hit) word = (staged && k == P.lds_k) ? ldsw[hs >> sh] : P.cwords[k][hs >> sh];
+          if (stg[k]) {
+            if (hit) word = ldsw[soff[k] + (hs >> sh)];
+          } else if (hit) {
+            word = P.cwords[k][hs >> sh];
+          }
           const uint32_t ent = (word >> ((hs & ((1 << sh) - 1)) * w)) & ((1u << w) - 1u);
           hit = hit && ent != 0u;
           v[k] = cvmin[k] + (int)ent - 1;
@@ -287,8 +315,11 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
 #pragma unroll
       for (int k = 0; k < 4; k++)
         if (kGen ? ((P.pmask >> k & 1) && P.key[k] && P.ht[k]) : (SHAPE >> k & 1))
-          load_col<BLOCK, VPT, MODE>(P.cache, P.key[k], seg, start, vec, n, lo, sel, kv[k]);
-      if (SINK != COMPACT) {
+          if (!kCompact || k == kFirst || cb[k])
+            load_col<BLOCK, VPT, MODE>(P.cache, P.key[k], seg, start, vec, n, lo, sel, kv[k]);
+      // with survivor compaction, the other keys and the aggregates are
+      // gathered by row in the ring rounds (late materialization)
+      if (SINK != COMPACT && !kCompact) {
 #pragma unroll
         for (int c = 0; c < 2; c++)
           if (P.aggr[c]) load_col<BLOCK, VPT, MODE>(P.cache, P.aggr[c], seg, start, vec, n, lo, sel, av[c]);
@@ -329,7 +360,23 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
     // ---------------------------------------------------------- probes
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      if (kCompact && k != kFirst) continue;
+      if (kCompact && k != kFirst) {
+        // a later table with an existence bitmap: test the bitmap here, for
+        // all items of the tile at once, so that only rows whose slot is
+        // filled reach the ring (whose rounds then skip the bitmap)
+        if (!(SHAPE >> k & 1) || !cb[k]) continue;
+        uint32_t bw[IPT];
+        int hb[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+          hb[j] = (sel >> j & 1u) ? phash(kv[k][j], P.len[k], P.minkey[k]) : 0;
+          bw[j] = P.cbits[k][hb[j] >> 5];
+        }
+#pragma unroll
+        for (int j = 0; j < IPT; j++)
+          if (!(bw[j] >> (hb[j] & 31) & 1u)) sel &= ~(1u << j);
+        continue;
+      }
       const bool probed = kGen ? ((P.pmask >> k & 1) && P.key[k] && P.ht[k]) : (SHAPE >> k & 1);
       if (!probed) {
         const int dflt = SINK == COMPACT ? 1 : 0;  // BlockSetValue(...,1) / (...,0)
@@ -372,9 +419,10 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
         const int w = cw[k], sh = csh[k], vmin = cvmin[k];
         const uint32_t emask = (1u << w) - 1u;
         uint32_t wv[IPT];
-        if (staged && k == P.lds_k) {
+        if (stg[k]) {
+          const uint32_t* lw = ldsw + soff[k];
 #pragma unroll
-          for (int j = 0; j < IPT; j++) wv[j] = ldsw[hs[j] >> sh];
+          for (int j = 0; j < IPT; j++) wv[j] = lw[hs[j] >> sh];
         } else {
           const uint32_t* cwk = P.cwords[k];
 #pragma unroll
@@ -425,9 +473,8 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
           const int e = (ring_tail + pos) & (kRing - 1);
 #pragma unroll
           for (int k = 0; k < 4; k++)
-            if (SHAPE >> k & 1) ring[k * kRing + e] = kv[k][j];
-          ring[4 * kRing + e] = P.aggr[0] ? av[0][j] : 0;
-          ring[5 * kRing + e] = P.aggr[1] ? av[1][j] : 0;
+            if ((SHAPE >> k & 1) && (k == kFirst || cb[k])) ring[k * kRing + e] = kv[k][j];
+          ring[4 * kRing + e] = base_row + item_row<BLOCK>(j);
         }
         ring_tail += __popcll(b);
         if (ring_tail - ring_head >= kWave) {
@@ -597,6 +644,9 @@ constexpr int kLdsBlock = MORDRED_LDS_BLOCK;
 // use fewer, larger workgroups
 constexpr int kAggrBlock = MORDRED_AGGR_BLOCK;
 constexpr size_t kLdsBudget = 80 * 1024;   // dynamic LDS of one workgroup (2 per CU)
+// The 1024-thread group-by kernel runs one workgroup per CU (its registers
+// allow no second), so it may use the whole 160 KiB less the static arrays.
+constexpr size_t kLdsBudgetOneGroup = 159 * 1024;
 constexpr int kCompactMinLen = 1 << 17;    // hash tables above 1 MB get a compact table
 
 // Workgroups the device keeps resident for kernel f (kernels.hip, cached).
@@ -617,14 +667,18 @@ inline size_t group_lds_bytes(int sink, int total_val) {
   return ((size_t)total_val + ((size_t)total_val + 63) / 64) * sizeof(unsigned long long);
 }
 
-inline size_t staged_lds_bytes(int len, int max_w) {
-  const size_t words = ((size_t)len * max_w + 31) / 32;
-  return ((words + 1) / 2) * sizeof(unsigned long long);
+// 32-bit words of a compact table of len entries at width w (entries never
+// straddle words).
+inline size_t staged_words(int len, int w) {
+  const int per = 32 / w;
+  return ((size_t)len + per - 1) / per;
 }
+
+inline size_t ring_lds_bytes(int block) { return (size_t)(block / kWave) * kRing * kRingFields * sizeof(int); }
 
 inline size_t pipe_lds_bytes(const Pipe& P, int sink) {
   size_t b = group_lds_bytes(sink, P.total_val);
-  if (P.lds_k >= 0) b += staged_lds_bytes(P.len[P.lds_k], P.lds_max_w);
+  b += (((size_t)P.lds_words + 1) / 2) * sizeof(unsigned long long);
   return b;
 }
 
@@ -635,7 +689,7 @@ int launch_pipe(const Pipe& Pin, hipStream_t stream) {
   size_t lds = pipe_lds_bytes(P, SINK);
   if (shape_compacts(MODE, SINK, SHAPE)) {
     P.ring_off = (int)(lds / sizeof(unsigned long long));
-    lds += (size_t)(BLOCK / kWave) * kRing * kRingFields * sizeof(int);
+    lds += ring_lds_bytes(BLOCK);
   }
   const void* f = reinterpret_cast<const void*>(&pipeline_kernel<BLOCK, VPT, MODE, SINK, SHAPE>);
   if (lds > 64 * 1024) {
