@@ -117,7 +117,7 @@ __device__ __forceinline__ void choose_width(int vmin, int vmax, int any, int& w
   else { w = 0; shift = 0; }
 }
 
-// blockIdx.y = table; grid-stride over packed words
+// blockIdx.y = table; grid-stride over slots
 __global__ __launch_bounds__(kPrepBlock) void prep_pack_kernel(Prep P) {
   const int t = blockIdx.y;
   const int len = P.len[t];
@@ -154,32 +154,31 @@ __global__ __launch_bounds__(kPrepBlock) void prep_pack_kernel(Prep P) {
     h.w = w; h.shift = shift; h.vmin = vmin; h.any = any;
     *P.hdr[t] = h;
   }
-  if (P.bits[t]) {
-    const int nb = (len + 31) >> 5;
-    for (int bi = blockIdx.x * kPrepBlock + threadIdx.x; bi < nb; bi += gridDim.x * kPrepBlock) {
-      uint32_t word = 0;
-      for (int e = 0; e < 32; e++) {
-        const int s = (bi << 5) + e;
-        int r;
-        if (s < len && slot_result(P.ht[t], s, P.row_probe, r)) word |= 1u << e;
-      }
-      P.bits[t][bi] = word;
-    }
-  }
-  if (w == 0) return;
-  const int epw = 1 << shift;
-  const int nwords = (len + epw - 1) >> shift;
+  // One slot per lane, one coalesced pass: the bitmap words of a wave's 64
+  // slots come from one ballot, a packed word from an OR over the 2^shift
+  // consecutive lanes that hold its entries (groups never straddle a wave).
   const int* ht = P.ht[t];
-  for (int wi = blockIdx.x * kPrepBlock + threadIdx.x; wi < nwords; wi += gridDim.x * kPrepBlock) {
-    uint32_t word = 0;
-    const int s0 = wi << shift;
-    for (int e = 0; e < epw; e++) {
-      const int s = s0 + e;
-      int r;
-      if (s < len && slot_result(ht, s, P.row_probe, r))
-        word |= (uint32_t)(r - vmin + 1) << (e * w);
+  uint32_t* bits = P.bits[t];
+  uint32_t* words = w ? P.words[t] : nullptr;
+  const int nb = (len + 31) >> 5;
+  const int epw = 1 << shift;
+  for (int base = blockIdx.x * kPrepBlock; base < len; base += gridDim.x * kPrepBlock) {
+    const int s = base + (int)threadIdx.x;
+    int r = 0;
+    const bool hit = s < len && slot_result(ht, s, P.row_probe, r);
+    if (bits) {
+      const unsigned long long b = __ballot(hit);
+      if (lane == 0) {
+        const int wb = s >> 5;
+        if (wb < nb) bits[wb] = (uint32_t)b;
+        if (wb + 1 < nb) bits[wb + 1] = (uint32_t)(b >> 32);
+      }
     }
-    P.words[t][wi] = word;
+    if (words) {
+      uint32_t c = hit ? (uint32_t)(r - vmin + 1) << ((s & (epw - 1)) * w) : 0u;
+      for (int d = 1; d < epw; d <<= 1) c |= __shfl_xor(c, d, kWave);
+      if ((s & (epw - 1)) == 0 && s < len) words[s >> shift] = c;
+    }
   }
 }
 

@@ -505,3 +505,23 @@ def test_compact_tables_probe_rows(big_part):
     lib.oracle_probe_CPU(p, Offset(*[iptr(a) for a in ro]), n, iptr(rt), 0, sptr(OQ.segment_group(n)), CONV_GPU)
     assert m == int(rt[0]) > 0
     np.testing.assert_array_equal(_sorted_rows([t.cpu().numpy() for t in outs], m), _sorted_rows(ro, m))
+
+
+def test_lds_staged_tables_and_late_columns(cuda):
+    """A fact scan long enough (>= 256 rows per date slot) that the date table
+    is compacted and staged in LDS next to the supplier table, with the part
+    table's existence bitmap tested per tile and the date keys and revenue
+    gathered by row for the survivors; grouped by (brand, year)."""
+    from mordred_amd import query as Q
+    q = Q.q21(5_000, 300_000)
+    n = 16 * SEG + 9_999
+    assert n >= 256 * q.join(4).dim_len
+    data = Q.make_data(q, n, seed=78, S=5_000, P=300_000)
+    cache = Q.load_cache(data, q, shuffle_seed=12)
+    print("[gpu] LDS-staged tables: fused group-by", flush=True)
+    gq = Q.GpuQuery(q, data, cache)
+    ref = oracle_res(q, data)
+    for _ in range(2):
+        gq.run()
+        sync()
+        np.testing.assert_array_equal(gq.result(), ref)
