@@ -101,6 +101,49 @@ __device__ __forceinline__ void load_col(const int* cache, const int* col, int s
   }
 }
 
+// Segment mode: the cache slot of each column of the current segment, loaded
+// once per segment (all columns at once, made wave-uniform) instead of once
+// per column per tile: a slot load followed by a wait would also wait for the
+// data loads of the columns issued before it (loads complete in issue order).
+template <int N>
+struct SegSlots {
+  int gi = -1;       // segment-group position of the cached slots
+  int seg = 0;
+  int slot[N];
+};
+
+template <int N>
+__device__ __forceinline__ void seg_slots(SegSlots<N>& S, const short* seg_group, int start_offset, int start,
+                                          const int* const (&cols)[N], unsigned used) {
+  const int gi = start >> kSegLog;
+  if (gi == S.gi) return;
+  S.gi = gi;
+  const int seg = seg_group ? (int)seg_group[gi] : (start_offset + start) >> kSegLog;
+  S.seg = __builtin_amdgcn_readfirstlane(seg);
+  int v[N];
+#pragma unroll
+  for (int c = 0; c < N; c++) v[c] = (used >> c & 1u) ? cols[c][S.seg] : 0;
+#pragma unroll
+  for (int c = 0; c < N; c++) S.slot[c] = __builtin_amdgcn_readfirstlane(v[c]);
+}
+
+// Contiguous tile range of this workgroup (persistent grid of balanced_grid
+// size): consecutive tiles share a segment.
+__device__ __forceinline__ void tile_range(int ntiles, int& t0, int& t1) {
+  const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+  t0 = (int)blockIdx.x * per;
+  t1 = min(ntiles, t0 + per);
+}
+
+// Tile load of a contiguous (SEG / PLAIN) tile; SEG takes the column's slot.
+template <int BLOCK, int VPT, int MODE>
+__device__ __forceinline__ void load_eager(const int* cache, const int* col, int slot, int start, bool vec,
+                                           int n, int (&x)[4 * VPT]) {
+  static_assert(MODE != OFF, "contiguous tiles only");
+  if (MODE == SEG) load_tile<BLOCK, VPT>(cache + ((size_t)slot << kSegLog) + (start & kSegMask), vec, n, x);
+  else load_tile<BLOCK, VPT>(col + start, vec, n, x);
+}
+
 template <int VPT>
 __device__ __forceinline__ uint32_t pred_between(const int (&x)[4 * VPT], int lo, int hi) {
   uint32_t m = 0;
@@ -280,7 +323,22 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
     }
   };
 
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  // slots 0-1 filter columns, 2-5 probe keys, 6-7 aggregates: those the
+  // tile loads below read
+  SegSlots<8> S;
+  const int* const scols[8] = {P.fcol[0], P.fcol[1], P.key[0], P.key[1], P.key[2], P.key[3], P.aggr[0], P.aggr[1]};
+  unsigned sused = 0;
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    if (kGen ? P.fcol[c] != nullptr : (SHAPE >> (4 + c) & 1)) sused |= 1u << c;
+    if (P.aggr[c] && SINK != COMPACT && !kCompact) sused |= 1u << (6 + c);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (kGen ? ((P.pmask >> k & 1) && P.key[k] && P.ht[k]) : (SHAPE >> k & 1)) sused |= 1u << (2 + k);
+  int tile0, tile1;
+  tile_range(ntiles, tile0, tile1);
+  for (int tile = tile0; tile < tile1; tile++) {
     const int start = tile * TILE;
     const int n = min(TILE, P.num_tuples - start);
     const bool full = n == TILE;
@@ -292,7 +350,8 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
     int seg = 0, base_row = 0;
     int lo[IPT];
     if (MODE == SEG) {
-      seg = P.seg_group ? (int)P.seg_group[start >> kSegLog] : (P.start_offset + start) >> kSegLog;
+      seg_slots(S, P.seg_group, P.start_offset, start, scols, sused);
+      seg = S.seg;
       base_row = seg * kSegSize + (start & kSegMask);
     } else if (MODE == PLAIN) {
       base_row = P.row_base + start;
@@ -307,22 +366,22 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
     // column values: fv filters, kv probe keys (replaced in place by the probe
     // result: group value or dimension row), av aggregates
     int fv[2][IPT], kv[4][IPT], av[2][IPT];
-    if (kEager) {
+    if constexpr (kEager) {
 #pragma unroll
       for (int c = 0; c < 2; c++)
         if (kGen ? P.fcol[c] != nullptr : (SHAPE >> (4 + c) & 1))
-          load_col<BLOCK, VPT, MODE>(P.cache, P.fcol[c], seg, start, vec, n, lo, sel, fv[c]);
+          load_eager<BLOCK, VPT, MODE>(P.cache, P.fcol[c], S.slot[c], start, vec, n, fv[c]);
 #pragma unroll
       for (int k = 0; k < 4; k++)
         if (kGen ? ((P.pmask >> k & 1) && P.key[k] && P.ht[k]) : (SHAPE >> k & 1))
           if (!kCompact || k == kFirst || cb[k])
-            load_col<BLOCK, VPT, MODE>(P.cache, P.key[k], seg, start, vec, n, lo, sel, kv[k]);
+            load_eager<BLOCK, VPT, MODE>(P.cache, P.key[k], S.slot[2 + k], start, vec, n, kv[k]);
       // with survivor compaction, the other keys and the aggregates are
       // gathered by row in the ring rounds (late materialization)
       if (SINK != COMPACT && !kCompact) {
 #pragma unroll
         for (int c = 0; c < 2; c++)
-          if (P.aggr[c]) load_col<BLOCK, VPT, MODE>(P.cache, P.aggr[c], seg, start, vec, n, lo, sel, av[c]);
+          if (P.aggr[c]) load_eager<BLOCK, VPT, MODE>(P.cache, P.aggr[c], S.slot[6 + c], start, vec, n, av[c]);
       }
     }
 
@@ -582,7 +641,12 @@ __global__ __launch_bounds__(BLOCK) void build_kernel(Build B) {
   constexpr int IPT = 4 * VPT;
   constexpr int TILE = BLOCK * IPT;
   const int ntiles = (B.num_tuples + TILE - 1) / TILE;
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  SegSlots<3> S;  // filter, key, value columns
+  const int* const scols[3] = {B.fcol, B.key, B.val};
+  const unsigned sused = (B.fcol ? 1u : 0u) | 2u | (B.val ? 4u : 0u);
+  int tile0, tile1;
+  tile_range(ntiles, tile0, tile1);
+  for (int tile = tile0; tile < tile1; tile++) {
     const int start = tile * TILE;
     const int n = min(TILE, B.num_tuples - start);
     const bool full = n == TILE;
@@ -592,7 +656,8 @@ __global__ __launch_bounds__(BLOCK) void build_kernel(Build B) {
     uint32_t sel = rmask;
     int seg = 0, base_row = 0, row[IPT];
     if (MODE == SEG) {
-      seg = B.seg_group ? (int)B.seg_group[start >> kSegLog] : (B.start_offset + start) >> kSegLog;
+      seg_slots(S, B.seg_group, B.start_offset, start, scols, sused);
+      seg = S.seg;
       base_row = seg * kSegSize + (start & kSegMask);
     } else if (MODE == PLAIN) {
       base_row = B.row_base + start;
@@ -604,13 +669,19 @@ __global__ __launch_bounds__(BLOCK) void build_kernel(Build B) {
       for (int j = 0; j < IPT; j++) row[j] = base_row + item_row<BLOCK>(j);
     }
     int x[IPT], v[IPT];
-    if (MODE != OFF && B.fcol) {
-      load_col<BLOCK, VPT, MODE>(B.cache, B.fcol, seg, start, vec, n, row, sel, x);
-      sel &= pred_mode<VPT>(x, B.fmode, B.c1, B.c2, rmask);
+    if constexpr (MODE != OFF) {
+      if (B.fcol) {
+        load_eager<BLOCK, VPT, MODE>(B.cache, B.fcol, S.slot[0], start, vec, n, x);
+        sel &= pred_mode<VPT>(x, B.fmode, B.c1, B.c2, rmask);
+      }
+      if (!wave_any(sel)) continue;
+      load_eager<BLOCK, VPT, MODE>(B.cache, B.key, S.slot[1], start, vec, n, x);
+      if (B.val) load_eager<BLOCK, VPT, MODE>(B.cache, B.val, S.slot[2], start, vec, n, v);
+    } else {
+      if (!wave_any(sel)) continue;
+      load_col<BLOCK, VPT, MODE>(B.cache, B.key, seg, start, vec, n, row, sel, x);
+      if (B.val) load_col<BLOCK, VPT, MODE>(B.cache, B.val, seg, start, vec, n, row, sel, v);
     }
-    if (!wave_any(sel)) continue;
-    load_col<BLOCK, VPT, MODE>(B.cache, B.key, seg, start, vec, n, row, sel, x);
-    if (B.val) load_col<BLOCK, VPT, MODE>(B.cache, B.val, seg, start, vec, n, row, sel, v);
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
       if (!(sel >> j & 1u)) continue;
