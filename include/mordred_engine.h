@@ -202,8 +202,8 @@ int mordred_engine_batch_tune(mordred_engine* e, const mordred_star_query* qs, i
  *   MORDRED_SERVE_CONTINUOUS: a circular shared scan.  The GPU walks the fact
  *     table in launches of consecutive segments; every launch runs the shared
  *     scan for all active queries.  A request is admitted at the next launch
- *     boundary when a query slot is free and its result and touched-bit memory
- *     fits the HBM budget, and finishes once the scan has covered every
+ *     boundary when a query slot is free and its result rows (24 bytes per
+ *     group) fit the HBM budget, and finishes once the scan has covered every
  *     segment exactly once for it (launches never cross an active query's
  *     starting segment), so its result equals its own run bit for bit.
  *     Launch sizes (segments per launch) are chosen per launch so that the

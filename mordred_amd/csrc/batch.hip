@@ -2,8 +2,8 @@
 // and the batch descriptor).
 //
 //   batch_fill_kernel     clears the packed tables of the used dimensions (when
-//                         rebuilt), and the result rows and global touched
-//                         bits of the queries being started;
+//                         rebuilt), and the result rows of the queries being
+//                         started;
 //   batch_records_kernel  one pass over a dimension: slot and packed entry
 //                         (the builds of all the batch's queries in one);
 //   batch_kernel          the persistent shared scan of the fact columns.
@@ -19,7 +19,7 @@ namespace mordred {
 
 
 // Clears the packed tables of the used dimensions when they are rebuilt, and
-// the result rows and global touched bits of the queries being (re)started.
+// the result rows of the queries being (re)started.
 __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __restrict__ D, unsigned* status,
                                                          uint4* r0, uint4* r1, uint4* r2, uint4* r3,
                                                          int n0, int n1, int n2, int n3) {
@@ -44,11 +44,6 @@ __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __rest
     int* res = reinterpret_cast<int*>(Q.res);
     const long long ni = 6 * (long long)Q.total_val;
     for (long long i = t; i < ni; i += stride) res[i] = 0;
-    if (!(Q.flags & kQueryAggr)) {
-      unsigned* tw = D->touch_arena + Q.gtouch;
-      const int nw = (Q.total_val + 31) / 32;
-      for (long long i = t; i < nw; i += stride) tw[i] = 0;
-    }
   }
 }
 
@@ -91,14 +86,11 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   unsigned long long* llut = reinterpret_cast<unsigned long long*>(lds_raw + nq * sizeof(BatchQuery));
   unsigned long long* lsum = llut + D->nlut;
   const int nsum = D->lds_sum_words;
-  uint32_t* ltouch = reinterpret_cast<uint32_t*>(lsum + nsum);
-  const int ntouch = D->lds_touch_words;
   {
     const uint4* src = reinterpret_cast<const uint4*>(D->q);
     uint4* dst = reinterpret_cast<uint4*>(lq);
     for (int i = threadIdx.x; i < nq * (int)(sizeof(BatchQuery) / 16); i += BLOCK) dst[i] = src[i];
     for (int i = threadIdx.x; i < nsum; i += BLOCK) lsum[i] = 0;
-    for (int i = threadIdx.x; i < ntouch; i += BLOCK) ltouch[i] = 0;
     const int np = D->npred;
     for (int i = threadIdx.x; i < np; i += BLOCK) {
       const BatchPred P = D->pred[i];
@@ -155,7 +147,6 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   const int usedmask = __builtin_amdgcn_readfirstlane(H.usedmask);
   const int nexpr = __builtin_amdgcn_readfirstlane(H.nexpr);
   const unsigned long long all = D->all;
-  unsigned* const touch_arena = D->touch_arena;
   const int lane = threadIdx.x & (kWave - 1);
 
   const int num_tuples = D->num_tuples;
@@ -397,21 +388,15 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
           if (e == Q.expr) val = ev[e][j];
         const unsigned long long add = (unsigned long long)(long long)val;
         int* res = reinterpret_cast<int*>(Q.res) + 6 * (size_t)h;
-        const uint32_t bit = 1u << (h & 31);
-        bool first = false;
-        if (Q.lds >= 0) {
-          atomicAdd(&lsum[Q.lds + h], add);
-          if (!(Q.flags & kQueryAggr)) first = !(atomicOr(&ltouch[Q.touch + (h >> 5)], bit) & bit);
-        } else {
-          atomicAdd(reinterpret_cast<unsigned long long*>(res + 4), add);
-          if (!(Q.flags & kQueryAggr)) {
-            unsigned* tw = touch_arena + Q.gtouch + (h >> 5);
-            first = !(*tw & bit) && !(atomicOr(tw, bit) & bit);
-          }
-        }
-        if (first) {
-          // the reference stores the keys for every row of the group
-          // (GPUProcessing.h:144-147); they are equal, so one store suffices
+        // The keys are stored whenever the sum the add returns was zero: at
+        // the first pair of the group in this workgroup (LDS) or overall
+        // (global), and again only if a running sum passes through zero.
+        // The reference stores them for every row of the group
+        // (GPUProcessing.h:144-147); they are equal, so any store suffices.
+        unsigned long long old;
+        if (Q.lds >= 0) old = atomicAdd(&lsum[Q.lds + h], add);
+        else old = atomicAdd(reinterpret_cast<unsigned long long*>(res + 4), add);
+        if (!(Q.flags & kQueryAggr) && old == 0ull) {
           int gv[4];
 #pragma unroll
           for (int d = 0; d < 4; d++)

@@ -51,10 +51,7 @@ struct BatchQuery {            // 64 bytes; staged in LDS by the scan kernel
   short expr;                  // index into BatchDesc::expr
   short flags;                 // kQueryAggr: no key words
   int lds;                     // u64 offset of its group array in LDS, -1 = global atomics
-  int touch;                   // u32 offset of its LDS touched bitmap (when lds >= 0)
-  int gtouch;                  // u32 offset of its touched bitmap in the global arena,
-                               // valid whatever the placement (a query of a request
-                               // loop may change placement between launches)
+  int unused[2];
   signed char attr[4];         // per dimension slot: -1 no group value, else its attribute
   // dense group index (GPUProcessing.h:142) as hbase + sum_d field_d * uniq[d]
   // with field_d = (packed entry >> shift[d]) & ((1 << bits[d]) - 1) and hbase
@@ -109,13 +106,11 @@ struct BatchDesc {
   int seg_begin, num_tuples;
   int rebuild;                 // record tables must be rebuilt (query set changed)
   int pad0;
-  unsigned long long reset;    // queries whose result / touched bits are cleared first
+  unsigned long long reset;    // queries whose result rows are cleared first
   int lds_sum_words;           // u64 words of LDS group arrays
-  int lds_touch_words;         // u32 words of LDS touched bitmaps
+  int pad2;
   long long res_ints;          // result arena ints
-  long long touch_words;       // global touched words
   int* res_arena;
-  unsigned* touch_arena;
   const int* col[kBatchMaxCols];
   int pred_begin[kBatchMaxCols + 1];
   BatchPred pred[kBatchMaxPred];
@@ -161,9 +156,7 @@ struct BatchConfig {
 };
 
 // Bytes of LDS a query's group array needs.
-inline size_t batch_group_lds(int total_val) {
-  return (size_t)total_val * 8 + (size_t)((total_val + 31) / 32) * 4;
-}
+inline size_t batch_group_lds(int total_val) { return (size_t)total_val * 8; }
 
 // Host launchers (batch.hip).  Every launch reads the batch from L.desc.
 // rec / rec_len: packed tables and their sizes in 16-byte units

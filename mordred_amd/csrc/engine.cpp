@@ -180,7 +180,7 @@ struct mordred_engine {
   DevBuf d_res, d_sg, d_ht[4];
   HostBuf h_stage;  // pinned scratch for segment groups / index updates
   // shared-scan batches: record tables, descriptor, result arena, the graph
-  DevBuf d_rec[4], d_desc, d_arena, d_touch, d_status;
+  DevBuf d_rec[4], d_desc, d_arena, d_status;
   HostBuf h_desc, h_arena;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -189,7 +189,7 @@ struct mordred_engine {
   mordred::BatchConfig batch_cfg{512, 1, 64 * 1024};
   bool batch_tuned = false;
   // request loop: persistent slot arenas and the learned cost model
-  DevBuf d_serve_res, d_serve_touch;
+  DevBuf d_serve_res;
   mordred_cost_model* cm = nullptr;
   std::string err;
 };
@@ -993,15 +993,12 @@ bool batch_probes(const mordred_star_query& q, const Resolved& r, int k) {
 }
 
 // Slots of a batch: query i holds bit i of every mask.  An empty slot
-// (qp[i] == NULL) takes no part.  With `fixed` set, slot i's result and
-// global touched bits sit at fixed strides of the given arenas (a serving
-// loop keeps them across launches); otherwise they are packed into the
-// engine's batch arenas.
+// (qp[i] == NULL) takes no part.  With `fixed` set, slot i's result rows sit
+// at a fixed stride of the given arena (a serving loop keeps them across
+// launches); otherwise they are packed into the engine's batch arena.
 struct SlotArenas {
   int* res = nullptr;
-  unsigned* touch = nullptr;
   long long res_stride = 0;    // ints per slot
-  long long touch_stride = 0;  // words per slot
 };
 
 int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, const mordred::BatchConfig& cfg,
@@ -1230,46 +1227,34 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       lds_used += need;
     }
   }
-  long long sum_words = 0, lds_touch = 0, g_touch = 0, res_ints = 0;
+  long long sum_words = 0, res_ints = 0;
   for (int i = 0; i < nq; i++) {
     BatchQuery& Q = D.q[i];
     if (!qp[i]) {
       Q.lds = -1;
       continue;
     }
-    const int tw = (b.G[i] + 31) / 32;
-    Q.gtouch = fixed ? (int)(i * fixed->touch_stride) : (int)g_touch;
-    g_touch += tw;
     if (in_lds[i]) {
       Q.lds = (int)sum_words;
-      Q.touch = (int)lds_touch;
       sum_words += b.G[i];
-      lds_touch += tw;
       b.lds_queries++;
     } else {
       Q.lds = -1;
-      Q.touch = Q.gtouch;
       b.global_queries++;
     }
-    if (fixed && (6 * (long long)b.G[i] > fixed->res_stride || tw > fixed->touch_stride))
+    if (fixed && 6 * (long long)b.G[i] > fixed->res_stride)
       return fail(e, "query result larger than its serving slot");
     b.res_off[i] = fixed ? i * fixed->res_stride : res_ints;
     res_ints += 6 * (long long)b.G[i];
   }
   D.lds_sum_words = (int)sum_words;
-  D.lds_touch_words = (int)lds_touch;
   if (fixed) {
     D.res_ints = nq * fixed->res_stride;
-    D.touch_words = nq * fixed->touch_stride;
     D.res_arena = fixed->res;
-    D.touch_arena = fixed->touch;
   } else {
     D.res_ints = (res_ints + 3) & ~3ll;
-    D.touch_words = (g_touch + 3) & ~3ll;
     HIP_OK(e->d_arena.reserve(std::max<long long>(16, D.res_ints * 4)));
-    HIP_OK(e->d_touch.reserve(std::max<long long>(16, D.touch_words * 4)));
     D.res_arena = static_cast<int*>(e->d_arena.p);
-    D.touch_arena = static_cast<unsigned*>(e->d_touch.p);
   }
   for (int i = 0; i < nq; i++) {
     D.q[i].res = (long long)(uintptr_t)(D.res_arena + b.res_off[i]);
@@ -2003,7 +1988,7 @@ int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n,
   const BatchConfig cfg = o->config.block > 0 ? BatchConfig{o->config.block, o->config.vpt, o->config.lds_bytes}
                                               : e->batch_cfg;
   if (!batch_config_ok(cfg)) return fail(e, "serve: bad batch configuration");
-  // per-request estimates and the slot arenas (result rows and touched bits)
+  // per-request estimates and the slot arena (result rows)
   std::vector<QueryEst> est(n);
   int gmax = 1;
   for (int i = 0; i < n; i++) {
@@ -2013,12 +1998,9 @@ int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n,
   }
   SlotArenas ar;
   ar.res_stride = ((6 * (long long)gmax) + 3) & ~3ll;
-  ar.touch_stride = (((gmax + 31) / 32) + 3) & ~3ll;
   HIP_OK(e->d_serve_res.reserve((size_t)nslot * ar.res_stride * sizeof(int)));
-  HIP_OK(e->d_serve_touch.reserve((size_t)nslot * ar.touch_stride * sizeof(unsigned)));
   ar.res = static_cast<int*>(e->d_serve_res.p);
-  ar.touch = static_cast<unsigned*>(e->d_serve_touch.p);
-  auto state_bytes = [&](int i) { return 24.0 * est[i].G + 4.0 * ((est[i].G + 31) / 32); };
+  auto state_bytes = [&](int i) { return 24.0 * est[i].G; };
   if (o->learned && !e->cm) {
     e->cm = mordred_cost_model_create(e->device, 12345);
     if (!e->cm) return fail(e, "serve: cost model creation failed");
