@@ -197,14 +197,19 @@ def batch_bench(args):
     t0 = time.time()
     eng.add_mix_data(data, qs)
     log(f"loaded into the segment cache in {time.time() - t0:.1f}s")
-    best, table = eng.batch_tune(qs, reps=2)
-    log(f"autotuner: best {best}; " + ", ".join(f"{c}:{ms:.2f}" for c, ms in table))
+    if args.batch_config:
+        best = tuple(int(x) for x in args.batch_config.split(","))
+        table = []
+        log(f"launch configuration {best} (autotuner skipped)")
+    else:
+        best, table = eng.batch_tune(qs, reps=2)
+        log(f"autotuner: best {best}; " + ", ".join(f"{c}:{ms:.2f}" for c, ms in table))
     for _ in range(args.warmup):
-        eng.batch(qs)
+        eng.batch(qs, best)
     tot, gpu = [], []
     outs = st = None
     for _ in range(args.steps):
-        outs, st = eng.batch(qs)
+        outs, st = eng.batch(qs, best)
         tot.append(st["total_ms"])
         gpu.append(st["gpu_ms"])
     batch_ms = statistics.median(tot)
@@ -503,6 +508,8 @@ def main():
     ap.add_argument("--rates", default="10,50,200,500,1000,2000", help="serve: offered rates (q/s)")
     ap.add_argument("--target-chunk-ms", type=float, default=8.0, help="serve: launch-time target")
     ap.add_argument("--rows", type=int, default=None, help="override fact rows")
+    ap.add_argument("--batch-config", default=None,
+                    help="batch64: threads,rows-per-thread-group,lds-bytes (skips the autotuner)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity check")
     ap.add_argument("--copy-probe", action="store_true",
