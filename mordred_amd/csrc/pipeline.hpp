@@ -429,7 +429,9 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
 #pragma unroll
         for (int j = 0; j < IPT; j++) {
           hb[j] = (sel >> j & 1u) ? phash(kv[k][j], P.len[k], P.minkey[k]) : 0;
-          bw[j] = P.cbits[k][hb[j] >> 5];
+          // masked: lanes without a survivor issue no address
+          if (sel >> j & 1u) bw[j] = P.cbits[k][hb[j] >> 5];
+          else bw[j] = 0xffffffffu;
         }
 #pragma unroll
         for (int j = 0; j < IPT; j++)
@@ -463,8 +465,10 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
       const int len = P.len[k], mk = P.minkey[k];
       // All slot reads of the tile are issued before any is used, so the
       // 4*VPT probes of a thread overlap instead of paying one L2/MALL round
-      // trip each.  Items that are not probed read slot 0 (always valid), which
-      // the address coalescer folds into a single request per wave.
+      // trip each.  Compact-table reads in global memory are masked to the
+      // items still selected (lanes without one issue no address, which
+      // unloads the texture address unit); hash-table and LDS reads of
+      // unselected items read slot 0 (measured faster for them).
       int hs[IPT];
       uint32_t act = sel;
 #pragma unroll
@@ -485,7 +489,10 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
         } else {
           const uint32_t* cwk = P.cwords[k];
 #pragma unroll
-          for (int j = 0; j < IPT; j++) wv[j] = cwk[hs[j] >> sh];
+          for (int j = 0; j < IPT; j++) {
+            if (act >> j & 1u) wv[j] = cwk[hs[j] >> sh];
+            else wv[j] = 0u;
+          }
         }
 #pragma unroll
         for (int j = 0; j < IPT; j++) {
