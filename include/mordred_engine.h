@@ -144,17 +144,20 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
 /* ---- shared-scan batches ------------------------------------------------
  * No counterpart in the reference, which runs one query at a time: up to 64
  * star queries execute in one persistent kernel that reads the fact columns
- * once.  Each dimension is probed once per fact row through a record table
- * that holds, per hash slot, the mask of queries whose filter the row passes
- * and its group attributes; every query's result equals its own
- * mordred_engine_run (GPU convention) bit for bit.  The launch sequence
- * (clear, record builds, scan) is captured in a hipGraph on first use and
- * replayed for every later batch of the same dataset and configuration.
+ * once.  Each dimension is probed once per fact row through a packed
+ * attribute table: per hash slot a present bit and the attributes any query
+ * of the batch filters or groups on (1, 2 or 4 bytes).  The queries' filters
+ * on a dimension become per-attribute tables of 64-bit query masks (narrow
+ * attributes) or per-query range compares; every query's result equals its
+ * own mordred_engine_run (GPU convention) bit for bit.  The launch sequence
+ * (clear, packed-table builds, scan) is captured in a hipGraph on first use
+ * and replayed for every later batch of the same dataset and configuration.
  * Requirements: every fact column of the batch cached in HBM (all segments),
  * dimension columns GPU-resident, queries that join one dimension agree on
- * its key columns, dim_len and min_key, at most 2 distinct group columns per
- * dimension with a value span below 65536, at most 12 distinct fact columns
- * and 4 distinct aggregate expressions, collision-free dimension hashing. */
+ * its fact key, key column, dim_len and min_key, dimension keys within
+ * [min_key, min_key + dim_len) (no hash collisions), at most 3 filter and
+ * group columns per dimension packing into 31 bits, at most 12 distinct fact
+ * columns and 4 distinct aggregate expressions. */
 #define MORDRED_BATCH_MAX_QUERIES 64
 
 typedef struct mordred_batch_config {
