@@ -132,6 +132,26 @@ def split_bench(args):
         runs[name] = {"ms_median": statistics.median(times), "stats": last[1],
                       "rows_per_s": n / (statistics.median(times) * 1e-3), "res": last[0]}
         log(f"{name}: {statistics.median(times):.1f} ms  {last[1]}")
+    # learned placement: train online (exploring random candidate plans), then
+    # let the model rank the plans while it beats the analytic model
+    eng.set_learned_placement(E.LEARNED_EXPLORE)
+    for _ in range(args.learn_runs):
+        eng.run(q, E.PLACE_COST_SPLIT)
+    eng.set_learned_placement(E.LEARNED_USE)
+    times, last, chose = [], None, {0: 0, 1: 0}
+    for _ in range(args.steps):
+        r, st = eng.run(q, E.PLACE_COST_SPLIT)
+        times.append(st["total_ms"])
+        chose[st["plan_model"]] = chose.get(st["plan_model"], 0) + 1
+        last = (r, st)
+    runs["cost_split_learned"] = {"ms_median": statistics.median(times), "stats": last[1],
+                                  "rows_per_s": n / (statistics.median(times) * 1e-3), "res": last[0]}
+    learned = {"train_runs": args.learn_runs, "model": eng.placement_model_stats(),
+               "plans_by_learned_model": chose.get(1, 0), "plans_by_analytic_model": chose.get(0, 0),
+               "streamed_segments": last[1]["plan_streamed"], "predicted_ms": last[1]["plan_predicted_ms"],
+               "analytic_ms": last[1]["plan_analytic_ms"]}
+    log(f"cost_split_learned: {statistics.median(times):.1f} ms  {learned}")
+    eng.set_learned_placement(E.LEARNED_OFF)
     best = min(runs, key=lambda k: runs[k]["ms_median"])
     out = {
         "metric": METRIC + " (segment cache + CPU/GPU split)",
@@ -151,6 +171,7 @@ def split_bench(args):
                    "group_by": "(c_nation, s_nation, d_year) 25x25x7 int64 SUM(revenue - supplycost)",
                    "placement": best, "cpu_threads": threads},
         "calibrated_bandwidth": {"cpu_Bps": bw_cpu, "pcie_h2d_Bps": bw_pci},
+        "learned_placement": learned,
         "placements": {k: {"ms": v["ms_median"], "rows_per_s": v["rows_per_s"],
                            "stats": {a: b for a, b in v["stats"].items()}} for k, v in runs.items()},
     }
@@ -508,6 +529,8 @@ def main():
     ap.add_argument("--rates", default="10,50,200,500,1000,2000", help="serve: offered rates (q/s)")
     ap.add_argument("--target-chunk-ms", type=float, default=8.0, help="serve: launch-time target")
     ap.add_argument("--rows", type=int, default=None, help="override fact rows")
+    ap.add_argument("--learn-runs", type=int, default=48,
+                    help="q4way-split: runs that train the learned placement model before it is used")
     ap.add_argument("--batch-config", default=None,
                     help="batch64: threads,rows-per-thread-group,lds-bytes (skips the autotuner)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")

@@ -79,6 +79,11 @@ typedef struct mordred_run_stats {
   double est_cpu_ms_per_segment;    /* analytic model */
   double est_stream_ms_per_segment; /* analytic model */
   int skipped_segments;             /* fact segments excluded by min/max */
+  /* COST_SPLIT with a learned placement model (mordred_engine_set_learned_placement) */
+  int plan_model;                   /* 0 analytic, 1 learned, 2 explored at random */
+  int plan_streamed;                /* non-resident segments the plan streams */
+  double plan_predicted_ms;         /* learned model's time for the plan (plan_model 1) */
+  double plan_analytic_ms;          /* analytic model's time for the plan */
 } mordred_run_stats;
 
 /* Engine on HIP device `device` with an HBM segment cache of cache_segments
@@ -126,6 +131,31 @@ int mordred_engine_store_segment_minmax(mordred_engine* e, const char* name, con
 /* mins / maxs receive one entry per segment (mordred_engine_column_segments). */
 int mordred_engine_segment_minmax(mordred_engine* e, const char* name, int* mins, int* maxs);
 int mordred_engine_set_skipping(mordred_engine* e, int on);
+
+/* ---- learned placement -------------------------------------------------
+ * No counterpart in the reference, whose optimizer places segments with the
+ * analytic model alone (CostModel.cu:283-440, QueryOptimizer.cu:1718-1747).
+ * With a mode other than OFF, MORDRED_PLACE_COST_SPLIT ranks candidate plans
+ * (stream k = n*c/8 of the n non-resident segments, c = 0..8, plus the
+ * analytic k) with an fp32 MLP (mordred_costmodel.h) that predicts the run's
+ * time from query and plan features.  It is trained on the measured times of
+ * the runs themselves, scored against the analytic model on every new run,
+ * and chooses a plan only while it is the more accurate of the two (else the
+ * analytic plan runs).  EXPLORE also runs a random candidate one time in
+ * four, so that the model sees more than the plans it picks.  Every plan
+ * gives the same exact result. */
+enum { MORDRED_LEARNED_OFF = 0, MORDRED_LEARNED_USE = 1, MORDRED_LEARNED_EXPLORE = 2 };
+
+typedef struct mordred_placement_stats {
+  long long samples;      /* runs observed */
+  int scored;
+  double learned_error;   /* EWMA |log error| on fresh runs */
+  double analytic_error;
+  int trusted;
+} mordred_placement_stats;
+
+int mordred_engine_set_learned_placement(mordred_engine* e, int mode);
+int mordred_engine_placement_model_stats(mordred_engine* e, mordred_placement_stats* st);
 
 /* Analytic-model bandwidths in bytes per second (CostModel.h:5-6 defaults
  * 42e9 and 12e9). */

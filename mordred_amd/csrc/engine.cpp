@@ -191,6 +191,12 @@ struct mordred_engine {
   // request loop: persistent slot arenas and the learned cost model
   DevBuf d_serve_res;
   mordred_cost_model* cm = nullptr;
+  // learned plan model for MORDRED_PLACE_COST_SPLIT (0 off, 1 use when
+  // trusted, 2 also explore), trained on the runs' measured times
+  mordred_cost_model* cm_place = nullptr;
+  int place_mode = 0;
+  int place_since_train = 0;
+  unsigned long long place_rng = 0x2545F4914F6CDD1Dull;
   std::string err;
 };
 
@@ -551,6 +557,40 @@ double cpu_cost(const Resolved& r, const Selectivity& sel, double L, double bw_c
 // CostModel::transfer_cost(M): every fact column of the segment over PCIe.
 double transfer_cost(const Resolved& r, double L, double bw_pci) {
   return L * 4 * (double)r.fact_cols.size() / bw_pci;
+}
+
+// ------------------------------------------------------------ learned split
+// Plans of MORDRED_PLACE_COST_SPLIT: stream k of the n non-resident segments
+// over PCIe, the other n - k on the CPU operators.  Every plan gives the same
+// exact result; only the time differs.  The analytic model times a plan as
+// the slower of its two concurrent sides; the learned model (an fp32 MLP,
+// mordred_costmodel.h) predicts the whole run's time from the features below
+// and is trained on the measured times of the runs themselves.
+double split_analytic_ms(int n, int k, double t_cpu, double t_str) {
+  return std::max((n - k) * t_cpu, k * t_str);
+}
+
+void split_features(const mordred_engine* e, const Resolved& r, const Selectivity& sel, int cached, int n, int k,
+                    double t_cpu, double t_str, float* f) {
+  std::fill(f, f + MORDRED_COST_FEATURES, 0.f);
+  const mordred_star_query& q = *r.q;
+  int njoins = 0;
+  for (int j = 0; j < 4; j++) njoins += r.join[j] != nullptr;
+  f[0] = n > 0 ? (float)k / (float)n : 0.f;
+  f[1] = (float)n / 1024.f;
+  f[2] = (float)cached / 1024.f;
+  for (int j = 0; j < 4; j++) f[3 + j] = r.join[j] ? (float)sel.join[j] : 1.f;
+  f[7] = (float)sel.filt[0];
+  f[8] = (float)sel.filt[1];
+  f[9] = (float)r.fact_cols.size() / 8.f;
+  f[10] = (float)njoins / 4.f;
+  f[11] = (float)q.nfilters / 2.f;
+  f[12] = (float)(std::log2((double)std::max(1, q.total_val)) / 16.0);
+  f[13] = q.kind == MORDRED_KIND_GROUPBY ? 1.f : 0.f;
+  f[14] = (float)std::log(std::max(t_cpu, 1e-3));
+  f[15] = (float)std::log(std::max(t_str, 1e-3));
+  f[16] = (float)std::log(std::max(split_analytic_ms(n, k, t_cpu, t_str), 1e-3));
+  f[17] = (float)(std::log2((double)std::max(1, e->pool->size())) / 5.0);
 }
 
 // ------------------------------------------------------------ CPU operators
@@ -1535,6 +1575,7 @@ void mordred_engine_destroy(mordred_engine* e) {
   for (auto& kv : e->cols) free_column(e, kv.second.get());
   e->cols.clear();
   if (e->cm) mordred_cost_model_destroy(e->cm);
+  if (e->cm_place) mordred_cost_model_destroy(e->cm_place);
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
@@ -1812,7 +1853,41 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     // both sides run concurrently: stream k segments so that k*t_str and
     // (n-k)*t_cpu finish together
     const int n = (int)other.size();
-    const int k = (int)std::min<double>(n, std::max(0.0, n * t_cpu / (t_cpu + t_str) + 0.5));
+    int k = (int)std::min<double>(n, std::max(0.0, n * t_cpu / (t_cpu + t_str) + 0.5));
+    if (e->cm_place && e->place_mode > 0 && n > 0) {
+      // candidate plans: k = n*c/8 for c = 0..8, and the analytic k
+      std::vector<int> ks;
+      for (int c = 0; c <= 8; c++) ks.push_back((int)((long long)n * c / 8));
+      ks.push_back(k);
+      std::sort(ks.begin(), ks.end());
+      ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+      std::vector<float> xs(ks.size() * MORDRED_COST_FEATURES), pred(ks.size());
+      for (size_t c = 0; c < ks.size(); c++)
+        split_features(e, r, sel, (int)g.gpu_segs.size(), n, ks[c], t_cpu, t_str, &xs[c * MORDRED_COST_FEATURES]);
+      mordred_cost_model_stats cs{};
+      // a model error leaves the analytic plan (the GPU half already runs)
+      if (mordred_cost_model_get_stats(e->cm_place, &cs) == 0 && cs.trusted &&
+          mordred_cost_model_predict(e->cm_place, xs.data(), (int)ks.size(), pred.data()) == 0) {
+        size_t best = 0;
+        for (size_t c = 1; c < ks.size(); c++)
+          if (pred[c] < pred[best]) best = c;
+        k = ks[best];
+        st.plan_model = 1;
+        st.plan_predicted_ms = std::exp((double)pred[best]);
+      }
+      if (e->place_mode == 2) {
+        // exploration: one plan in four is drawn at random
+        e->place_rng ^= e->place_rng << 13;
+        e->place_rng ^= e->place_rng >> 7;
+        e->place_rng ^= e->place_rng << 17;
+        if ((e->place_rng & 3) == 0) {
+          k = ks[(size_t)((e->place_rng >> 8) % ks.size())];
+          st.plan_model = 2;
+        }
+      }
+      st.plan_analytic_ms = split_analytic_ms(n, k, t_cpu, t_str);
+      st.plan_streamed = k;
+    }
     stream_segs.assign(other.begin(), other.begin() + k);
     cpu_segs.assign(other.begin() + k, other.end());
   } else {
@@ -1858,7 +1933,47 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   }
   st.merge_ms = ms_since(tm0);
   st.total_ms = ms_since(t0);
+  // learn from the measured run (a plan the model was given)
+  if (placement == MORDRED_PLACE_COST_SPLIT && e->cm_place && e->place_mode > 0 && !other.empty() &&
+      st.plan_streamed == st.stream_segments) {
+    float f[MORDRED_COST_FEATURES];
+    split_features(e, r, sel, st.gpu_segments, (int)other.size(), st.plan_streamed, t_cpu, t_str, f);
+    if (mordred_cost_model_observe(e->cm_place, f, std::max(1e-3, st.total_ms), std::max(1e-3, st.plan_analytic_ms)))
+      return fail(e, std::string("placement model: ") + mordred_cost_model_last_error(e->cm_place));
+    if (++e->place_since_train >= 8) {
+      e->place_since_train = 0;
+      if (mordred_cost_model_train(e->cm_place, 32, 64, 2e-3f, nullptr))
+        return fail(e, std::string("placement model: ") + mordred_cost_model_last_error(e->cm_place));
+    }
+  }
   if (stats) *stats = st;
+  return 0;
+}
+
+int mordred_engine_set_learned_placement(mordred_engine* e, int mode) {
+  if (!e) return -1;
+  if (mode < MORDRED_LEARNED_OFF || mode > MORDRED_LEARNED_EXPLORE) return fail(e, "bad learned placement mode");
+  if (mode != MORDRED_LEARNED_OFF && e->device < 0) return fail(e, "the learned placement model needs a GPU");
+  if (mode != MORDRED_LEARNED_OFF && !e->cm_place) {
+    e->cm_place = mordred_cost_model_create(e->device, 777);
+    if (!e->cm_place) return fail(e, "placement model creation failed");
+  }
+  e->place_mode = mode;
+  return 0;
+}
+
+int mordred_engine_placement_model_stats(mordred_engine* e, mordred_placement_stats* st) {
+  if (!e || !st) return e ? fail(e, "stats is NULL") : -1;
+  *st = mordred_placement_stats{};
+  if (!e->cm_place) return 0;
+  mordred_cost_model_stats cs{};
+  const int rc = mordred_cost_model_get_stats(e->cm_place, &cs);
+  if (rc) return fail(e, "placement model stats failed");
+  st->samples = cs.samples;
+  st->scored = cs.scored;
+  st->learned_error = cs.ewma_abs_log_error;
+  st->analytic_error = cs.ewma_abs_log_error_analytic;
+  st->trusted = cs.trusted;
   return 0;
 }
 

@@ -37,10 +37,22 @@ class RunStats(C.Structure):
                 ("gpu_rows", C.c_longlong), ("cpu_rows", C.c_longlong), ("stream_rows", C.c_longlong),
                 ("gpu_segments", C.c_int), ("cpu_segments", C.c_int), ("stream_segments", C.c_int),
                 ("est_cpu_ms_per_segment", C.c_double), ("est_stream_ms_per_segment", C.c_double),
-                ("skipped_segments", C.c_int)]
+                ("skipped_segments", C.c_int), ("plan_model", C.c_int), ("plan_streamed", C.c_int),
+                ("plan_predicted_ms", C.c_double), ("plan_analytic_ms", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class PlacementStats(C.Structure):
+    _fields_ = [("samples", C.c_longlong), ("scored", C.c_int), ("learned_error", C.c_double),
+                ("analytic_error", C.c_double), ("trusted", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+LEARNED_OFF, LEARNED_USE, LEARNED_EXPLORE = 0, 1, 2
 
 
 class BatchConfig(C.Structure):
@@ -125,6 +137,8 @@ _SIGS = {
     "mordred_engine_store_segment_minmax": ([C.c_void_p, C.c_char_p, C.c_char_p], C.c_int),
     "mordred_engine_segment_minmax": ([C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p], C.c_int),
     "mordred_engine_set_skipping": ([C.c_void_p, C.c_int], C.c_int),
+    "mordred_engine_set_learned_placement": ([C.c_void_p, C.c_int], C.c_int),
+    "mordred_engine_placement_model_stats": ([C.c_void_p, C.POINTER(PlacementStats)], C.c_int),
     "mordred_engine_place": ([C.c_void_p, C.POINTER(StarQueryC), C.c_double], C.c_int),
     "mordred_engine_run": ([C.c_void_p, C.POINTER(StarQueryC), C.c_int, C.c_void_p, C.POINTER(RunStats)], C.c_int),
     "mordred_engine_batch": ([C.c_void_p, C.POINTER(StarQueryC), C.c_int, C.POINTER(C.c_void_p),
@@ -254,6 +268,17 @@ class Engine:
 
     def set_skipping(self, on: bool):
         self._check(self.lib.mordred_engine_set_skipping(self.h, int(on)), "set_skipping")
+
+    # ---- learned placement
+    def set_learned_placement(self, mode: int):
+        """LEARNED_OFF, LEARNED_USE (plans ranked by the MLP while it is the more
+        accurate model) or LEARNED_EXPLORE (also a random plan one run in four)."""
+        self._check(self.lib.mordred_engine_set_learned_placement(self.h, mode), "set_learned_placement")
+
+    def placement_model_stats(self) -> dict:
+        st = PlacementStats()
+        self._check(self.lib.mordred_engine_placement_model_stats(self.h, C.byref(st)), "placement_model_stats")
+        return st.as_dict()
 
     def place(self, query, fraction: float):
         qc = star_query(query)

@@ -67,3 +67,13 @@ def test_bad_queries_are_rejected():
     with pytest.raises(E.EngineError, match="duplicate"):
         eng.add_column("a", np.arange(3, dtype=np.int32))
     eng.close()
+
+
+def test_learned_placement_needs_a_gpu(host_engine):
+    q, data, eng = host_engine
+    eng.set_learned_placement(E.LEARNED_OFF)
+    with pytest.raises(E.EngineError, match="needs a GPU"):
+        eng.set_learned_placement(E.LEARNED_USE)
+    with pytest.raises(E.EngineError, match="bad learned placement mode"):
+        eng.set_learned_placement(7)
+    assert eng.placement_model_stats()["samples"] == 0

@@ -122,3 +122,31 @@ def test_skipping_on_gpu_placements(sorted_engine, placement):
     np.testing.assert_array_equal(res, ref)
     assert st["skipped_segments"] == 0
     eng.set_skipping(True)
+
+
+# ---------------------------------------------------------------- learned placement
+def test_learned_split_plans_are_exact(gengine):
+    """COST_SPLIT with the learned placement model: exploration runs random
+    candidate plans, and every plan returns the oracle's answer; every run is
+    observed by the model."""
+    q, data, eng, ref = gengine
+    print(f"[gpu] learned split {q.name}", flush=True)
+    eng.place(q, 0.5)
+    eng.set_learned_placement(E.LEARNED_EXPLORE)
+    seen_models, seen_k = set(), set()
+    runs = 40
+    for _ in range(runs):
+        res, st = eng.run(q, E.PLACE_COST_SPLIT)
+        np.testing.assert_array_equal(res, ref)
+        seen_models.add(st["plan_model"])
+        seen_k.add(st["plan_streamed"])
+        assert st["plan_streamed"] == st["stream_segments"]
+        assert st["plan_analytic_ms"] >= 0
+    ms = eng.placement_model_stats()
+    assert ms["samples"] >= runs and ms["scored"] >= runs - 1
+    assert 2 in seen_models and len(seen_k) >= 2
+    eng.set_learned_placement(E.LEARNED_USE)
+    res, st = eng.run(q, E.PLACE_COST_SPLIT)
+    np.testing.assert_array_equal(res, ref)
+    assert st["plan_model"] in (0, 1)
+    eng.set_learned_placement(E.LEARNED_OFF)
