@@ -63,7 +63,7 @@ typedef struct mordred_star_query {
 } mordred_star_query;
 
 enum {
-  MORDRED_PLACE_RESIDENCY = 0,  /* reference: cached groups on GPU, the rest on CPU */
+  MORDRED_PLACE_RESIDENCY = 0,  /* reference: cached groups on GPU, partly cached ones mixed, the rest on CPU */
   MORDRED_PLACE_COST_SPLIT = 1, /* cost model splits non-resident segments CPU / PCIe+GPU */
   MORDRED_PLACE_GPU_ONLY = 2,   /* every non-resident segment streamed to the GPU */
   MORDRED_PLACE_CPU_ONLY = 3    /* every segment on the CPU operators */
@@ -84,6 +84,13 @@ typedef struct mordred_run_stats {
   int plan_streamed;                /* non-resident segments the plan streams */
   double plan_predicted_ms;         /* learned model's time for the plan (plan_model 1) */
   double plan_analytic_ms;          /* analytic model's time for the plan */
+  /* partly cached segments run as mixed pipelines: the filters and probes whose
+   * fact column is cached on the GPU (into offset lists), the rest of the
+   * pipeline on the CPU operators over the survivors */
+  int mixed_segments;
+  long long mixed_rows;             /* fact rows of the mixed segments */
+  long long mixed_survivors;        /* rows that left the GPU operators */
+  double est_mixed_ms_per_segment;  /* analytic model (COST_SPLIT) */
 } mordred_run_stats;
 
 /* Engine on HIP device `device` with an HBM segment cache of cache_segments

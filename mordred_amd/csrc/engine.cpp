@@ -177,7 +177,7 @@ struct mordred_engine {
   double bw_cpu = 42e9, bw_pci = 12e9;  // CostModel.h:5-6 (bytes per second)
   bool skipping = true;                 // segment skipping (main.cu:21)
   std::unordered_map<long long, double> seg_weight;  // (column id << 32 | segment) -> weight
-  DevBuf d_res, d_sg, d_ht[4];
+  DevBuf d_res, d_sg, d_ht[4], d_mix;
   HostBuf h_stage;  // pinned scratch for segment groups / index updates
   // shared-scan batches: record tables, descriptor, result arena, the graph
   DevBuf d_rec[4], d_desc, d_arena, d_status;
@@ -598,6 +598,23 @@ void split_features(const mordred_engine* e, const Resolved& r, const Selectivit
 // CPUProcessing.cu build_CPU (:1034-1104), filter_probe_group_by_CPU
 // (:723-866) and filter_probe_aggr_CPU (:1628-1710), with per-thread dense
 // accumulators instead of one atomic per row.
+// Partly cached segments that share one residency pattern, run as a mixed
+// pipeline (the reference's per-operator placement, QueryOptimizer.cu:
+// 1718-1747, executeTableFact_v1 in QueryProcessing.cu:201): the fact filters
+// and probes whose columns are cached run first on the GPU
+// (filter_probe_GPU2 into offset lists), the survivors' row ids and dimension
+// rows go to the host, and the CPU operators finish the rest of the pipeline.
+struct MixedGroup {
+  std::vector<int> segs;
+  bool gpu_filter[2] = {false, false};
+  bool gpu_probe[4] = {false, false, false, false};
+  long long rows = 0;        // fact rows of the group
+  // from the GPU side
+  int total = 0;             // survivors of the GPU operators
+  std::vector<int> lo;       // their fact row ids
+  std::vector<int> dim[4];   // their dimension rows, for GPU-probed tables with a group value
+};
+
 struct CpuTables {
   std::vector<int> ht[4];
 };
@@ -707,6 +724,86 @@ void cpu_fact(mordred_engine* e, const Resolved& r, const CpuTables& t, const st
     long long s = 0;
     for (auto& A : acc) s += A.scalar;
     reinterpret_cast<long long*>(res)[2] += s;
+    return;
+  }
+  for (auto& A : acc) {
+    if (A.sum.empty()) continue;
+    for (int h = 0; h < G; h++) {
+      if (!A.touched[h]) continue;
+      for (int k = 0; k < 4; k++)
+        if (A.key[4 * (size_t)h + k] != 0) res[6 * (size_t)h + k] = A.key[4 * (size_t)h + k];
+      reinterpret_cast<long long*>(res + 6 * (size_t)h + 4)[0] += A.sum[h];
+    }
+  }
+}
+
+// The CPU part of a mixed pipeline (the offset-list CPU operators of
+// CPUProcessing.cu, e.g. probe_group_by_CPU2 :301-438): for each survivor of
+// the GPU operators, the remaining fact filters and probes, then the group-by
+// or aggregate, accumulated into res as cpu_fact does.  A table probed on the
+// GPU contributes the group value of the dimension row the GPU found (the
+// value its hash table holds for that row).
+void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const MixedGroup& m, int* res) {
+  const mordred_star_query& q = *r.q;
+  const int G = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
+  const int nw = e->pool->size();
+  std::vector<CpuAcc> acc(nw);
+  const long long total = m.total;
+  const int tasks = (int)((total + kCpuChunk - 1) / kCpuChunk);
+  int len[4], mk[4], mv[4], uq[4];
+  for (int k = 0; k < 4; k++) {
+    len[k] = r.join[k] ? r.join[k]->dim_len : 1;
+    mk[k] = r.join[k] ? r.join[k]->min_key : 0;
+    mv[k] = r.join[k] && r.join[k]->group_col ? r.join[k]->min_val : 0;
+    uq[k] = r.join[k] && r.join[k]->group_col ? r.join[k]->unique_val : 0;
+  }
+  e->pool->run(tasks, [&](int task, int w) {
+    CpuAcc& A = acc[w];
+    if (q.kind == MORDRED_KIND_GROUPBY && A.sum.empty()) {
+      A.sum.assign(G, 0);
+      A.key.assign(4 * (size_t)G, 0);
+      A.touched.assign(G, 0);
+    }
+    const long long b = (long long)task * kCpuChunk, en = std::min(total, b + kCpuChunk);
+    for (long long i = b; i < en; i++) {
+      const long long row = m.lo[i];
+      bool ok = true;
+      for (int f = 0; f < q.nfilters && ok; f++) {
+        if (m.gpu_filter[f]) continue;
+        const int x = r.filt[f]->host[row];
+        ok = x >= q.filters[f].lo && x <= q.filters[f].hi;
+      }
+      if (!ok) continue;
+      int v[4] = {0, 0, 0, 0};
+      for (int k = 0; k < 4 && ok; k++) {
+        if (!r.join[k]) continue;
+        if (m.gpu_probe[k]) {
+          if (r.dval[k]) v[k] = r.dval[k]->host[m.dim[k][i]];
+          continue;
+        }
+        const int h = (r.key[k]->host[row] - mk[k]) % len[k];
+        const int lo = t.ht[k][2 * (size_t)h], hi = t.ht[k][2 * (size_t)h + 1];
+        if ((lo | hi) == 0) ok = false;
+        else v[k] = lo;
+      }
+      if (!ok) continue;
+      const int x1 = r.a1->host[row], x2 = r.a2 ? r.a2->host[row] : 0;
+      if (q.kind == MORDRED_KIND_AGGR) {
+        A.scalar += (long long)(int)((unsigned)x1 * (unsigned)x2);
+        continue;
+      }
+      const int h = ((v[0] - mv[0]) * uq[0] + (v[1] - mv[1]) * uq[1] + (v[2] - mv[2]) * uq[2] +
+                     (v[3] - mv[3]) * uq[3]) % G;
+      A.sum[h] += (long long)(int)((unsigned)x1 - (unsigned)x2);
+      A.touched[h] = 1;
+      for (int k = 0; k < 4; k++)
+        if (v[k] != 0) A.key[4 * (size_t)h + k] = v[k];
+    }
+  });
+  if (q.kind == MORDRED_KIND_AGGR) {
+    long long sum = 0;
+    for (auto& A : acc) sum += A.scalar;
+    reinterpret_cast<long long*>(res)[2] += sum;
     return;
   }
   for (auto& A : acc) {
@@ -847,14 +944,31 @@ Selectivity estimate_selectivity(const Resolved& r, const CpuTables& t) {
 // until earlier commands drain, which on the calling thread would serialise
 // the PCIe stream and the CPU group.  The builds and the cached segment group
 // are issued immediately; the streamed segments once the split is known.
+struct SplitPlan {
+  std::vector<int> stream;          // segments streamed over PCIe
+  std::vector<MixedGroup>* mixed;   // owned by the caller of the run
+};
+
 struct GpuPart {
   std::vector<int> gpu_segs;              // cached segment group
-  std::future<std::vector<int>> streamed; // streamed segments, from the CPU side
+  std::future<SplitPlan> streamed;        // the split, from the CPU side
+  std::promise<bool> mixed_done;          // mixed groups' offsets are on the host
   std::vector<int> res;                   // partial result (res_ints)
   int rc = 0;
   double ms = 0;
   long long gpu_rows = 0, stream_rows = 0;
   int gpu_segments = 0, stream_segments = 0;
+};
+
+// Sets the promise (false) on every exit that has not set it.
+struct PromiseGuard {
+  std::promise<bool>& p;
+  bool set = false;
+  void done(bool v) {
+    if (!set) p.set_value(v);
+    set = true;
+  }
+  ~PromiseGuard() { done(false); }
 };
 
 // Staging slots borrowed from the free list for one run; on every exit the
@@ -933,7 +1047,9 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
     g.gpu_segments = (int)g.gpu_segs.size();
   }
 
-  std::vector<int> seg_order = g.streamed.get();
+  PromiseGuard mixed_signal{g.mixed_done};
+  SplitPlan plan = g.streamed.get();
+  std::vector<int>& seg_order = plan.stream;
   SlotLoan loan{e, {}};
   if (!seg_order.empty()) {
     // Two staging sets of kStreamBatch segments x fact columns.  Batch i uses
@@ -991,6 +1107,93 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
     }
     g.stream_segments = (int)seg_order.size();
   }
+  // mixed groups: the GPU operators into offset lists, then to the host
+  if (plan.mixed && !plan.mixed->empty()) {
+    // device scratch: [row ids | dimension rows of tables 0-3] x cap ints,
+    // the survivor count, the group's segment list
+    size_t cap = 0, msegs = 0;
+    for (const MixedGroup& m : *plan.mixed) {
+      cap = std::max<size_t>(cap, (size_t)m.rows);
+      msegs = std::max<size_t>(msegs, m.segs.size());
+    }
+    // (plus a row-id list between a filter call and a probe call)
+    HIP_OK(e->d_mix.reserve((6 * cap + 16) * sizeof(int) + msegs * sizeof(short)));
+    int* dm = static_cast<int*>(e->d_mix.p);
+    int* d_total = dm + 5 * cap;
+    int* d_flo = dm + 5 * cap + 16;
+    short* d_msg = reinterpret_cast<short*>(dm + 6 * cap + 16);
+    for (MixedGroup& m : *plan.mixed) {
+      order_last_partial(r, m.segs);
+      std::vector<short> sgl(m.segs.begin(), m.segs.end());
+      HIP_OK(hipMemcpyAsync(d_msg, sgl.data(), sgl.size() * sizeof(short), hipMemcpyHostToDevice, e->s_main));
+      HIP_OK(hipMemsetAsync(d_total, 0, sizeof(int), e->s_main));
+      filterArgsGPU f{};
+      bool any_f = false;
+      for (int i = 0; i < 2; i++) {
+        if (!m.gpu_filter[i]) continue;
+        any_f = true;
+        int* const idx = r.filt[i]->d_idx;
+        if (i == 0) { f.filter_idx1 = idx; f.compare1 = r.q->filters[i].lo; f.compare2 = r.q->filters[i].hi; f.mode1 = 1; }
+        else { f.filter_idx2 = idx; f.compare3 = r.q->filters[i].lo; f.compare4 = r.q->filters[i].hi; f.mode2 = 1; }
+      }
+      probeArgsGPU pa{};
+      offsetGPU out{};
+      out.lo_off = dm;
+      for (int k = 0; k < 4; k++) {
+        if (!m.gpu_probe[k]) continue;
+        const mordred_join& j = *r.join[k];
+        int* key = r.key[k]->d_idx;
+        int* dout = r.dval[k] ? dm + (size_t)(k + 1) * cap : nullptr;
+        switch (k) {
+          case 0: pa.key_idx1 = key; pa.ht1 = ht[0]; pa.dim_len1 = j.dim_len; pa.min_key1 = j.min_key; out.dim_off1 = dout; break;
+          case 1: pa.key_idx2 = key; pa.ht2 = ht[1]; pa.dim_len2 = j.dim_len; pa.min_key2 = j.min_key; out.dim_off2 = dout; break;
+          case 2: pa.key_idx3 = key; pa.ht3 = ht[2]; pa.dim_len3 = j.dim_len; pa.min_key3 = j.min_key; out.dim_off3 = dout; break;
+          default: pa.key_idx4 = key; pa.ht4 = ht[3]; pa.dim_len4 = j.dim_len; pa.min_key4 = j.min_key; out.dim_off4 = dout; break;
+        }
+      }
+      // the reference's kernel for each operator set: filter_GPU2 (filters),
+      // probe_GPU2 (probes of any tables), filter_probe_GPU2 (filters + the
+      // date probe), or filter_GPU2 then probe_GPU3 over its row ids
+      bool any_p = false, date_only = true;
+      for (int k = 0; k < 4; k++)
+        if (m.gpu_probe[k]) any_p = true, date_only &= k == 3;
+      const mordred_stream_t ms = (mordred_stream_t)e->s_main;
+      int krc;
+      if (!any_p) {
+        krc = mordred_filter_GPU2(e->cache, &f, dm, (int)m.rows, d_total, 0, d_msg, ms);
+      } else if (!any_f) {
+        krc = mordred_probe_GPU2(e->cache, &pa, &out, (int)m.rows, d_total, 0, d_msg, ms);
+      } else if (date_only) {
+        krc = mordred_filter_probe_GPU2(e->cache, &f, &pa, &out, (int)m.rows, d_total, 0, d_msg, ms);
+      } else {
+        krc = mordred_filter_GPU2(e->cache, &f, d_flo, (int)m.rows, d_total, 0, d_msg, ms);
+        int nf = 0;
+        if (!krc) {
+          HIP_OK(hipMemcpyAsync(&nf, d_total, sizeof(int), hipMemcpyDeviceToHost, e->s_main));
+          HIP_OK(hipStreamSynchronize(e->s_main));
+          HIP_OK(hipMemsetAsync(d_total, 0, sizeof(int), e->s_main));
+          offsetGPU in{};
+          in.lo_off = d_flo;
+          krc = mordred_probe_GPU3(e->cache, &in, &pa, &out, nf, d_total, ms);
+        }
+      }
+      if (krc) return hip_fail(e, (hipError_t)krc, "mixed pipeline GPU operators");
+      HIP_OK(hipMemcpyAsync(&m.total, d_total, sizeof(int), hipMemcpyDeviceToHost, e->s_main));
+      HIP_OK(hipStreamSynchronize(e->s_main));
+      m.lo.resize(m.total);
+      if (m.total > 0)
+        HIP_OK(hipMemcpyAsync(m.lo.data(), dm, (size_t)m.total * sizeof(int), hipMemcpyDeviceToHost, e->s_main));
+      for (int k = 0; k < 4; k++) {
+        if (!m.gpu_probe[k] || !r.dval[k]) continue;
+        m.dim[k].resize(m.total);
+        if (m.total > 0)
+          HIP_OK(hipMemcpyAsync(m.dim[k].data(), dm + (size_t)(k + 1) * cap, (size_t)m.total * sizeof(int),
+                                hipMemcpyDeviceToHost, e->s_main));
+      }
+      HIP_OK(hipStreamSynchronize(e->s_main));
+    }
+  }
+  mixed_signal.done(true);
   for (hipStream_t s : {e->s_main, e->s_copy[0], e->s_copy[1]}) HIP_OK(hipStreamSynchronize(s));
   g.res.assign(res_ints, 0);
   HIP_OK(hipMemcpy(g.res.data(), d_res, res_ints * sizeof(int), hipMemcpyDeviceToHost));
@@ -1819,14 +2022,43 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     if (placement != MORDRED_PLACE_CPU_ONLY && segment_cached(r, s)) g.gpu_segs.push_back(s);
     else other.push_back(s);
   }
+  // partly cached segments, grouped by which of the query's filters and
+  // probes have their fact column cached (candidates for a mixed pipeline)
+  std::vector<MixedGroup> mixed;
+  if (placement == MORDRED_PLACE_RESIDENCY || placement == MORDRED_PLACE_COST_SPLIT) {
+    std::vector<int> plain;
+    for (int s : other) {
+      MixedGroup pat;
+      bool any = false;
+      for (int i = 0; i < r.q->nfilters; i++) any |= pat.gpu_filter[i] = r.filt[i]->slot[s] >= 0;
+      for (int k = 0; k < 4; k++) any |= pat.gpu_probe[k] = r.join[k] && r.key[k]->slot[s] >= 0;
+      if (!any) {
+        plain.push_back(s);
+        continue;
+      }
+      MixedGroup* grp = nullptr;
+      for (MixedGroup& m : mixed)
+        if (std::equal(m.gpu_filter, m.gpu_filter + 2, pat.gpu_filter) &&
+            std::equal(m.gpu_probe, m.gpu_probe + 4, pat.gpu_probe))
+          grp = &m;
+      if (!grp) {
+        mixed.push_back(pat);
+        grp = &mixed.back();
+      }
+      grp->segs.push_back(s);
+    }
+    other.swap(plain);
+    for (MixedGroup& m : mixed) m.rows = group_rows(r, m.segs);
+  }
   const bool may_stream = !other.empty() &&
                           (placement == MORDRED_PLACE_GPU_ONLY || placement == MORDRED_PLACE_COST_SPLIT);
-  const bool use_gpu = !g.gpu_segs.empty() || may_stream;
+  const bool use_gpu = !g.gpu_segs.empty() || may_stream || !mixed.empty();
   const size_t staging = 2 * (size_t)kStreamBatch * r.fact_cols.size();
 
   // ---- GPU half on its own thread (builds + cached group start now)
-  std::promise<std::vector<int>> streamed;
+  std::promise<SplitPlan> streamed;
   g.streamed = streamed.get_future();
+  std::future<bool> mixed_ready = g.mixed_done.get_future();
   std::thread feeder;
   if (use_gpu) feeder = std::thread([&] { g.rc = gpu_part(e, r, g, res_ints); });
 
@@ -1835,7 +2067,8 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   CpuTables t;
   Selectivity sel;
   bool cpu_tables = false;
-  if ((placement != MORDRED_PLACE_GPU_ONLY && !other.empty()) || placement == MORDRED_PLACE_COST_SPLIT) {
+  if ((placement != MORDRED_PLACE_GPU_ONLY && (!other.empty() || !mixed.empty())) ||
+      placement == MORDRED_PLACE_COST_SPLIT) {
     cpu_build(e, r, t);
     sel = estimate_selectivity(r, t);
     cpu_tables = true;
@@ -1847,6 +2080,37 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   const double t_str = transfer_cost(r, kSeg, e->bw_pci) * 1e3;
   st.est_cpu_ms_per_segment = t_cpu;
   st.est_stream_ms_per_segment = t_str;
+  if (placement == MORDRED_PLACE_COST_SPLIT && !mixed.empty()) {
+    // a mixed group stays mixed when the analytic model times it below both
+    // whole-segment plans: the GPU operators are free (CostModel.cu:283-363),
+    // the survivors' offsets cross PCIe, and the CPU operators touch one cache
+    // line per survivor and column (random access)
+    std::vector<MixedGroup> keep;
+    for (MixedGroup& m : mixed) {
+      double surv = 1;
+      int outs = 1, lines = 1 + (r.a2 ? 2 : 1);
+      for (int i = 0; i < r.q->nfilters; i++) {
+        if (m.gpu_filter[i]) surv *= sel.filt[i];
+        else lines++;
+      }
+      for (int k = 0; k < 4; k++) {
+        if (!r.join[k]) continue;
+        if (m.gpu_probe[k]) {
+          surv *= sel.join[k];
+          if (r.dval[k]) outs++, lines++;
+        } else {
+          lines += 2;
+        }
+      }
+      const double L = (double)kSeg * surv;
+      const double t_mix = (L * 4.0 * outs / e->bw_pci + L * kCacheLine * lines / e->bw_cpu) * 1e3;
+      st.est_mixed_ms_per_segment = t_mix;
+      if (t_mix < std::min(t_cpu, t_str)) keep.push_back(std::move(m));
+      else other.insert(other.end(), m.segs.begin(), m.segs.end());
+    }
+    mixed.swap(keep);
+    std::sort(other.begin(), other.end());
+  }
   if (placement == MORDRED_PLACE_GPU_ONLY) {
     stream_segs = other;
   } else if (placement == MORDRED_PLACE_COST_SPLIT) {
@@ -1901,7 +2165,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     stream_segs.clear();
   }
   std::sort(cpu_segs.begin(), cpu_segs.end());
-  streamed.set_value(stream_segs);
+  streamed.set_value(SplitPlan{stream_segs, &mixed});
 
   // ---- CPU: the CPU segment group, concurrently with the GPU
   std::vector<int> cpu_res;
@@ -1910,6 +2174,20 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     cpu_fact(e, r, t, cpu_segs, cpu_res.data());
     st.cpu_rows = group_rows(r, cpu_segs);
     st.cpu_segments = (int)cpu_segs.size();
+  }
+  // ---- mixed groups: the CPU operators over the GPU operators' survivors
+  if (!mixed.empty()) {
+    if (!mixed_ready.get()) {
+      if (feeder.joinable()) feeder.join();
+      return g.rc ? g.rc : fail(e, "mixed pipeline: GPU part failed");
+    }
+    if (cpu_res.empty()) cpu_res.assign(res_ints, 0);
+    for (const MixedGroup& m : mixed) {
+      cpu_suffix(e, r, t, m, cpu_res.data());
+      st.mixed_segments += (int)m.segs.size();
+      st.mixed_rows += m.rows;
+      st.mixed_survivors += m.total;
+    }
   }
   st.cpu_ms = cpu_tables ? ms_since(tc0) : 0.0;
 

@@ -38,7 +38,9 @@ class RunStats(C.Structure):
                 ("gpu_segments", C.c_int), ("cpu_segments", C.c_int), ("stream_segments", C.c_int),
                 ("est_cpu_ms_per_segment", C.c_double), ("est_stream_ms_per_segment", C.c_double),
                 ("skipped_segments", C.c_int), ("plan_model", C.c_int), ("plan_streamed", C.c_int),
-                ("plan_predicted_ms", C.c_double), ("plan_analytic_ms", C.c_double)]
+                ("plan_predicted_ms", C.c_double), ("plan_analytic_ms", C.c_double),
+                ("mixed_segments", C.c_int), ("mixed_rows", C.c_longlong), ("mixed_survivors", C.c_longlong),
+                ("est_mixed_ms_per_segment", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -150,3 +150,89 @@ def test_learned_split_plans_are_exact(gengine):
     np.testing.assert_array_equal(res, ref)
     assert st["plan_model"] in (0, 1)
     eng.set_learned_placement(E.LEARNED_OFF)
+
+
+# ---------------------------------------------------------------- mixed pipelines
+@pytest.fixture(scope="module", params=["q11", "q21", "q4way"])
+def fresh_engine(request, cuda):
+    """An engine with nothing of the fact table cached."""
+    q, data = cases.star(request.param, 5 * SEG + 77_777, seed=9)
+    eng = E.Engine(device=0, cache_segments=256, threads=8)
+    eng.add_data(data, q)
+    ref = OQ.run_query(q, data, oracle.CONV_GPU)
+    yield q, data, eng, ref
+    eng.close()
+
+
+def _uncache_all(eng, q):
+    for c in q.fact_columns():
+        for s in range(eng.segments(c)):
+            eng.evict_segment(c, s)
+
+
+def test_mixed_pipeline_first_operator_cached(fresh_engine):
+    """Only the first operator's fact column is cached (a filter column, or the
+    first join's key): the GPU runs that operator over every segment, the CPU
+    operators finish the survivors."""
+    q, data, eng, ref = fresh_engine
+    print(f"[gpu] mixed pipeline {q.name}", flush=True)
+    _uncache_all(eng, q)
+    col = q.fact_filters[0][0] if q.fact_filters else q.joins[0].fact_key
+    nseg = eng.segments(col)
+    for s in range(nseg):
+        eng.cache_segment(col, s)
+    res, st = eng.run(q, E.PLACE_RESIDENCY)
+    np.testing.assert_array_equal(res, ref)
+    assert st["mixed_segments"] == nseg and st["mixed_rows"] == data.n
+    assert st["gpu_segments"] == 0 and st["cpu_segments"] == 0
+    assert 0 < st["mixed_survivors"] < data.n
+
+
+def test_mixed_with_cached_and_plain_segments(fresh_engine):
+    """Segments 0-1 fully cached (fused kernel), the key of the last join cached
+    in segments 2-3 (mixed), the join keys alternating in the rest (two more
+    residency patterns), the partial last segment uncached (CPU operators)."""
+    q, data, eng, ref = fresh_engine
+    _uncache_all(eng, q)
+    cols = q.fact_columns()
+    nseg = eng.segments(cols[0])
+    for s in (0, 1):
+        for c in cols:
+            eng.cache_segment(c, s)
+    keys = [j.fact_key for j in q.joins]
+    for s in (2, 3):
+        eng.cache_segment(keys[-1], s)
+    for s in range(4, nseg - 1):
+        eng.cache_segment(keys[s % len(keys)], s)
+    res, st = eng.run(q, E.PLACE_RESIDENCY)
+    np.testing.assert_array_equal(res, ref)
+    assert st["gpu_segments"] == 2 and st["cpu_segments"] == 1
+    assert st["mixed_segments"] == nseg - 3
+    res, st = eng.run(q, E.PLACE_COST_SPLIT)
+    np.testing.assert_array_equal(res, ref)
+    assert st["gpu_segments"] == 2
+    assert st["mixed_segments"] + st["cpu_segments"] + st["stream_segments"] == nseg - 2
+    res, st = eng.run(q, E.PLACE_GPU_ONLY)
+    np.testing.assert_array_equal(res, ref)
+    assert st["mixed_segments"] == 0 and st["stream_segments"] == nseg - 2
+
+
+def test_mixed_filter_then_probe(cuda):
+    """A fact filter and a non-date join both cached: the GPU runs filter_GPU2,
+    then probe_GPU3 over its row ids; the CPU finishes the part and date
+    probes and the group-by."""
+    import dataclasses
+    q, data = cases.star("q21", 4 * SEG + 4_321, seed=12)
+    q = dataclasses.replace(q, fact_filters=[("lo_revenue", 1_000_000, 6_000_000)])
+    eng = E.Engine(device=0, cache_segments=64, threads=8)
+    eng.add_data(data, q)
+    ref = OQ.run_query(q, data, oracle.CONV_GPU)
+    nseg = eng.segments("lo_revenue")
+    for s in range(nseg):
+        eng.cache_segment("lo_revenue", s)
+        eng.cache_segment(q.joins[0].fact_key, s)
+    print("[gpu] mixed filter then probe", flush=True)
+    res, st = eng.run(q, E.PLACE_RESIDENCY)
+    np.testing.assert_array_equal(res, ref)
+    assert st["mixed_segments"] == nseg and st["gpu_segments"] == 0
+    eng.close()
