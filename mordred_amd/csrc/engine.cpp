@@ -42,6 +42,7 @@ constexpr int kSegLog = 20;
 constexpr int kSeg = 1 << kSegLog;
 constexpr int kCpuChunk = 1 << 16;   // rows per CPU task
 constexpr int kStreamBatch = 4;      // segments per streamed batch
+constexpr int kMixedChunk = 64;      // segments per GPU call of a mixed pipeline
 constexpr double kCacheLine = 64.0;  // CostModel.h:4
 
 using Clock = std::chrono::steady_clock;
@@ -1107,26 +1108,20 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
     }
     g.stream_segments = (int)seg_order.size();
   }
-  // mixed groups: the GPU operators into offset lists, then to the host
+  // mixed groups: the GPU operators into offset lists, then to the host, in
+  // chunks of at most kMixedChunk segments (bounded device scratch)
   if (plan.mixed && !plan.mixed->empty()) {
-    // device scratch: [row ids | dimension rows of tables 0-3] x cap ints,
-    // the survivor count, the group's segment list
-    size_t cap = 0, msegs = 0;
-    for (const MixedGroup& m : *plan.mixed) {
-      cap = std::max<size_t>(cap, (size_t)m.rows);
-      msegs = std::max<size_t>(msegs, m.segs.size());
-    }
-    // (plus a row-id list between a filter call and a probe call)
-    HIP_OK(e->d_mix.reserve((6 * cap + 16) * sizeof(int) + msegs * sizeof(short)));
+    // device scratch: [row ids | dimension rows of tables 0-3 | a row-id list
+    // between a filter call and a probe call] x cap ints, the survivor count,
+    // the chunk's segment list
+    const size_t cap = (size_t)kMixedChunk * kSeg;
+    HIP_OK(e->d_mix.reserve((6 * cap + 16) * sizeof(int) + kMixedChunk * sizeof(short)));
     int* dm = static_cast<int*>(e->d_mix.p);
     int* d_total = dm + 5 * cap;
     int* d_flo = dm + 5 * cap + 16;
     short* d_msg = reinterpret_cast<short*>(dm + 6 * cap + 16);
     for (MixedGroup& m : *plan.mixed) {
       order_last_partial(r, m.segs);
-      std::vector<short> sgl(m.segs.begin(), m.segs.end());
-      HIP_OK(hipMemcpyAsync(d_msg, sgl.data(), sgl.size() * sizeof(short), hipMemcpyHostToDevice, e->s_main));
-      HIP_OK(hipMemsetAsync(d_total, 0, sizeof(int), e->s_main));
       filterArgsGPU f{};
       bool any_f = false;
       for (int i = 0; i < 2; i++) {
@@ -1158,39 +1153,51 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
       for (int k = 0; k < 4; k++)
         if (m.gpu_probe[k]) any_p = true, date_only &= k == 3;
       const mordred_stream_t ms = (mordred_stream_t)e->s_main;
-      int krc;
-      if (!any_p) {
-        krc = mordred_filter_GPU2(e->cache, &f, dm, (int)m.rows, d_total, 0, d_msg, ms);
-      } else if (!any_f) {
-        krc = mordred_probe_GPU2(e->cache, &pa, &out, (int)m.rows, d_total, 0, d_msg, ms);
-      } else if (date_only) {
-        krc = mordred_filter_probe_GPU2(e->cache, &f, &pa, &out, (int)m.rows, d_total, 0, d_msg, ms);
-      } else {
-        krc = mordred_filter_GPU2(e->cache, &f, d_flo, (int)m.rows, d_total, 0, d_msg, ms);
-        int nf = 0;
-        if (!krc) {
-          HIP_OK(hipMemcpyAsync(&nf, d_total, sizeof(int), hipMemcpyDeviceToHost, e->s_main));
-          HIP_OK(hipStreamSynchronize(e->s_main));
-          HIP_OK(hipMemsetAsync(d_total, 0, sizeof(int), e->s_main));
-          offsetGPU in{};
-          in.lo_off = d_flo;
-          krc = mordred_probe_GPU3(e->cache, &in, &pa, &out, nf, d_total, ms);
+      m.total = 0;
+      for (size_t c0 = 0; c0 < m.segs.size(); c0 += kMixedChunk) {
+        const size_t c1 = std::min(m.segs.size(), c0 + kMixedChunk);
+        std::vector<short> sgl(m.segs.begin() + c0, m.segs.begin() + c1);
+        long long rows = 0;
+        for (size_t i = c0; i < c1; i++) rows += segment_rows(r, m.segs[i]);
+        HIP_OK(hipMemcpyAsync(d_msg, sgl.data(), sgl.size() * sizeof(short), hipMemcpyHostToDevice, e->s_main));
+        HIP_OK(hipMemsetAsync(d_total, 0, sizeof(int), e->s_main));
+        int krc;
+        if (!any_p) {
+          krc = mordred_filter_GPU2(e->cache, &f, dm, (int)rows, d_total, 0, d_msg, ms);
+        } else if (!any_f) {
+          krc = mordred_probe_GPU2(e->cache, &pa, &out, (int)rows, d_total, 0, d_msg, ms);
+        } else if (date_only) {
+          krc = mordred_filter_probe_GPU2(e->cache, &f, &pa, &out, (int)rows, d_total, 0, d_msg, ms);
+        } else {
+          krc = mordred_filter_GPU2(e->cache, &f, d_flo, (int)rows, d_total, 0, d_msg, ms);
+          int nf = 0;
+          if (!krc) {
+            HIP_OK(hipMemcpyAsync(&nf, d_total, sizeof(int), hipMemcpyDeviceToHost, e->s_main));
+            HIP_OK(hipStreamSynchronize(e->s_main));
+            HIP_OK(hipMemsetAsync(d_total, 0, sizeof(int), e->s_main));
+            offsetGPU in{};
+            in.lo_off = d_flo;
+            krc = mordred_probe_GPU3(e->cache, &in, &pa, &out, nf, d_total, ms);
+          }
         }
+        if (krc) return hip_fail(e, (hipError_t)krc, "mixed pipeline GPU operators");
+        int got = 0;
+        HIP_OK(hipMemcpyAsync(&got, d_total, sizeof(int), hipMemcpyDeviceToHost, e->s_main));
+        HIP_OK(hipStreamSynchronize(e->s_main));
+        const size_t at = (size_t)m.total;
+        m.total += got;
+        m.lo.resize(m.total);
+        if (got > 0)
+          HIP_OK(hipMemcpyAsync(m.lo.data() + at, dm, (size_t)got * sizeof(int), hipMemcpyDeviceToHost, e->s_main));
+        for (int k = 0; k < 4; k++) {
+          if (!m.gpu_probe[k] || !r.dval[k]) continue;
+          m.dim[k].resize(m.total);
+          if (got > 0)
+            HIP_OK(hipMemcpyAsync(m.dim[k].data() + at, dm + (size_t)(k + 1) * cap, (size_t)got * sizeof(int),
+                                  hipMemcpyDeviceToHost, e->s_main));
+        }
+        HIP_OK(hipStreamSynchronize(e->s_main));
       }
-      if (krc) return hip_fail(e, (hipError_t)krc, "mixed pipeline GPU operators");
-      HIP_OK(hipMemcpyAsync(&m.total, d_total, sizeof(int), hipMemcpyDeviceToHost, e->s_main));
-      HIP_OK(hipStreamSynchronize(e->s_main));
-      m.lo.resize(m.total);
-      if (m.total > 0)
-        HIP_OK(hipMemcpyAsync(m.lo.data(), dm, (size_t)m.total * sizeof(int), hipMemcpyDeviceToHost, e->s_main));
-      for (int k = 0; k < 4; k++) {
-        if (!m.gpu_probe[k] || !r.dval[k]) continue;
-        m.dim[k].resize(m.total);
-        if (m.total > 0)
-          HIP_OK(hipMemcpyAsync(m.dim[k].data(), dm + (size_t)(k + 1) * cap, (size_t)m.total * sizeof(int),
-                                hipMemcpyDeviceToHost, e->s_main));
-      }
-      HIP_OK(hipStreamSynchronize(e->s_main));
     }
   }
   mixed_signal.done(true);
