@@ -137,7 +137,8 @@ static ProbePlan plan_probes(const Pipe& P, int mode, int sink) {
   ProbePlan plan{0, 0, 0};
   const bool filtered = P.fkind != F_NONE && (P.fcol[0] || P.fcol[1]);
   size_t fixed = group_lds_bytes(sink, P.total_val);
-  if (shape_compacts(mode, sink, shape_of(P, mode, sink))) fixed += ring_lds_bytes(sink_block(sink));
+  const int shape = shape_of(P, mode, sink);
+  if (shape_compacts(mode, sink, shape)) fixed += ring_lds_bytes(sink_block(sink), shape);
   const size_t avail = lds_cap(sink) > fixed ? lds_cap(sink) - fixed : 0;
   size_t want = 0;
   int first = -1;
@@ -238,10 +239,11 @@ static void* build_compact(Pipe& P, const ProbePlan& plan, int sink, hipStream_t
 // sink < 0: group-by, placed in LDS when the dense array fits.
 static int dispatch(Pipe P, int mode, int sink, hipStream_t stream) {
   if (P.num_tuples == 0) return hipSuccess;
-  if (sink < 0)
-    sink = group_lds_bytes(GROUPBY_LDS, P.total_val) + ring_lds_bytes(kLdsBlock) <= kLdsBudgetOneGroup
-               ? GROUPBY_LDS
-               : GROUPBY_GLOBAL;
+  if (sink < 0) {
+    const int shape = shape_of(P, mode, GROUPBY_LDS);
+    const size_t ring = shape_compacts(mode, GROUPBY_LDS, shape) ? ring_lds_bytes(kLdsBlock, shape) : 0;
+    sink = group_lds_bytes(GROUPBY_LDS, P.total_val) + ring <= kLdsBudgetOneGroup ? GROUPBY_LDS : GROUPBY_GLOBAL;
+  }
   const ProbePlan plan = plan_probes(P, mode, sink);
   void* scratch = build_compact(P, plan, sink, stream);
   if (scratch && plan.stage) {

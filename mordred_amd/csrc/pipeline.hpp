@@ -185,7 +185,11 @@ constexpr int kShapeGeneric = -1;
 // the remaining keys and the aggregates by row (late materialization: only
 // cache lines that hold a survivor are read).
 constexpr int kRing = 128;   // entries per wave (a round of 64 plus one item of carry)
-constexpr int kRingFields = 5;  // table 0-3 value or key, row id
+// Ring fields of a shape: one per probed table (its value or key), then the
+// row id; table k of the shape uses field ring_field(shape, k).
+constexpr int popc4(int s) { return (s & 1) + (s >> 1 & 1) + (s >> 2 & 1) + (s >> 3 & 1); }
+constexpr int ring_fields(int shape) { return popc4(shape & 0xF) + 1; }
+constexpr int ring_field(int shape, int k) { return popc4(shape & 0xF & ((1 << k) - 1)); }
 constexpr bool shape_compacts(int mode, int sink, int shape) {
   return shape >= 0 && mode == SEG && (sink == GROUPBY_LDS || sink == AGGR) &&
          ((shape & 1) + (shape >> 1 & 1) + (shape >> 2 & 1) + (shape >> 3 & 1)) >= 2;
@@ -216,7 +220,8 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
   constexpr bool kCompact = shape_compacts(MODE, SINK, SHAPE);
   constexpr int kFirst = shape_first(SHAPE);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  int* ring = kCompact ? reinterpret_cast<int*>(lds + P.ring_off) + wave * (kRing * kRingFields) : nullptr;
+  constexpr int kRowField = kCompact ? ring_fields(SHAPE) - 1 : 0;
+  int* ring = kCompact ? reinterpret_cast<int*>(lds + P.ring_off) + wave * (kRing * ring_fields(SHAPE)) : nullptr;
   int ring_head = 0, ring_tail = 0;  // wave-uniform
 
   // per-table compact probe parameters (workgroup-uniform); cb: probe the
@@ -272,8 +277,8 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
       int v[4] = {0, 0, 0, 0};
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        if ((SHAPE >> k & 1) && (k == kFirst || cb[k])) v[k] = ring[k * kRing + e];
-      const int row = ring[4 * kRing + e];
+        if ((SHAPE >> k & 1) && (k == kFirst || cb[k])) v[k] = ring[ring_field(SHAPE, k) * kRing + e];
+      const int row = ring[kRowField * kRing + e];
       __builtin_amdgcn_wave_barrier();
       // late columns of the entry's row: segment slot, then value
       const int rseg = hit ? row >> kSegLog : 0, roff = row & kSegMask;
@@ -539,8 +544,8 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
           const int e = (ring_tail + pos) & (kRing - 1);
 #pragma unroll
           for (int k = 0; k < 4; k++)
-            if ((SHAPE >> k & 1) && (k == kFirst || cb[k])) ring[k * kRing + e] = kv[k][j];
-          ring[4 * kRing + e] = base_row + item_row<BLOCK>(j);
+            if ((SHAPE >> k & 1) && (k == kFirst || cb[k])) ring[ring_field(SHAPE, k) * kRing + e] = kv[k][j];
+          ring[kRowField * kRing + e] = base_row + item_row<BLOCK>(j);
         }
         ring_tail += __popcll(b);
         if (ring_tail - ring_head >= kWave) {
@@ -752,7 +757,9 @@ inline size_t staged_words(int len, int w) {
   return ((size_t)len + per - 1) / per;
 }
 
-inline size_t ring_lds_bytes(int block) { return (size_t)(block / kWave) * kRing * kRingFields * sizeof(int); }
+inline size_t ring_lds_bytes(int block, int shape) {
+  return (size_t)(block / kWave) * kRing * ring_fields(shape) * sizeof(int);
+}
 
 inline size_t pipe_lds_bytes(const Pipe& P, int sink) {
   size_t b = group_lds_bytes(sink, P.total_val);
@@ -767,7 +774,7 @@ int launch_pipe(const Pipe& Pin, hipStream_t stream) {
   size_t lds = pipe_lds_bytes(P, SINK);
   if (shape_compacts(MODE, SINK, SHAPE)) {
     P.ring_off = (int)(lds / sizeof(unsigned long long));
-    lds += ring_lds_bytes(BLOCK);
+    lds += ring_lds_bytes(BLOCK, SHAPE);
   }
   const void* f = reinterpret_cast<const void*>(&pipeline_kernel<BLOCK, VPT, MODE, SINK, SHAPE>);
   if (lds > 64 * 1024) {
