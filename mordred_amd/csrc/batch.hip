@@ -393,9 +393,15 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         // (global), and again only if a running sum passes through zero.
         // The reference stores them for every row of the group
         // (GPUProcessing.h:144-147); they are equal, so any store suffices.
+        // (scoped atomics keep the LDS add a ds_ instruction: one atomicAdd
+        // on either pointer is merged into a slower flat atomic)
         unsigned long long old;
-        if (Q.lds >= 0) old = atomicAdd(&lsum[Q.lds + h], add);
-        else old = atomicAdd(reinterpret_cast<unsigned long long*>(res + 4), add);
+        if (Q.lds >= 0)
+          old = __hip_atomic_fetch_add(&lsum[Q.lds + h], add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+          old = __hip_atomic_fetch_add(
+              (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(res + 4),
+              add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (!(Q.flags & kQueryAggr) && old == 0ull) {
           int gv[4];
 #pragma unroll
