@@ -373,10 +373,10 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         const int q = __builtin_ctzll(m);
         m &= m - 1;
         const BatchQuery& Q = lq[q];
-        const int h = mod_pos(Q.hbase + (int)((attrw[0][j] >> Q.shift[0]) & ((1u << Q.bits[0]) - 1u)) * Q.uniq[0] +
-                                  (int)((attrw[1][j] >> Q.shift[1]) & ((1u << Q.bits[1]) - 1u)) * Q.uniq[1] +
-                                  (int)((attrw[2][j] >> Q.shift[2]) & ((1u << Q.bits[2]) - 1u)) * Q.uniq[2] +
-                                  (int)((attrw[3][j] >> Q.shift[3]) & ((1u << Q.bits[3]) - 1u)) * Q.uniq[3],
+        const int h = mod_pos(Q.hbase + (int)__umul24((attrw[0][j] >> Q.shift[0]) & Q.mask[0], (unsigned)Q.uniq[0]) +
+                                  (int)__umul24((attrw[1][j] >> Q.shift[1]) & Q.mask[1], (unsigned)Q.uniq[1]) +
+                                  (int)__umul24((attrw[2][j] >> Q.shift[2]) & Q.mask[2], (unsigned)Q.uniq[2]) +
+                                  (int)__umul24((attrw[3][j] >> Q.shift[3]) & Q.mask[3], (unsigned)Q.uniq[3]),
                               Q.total_val);
         // an index outside [0, total_val) (group values below min_val) would
         // write outside the query's result; the reference does, here the pair
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
 #pragma unroll
           for (int d = 0; d < 4; d++)
             gv[d] = Q.attr[d] < 0 ? 0
-                                  : H.dim[d].base[Q.attr[d]] + (int)((attrw[d][j] >> Q.shift[d]) & ((1u << Q.bits[d]) - 1u));
+                                  : H.dim[d].base[Q.attr[d]] + (int)((attrw[d][j] >> Q.shift[d]) & Q.mask[d]);
           int2* key = reinterpret_cast<int2*>(res);
           key[0] = make_int2(gv[0], gv[1]);
           key[1] = make_int2(gv[2], gv[3]);

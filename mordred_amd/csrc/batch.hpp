@@ -45,24 +45,26 @@ constexpr int kBatchLutSpan = 2048; // widest attribute span given a mask table
 enum BatchExprKind { kExprDiff = 0, kExprProduct = 1 };  // a1 - a2 (group-by), a1 * a2 (aggregate)
 enum BatchQueryFlags { kQueryAggr = 1 };
 
+// The fields a (row, query) pair reads sit in the first 48 bytes (three
+// 16-byte LDS reads); attr is read only when the keys of a group are stored.
 struct BatchQuery {            // 64 bytes; staged in LDS by the scan kernel
   long long res;               // device address of the query's 6*total_val ints
   int total_val;
-  short expr;                  // index into BatchDesc::expr
-  short flags;                 // kQueryAggr: no key words
-  int lds;                     // u64 offset of its group array in LDS, -1 = global atomics
-  int unused[2];
-  signed char attr[4];         // per dimension slot: -1 no group value, else its attribute
+  short lds;                   // u64 offset of its group array in LDS, -1 = global atomics
+  unsigned char expr;          // index into BatchDesc::expr
+  unsigned char flags;         // kQueryAggr: no key words
   // dense group index (GPUProcessing.h:142) as hbase + sum_d field_d * uniq[d]
-  // with field_d = (packed entry >> shift[d]) & ((1 << bits[d]) - 1) and hbase
-  // = sum over grouped dimensions of (attribute base - min_val) * unique_val;
-  // bits[d] = uniq[d] = 0 for a dimension without group value (identical in
-  // 32-bit wrap-around)
+  // with field_d = (packed entry >> shift[d]) & mask[d] and hbase = sum over
+  // grouped dimensions of (attribute base - min_val) * unique_val; mask[d] =
+  // uniq[d] = 0 for a dimension without group value (identical in 32-bit
+  // wrap-around).  field_d < 2^16 and uniq[d] < 2^24, so the products are
+  // 24-bit multiplies.
   int hbase;
   unsigned char shift[4];
-  unsigned char bits[4];
-  int pad;
+  unsigned short mask[4];
   int uniq[4];
+  signed char attr[4];         // per dimension slot: -1 no group value, else its attribute
+  int unused[3];
 };
 static_assert(sizeof(BatchQuery) == 64, "BatchQuery is staged as 4 x 16 bytes");
 

@@ -1430,13 +1430,13 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     }
     BatchQuery& Q = D.q[i];
     Q.total_val = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
-    Q.expr = (short)ex;
-    Q.flags = (short)(q.kind == MORDRED_KIND_GROUPBY ? 0 : kQueryAggr);
+    Q.expr = (unsigned char)ex;
+    Q.flags = (unsigned char)(q.kind == MORDRED_KIND_GROUPBY ? 0 : kQueryAggr);
     unsigned hbase = 0;
     for (int k = 0; k < 4; k++) {
       Q.attr[k] = -1;
       Q.shift[k] = 0;
-      Q.bits[k] = 0;
+      Q.mask[k] = 0;
       Q.uniq[k] = 0;
       if (q.kind != MORDRED_KIND_GROUPBY || !r.join[k] || !r.dval[k]) continue;
       int a = 0;
@@ -1444,8 +1444,10 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       const BatchAttr& A = D.dim[k].attr[a];
       Q.attr[k] = (signed char)a;
       Q.shift[k] = (unsigned char)A.off;
-      Q.bits[k] = (unsigned char)A.bits;
+      Q.mask[k] = (unsigned short)((1u << A.bits) - 1u);
       Q.uniq[k] = r.join[k]->unique_val;
+      if (A.bits > 16 || Q.uniq[k] < 0 || Q.uniq[k] >= (1 << 24))
+        return fail(e, "batch group index needs attribute fields of at most 16 bits and unique_val in [0, 2^24)");
       hbase += ((unsigned)A.base - (unsigned)r.join[k]->min_val) * (unsigned)r.join[k]->unique_val;
     }
     Q.hbase = (int)hbase;
@@ -1485,7 +1487,7 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       continue;
     }
     if (in_lds[i]) {
-      Q.lds = (int)sum_words;
+      Q.lds = (short)sum_words;
       sum_words += b.G[i];
       b.lds_queries++;
     } else {

@@ -174,7 +174,7 @@ static int run_pipe(const Pipe& P, int sink, hipStream_t stream) {
   switch (sink) {
     case GROUPBY_LDS: return launch_pipe<kLdsBlock, kVpt, MODE, GROUPBY_LDS>(P, stream);
     case GROUPBY_GLOBAL: return launch_pipe<kBlock, kVpt, MODE, GROUPBY_GLOBAL>(P, stream);
-    case AGGR: return launch_pipe<kAggrBlock, kVpt, MODE, AGGR>(P, stream);
+    case AGGR: return launch_pipe<kAggrBlock, kAggrVpt, MODE, AGGR>(P, stream);
     default: return launch_pipe<kBlock, kVpt, MODE, COMPACT>(P, stream);
   }
 }

@@ -714,6 +714,9 @@ __global__ __launch_bounds__(BLOCK) void build_kernel(Build B) {
 #ifndef MORDRED_VPT
 #define MORDRED_VPT 2
 #endif
+#ifndef MORDRED_AGGR_VPT
+#define MORDRED_AGGR_VPT 2
+#endif
 #ifndef MORDRED_AGGR_BLOCK
 #define MORDRED_AGGR_BLOCK 512
 #endif
@@ -726,6 +729,7 @@ constexpr int kLdsBlock = MORDRED_LDS_BLOCK;
 // scalar aggregates end with one same-address atomic per workgroup, so they
 // use fewer, larger workgroups
 constexpr int kAggrBlock = MORDRED_AGGR_BLOCK;
+constexpr int kAggrVpt = MORDRED_AGGR_VPT;
 constexpr size_t kLdsBudget = 80 * 1024;   // dynamic LDS of one workgroup (2 per CU)
 // The 1024-thread group-by kernel runs one workgroup per CU (its registers
 // allow no second), so it may use the whole 160 KiB less the static arrays.

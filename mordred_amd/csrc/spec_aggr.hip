@@ -7,7 +7,7 @@ namespace mordred {
 
 template <int S>
 static int go(const Pipe& P, hipStream_t s) {
-  return launch_pipe<kAggrBlock, kVpt, SEG, AGGR, S>(P, s);
+  return launch_pipe<kAggrBlock, kAggrVpt, SEG, AGGR, S>(P, s);
 }
 
 int launch_aggr_spec(const Pipe& P, int shape, hipStream_t stream) {
