@@ -22,6 +22,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <future>
 #include <memory>
@@ -43,6 +44,7 @@ constexpr int kSeg = 1 << kSegLog;
 constexpr int kCpuChunk = 1 << 16;   // rows per CPU task
 constexpr int kStreamBatch = 4;      // segments per streamed batch
 constexpr int kMixedChunk = 64;      // segments per GPU call of a mixed pipeline
+constexpr int kSuffixTask = 8192;    // survivors per CPU task of a mixed pipeline
 constexpr double kCacheLine = 64.0;  // CostModel.h:4
 
 using Clock = std::chrono::steady_clock;
@@ -610,10 +612,49 @@ struct MixedGroup {
   bool gpu_filter[2] = {false, false};
   bool gpu_probe[4] = {false, false, false, false};
   long long rows = 0;        // fact rows of the group
-  // from the GPU side
-  int total = 0;             // survivors of the GPU operators
-  std::vector<int> lo;       // their fact row ids
-  std::vector<int> dim[4];   // their dimension rows, for GPU-probed tables with a group value
+  long long survivors = 0;   // rows that left the GPU operators (all chunks)
+};
+
+// Survivors of one chunk of a mixed group, on the host.
+struct MixedChunk {
+  int group = 0;
+  std::vector<int> lo;       // fact row ids
+  std::vector<int> dim[4];   // dimension rows, for GPU-probed tables with a group value
+};
+
+// Chunks pass from the GPU side to the CPU side as each one lands on the
+// host, so the CPU operators on one chunk overlap the GPU operators on the
+// next.
+struct MixedFeed {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<MixedChunk> ready;
+  bool done = false, ok = true;
+  void push(MixedChunk&& c) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      ready.push_back(std::move(c));
+    }
+    cv.notify_one();
+  }
+  void finish(bool success) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      if (done) return;
+      done = true;
+      ok = success;
+    }
+    cv.notify_all();
+  }
+  // false when no chunk is left (check ok)
+  bool pop(MixedChunk& c) {
+    std::unique_lock<std::mutex> lk(mu);
+    cv.wait(lk, [&] { return !ready.empty() || done; });
+    if (ready.empty()) return false;
+    c = std::move(ready.front());
+    ready.pop_front();
+    return true;
+  }
 };
 
 struct CpuTables {
@@ -741,16 +782,16 @@ void cpu_fact(mordred_engine* e, const Resolved& r, const CpuTables& t, const st
 // The CPU part of a mixed pipeline (the offset-list CPU operators of
 // CPUProcessing.cu, e.g. probe_group_by_CPU2 :301-438): for each survivor of
 // the GPU operators, the remaining fact filters and probes, then the group-by
-// or aggregate, accumulated into res as cpu_fact does.  A table probed on the
+// or aggregate, into per-worker accumulators that persist across the chunks
+// of a run (cpu_combine adds them into res once).  A table probed on the
 // GPU contributes the group value of the dimension row the GPU found (the
 // value its hash table holds for that row).
-void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const MixedGroup& m, int* res) {
+void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const MixedGroup& m, const MixedChunk& c,
+                std::vector<CpuAcc>& acc) {
   const mordred_star_query& q = *r.q;
   const int G = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
-  const int nw = e->pool->size();
-  std::vector<CpuAcc> acc(nw);
-  const long long total = m.total;
-  const int tasks = (int)((total + kCpuChunk - 1) / kCpuChunk);
+  const long long total = (long long)c.lo.size();
+  const int tasks = (int)((total + kSuffixTask - 1) / kSuffixTask);
   int len[4], mk[4], mv[4], uq[4];
   for (int k = 0; k < 4; k++) {
     len[k] = r.join[k] ? r.join[k]->dim_len : 1;
@@ -765,9 +806,9 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
       A.key.assign(4 * (size_t)G, 0);
       A.touched.assign(G, 0);
     }
-    const long long b = (long long)task * kCpuChunk, en = std::min(total, b + kCpuChunk);
+    const long long b = (long long)task * kSuffixTask, en = std::min(total, b + kSuffixTask);
     for (long long i = b; i < en; i++) {
-      const long long row = m.lo[i];
+      const long long row = c.lo[i];
       bool ok = true;
       for (int f = 0; f < q.nfilters && ok; f++) {
         if (m.gpu_filter[f]) continue;
@@ -779,7 +820,7 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
       for (int k = 0; k < 4 && ok; k++) {
         if (!r.join[k]) continue;
         if (m.gpu_probe[k]) {
-          if (r.dval[k]) v[k] = r.dval[k]->host[m.dim[k][i]];
+          if (r.dval[k]) v[k] = r.dval[k]->host[c.dim[k][i]];
           continue;
         }
         const int h = (r.key[k]->host[row] - mk[k]) % len[k];
@@ -801,6 +842,12 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
         if (v[k] != 0) A.key[4 * (size_t)h + k] = v[k];
     }
   });
+}
+
+// Adds per-worker accumulators into res (CPU convention: non-zero keys only).
+void cpu_combine(const Resolved& r, std::vector<CpuAcc>& acc, int* res) {
+  const mordred_star_query& q = *r.q;
+  const int G = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
   if (q.kind == MORDRED_KIND_AGGR) {
     long long sum = 0;
     for (auto& A : acc) sum += A.scalar;
@@ -953,7 +1000,7 @@ struct SplitPlan {
 struct GpuPart {
   std::vector<int> gpu_segs;              // cached segment group
   std::future<SplitPlan> streamed;        // the split, from the CPU side
-  std::promise<bool> mixed_done;          // mixed groups' offsets are on the host
+  MixedFeed* feed = nullptr;              // mixed groups' survivors, chunk by chunk
   std::vector<int> res;                   // partial result (res_ints)
   int rc = 0;
   double ms = 0;
@@ -961,15 +1008,12 @@ struct GpuPart {
   int gpu_segments = 0, stream_segments = 0;
 };
 
-// Sets the promise (false) on every exit that has not set it.
-struct PromiseGuard {
-  std::promise<bool>& p;
-  bool set = false;
-  void done(bool v) {
-    if (!set) p.set_value(v);
-    set = true;
+// Ends the feed (as failed) on every exit that has not ended it.
+struct FeedGuard {
+  MixedFeed* f = nullptr;
+  ~FeedGuard() {
+    if (f) f->finish(false);
   }
-  ~PromiseGuard() { done(false); }
 };
 
 // Staging slots borrowed from the free list for one run; on every exit the
@@ -995,6 +1039,7 @@ struct Events {
 inline size_t align64(size_t b) { return (b + 63) & ~size_t(63); }
 
 int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) {
+  FeedGuard feed_guard{g.feed};  // every early return ends the feed as failed
   const auto t0 = Clock::now();
   HIP_OK(hipSetDevice(e->device));
   HIP_OK(e->d_res.reserve(res_ints * sizeof(int)));
@@ -1048,7 +1093,6 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
     g.gpu_segments = (int)g.gpu_segs.size();
   }
 
-  PromiseGuard mixed_signal{g.mixed_done};
   SplitPlan plan = g.streamed.get();
   std::vector<int>& seg_order = plan.stream;
   SlotLoan loan{e, {}};
@@ -1153,7 +1197,7 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
       for (int k = 0; k < 4; k++)
         if (m.gpu_probe[k]) any_p = true, date_only &= k == 3;
       const mordred_stream_t ms = (mordred_stream_t)e->s_main;
-      m.total = 0;
+      m.survivors = 0;
       for (size_t c0 = 0; c0 < m.segs.size(); c0 += kMixedChunk) {
         const size_t c1 = std::min(m.segs.size(), c0 + kMixedChunk);
         std::vector<short> sgl(m.segs.begin() + c0, m.segs.begin() + c1);
@@ -1184,23 +1228,25 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
         int got = 0;
         HIP_OK(hipMemcpyAsync(&got, d_total, sizeof(int), hipMemcpyDeviceToHost, e->s_main));
         HIP_OK(hipStreamSynchronize(e->s_main));
-        const size_t at = (size_t)m.total;
-        m.total += got;
-        m.lo.resize(m.total);
+        MixedChunk ch;
+        ch.group = (int)(&m - plan.mixed->data());
+        ch.lo.resize(got);
         if (got > 0)
-          HIP_OK(hipMemcpyAsync(m.lo.data() + at, dm, (size_t)got * sizeof(int), hipMemcpyDeviceToHost, e->s_main));
+          HIP_OK(hipMemcpyAsync(ch.lo.data(), dm, (size_t)got * sizeof(int), hipMemcpyDeviceToHost, e->s_main));
         for (int k = 0; k < 4; k++) {
           if (!m.gpu_probe[k] || !r.dval[k]) continue;
-          m.dim[k].resize(m.total);
+          ch.dim[k].resize(got);
           if (got > 0)
-            HIP_OK(hipMemcpyAsync(m.dim[k].data() + at, dm + (size_t)(k + 1) * cap, (size_t)got * sizeof(int),
+            HIP_OK(hipMemcpyAsync(ch.dim[k].data(), dm + (size_t)(k + 1) * cap, (size_t)got * sizeof(int),
                                   hipMemcpyDeviceToHost, e->s_main));
         }
         HIP_OK(hipStreamSynchronize(e->s_main));
+        m.survivors += got;
+        g.feed->push(std::move(ch));
       }
     }
   }
-  mixed_signal.done(true);
+  if (g.feed) g.feed->finish(true);
   for (hipStream_t s : {e->s_main, e->s_copy[0], e->s_copy[1]}) HIP_OK(hipStreamSynchronize(s));
   g.res.assign(res_ints, 0);
   HIP_OK(hipMemcpy(g.res.data(), d_res, res_ints * sizeof(int), hipMemcpyDeviceToHost));
@@ -2067,7 +2113,8 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   // ---- GPU half on its own thread (builds + cached group start now)
   std::promise<SplitPlan> streamed;
   g.streamed = streamed.get_future();
-  std::future<bool> mixed_ready = g.mixed_done.get_future();
+  MixedFeed feed;
+  g.feed = &feed;
   std::thread feeder;
   if (use_gpu) feeder = std::thread([&] { g.rc = gpu_part(e, r, g, res_ints); });
 
@@ -2186,16 +2233,19 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   }
   // ---- mixed groups: the CPU operators over the GPU operators' survivors
   if (!mixed.empty()) {
-    if (!mixed_ready.get()) {
+    if (cpu_res.empty()) cpu_res.assign(res_ints, 0);
+    std::vector<CpuAcc> acc(e->pool->size());
+    MixedChunk ch;
+    while (feed.pop(ch)) cpu_suffix(e, r, t, mixed[ch.group], ch, acc);
+    cpu_combine(r, acc, cpu_res.data());
+    if (!feed.ok) {
       if (feeder.joinable()) feeder.join();
       return g.rc ? g.rc : fail(e, "mixed pipeline: GPU part failed");
     }
-    if (cpu_res.empty()) cpu_res.assign(res_ints, 0);
     for (const MixedGroup& m : mixed) {
-      cpu_suffix(e, r, t, m, cpu_res.data());
       st.mixed_segments += (int)m.segs.size();
       st.mixed_rows += m.rows;
-      st.mixed_survivors += m.total;
+      st.mixed_survivors += m.survivors;
     }
   }
   st.cpu_ms = cpu_tables ? ms_since(tc0) : 0.0;
