@@ -122,8 +122,11 @@ def split_bench(args):
     bw_cpu = 42e9 * st["est_cpu_ms_per_segment"] / cpu_ms_seg
     eng.set_bandwidth(bw_cpu, bw_pci)
     runs = {}
-    for name, pl in (("residency", E.PLACE_RESIDENCY), ("cost_split", E.PLACE_COST_SPLIT),
-                     ("gpu_only", E.PLACE_GPU_ONLY), ("cpu_only", E.PLACE_CPU_ONLY)):
+    for name, pl in (("residency", E.PLACE_RESIDENCY), ("cost_split_whole", E.PLACE_COST_SPLIT),
+                     ("cost_split", E.PLACE_COST_SPLIT), ("gpu_only", E.PLACE_GPU_ONLY),
+                     ("cpu_only", E.PLACE_CPU_ONLY)):
+        # cost_split_whole: the split streams whole segments only
+        eng.set_prefix_streaming(name != "cost_split_whole")
         times, last = [], None
         for _ in range(args.steps):
             r, st = eng.run(q, pl)

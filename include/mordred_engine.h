@@ -91,6 +91,14 @@ typedef struct mordred_run_stats {
   long long mixed_rows;             /* fact rows of the mixed segments */
   long long mixed_survivors;        /* rows that left the GPU operators */
   double est_mixed_ms_per_segment;  /* analytic model (COST_SPLIT) */
+  /* COST_SPLIT prefix streaming: of these non-resident segments only the
+   * columns of the fact filters and selective probes crossed PCIe; those
+   * operators ran on the GPU, the CPU operators finished the survivors */
+  int prefix_segments;
+  long long prefix_rows, prefix_survivors;
+  double est_prefix_ms_per_segment;      /* analytic model: transfers */
+  double est_prefix_cpu_ms_per_segment;  /* analytic model: CPU operators */
+  double suffix_ms;                  /* CPU operators over mixed / prefix survivors */
 } mordred_run_stats;
 
 /* Engine on HIP device `device` with an HBM segment cache of cache_segments
@@ -138,6 +146,11 @@ int mordred_engine_store_segment_minmax(mordred_engine* e, const char* name, con
 /* mins / maxs receive one entry per segment (mordred_engine_column_segments). */
 int mordred_engine_segment_minmax(mordred_engine* e, const char* name, int* mins, int* maxs);
 int mordred_engine_set_skipping(mordred_engine* e, int on);
+/* COST_SPLIT prefix streaming (default on): the GPU side of the split may
+ * stream only the columns of the fact filters and selective probes of a
+ * non-resident segment and hand the survivors to the CPU operators, when the
+ * analytic model prices that below streaming the whole segment. */
+int mordred_engine_set_prefix_streaming(mordred_engine* e, int on);
 
 /* ---- learned placement -------------------------------------------------
  * No counterpart in the reference, whose optimizer places segments with the

@@ -179,8 +179,10 @@ struct mordred_engine {
   short* d_identity = nullptr;  // device segment group 0, 1, 2, ...
   double bw_cpu = 42e9, bw_pci = 12e9;  // CostModel.h:5-6 (bytes per second)
   bool skipping = true;                 // segment skipping (main.cu:21)
+  bool prefix_stream = true;            // COST_SPLIT prefix streaming
+  double suffix_ns_per_line = 0;        // measured CPU-operator time per survivor line (0 = none yet)
   std::unordered_map<long long, double> seg_weight;  // (column id << 32 | segment) -> weight
-  DevBuf d_res, d_sg, d_ht[4], d_mix;
+  DevBuf d_res, d_sg, d_ht[4], d_mix, d_pmix;
   HostBuf h_stage;  // pinned scratch for segment groups / index updates
   // shared-scan batches: record tables, descriptor, result arena, the graph
   DevBuf d_rec[4], d_desc, d_arena, d_status;
@@ -569,12 +571,20 @@ double transfer_cost(const Resolved& r, double L, double bw_pci) {
 // the slower of its two concurrent sides; the learned model (an fp32 MLP,
 // mordred_costmodel.h) predicts the whole run's time from the features below
 // and is trained on the measured times of the runs themselves.
-double split_analytic_ms(int n, int k, double t_cpu, double t_str) {
-  return std::max((n - k) * t_cpu, k * t_str);
+// A segment sent to the GPU side costs t_gpu there and, when it is prefix
+// streamed, t_gcpu of CPU operator time over its survivors.
+double split_analytic_ms(int n, int k, double t_cpu, double t_gpu, double t_gcpu = 0) {
+  return std::max((n - k) * t_cpu + k * t_gcpu, k * t_gpu);
+}
+
+// The k of the two sides finishing together: k*t_gpu = (n-k)*t_cpu + k*t_gcpu.
+int split_balance(int n, double t_cpu, double t_gpu, double t_gcpu = 0) {
+  if (n <= 0 || t_gcpu >= t_cpu) return 0;
+  return (int)std::min<double>(n, std::max(0.0, n * t_cpu / (t_gpu + t_cpu - t_gcpu) + 0.5));
 }
 
 void split_features(const mordred_engine* e, const Resolved& r, const Selectivity& sel, int cached, int n, int k,
-                    double t_cpu, double t_str, float* f) {
+                    double t_cpu, double t_str, double t_gcpu, float* f) {
   std::fill(f, f + MORDRED_COST_FEATURES, 0.f);
   const mordred_star_query& q = *r.q;
   int njoins = 0;
@@ -592,8 +602,9 @@ void split_features(const mordred_engine* e, const Resolved& r, const Selectivit
   f[13] = q.kind == MORDRED_KIND_GROUPBY ? 1.f : 0.f;
   f[14] = (float)std::log(std::max(t_cpu, 1e-3));
   f[15] = (float)std::log(std::max(t_str, 1e-3));
-  f[16] = (float)std::log(std::max(split_analytic_ms(n, k, t_cpu, t_str), 1e-3));
+  f[16] = (float)std::log(std::max(split_analytic_ms(n, k, t_cpu, t_str, t_gcpu), 1e-3));
   f[17] = (float)(std::log2((double)std::max(1, e->pool->size())) / 5.0);
+  f[18] = (float)std::log(std::max(t_gcpu, 1e-3));
 }
 
 // ------------------------------------------------------------ CPU operators
@@ -614,6 +625,46 @@ struct MixedGroup {
   long long rows = 0;        // fact rows of the group
   long long survivors = 0;   // rows that left the GPU operators (all chunks)
 };
+
+// Cache lines the CPU operators of a mixed pipeline touch per survivor: one
+// per remaining column or dimension value, two per remaining probe.
+int mixed_lines(const Resolved& r, const MixedGroup& m) {
+  int lines = 1 + (r.a2 ? 2 : 1);
+  for (int i = 0; i < r.q->nfilters; i++) lines += !m.gpu_filter[i];
+  for (int k = 0; k < 4; k++)
+    if (r.join[k]) lines += m.gpu_probe[k] ? (r.dval[k] ? 1 : 0) : 2;
+  return lines;
+}
+
+// Analytic time per segment of a mixed pipeline (CostModel.cu:283-363 costs
+// the GPU operators at zero), as its two sides: `gpu` the transfers (the
+// survivors' offsets and, with prefix set, the GPU operators' fact columns
+// first), `cpu` the CPU operators, one random cache line per survivor and
+// mixed_lines -- priced at the engine's measured time per line once a run
+// has calibrated it, else at the CPU bandwidth.
+struct MixedCost {
+  double gpu = 0, cpu = 0;
+  double total() const { return gpu + cpu; }
+};
+MixedCost mixed_cost(const mordred_engine* e, const Resolved& r, const Selectivity& sel, const MixedGroup& m,
+                     bool prefix) {
+  double surv = 1;
+  int outs = 1, cols = 0;
+  for (int i = 0; i < r.q->nfilters; i++)
+    if (m.gpu_filter[i]) surv *= sel.filt[i], cols++;
+  for (int k = 0; k < 4; k++) {
+    if (!r.join[k] || !m.gpu_probe[k]) continue;
+    surv *= sel.join[k];
+    cols++;
+    outs += r.dval[k] != nullptr;
+  }
+  const double L = (double)kSeg * surv;
+  const double per_line = e->suffix_ns_per_line > 0 ? e->suffix_ns_per_line * 1e-9 : kCacheLine / e->bw_cpu;
+  MixedCost c;
+  c.gpu = (L * 4.0 * outs + (prefix ? (double)kSeg * 4.0 * cols : 0.0)) / e->bw_pci * 1e3;
+  c.cpu = L * mixed_lines(r, m) * per_line * 1e3;
+  return c;
+}
 
 // Survivors of one chunk of a mixed group, on the host.
 struct MixedChunk {
@@ -791,7 +842,11 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
   const mordred_star_query& q = *r.q;
   const int G = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
   const long long total = (long long)c.lo.size();
-  const int tasks = (int)((total + kSuffixTask - 1) / kSuffixTask);
+  // at least two tasks per worker (a chunk of a few segments holds only tens
+  // of thousands of survivors)
+  const long long per = std::max<long long>(
+      1024, std::min<long long>(kSuffixTask, (total + 2 * e->pool->size() - 1) / (2 * e->pool->size())));
+  const int tasks = (int)((total + per - 1) / per);
   int len[4], mk[4], mv[4], uq[4];
   for (int k = 0; k < 4; k++) {
     len[k] = r.join[k] ? r.join[k]->dim_len : 1;
@@ -799,6 +854,17 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
     mv[k] = r.join[k] && r.join[k]->group_col ? r.join[k]->min_val : 0;
     uq[k] = r.join[k] && r.join[k]->group_col ? r.join[k]->unique_val : 0;
   }
+  // fact columns read at random rows (prefetched kPrefetch survivors ahead:
+  // each access is a cache and TLB miss, so overlap several per thread)
+  constexpr int kPrefetch = 16;
+  const int* pcol[8];
+  int npc = 0;
+  for (int f = 0; f < q.nfilters; f++)
+    if (!m.gpu_filter[f]) pcol[npc++] = r.filt[f]->host;
+  for (int k = 0; k < 4; k++)
+    if (r.join[k] && !m.gpu_probe[k]) pcol[npc++] = r.key[k]->host;
+  pcol[npc++] = r.a1->host;
+  if (r.a2) pcol[npc++] = r.a2->host;
   e->pool->run(tasks, [&](int task, int w) {
     CpuAcc& A = acc[w];
     if (q.kind == MORDRED_KIND_GROUPBY && A.sum.empty()) {
@@ -806,8 +872,14 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
       A.key.assign(4 * (size_t)G, 0);
       A.touched.assign(G, 0);
     }
-    const long long b = (long long)task * kSuffixTask, en = std::min(total, b + kSuffixTask);
+    const long long b = (long long)task * per, en = std::min(total, b + per);
     for (long long i = b; i < en; i++) {
+      if (i + kPrefetch < en) {
+        const long long prow = c.lo[i + kPrefetch];
+        for (int j = 0; j < npc; j++) __builtin_prefetch(pcol[j] + prow);
+        for (int k = 0; k < 4; k++)
+          if (m.gpu_probe[k] && r.dval[k]) __builtin_prefetch(r.dval[k]->host + c.dim[k][i + kPrefetch]);
+      }
       const long long row = c.lo[i];
       bool ok = true;
       for (int f = 0; f < q.nfilters && ok; f++) {
@@ -993,8 +1065,10 @@ Selectivity estimate_selectivity(const Resolved& r, const CpuTables& t) {
 // the PCIe stream and the CPU group.  The builds and the cached segment group
 // are issued immediately; the streamed segments once the split is known.
 struct SplitPlan {
-  std::vector<int> stream;          // segments streamed over PCIe
-  std::vector<MixedGroup>* mixed;   // owned by the caller of the run
+  std::vector<int> stream;          // segments streamed whole over PCIe
+  std::vector<MixedGroup>* mixed;   // partly cached groups (owned by the caller of the run)
+  std::vector<int> pstream;         // segments of which only pgroup's GPU operators' columns are streamed
+  MixedGroup* pgroup;               // (owned by the caller of the run)
 };
 
 struct GpuPart {
@@ -1005,7 +1079,8 @@ struct GpuPart {
   int rc = 0;
   double ms = 0;
   long long gpu_rows = 0, stream_rows = 0;
-  int gpu_segments = 0, stream_segments = 0;
+  int gpu_segments = 0, stream_segments = 0, prefix_segments = 0;
+  long long prefix_rows = 0;
 };
 
 // Ends the feed (as failed) on every exit that has not ended it.
@@ -1037,6 +1112,97 @@ struct Events {
 };
 
 inline size_t align64(size_t b) { return (b + 63) & ~size_t(63); }
+
+// The GPU operators of a mixed pipeline over n rows of the segment group d_sg,
+// on stream s: the reference's kernel for the operator set (filter_GPU2 for
+// filters, probe_GPU2 for probes of any tables, filter_probe_GPU2 for filters
+// plus the date probe, else filter_GPU2 then probe_GPU3 over its row ids).
+// fidx / kidx are the segment tables of the filter and key columns (a
+// column's own, or staged ones).  Survivors' row ids go to out.lo_off, the
+// dimension rows of tables with a group value to out.dim_off*, their count to
+// *d_total (zeroed here).
+int mixed_gpu_ops(mordred_engine* e, const Resolved& r, int* const ht[4], const MixedGroup& m, int* const fidx[2],
+                  int* const kidx[4], long long n, const short* d_sg, int* d_total, int* d_flo,
+                  const offsetGPU& out, hipStream_t s) {
+  filterArgsGPU f{};
+  bool any_f = false;
+  for (int i = 0; i < 2; i++) {
+    if (!m.gpu_filter[i]) continue;
+    any_f = true;
+    if (i == 0) {
+      f.filter_idx1 = fidx[0];
+      f.compare1 = r.q->filters[0].lo;
+      f.compare2 = r.q->filters[0].hi;
+      f.mode1 = 1;
+    } else {
+      f.filter_idx2 = fidx[1];
+      f.compare3 = r.q->filters[1].lo;
+      f.compare4 = r.q->filters[1].hi;
+      f.mode2 = 1;
+    }
+  }
+  probeArgsGPU pa{};
+  bool any_p = false, date_only = true;
+  for (int k = 0; k < 4; k++) {
+    if (!m.gpu_probe[k]) continue;
+    any_p = true;
+    date_only &= k == 3;
+    const mordred_join& j = *r.join[k];
+    switch (k) {
+      case 0: pa.key_idx1 = kidx[0]; pa.ht1 = ht[0]; pa.dim_len1 = j.dim_len; pa.min_key1 = j.min_key; break;
+      case 1: pa.key_idx2 = kidx[1]; pa.ht2 = ht[1]; pa.dim_len2 = j.dim_len; pa.min_key2 = j.min_key; break;
+      case 2: pa.key_idx3 = kidx[2]; pa.ht3 = ht[2]; pa.dim_len3 = j.dim_len; pa.min_key3 = j.min_key; break;
+      default: pa.key_idx4 = kidx[3]; pa.ht4 = ht[3]; pa.dim_len4 = j.dim_len; pa.min_key4 = j.min_key; break;
+    }
+  }
+  HIP_OK(hipMemsetAsync(d_total, 0, sizeof(int), s));
+  const mordred_stream_t ms = (mordred_stream_t)s;
+  int krc;
+  if (!any_p) {
+    krc = mordred_filter_GPU2(e->cache, &f, out.lo_off, (int)n, d_total, 0, d_sg, ms);
+  } else if (!any_f) {
+    krc = mordred_probe_GPU2(e->cache, &pa, &out, (int)n, d_total, 0, d_sg, ms);
+  } else if (date_only) {
+    krc = mordred_filter_probe_GPU2(e->cache, &f, &pa, &out, (int)n, d_total, 0, d_sg, ms);
+  } else {
+    krc = mordred_filter_GPU2(e->cache, &f, d_flo, (int)n, d_total, 0, d_sg, ms);
+    if (!krc) {
+      int nf = 0;
+      HIP_OK(hipMemcpyAsync(&nf, d_total, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      HIP_OK(hipMemsetAsync(d_total, 0, sizeof(int), s));
+      offsetGPU in{};
+      in.lo_off = d_flo;
+      krc = mordred_probe_GPU3(e->cache, &in, &pa, &out, nf, d_total, ms);
+    }
+  }
+  if (krc) return hip_fail(e, (hipError_t)krc, "mixed pipeline GPU operators");
+  return 0;
+}
+
+// Copies a finished batch's survivors to the host (waits for stream s) and
+// hands them to the feed.
+int mixed_collect(mordred_engine* e, const Resolved& r, const MixedGroup& m, int group, const int* d_total,
+                  const offsetGPU& out, hipStream_t s, MixedFeed* feed, long long& survivors) {
+  int got = 0;
+  HIP_OK(hipMemcpyAsync(&got, d_total, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  MixedChunk ch;
+  ch.group = group;
+  ch.lo.resize(got);
+  if (got > 0) HIP_OK(hipMemcpyAsync(ch.lo.data(), out.lo_off, (size_t)got * sizeof(int), hipMemcpyDeviceToHost, s));
+  int* const douts[4] = {out.dim_off1, out.dim_off2, out.dim_off3, out.dim_off4};
+  for (int k = 0; k < 4; k++) {
+    if (!m.gpu_probe[k] || !r.dval[k]) continue;
+    ch.dim[k].resize(got);
+    if (got > 0)
+      HIP_OK(hipMemcpyAsync(ch.dim[k].data(), douts[k], (size_t)got * sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  HIP_OK(hipStreamSynchronize(s));
+  survivors += got;
+  feed->push(std::move(ch));
+  return 0;
+}
 
 int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) {
   FeedGuard feed_guard{g.feed};  // every early return ends the feed as failed
@@ -1073,7 +1239,7 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
   const size_t nseg = (size_t)r.nseg, nc = r.fact_cols.size();
   const size_t sg_bytes = align64(nseg * sizeof(short)), tab_bytes = align64(nc * nseg * sizeof(int));
   const size_t batch_bytes = align64(kStreamBatch * sizeof(short));
-  const size_t max_batches = (nseg + kStreamBatch - 1) / kStreamBatch;
+  const size_t max_batches = nseg / kStreamBatch + 2;  // whole and prefix batches
   const size_t scratch = sg_bytes + tab_bytes + max_batches * batch_bytes;
   HIP_OK(e->h_stage.reserve(scratch));
   HIP_OK(e->d_sg.reserve(scratch));
@@ -1095,14 +1261,29 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
 
   SplitPlan plan = g.streamed.get();
   std::vector<int>& seg_order = plan.stream;
+  std::vector<int>& pseg = plan.pstream;
   SlotLoan loan{e, {}};
-  if (!seg_order.empty()) {
+  if (!seg_order.empty() || !pseg.empty()) {
     // Two staging sets of kStreamBatch segments x fact columns.  Batch i uses
     // set i % 2 on copy stream i % 2, so a set is refilled only after the same
     // stream has run the kernel of the batch before it.  The staged tables map
     // every streamed segment to the slot of its batch; other entries keep the
     // column's own slot (or 0) and are not read by the streamed launches.
+    // Whole-segment batches come first, then the prefix batches, whose
+    // segments carry only the columns of the prefix group's GPU operators.
     order_last_partial(r, seg_order);
+    order_last_partial(r, pseg);
+    struct Batch {
+      bool prefix;
+      size_t first, cnt;  // into `all`
+    };
+    std::vector<int> all(seg_order);
+    all.insert(all.end(), pseg.begin(), pseg.end());
+    std::vector<Batch> batches;
+    for (size_t i = 0; i < seg_order.size(); i += kStreamBatch)
+      batches.push_back({false, i, std::min<size_t>(kStreamBatch, seg_order.size() - i)});
+    for (size_t i = 0; i < pseg.size(); i += kStreamBatch)
+      batches.push_back({true, seg_order.size() + i, std::min<size_t>(kStreamBatch, pseg.size() - i)});
     for (size_t i = 0; i < 2 * kStreamBatch * nc; i++) {
       loan.slots.push_back(e->free_slots.back());
       e->free_slots.pop_back();
@@ -1112,10 +1293,9 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
     for (size_t c = 0; c < nc; c++) {
       const std::vector<int>& mirror = r.fact_cols[c]->slot;
       for (size_t s = 0; s < nseg; s++) h_tab[c * nseg + s] = std::max(0, mirror[s]);
-      for (size_t i = 0; i < seg_order.size(); i++) {
-        const size_t bi = i / kStreamBatch, within = i % kStreamBatch;
-        h_tab[c * nseg + seg_order[i]] = loan.slots[((bi & 1) * kStreamBatch + within) * nc + c];
-      }
+      for (size_t bi = 0; bi < batches.size(); bi++)
+        for (size_t i = 0; i < batches[bi].cnt; i++)
+          h_tab[c * nseg + all[batches[bi].first + i]] = loan.slots[((bi & 1) * kStreamBatch + i) * nc + c];
     }
     for (int b = 0; b < 2; b++) HIP_OK(hipStreamWaitEvent(e->s_copy[b], ev.built, 0));
     HIP_OK(hipMemcpyAsync(d_tab, h_tab, nc * nseg * sizeof(int), hipMemcpyHostToDevice, e->s_copy[0]));
@@ -1124,33 +1304,90 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
     HIP_OK(hipStreamWaitEvent(e->s_copy[1], ev.tables, 0));
     GpuArgs as;
     gpu_args(r, ht, d_tab, as);
+    // prefix batches: the staged tables of the operators' columns, the columns
+    // to copy, and per staging set an output area [row ids | dimension rows x 4
+    // | filtered row ids] of kStreamBatch segments, the count
+    auto staged_of = [&](const Column* c) -> int* {
+      for (size_t i = 0; i < nc; i++)
+        if (r.fact_cols[i] == c) return d_tab + i * nseg;
+      return nullptr;
+    };
+    int* pf[2] = {nullptr, nullptr};
+    int* pk[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<char> pcopy(nc, 0);
+    const size_t pcap = (size_t)kStreamBatch * kSeg;
+    offsetGPU pout[2] = {};
+    int* ptotal[2] = {nullptr, nullptr};
+    int* pflo[2] = {nullptr, nullptr};
+    if (!pseg.empty()) {
+      const MixedGroup& m = *plan.pgroup;
+      for (int i = 0; i < 2; i++)
+        if (m.gpu_filter[i]) pf[i] = staged_of(r.filt[i]);
+      for (int k = 0; k < 4; k++)
+        if (m.gpu_probe[k]) pk[k] = staged_of(r.key[k]);
+      for (size_t c = 0; c < nc; c++) {
+        for (int i = 0; i < 2; i++) pcopy[c] |= m.gpu_filter[i] && r.fact_cols[c] == r.filt[i];
+        for (int k = 0; k < 4; k++) pcopy[c] |= m.gpu_probe[k] && r.fact_cols[c] == r.key[k];
+      }
+      const size_t per_set = 6 * pcap + 16;
+      HIP_OK(e->d_pmix.reserve(2 * per_set * sizeof(int)));
+      for (int set = 0; set < 2; set++) {
+        int* base = static_cast<int*>(e->d_pmix.p) + set * per_set;
+        pout[set].lo_off = base;
+        int** douts[4] = {&pout[set].dim_off1, &pout[set].dim_off2, &pout[set].dim_off3, &pout[set].dim_off4};
+        for (int k = 0; k < 4; k++)
+          if (m.gpu_probe[k] && r.dval[k]) *douts[k] = base + (size_t)(k + 1) * pcap;
+        pflo[set] = base + 5 * pcap;
+        ptotal[set] = base + 6 * pcap;
+      }
+    }
     char* hb = hs + sg_bytes + tab_bytes;
     char* db = ds + sg_bytes + tab_bytes;
-    const size_t nbatches = (seg_order.size() + kStreamBatch - 1) / kStreamBatch;
-    for (size_t bi = 0; bi < nbatches; bi++) {
+    long long psurv = 0;
+    int pending = -1;  // prefix batch whose survivors are still on the device
+    for (size_t bi = 0; bi < batches.size(); bi++) {
       const int set = (int)(bi & 1);
       hipStream_t s = e->s_copy[set];
-      const size_t first = bi * kStreamBatch;
-      const size_t cnt = std::min<size_t>(kStreamBatch, seg_order.size() - first);
+      const Batch& B = batches[bi];
       short* h_bsg = reinterpret_cast<short*>(hb + bi * batch_bytes);
       short* d_bsg = reinterpret_cast<short*>(db + bi * batch_bytes);
       long long rows = 0;
-      for (size_t i = 0; i < cnt; i++) {
-        const int seg = seg_order[first + i];
+      for (size_t i = 0; i < B.cnt; i++) {
+        const int seg = all[B.first + i];
         h_bsg[i] = (short)seg;
         rows += segment_rows(r, seg);
         for (size_t c = 0; c < nc; c++) {
+          if (B.prefix && !pcopy[c]) continue;
           const int slot = loan.slots[(set * kStreamBatch + i) * nc + c];
           HIP_OK(hipMemcpyAsync(e->cache + (size_t)slot * kSeg, r.fact_cols[c]->host + (size_t)seg * kSeg,
                                 segment_rows(r, seg) * sizeof(int), hipMemcpyHostToDevice, s));
         }
       }
-      HIP_OK(hipMemcpyAsync(d_bsg, h_bsg, cnt * sizeof(short), hipMemcpyHostToDevice, s));
-      const int frc = launch_fact(e, r, as, (int)rows, d_res, d_bsg, s);
-      if (frc) return frc;
-      g.stream_rows += rows;
+      HIP_OK(hipMemcpyAsync(d_bsg, h_bsg, B.cnt * sizeof(short), hipMemcpyHostToDevice, s));
+      if (!B.prefix) {
+        const int frc = launch_fact(e, r, as, (int)rows, d_res, d_bsg, s);
+        if (frc) return frc;
+        g.stream_rows += rows;
+        continue;
+      }
+      int rc = mixed_gpu_ops(e, r, ht, *plan.pgroup, pf, pk, rows, d_bsg, ptotal[set], pflo[set], pout[set], s);
+      if (rc) return rc;
+      g.prefix_rows += rows;
+      // the previous prefix batch (other set) finishes while this one runs
+      if (pending >= 0) {
+        rc = mixed_collect(e, r, *plan.pgroup, -1, ptotal[pending], pout[pending], e->s_copy[pending], g.feed, psurv);
+        if (rc) return rc;
+      }
+      pending = set;
     }
+    if (pending >= 0) {
+      const int rc =
+          mixed_collect(e, r, *plan.pgroup, -1, ptotal[pending], pout[pending], e->s_copy[pending], g.feed, psurv);
+      if (rc) return rc;
+    }
+    if (plan.pgroup) plan.pgroup->survivors = psurv;
     g.stream_segments = (int)seg_order.size();
+    g.prefix_segments = (int)pseg.size();
   }
   // mixed groups: the GPU operators into offset lists, then to the host, in
   // chunks of at most kMixedChunk segments (bounded device scratch)
@@ -1164,39 +1401,21 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
     int* d_total = dm + 5 * cap;
     int* d_flo = dm + 5 * cap + 16;
     short* d_msg = reinterpret_cast<short*>(dm + 6 * cap + 16);
-    for (MixedGroup& m : *plan.mixed) {
+    for (size_t gi = 0; gi < plan.mixed->size(); gi++) {
+      MixedGroup& m = (*plan.mixed)[gi];
       order_last_partial(r, m.segs);
-      filterArgsGPU f{};
-      bool any_f = false;
-      for (int i = 0; i < 2; i++) {
-        if (!m.gpu_filter[i]) continue;
-        any_f = true;
-        int* const idx = r.filt[i]->d_idx;
-        if (i == 0) { f.filter_idx1 = idx; f.compare1 = r.q->filters[i].lo; f.compare2 = r.q->filters[i].hi; f.mode1 = 1; }
-        else { f.filter_idx2 = idx; f.compare3 = r.q->filters[i].lo; f.compare4 = r.q->filters[i].hi; f.mode2 = 1; }
-      }
-      probeArgsGPU pa{};
+      int* fidx[2] = {nullptr, nullptr};
+      int* kidx[4] = {nullptr, nullptr, nullptr, nullptr};
+      for (int i = 0; i < 2; i++)
+        if (m.gpu_filter[i]) fidx[i] = r.filt[i]->d_idx;
       offsetGPU out{};
       out.lo_off = dm;
+      int** douts[4] = {&out.dim_off1, &out.dim_off2, &out.dim_off3, &out.dim_off4};
       for (int k = 0; k < 4; k++) {
         if (!m.gpu_probe[k]) continue;
-        const mordred_join& j = *r.join[k];
-        int* key = r.key[k]->d_idx;
-        int* dout = r.dval[k] ? dm + (size_t)(k + 1) * cap : nullptr;
-        switch (k) {
-          case 0: pa.key_idx1 = key; pa.ht1 = ht[0]; pa.dim_len1 = j.dim_len; pa.min_key1 = j.min_key; out.dim_off1 = dout; break;
-          case 1: pa.key_idx2 = key; pa.ht2 = ht[1]; pa.dim_len2 = j.dim_len; pa.min_key2 = j.min_key; out.dim_off2 = dout; break;
-          case 2: pa.key_idx3 = key; pa.ht3 = ht[2]; pa.dim_len3 = j.dim_len; pa.min_key3 = j.min_key; out.dim_off3 = dout; break;
-          default: pa.key_idx4 = key; pa.ht4 = ht[3]; pa.dim_len4 = j.dim_len; pa.min_key4 = j.min_key; out.dim_off4 = dout; break;
-        }
+        kidx[k] = r.key[k]->d_idx;
+        if (r.dval[k]) *douts[k] = dm + (size_t)(k + 1) * cap;
       }
-      // the reference's kernel for each operator set: filter_GPU2 (filters),
-      // probe_GPU2 (probes of any tables), filter_probe_GPU2 (filters + the
-      // date probe), or filter_GPU2 then probe_GPU3 over its row ids
-      bool any_p = false, date_only = true;
-      for (int k = 0; k < 4; k++)
-        if (m.gpu_probe[k]) any_p = true, date_only &= k == 3;
-      const mordred_stream_t ms = (mordred_stream_t)e->s_main;
       m.survivors = 0;
       for (size_t c0 = 0; c0 < m.segs.size(); c0 += kMixedChunk) {
         const size_t c1 = std::min(m.segs.size(), c0 + kMixedChunk);
@@ -1204,45 +1423,9 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
         long long rows = 0;
         for (size_t i = c0; i < c1; i++) rows += segment_rows(r, m.segs[i]);
         HIP_OK(hipMemcpyAsync(d_msg, sgl.data(), sgl.size() * sizeof(short), hipMemcpyHostToDevice, e->s_main));
-        HIP_OK(hipMemsetAsync(d_total, 0, sizeof(int), e->s_main));
-        int krc;
-        if (!any_p) {
-          krc = mordred_filter_GPU2(e->cache, &f, dm, (int)rows, d_total, 0, d_msg, ms);
-        } else if (!any_f) {
-          krc = mordred_probe_GPU2(e->cache, &pa, &out, (int)rows, d_total, 0, d_msg, ms);
-        } else if (date_only) {
-          krc = mordred_filter_probe_GPU2(e->cache, &f, &pa, &out, (int)rows, d_total, 0, d_msg, ms);
-        } else {
-          krc = mordred_filter_GPU2(e->cache, &f, d_flo, (int)rows, d_total, 0, d_msg, ms);
-          int nf = 0;
-          if (!krc) {
-            HIP_OK(hipMemcpyAsync(&nf, d_total, sizeof(int), hipMemcpyDeviceToHost, e->s_main));
-            HIP_OK(hipStreamSynchronize(e->s_main));
-            HIP_OK(hipMemsetAsync(d_total, 0, sizeof(int), e->s_main));
-            offsetGPU in{};
-            in.lo_off = d_flo;
-            krc = mordred_probe_GPU3(e->cache, &in, &pa, &out, nf, d_total, ms);
-          }
-        }
-        if (krc) return hip_fail(e, (hipError_t)krc, "mixed pipeline GPU operators");
-        int got = 0;
-        HIP_OK(hipMemcpyAsync(&got, d_total, sizeof(int), hipMemcpyDeviceToHost, e->s_main));
-        HIP_OK(hipStreamSynchronize(e->s_main));
-        MixedChunk ch;
-        ch.group = (int)(&m - plan.mixed->data());
-        ch.lo.resize(got);
-        if (got > 0)
-          HIP_OK(hipMemcpyAsync(ch.lo.data(), dm, (size_t)got * sizeof(int), hipMemcpyDeviceToHost, e->s_main));
-        for (int k = 0; k < 4; k++) {
-          if (!m.gpu_probe[k] || !r.dval[k]) continue;
-          ch.dim[k].resize(got);
-          if (got > 0)
-            HIP_OK(hipMemcpyAsync(ch.dim[k].data(), dm + (size_t)(k + 1) * cap, (size_t)got * sizeof(int),
-                                  hipMemcpyDeviceToHost, e->s_main));
-        }
-        HIP_OK(hipStreamSynchronize(e->s_main));
-        m.survivors += got;
-        g.feed->push(std::move(ch));
+        int rc = mixed_gpu_ops(e, r, ht, m, fidx, kidx, rows, d_msg, d_total, d_flo, out, e->s_main);
+        if (!rc) rc = mixed_collect(e, r, m, (int)gi, d_total, out, e->s_main, g.feed, m.survivors);
+        if (rc) return rc;
       }
     }
   }
@@ -1999,6 +2182,12 @@ int mordred_engine_set_skipping(mordred_engine* e, int on) {
   return 0;
 }
 
+int mordred_engine_set_prefix_streaming(mordred_engine* e, int on) {
+  if (!e) return -1;
+  e->prefix_stream = on != 0;
+  return 0;
+}
+
 int mordred_engine_place(mordred_engine* e, const mordred_star_query* q, double fraction) {
   if (!e) return -1;
   if (e->device < 0) return fail(e, "host-only engine has no segment cache");
@@ -2143,23 +2332,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     // line per survivor and column (random access)
     std::vector<MixedGroup> keep;
     for (MixedGroup& m : mixed) {
-      double surv = 1;
-      int outs = 1, lines = 1 + (r.a2 ? 2 : 1);
-      for (int i = 0; i < r.q->nfilters; i++) {
-        if (m.gpu_filter[i]) surv *= sel.filt[i];
-        else lines++;
-      }
-      for (int k = 0; k < 4; k++) {
-        if (!r.join[k]) continue;
-        if (m.gpu_probe[k]) {
-          surv *= sel.join[k];
-          if (r.dval[k]) outs++, lines++;
-        } else {
-          lines += 2;
-        }
-      }
-      const double L = (double)kSeg * surv;
-      const double t_mix = (L * 4.0 * outs / e->bw_pci + L * kCacheLine * lines / e->bw_cpu) * 1e3;
+      const double t_mix = mixed_cost(e, r, sel, m, false).total();
       st.est_mixed_ms_per_segment = t_mix;
       if (t_mix < std::min(t_cpu, t_str)) keep.push_back(std::move(m));
       else other.insert(other.end(), m.segs.begin(), m.segs.end());
@@ -2167,13 +2340,37 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     mixed.swap(keep);
     std::sort(other.begin(), other.end());
   }
+  // prefix streaming: of a whole non-resident segment, only the columns of the
+  // fact filters and of the selective probes (those keeping under half of
+  // the dimension) cross PCIe; those operators run on the GPU and the CPU
+  // operators finish the survivors, as in a mixed pipeline
+  // (the survivors' CPU operators run on the CPU side of the split)
+  MixedGroup pgroup;
+  std::vector<int> pstream_segs;
+  bool prefix_plan = false;
+  if (placement == MORDRED_PLACE_COST_SPLIT && e->prefix_stream && !other.empty()) {
+    bool any = false;
+    for (int i = 0; i < r.q->nfilters; i++) any |= pgroup.gpu_filter[i] = true;
+    for (int k = 0; k < 4; k++) any |= pgroup.gpu_probe[k] = r.join[k] && sel.join[k] < 0.5;
+    if (any) {
+      const MixedCost pc = mixed_cost(e, r, sel, pgroup, true);
+      st.est_prefix_ms_per_segment = pc.gpu;
+      st.est_prefix_cpu_ms_per_segment = pc.cpu;
+      const int n = (int)other.size();
+      const int kw = split_balance(n, t_cpu, t_str), kp = split_balance(n, t_cpu, pc.gpu, pc.cpu);
+      prefix_plan = split_analytic_ms(n, kp, t_cpu, pc.gpu, pc.cpu) < split_analytic_ms(n, kw, t_cpu, t_str);
+    }
+  }
+  // the GPU side's time per segment, and the CPU time it leaves per segment
+  const double t_gpu = prefix_plan ? st.est_prefix_ms_per_segment : t_str;
+  const double t_gcpu = prefix_plan ? st.est_prefix_cpu_ms_per_segment : 0.0;
   if (placement == MORDRED_PLACE_GPU_ONLY) {
     stream_segs = other;
   } else if (placement == MORDRED_PLACE_COST_SPLIT) {
-    // both sides run concurrently: stream k segments so that k*t_str and
-    // (n-k)*t_cpu finish together
+    // both sides run concurrently: send k segments to the GPU side so that
+    // both finish together
     const int n = (int)other.size();
-    int k = (int)std::min<double>(n, std::max(0.0, n * t_cpu / (t_cpu + t_str) + 0.5));
+    int k = split_balance(n, t_cpu, t_gpu, t_gcpu);
     if (e->cm_place && e->place_mode > 0 && n > 0) {
       // candidate plans: k = n*c/8 for c = 0..8, and the analytic k
       std::vector<int> ks;
@@ -2183,7 +2380,8 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
       ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
       std::vector<float> xs(ks.size() * MORDRED_COST_FEATURES), pred(ks.size());
       for (size_t c = 0; c < ks.size(); c++)
-        split_features(e, r, sel, (int)g.gpu_segs.size(), n, ks[c], t_cpu, t_str, &xs[c * MORDRED_COST_FEATURES]);
+        split_features(e, r, sel, (int)g.gpu_segs.size(), n, ks[c], t_cpu, t_gpu, t_gcpu,
+                       &xs[c * MORDRED_COST_FEATURES]);
       mordred_cost_model_stats cs{};
       // a model error leaves the analytic plan (the GPU half already runs)
       if (mordred_cost_model_get_stats(e->cm_place, &cs) == 0 && cs.trusted &&
@@ -2205,23 +2403,26 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
           st.plan_model = 2;
         }
       }
-      st.plan_analytic_ms = split_analytic_ms(n, k, t_cpu, t_str);
+      st.plan_analytic_ms = split_analytic_ms(n, k, t_cpu, t_gpu, t_gcpu);
       st.plan_streamed = k;
     }
-    stream_segs.assign(other.begin(), other.begin() + k);
+    if (prefix_plan) pstream_segs.assign(other.begin(), other.begin() + k);
+    else stream_segs.assign(other.begin(), other.begin() + k);
     cpu_segs.assign(other.begin() + k, other.end());
   } else {
     cpu_segs = other;
   }
-  if (!stream_segs.empty() && e->free_slots.size() < staging) {
+  if ((!stream_segs.empty() || !pstream_segs.empty()) && e->free_slots.size() < staging) {
     // not enough free cache slots for two staging batches: CPU instead
     if (!cpu_tables) cpu_build(e, r, t);
     cpu_tables = true;
     cpu_segs.insert(cpu_segs.end(), stream_segs.begin(), stream_segs.end());
+    cpu_segs.insert(cpu_segs.end(), pstream_segs.begin(), pstream_segs.end());
     stream_segs.clear();
+    pstream_segs.clear();
   }
   std::sort(cpu_segs.begin(), cpu_segs.end());
-  streamed.set_value(SplitPlan{stream_segs, &mixed});
+  streamed.set_value(SplitPlan{stream_segs, &mixed, pstream_segs, &pgroup});
 
   // ---- CPU: the CPU segment group, concurrently with the GPU
   std::vector<int> cpu_res;
@@ -2232,20 +2433,31 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     st.cpu_segments = (int)cpu_segs.size();
   }
   // ---- mixed groups: the CPU operators over the GPU operators' survivors
-  if (!mixed.empty()) {
+  if (!mixed.empty() || !pstream_segs.empty()) {
     if (cpu_res.empty()) cpu_res.assign(res_ints, 0);
     std::vector<CpuAcc> acc(e->pool->size());
     MixedChunk ch;
-    while (feed.pop(ch)) cpu_suffix(e, r, t, mixed[ch.group], ch, acc);
+    while (feed.pop(ch)) {
+      const auto ts0 = Clock::now();
+      cpu_suffix(e, r, t, ch.group < 0 ? pgroup : mixed[ch.group], ch, acc);
+      st.suffix_ms += ms_since(ts0);
+    }
     cpu_combine(r, acc, cpu_res.data());
     if (!feed.ok) {
       if (feeder.joinable()) feeder.join();
       return g.rc ? g.rc : fail(e, "mixed pipeline: GPU part failed");
     }
+    double lines = (double)pgroup.survivors * mixed_lines(r, pgroup);
     for (const MixedGroup& m : mixed) {
       st.mixed_segments += (int)m.segs.size();
       st.mixed_rows += m.rows;
       st.mixed_survivors += m.survivors;
+      lines += (double)m.survivors * mixed_lines(r, m);
+    }
+    // calibrate the CPU operators' time per survivor line (moving average)
+    if (lines >= (1 << 18) && st.suffix_ms > 0) {
+      const double ns = st.suffix_ms * 1e6 / lines;
+      e->suffix_ns_per_line = e->suffix_ns_per_line > 0 ? 0.7 * e->suffix_ns_per_line + 0.3 * ns : ns;
     }
   }
   st.cpu_ms = cpu_tables ? ms_since(tc0) : 0.0;
@@ -2258,6 +2470,9 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   st.gpu_segments = g.gpu_segments;
   st.stream_rows = g.stream_rows;
   st.stream_segments = g.stream_segments;
+  st.prefix_segments = g.prefix_segments;
+  st.prefix_survivors = pgroup.survivors;
+  st.prefix_rows = g.prefix_rows;
   const auto tm0 = Clock::now();
   if (!cpu_res.empty() && !g.res.empty()) {
     if (q->kind == MORDRED_KIND_GROUPBY) merge(cpu_res.data(), g.res.data(), G);
@@ -2272,9 +2487,9 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   st.total_ms = ms_since(t0);
   // learn from the measured run (a plan the model was given)
   if (placement == MORDRED_PLACE_COST_SPLIT && e->cm_place && e->place_mode > 0 && !other.empty() &&
-      st.plan_streamed == st.stream_segments) {
+      st.plan_streamed == st.stream_segments + st.prefix_segments) {
     float f[MORDRED_COST_FEATURES];
-    split_features(e, r, sel, st.gpu_segments, (int)other.size(), st.plan_streamed, t_cpu, t_str, f);
+    split_features(e, r, sel, st.gpu_segments, (int)other.size(), st.plan_streamed, t_cpu, t_gpu, t_gcpu, f);
     if (mordred_cost_model_observe(e->cm_place, f, std::max(1e-3, st.total_ms), std::max(1e-3, st.plan_analytic_ms)))
       return fail(e, std::string("placement model: ") + mordred_cost_model_last_error(e->cm_place));
     if (++e->place_since_train >= 8) {

@@ -40,7 +40,10 @@ class RunStats(C.Structure):
                 ("skipped_segments", C.c_int), ("plan_model", C.c_int), ("plan_streamed", C.c_int),
                 ("plan_predicted_ms", C.c_double), ("plan_analytic_ms", C.c_double),
                 ("mixed_segments", C.c_int), ("mixed_rows", C.c_longlong), ("mixed_survivors", C.c_longlong),
-                ("est_mixed_ms_per_segment", C.c_double)]
+                ("est_mixed_ms_per_segment", C.c_double), ("prefix_segments", C.c_int),
+                ("prefix_rows", C.c_longlong), ("prefix_survivors", C.c_longlong),
+                ("est_prefix_ms_per_segment", C.c_double), ("est_prefix_cpu_ms_per_segment", C.c_double),
+                ("suffix_ms", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -139,6 +142,7 @@ _SIGS = {
     "mordred_engine_store_segment_minmax": ([C.c_void_p, C.c_char_p, C.c_char_p], C.c_int),
     "mordred_engine_segment_minmax": ([C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p], C.c_int),
     "mordred_engine_set_skipping": ([C.c_void_p, C.c_int], C.c_int),
+    "mordred_engine_set_prefix_streaming": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_set_learned_placement": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_placement_model_stats": ([C.c_void_p, C.POINTER(PlacementStats)], C.c_int),
     "mordred_engine_place": ([C.c_void_p, C.POINTER(StarQueryC), C.c_double], C.c_int),
@@ -270,6 +274,9 @@ class Engine:
 
     def set_skipping(self, on: bool):
         self._check(self.lib.mordred_engine_set_skipping(self.h, int(on)), "set_skipping")
+
+    def set_prefix_streaming(self, on: bool):
+        self._check(self.lib.mordred_engine_set_prefix_streaming(self.h, int(on)), "set_prefix_streaming")
 
     # ---- learned placement
     def set_learned_placement(self, mode: int):

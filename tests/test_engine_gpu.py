@@ -36,7 +36,7 @@ def test_placements_match_oracle(gengine, placement):
     eng.place(q, 0.5)
     res, st = eng.run(q, placement)
     np.testing.assert_array_equal(res, ref)
-    assert st["gpu_rows"] + st["cpu_rows"] + st["stream_rows"] == data.n
+    assert st["gpu_rows"] + st["cpu_rows"] + st["stream_rows"] + st["prefix_rows"] == data.n
     if placement == E.PLACE_RESIDENCY:
         assert st["gpu_segments"] == 3 and st["cpu_segments"] == 3 and st["stream_segments"] == 0
     if placement == E.PLACE_GPU_ONLY:
@@ -114,7 +114,8 @@ def test_skipping_on_gpu_placements(sorted_engine, placement):
     np.testing.assert_array_equal(res, ref)
     skipped = st["skipped_segments"]
     assert skipped >= 6
-    assert st["gpu_segments"] + st["cpu_segments"] + st["stream_segments"] == 10 - skipped
+    assert (st["gpu_segments"] + st["cpu_segments"] + st["stream_segments"] + st["prefix_segments"]
+            == 10 - skipped)
     if placement == E.PLACE_RESIDENCY:
         assert st["cpu_segments"] == 0 and st["gpu_segments"] == 10 - skipped
     eng.set_skipping(False)
@@ -140,7 +141,7 @@ def test_learned_split_plans_are_exact(gengine):
         np.testing.assert_array_equal(res, ref)
         seen_models.add(st["plan_model"])
         seen_k.add(st["plan_streamed"])
-        assert st["plan_streamed"] == st["stream_segments"]
+        assert st["plan_streamed"] == st["stream_segments"] + st["prefix_segments"]
         assert st["plan_analytic_ms"] >= 0
     ms = eng.placement_model_stats()
     assert ms["samples"] >= runs and ms["scored"] >= runs - 1
@@ -211,7 +212,8 @@ def test_mixed_with_cached_and_plain_segments(fresh_engine):
     res, st = eng.run(q, E.PLACE_COST_SPLIT)
     np.testing.assert_array_equal(res, ref)
     assert st["gpu_segments"] == 2
-    assert st["mixed_segments"] + st["cpu_segments"] + st["stream_segments"] == nseg - 2
+    assert (st["mixed_segments"] + st["cpu_segments"] + st["stream_segments"] + st["prefix_segments"]
+            == nseg - 2)
     res, st = eng.run(q, E.PLACE_GPU_ONLY)
     np.testing.assert_array_equal(res, ref)
     assert st["mixed_segments"] == 0 and st["stream_segments"] == nseg - 2
@@ -236,3 +238,32 @@ def test_mixed_filter_then_probe(cuda):
     np.testing.assert_array_equal(res, ref)
     assert st["mixed_segments"] == nseg and st["gpu_segments"] == 0
     eng.close()
+
+
+def test_prefix_streaming_cost_split(fresh_engine):
+    """COST_SPLIT over an uncached table: when the analytic model prices it
+    below whole-segment streaming, the GPU side of the split streams only the
+    columns of the fact filters and selective probes, runs those operators,
+    and the CPU operators finish the survivors."""
+    q, data, eng, ref = fresh_engine
+    print(f"[gpu] prefix streaming {q.name}", flush=True)
+    _uncache_all(eng, q)
+    res, st = eng.run(q, E.PLACE_COST_SPLIT)
+    np.testing.assert_array_equal(res, ref)
+    nseg = eng.segments(q.fact_columns()[0])
+    assert st["gpu_segments"] == 0 and st["mixed_segments"] == 0
+    assert st["cpu_segments"] + st["stream_segments"] + st["prefix_segments"] == nseg
+    assert st["cpu_rows"] + st["stream_rows"] + st["prefix_rows"] == data.n
+    # one plan or the other for the GPU side of the split
+    assert st["stream_segments"] == 0 or st["prefix_segments"] == 0
+    if len(q.joins) > 1:
+        # the part and supplier probes keep a small fraction of the rows
+        assert 0 < st["est_prefix_ms_per_segment"] < st["est_stream_ms_per_segment"]
+        assert st["prefix_segments"] > 0
+        assert 0 < st["prefix_survivors"] < st["prefix_rows"] // 10
+    eng.set_prefix_streaming(False)
+    res, st = eng.run(q, E.PLACE_COST_SPLIT)
+    eng.set_prefix_streaming(True)
+    np.testing.assert_array_equal(res, ref)
+    assert st["prefix_segments"] == 0
+    assert st["cpu_segments"] + st["stream_segments"] == nseg
