@@ -69,6 +69,25 @@ enum {
   MORDRED_PLACE_CPU_ONLY = 3    /* every segment on the CPU operators */
 };
 
+/* Operator order of a partly cached segment group: which device runs each
+ * filter, probe and the group-by, and in what order (each change of device
+ * copies the row ids and dimension rows kept so far over PCIe, as
+ * switch_device_fact, CPUGPUProcessing.cu:114-136).  A filter or probe runs
+ * on the GPU iff its fact column is cached in every segment of the group. */
+enum {
+  /* every GPU filter and probe first, then one switch; the CPU operators
+   * finish, group-by included (default) */
+  MORDRED_ORDER_GPU_FIRST = 0,
+  /* executeTableFact_v1 (QueryProcessing.cu:201-437): CPU filters, GPU
+   * filters and probes, CPU probes, then the group-by on the GPU when its
+   * fact columns are cached, else on the CPU */
+  MORDRED_ORDER_CPU_FIRST = 1,
+  /* executeTableFact_v2 (QueryProcessing.cu:676-914): GPU filters, CPU
+   * filters and probes, GPU probes, then the group-by as above */
+  MORDRED_ORDER_ALTERNATE = 2
+};
+int mordred_engine_set_operator_order(mordred_engine* e, int order);
+
 typedef struct mordred_run_stats {
   double total_ms;         /* whole query, builds to merged result */
   double gpu_ms;           /* GPU part: builds, cached groups, streamed groups */
@@ -99,6 +118,8 @@ typedef struct mordred_run_stats {
   double est_prefix_ms_per_segment;      /* analytic model: transfers */
   double est_prefix_cpu_ms_per_segment;  /* analytic model: CPU operators */
   double suffix_ms;                  /* CPU operators over mixed / prefix survivors */
+  int operator_order;                /* MORDRED_ORDER_* the mixed groups ran with */
+  int order_switches;                /* device switches of ordered mixed pipelines (all chunks) */
 } mordred_run_stats;
 
 /* Engine on HIP device `device` with an HBM segment cache of cache_segments

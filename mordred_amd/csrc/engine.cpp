@@ -181,8 +181,11 @@ struct mordred_engine {
   bool skipping = true;                 // segment skipping (main.cu:21)
   bool prefix_stream = true;            // COST_SPLIT prefix streaming
   double suffix_ns_per_line = 0;        // measured CPU-operator time per survivor line (0 = none yet)
+  int op_order = MORDRED_ORDER_GPU_FIRST;  // operator order of partly cached segment groups
+  hipStream_t s_order = nullptr;        // the calling thread's stream for ordered pipelines
+  hipEvent_t ev_built = nullptr;        // recorded after a run's hash-table builds
   std::unordered_map<long long, double> seg_weight;  // (column id << 32 | segment) -> weight
-  DevBuf d_res, d_sg, d_ht[4], d_mix, d_pmix;
+  DevBuf d_res, d_sg, d_ht[4], d_mix, d_pmix, d_ord, d_res_ord;
   HostBuf h_stage;  // pinned scratch for segment groups / index updates
   // shared-scan batches: record tables, descriptor, result arena, the graph
   DevBuf d_rec[4], d_desc, d_arena, d_status;
@@ -1081,13 +1084,24 @@ struct GpuPart {
   long long gpu_rows = 0, stream_rows = 0;
   int gpu_segments = 0, stream_segments = 0, prefix_segments = 0;
   long long prefix_rows = 0;
+  // the hash tables are built (e->ev_built recorded): 0, or -1 if the GPU
+  // side failed first
+  std::promise<int> built;
+  bool built_set = false;
+  void signal_built(int v) {
+    if (built_set) return;
+    built_set = true;
+    built.set_value(v);
+  }
 };
 
 // Ends the feed (as failed) on every exit that has not ended it.
 struct FeedGuard {
   MixedFeed* f = nullptr;
+  GpuPart* g = nullptr;
   ~FeedGuard() {
     if (f) f->finish(false);
+    if (g) g->signal_built(-1);
   }
 };
 
@@ -1205,7 +1219,7 @@ int mixed_collect(mordred_engine* e, const Resolved& r, const MixedGroup& m, int
 }
 
 int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) {
-  FeedGuard feed_guard{g.feed};  // every early return ends the feed as failed
+  FeedGuard feed_guard{g.feed, &g};  // every early return ends the feed as failed
   const auto t0 = Clock::now();
   HIP_OK(hipSetDevice(e->device));
   HIP_OK(e->d_res.reserve(res_ints * sizeof(int)));
@@ -1233,6 +1247,8 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
   Events ev;
   HIP_OK(hipEventCreateWithFlags(&ev.built, hipEventDisableTiming));
   HIP_OK(hipEventRecord(ev.built, e->s_main));
+  HIP_OK(hipEventRecord(e->ev_built, e->s_main));
+  g.signal_built(0);
 
   // pinned + device scratch: [cached group: nseg shorts][staged segment tables:
   // nc x nseg ints][one segment group per streamed batch]
@@ -1434,6 +1450,331 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
   g.res.assign(res_ints, 0);
   HIP_OK(hipMemcpy(g.res.data(), d_res, res_ints * sizeof(int), hipMemcpyDeviceToHost));
   g.ms = ms_since(t0);
+  return 0;
+}
+
+// ------------------------------------------------------------ operator orders
+// A partly cached segment group run in the operator order of one of the
+// reference's fact pipelines (QueryProcessing.cu:201-437 CPU-first, :676-914
+// GPU-first with a switch back).  A stage runs a set of operators on one
+// device over the rows the previous stage kept; between stages on different
+// devices the row ids and the dimension rows found so far cross PCIe.  Runs
+// on the calling thread (which owns the CPU pool) on its own stream, chunk by
+// chunk, after the GPU side has built the hash tables.
+constexpr int kOrderChunk = 16;  // segments per chunk (bounds the device lists)
+
+struct OrderStage {
+  bool gpu = false;
+  bool filt[2] = {false, false};
+  bool probe[4] = {false, false, false, false};
+  bool group = false;
+  bool any_filter() const { return filt[0] || filt[1]; }
+  bool any_probe() const { return probe[0] || probe[1] || probe[2] || probe[3]; }
+  bool empty() const { return !group && !any_filter() && !any_probe(); }
+};
+
+std::vector<OrderStage> order_stages(const Resolved& r, const MixedGroup& m, int order) {
+  const int nf = r.q->nfilters;
+  // group-by on the GPU iff its fact columns are cached in every segment
+  bool ggpu = order != MORDRED_ORDER_GPU_FIRST;
+  for (int s : m.segs) ggpu = ggpu && r.a1->slot[s] >= 0 && (!r.a2 || r.a2->slot[s] >= 0);
+  OrderStage st[4];
+  auto cpu_f = [&](OrderStage& x) { for (int f = 0; f < nf; f++) x.filt[f] = !m.gpu_filter[f]; };
+  auto gpu_f = [&](OrderStage& x) { for (int f = 0; f < nf; f++) x.filt[f] = m.gpu_filter[f]; };
+  auto cpu_p = [&](OrderStage& x) { for (int k = 0; k < 4; k++) x.probe[k] = r.join[k] && !m.gpu_probe[k]; };
+  auto gpu_p = [&](OrderStage& x) { for (int k = 0; k < 4; k++) x.probe[k] = r.join[k] && m.gpu_probe[k]; };
+  if (order == MORDRED_ORDER_CPU_FIRST) {
+    cpu_f(st[0]);
+    st[1].gpu = true, gpu_f(st[1]), gpu_p(st[1]);
+    cpu_p(st[2]);
+  } else if (order == MORDRED_ORDER_ALTERNATE) {
+    st[0].gpu = true, gpu_f(st[0]);
+    cpu_f(st[1]), cpu_p(st[1]);
+    st[2].gpu = true, gpu_p(st[2]);
+  } else {
+    st[0].gpu = true, gpu_f(st[0]), gpu_p(st[0]);
+    cpu_f(st[1]), cpu_p(st[1]);
+  }
+  st[3].gpu = ggpu;
+  st[3].group = true;
+  std::vector<OrderStage> out;
+  for (const OrderStage& x : st) {
+    if (x.empty()) continue;
+    if (!out.empty() && out.back().gpu == x.gpu) {
+      OrderStage& y = out.back();
+      for (int f = 0; f < 2; f++) y.filt[f] |= x.filt[f];
+      for (int k = 0; k < 4; k++) y.probe[k] |= x.probe[k];
+      y.group |= x.group;
+    } else {
+      out.push_back(x);
+    }
+  }
+  return out;
+}
+
+// The rows a stage hands to the next: every row of the chunk (implicit), or
+// a list of fact row ids with the dimension rows of the probed tables that
+// have a group value, on the host or on the device.
+struct RowList {
+  bool implicit = true, on_gpu = false;
+  long long n = 0;
+  bool dim[4] = {false, false, false, false};
+  std::vector<int> lo, dims[4];  // host copy
+  int set = 0;                   // device copy: list set 0 or 1
+};
+
+// Device lists: two sets of [row ids | dimension rows 0-3] x cap ints, a row-id
+// list between a filter call and a probe call, the count, the segment list.
+struct OrderBufs {
+  size_t cap = 0;
+  int* set[2] = {nullptr, nullptr};
+  int* flo = nullptr;
+  int* total = nullptr;
+  short* sg = nullptr;
+  offsetGPU off(int which, const bool dim[4]) const {
+    offsetGPU o{};
+    int* b = set[which];
+    o.lo_off = b;
+    int** d[4] = {&o.dim_off1, &o.dim_off2, &o.dim_off3, &o.dim_off4};
+    for (int k = 0; k < 4; k++)
+      if (dim[k]) *d[k] = b + (size_t)(k + 1) * cap;
+    return o;
+  }
+};
+
+int order_count(mordred_engine* e, const OrderBufs& B, hipStream_t s, long long& n) {
+  int got = 0;
+  HIP_OK(hipMemcpyAsync(&got, B.total, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  n = got;
+  return 0;
+}
+
+// Moves a list to the other device (switch_device_fact).
+int order_switch(mordred_engine* e, RowList& L, const OrderBufs& B, hipStream_t s) {
+  const size_t bytes = (size_t)L.n * sizeof(int);
+  if (L.on_gpu) {
+    L.lo.resize(L.n);
+    int* b = B.set[L.set];
+    if (bytes) HIP_OK(hipMemcpyAsync(L.lo.data(), b, bytes, hipMemcpyDeviceToHost, s));
+    for (int k = 0; k < 4; k++) {
+      if (!L.dim[k]) continue;
+      L.dims[k].resize(L.n);
+      if (bytes)
+        HIP_OK(hipMemcpyAsync(L.dims[k].data(), b + (size_t)(k + 1) * B.cap, bytes, hipMemcpyDeviceToHost, s));
+    }
+    HIP_OK(hipStreamSynchronize(s));
+  } else {
+    L.set = 0;
+    int* b = B.set[0];
+    if (bytes) HIP_OK(hipMemcpyAsync(b, L.lo.data(), bytes, hipMemcpyHostToDevice, s));
+    for (int k = 0; k < 4; k++)
+      if (L.dim[k] && bytes)
+        HIP_OK(hipMemcpyAsync(b + (size_t)(k + 1) * B.cap, L.dims[k].data(), bytes, hipMemcpyHostToDevice, s));
+    HIP_OK(hipStreamSynchronize(s));  // the host lists are reused
+  }
+  L.on_gpu = !L.on_gpu;
+  return 0;
+}
+
+// A GPU stage (the reference's offset-mode kernels).  A group-by stage adds
+// into d_res; any other stage leaves a device list.
+int order_gpu_stage(mordred_engine* e, const Resolved& r, int* const ht[4], const OrderStage& st, RowList& L,
+                    long long rows, const OrderBufs& B, int* d_res, hipStream_t s) {
+  const mordred_stream_t ms = (mordred_stream_t)s;
+  GpuArgs a;
+  gpu_args(r, ht, nullptr, a);
+  filterArgsGPU f{};
+  for (int i = 0; i < 2; i++) {
+    if (!st.filt[i]) continue;
+    if (i == 0) f.filter_idx1 = r.filt[0]->d_idx, f.compare1 = r.q->filters[0].lo, f.compare2 = r.q->filters[0].hi, f.mode1 = 1;
+    else f.filter_idx2 = r.filt[1]->d_idx, f.compare3 = r.q->filters[1].lo, f.compare4 = r.q->filters[1].hi, f.mode2 = 1;
+  }
+  probeArgsGPU p{};
+  int** pk[4] = {&p.key_idx1, &p.key_idx2, &p.key_idx3, &p.key_idx4};
+  int** ph[4] = {&p.ht1, &p.ht2, &p.ht3, &p.ht4};
+  int* pl[4] = {&p.dim_len1, &p.dim_len2, &p.dim_len3, &p.dim_len4};
+  int* pm[4] = {&p.min_key1, &p.min_key2, &p.min_key3, &p.min_key4};
+  for (int k = 0; k < 4; k++) {
+    if (!st.probe[k]) continue;
+    *pk[k] = r.key[k]->d_idx, *ph[k] = ht[k], *pl[k] = r.join[k]->dim_len, *pm[k] = r.join[k]->min_key;
+  }
+  bool od[4];
+  for (int k = 0; k < 4; k++) od[k] = L.dim[k] || (st.probe[k] && r.dval[k]);
+  int krc = 0;
+  if (L.implicit) {
+    if (st.group) return fail(e, "operator order: a group-by stage over whole segments");
+    MixedGroup ops;
+    std::copy(st.filt, st.filt + 2, ops.gpu_filter);
+    std::copy(st.probe, st.probe + 4, ops.gpu_probe);
+    int* fidx[2] = {f.filter_idx1, f.filter_idx2};
+    int* kidx[4] = {p.key_idx1, p.key_idx2, p.key_idx3, p.key_idx4};
+    const offsetGPU out = B.off(0, od);
+    const int rc = mixed_gpu_ops(e, r, ht, ops, fidx, kidx, rows, B.sg, B.total, B.flo, out, s);
+    if (rc) return rc;
+    L.implicit = false, L.on_gpu = true, L.set = 0;
+    std::copy(od, od + 4, L.dim);
+    return order_count(e, B, s, L.n);
+  }
+  int cur = L.set;
+  if (st.any_filter()) {
+    for (int k = 0; k < 4; k++)
+      if (L.dim[k]) return fail(e, "operator order: a GPU filter after a probe");
+    HIP_OK(hipMemsetAsync(B.total, 0, sizeof(int), s));
+    krc = mordred_filter_GPU3(e->cache, B.set[cur], &f, B.set[1 - cur], (int)L.n, B.total, ms);
+    if (krc) return hip_fail(e, (hipError_t)krc, "operator order: filter_GPU3");
+    cur = 1 - cur;
+    if (int rc = order_count(e, B, s, L.n)) return rc;
+  }
+  const bool groupby = r.q->kind == MORDRED_KIND_GROUPBY;
+  // probes: compaction, unless the group-by kernel probes them itself
+  // (filter_probe_group_by_GPU3 probes every table; the aggregate kernels
+  // only the date table)
+  const bool fused = st.group && (groupby || (!st.probe[0] && !st.probe[1] && !st.probe[2]));
+  if (st.any_probe() && !fused) {
+    const offsetGPU in = B.off(cur, L.dim), out = B.off(1 - cur, od);
+    HIP_OK(hipMemsetAsync(B.total, 0, sizeof(int), s));
+    krc = mordred_probe_GPU3(e->cache, &in, &p, &out, (int)L.n, B.total, ms);
+    if (krc) return hip_fail(e, (hipError_t)krc, "operator order: probe_GPU3");
+    cur = 1 - cur;
+    std::copy(od, od + 4, L.dim);
+    if (int rc = order_count(e, B, s, L.n)) return rc;
+  }
+  L.set = cur;
+  if (!st.group) return 0;
+  const offsetGPU in = B.off(cur, L.dim);
+  const bool probes_left = fused && st.any_probe();
+  if (groupby) krc = mordred_filter_probe_group_by_GPU3(e->cache, &in, nullptr, probes_left ? &p : nullptr, &a.g,
+                                                        (int)L.n, d_res, ms);
+  else if (probes_left) krc = mordred_probe_aggr_GPU3(e->cache, &in, &p, &a.g, (int)L.n, d_res, ms);
+  else krc = mordred_aggregationGPU(e->cache, in.lo_off, &a.g, (int)L.n, d_res, ms);
+  if (krc) return hip_fail(e, (hipError_t)krc, "operator order: group-by kernel");
+  L.n = 0;
+  return 0;
+}
+
+// A CPU stage without the group-by: the stage's filters and probes over the
+// list (or every row of segs), keeping the passing rows and dimension rows.
+void order_cpu_compact(mordred_engine* e, const Resolved& r, const CpuTables& t, const OrderStage& st,
+                       const std::vector<int>& segs, RowList& L) {
+  const mordred_star_query& q = *r.q;
+  bool od[4];
+  for (int k = 0; k < 4; k++) od[k] = L.dim[k] || (st.probe[k] && r.dval[k]);
+  // tasks: (first row, end row) of a segment chunk, or a range of the list
+  std::vector<std::pair<long long, long long>> tasks;
+  if (L.implicit) {
+    for (int sg : segs)
+      for (long long b = (long long)sg * kSeg; b < (long long)sg * kSeg + segment_rows(r, sg); b += kCpuChunk)
+        tasks.push_back({b, std::min<long long>((long long)sg * kSeg + segment_rows(r, sg), b + kCpuChunk)});
+  } else {
+    const long long per = std::max<long long>(1024, (L.n + 2 * e->pool->size() - 1) / (2 * e->pool->size()));
+    for (long long b = 0; b < L.n; b += per) tasks.push_back({b, std::min(L.n, b + per)});
+  }
+  struct Out {
+    std::vector<int> lo, dims[4];
+  };
+  std::vector<Out> outs(tasks.size());
+  e->pool->run((int)tasks.size(), [&](int task, int) {
+    Out& o = outs[task];
+    for (long long i = tasks[task].first; i < tasks[task].second; i++) {
+      const long long row = L.implicit ? i : L.lo[i];
+      bool ok = true;
+      for (int fi = 0; fi < q.nfilters && ok; fi++) {
+        if (!st.filt[fi]) continue;
+        const int x = r.filt[fi]->host[row];
+        ok = x >= q.filters[fi].lo && x <= q.filters[fi].hi;
+      }
+      int drow[4] = {0, 0, 0, 0};
+      for (int k = 0; k < 4 && ok; k++) {
+        if (!st.probe[k]) continue;
+        const mordred_join& j = *r.join[k];
+        const int h = (r.key[k]->host[row] - j.min_key) % j.dim_len;
+        const int slot_row = t.ht[k][2 * (size_t)h + 1];  // dimension row + 1, 0 = empty
+        ok = slot_row != 0;
+        drow[k] = slot_row - 1;
+      }
+      if (!ok) continue;
+      o.lo.push_back((int)row);
+      for (int k = 0; k < 4; k++)
+        if (od[k]) o.dims[k].push_back(st.probe[k] ? drow[k] : L.dims[k][i]);
+    }
+  });
+  RowList R;
+  R.implicit = false;
+  std::copy(od, od + 4, R.dim);
+  size_t n = 0;
+  for (const Out& o : outs) n += o.lo.size();
+  R.lo.reserve(n);
+  for (int k = 0; k < 4; k++)
+    if (od[k]) R.dims[k].reserve(n);
+  for (Out& o : outs) {
+    R.lo.insert(R.lo.end(), o.lo.begin(), o.lo.end());
+    for (int k = 0; k < 4; k++)
+      if (od[k]) R.dims[k].insert(R.dims[k].end(), o.dims[k].begin(), o.dims[k].end());
+  }
+  R.n = (long long)n;
+  L = std::move(R);
+}
+
+// Runs the mixed groups in the engine's operator order.  CPU group-by stages
+// accumulate into acc, GPU ones into d_res (zeroed here).
+int order_groups(mordred_engine* e, const Resolved& r, const CpuTables& t, std::vector<MixedGroup>& mixed,
+                 int* d_res, size_t res_ints, std::vector<CpuAcc>& acc, mordred_run_stats& stats) {
+  HIP_OK(hipSetDevice(e->device));
+  hipStream_t s = e->s_order;
+  HIP_OK(hipStreamWaitEvent(s, e->ev_built, 0));
+  HIP_OK(hipMemsetAsync(d_res, 0, res_ints * sizeof(int), s));
+  int* ht[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int k = 0; k < 4; k++)
+    if (r.join[k]) ht[k] = static_cast<int*>(e->d_ht[k].p);
+  OrderBufs B;
+  B.cap = (size_t)kOrderChunk * kSeg;
+  HIP_OK(e->d_ord.reserve((11 * B.cap + 16) * sizeof(int) + kOrderChunk * sizeof(short)));
+  int* base = static_cast<int*>(e->d_ord.p);
+  B.set[0] = base;
+  B.set[1] = base + 5 * B.cap;
+  B.flo = base + 10 * B.cap;
+  B.total = base + 11 * B.cap;
+  B.sg = reinterpret_cast<short*>(base + 11 * B.cap + 16);
+  for (MixedGroup& m : mixed) {
+    order_last_partial(r, m.segs);
+    const std::vector<OrderStage> stages = order_stages(r, m, e->op_order);
+    if (stages.empty() || !stages.back().group) return fail(e, "operator order: no group-by stage");
+    m.survivors = 0;
+    for (size_t c0 = 0; c0 < m.segs.size(); c0 += kOrderChunk) {
+      const size_t c1 = std::min(m.segs.size(), c0 + kOrderChunk);
+      const std::vector<int> segs(m.segs.begin() + c0, m.segs.begin() + c1);
+      const std::vector<short> sgl(segs.begin(), segs.end());
+      const long long rows = group_rows(r, segs);
+      HIP_OK(hipMemcpyAsync(B.sg, sgl.data(), sgl.size() * sizeof(short), hipMemcpyHostToDevice, s));
+      RowList L;
+      L.n = rows;
+      for (const OrderStage& st : stages) {
+        if (!L.implicit && L.on_gpu != st.gpu) {
+          if (int rc = order_switch(e, L, B, s)) return rc;
+          stats.order_switches++;
+        }
+        if (st.group) m.survivors += L.n;
+        if (st.gpu) {
+          if (int rc = order_gpu_stage(e, r, ht, st, L, rows, B, d_res, s)) return rc;
+        } else if (st.group) {
+          if (L.implicit) return fail(e, "operator order: a CPU group-by stage over whole segments");
+          // the CPU operators of cpu_suffix: everything before this stage is done
+          MixedGroup done;
+          for (int f = 0; f < 2; f++) done.gpu_filter[f] = !st.filt[f];
+          for (int k = 0; k < 4; k++) done.gpu_probe[k] = r.join[k] && !st.probe[k];
+          MixedChunk ch;
+          ch.lo = std::move(L.lo);
+          for (int k = 0; k < 4; k++)
+            if (L.dim[k]) ch.dim[k] = std::move(L.dims[k]);
+          cpu_suffix(e, r, t, done, ch, acc);
+        } else {
+          order_cpu_compact(e, r, t, st, segs, L);
+        }
+      }
+    }
+  }
+  HIP_OK(hipStreamSynchronize(s));
   return 0;
 }
 
@@ -1989,7 +2330,8 @@ mordred_engine* mordred_engine_create(int device, int cache_segments, int thread
   ok = ok && (cache_segments == 0 ||
               hipMalloc(&e->cache, (size_t)cache_segments * kSeg * sizeof(int)) == hipSuccess);
   ok = ok && hipStreamCreate(&e->s_main) == hipSuccess && hipStreamCreate(&e->s_copy[0]) == hipSuccess &&
-       hipStreamCreate(&e->s_copy[1]) == hipSuccess;
+       hipStreamCreate(&e->s_copy[1]) == hipSuccess && hipStreamCreate(&e->s_order) == hipSuccess &&
+       hipEventCreateWithFlags(&e->ev_built, hipEventDisableTiming) == hipSuccess;
   if (ok) {
     std::vector<short> ident(kMaxIdentity);
     for (int i = 0; i < kMaxIdentity; i++) ident[i] = (short)i;
@@ -2021,7 +2363,8 @@ void mordred_engine_destroy(mordred_engine* e) {
   if (e->graph) (void)hipGraphDestroy(e->graph);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
   if (e->ev1) (void)hipEventDestroy(e->ev1);
-  for (hipStream_t s : {e->s_main, e->s_copy[0], e->s_copy[1]})
+  if (e->ev_built) (void)hipEventDestroy(e->ev_built);
+  for (hipStream_t s : {e->s_main, e->s_copy[0], e->s_copy[1], e->s_order})
     if (s) (void)hipStreamDestroy(s);
   if (e->cache) (void)hipFree(e->cache);
   if (e->d_identity) (void)hipFree(e->d_identity);
@@ -2182,6 +2525,13 @@ int mordred_engine_set_skipping(mordred_engine* e, int on) {
   return 0;
 }
 
+int mordred_engine_set_operator_order(mordred_engine* e, int order) {
+  if (!e) return -1;
+  if (order < MORDRED_ORDER_GPU_FIRST || order > MORDRED_ORDER_ALTERNATE) return fail(e, "bad operator order");
+  e->op_order = order;
+  return 0;
+}
+
 int mordred_engine_set_prefix_streaming(mordred_engine* e, int on) {
   if (!e) return -1;
   e->prefix_stream = on != 0;
@@ -2304,6 +2654,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   g.streamed = streamed.get_future();
   MixedFeed feed;
   g.feed = &feed;
+  std::future<int> built = g.built.get_future();
   std::thread feeder;
   if (use_gpu) feeder = std::thread([&] { g.rc = gpu_part(e, r, g, res_ints); });
 
@@ -2422,7 +2773,10 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     pstream_segs.clear();
   }
   std::sort(cpu_segs.begin(), cpu_segs.end());
-  streamed.set_value(SplitPlan{stream_segs, &mixed, pstream_segs, &pgroup});
+  // mixed groups in another operator order run on this thread (below)
+  const bool ordered = e->op_order != MORDRED_ORDER_GPU_FIRST && !mixed.empty();
+  std::vector<MixedGroup> no_mixed;
+  streamed.set_value(SplitPlan{stream_segs, ordered ? &no_mixed : &mixed, pstream_segs, &pgroup});
 
   // ---- CPU: the CPU segment group, concurrently with the GPU
   std::vector<int> cpu_res;
@@ -2433,9 +2787,29 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     st.cpu_segments = (int)cpu_segs.size();
   }
   // ---- mixed groups: the CPU operators over the GPU operators' survivors
+  std::vector<int> ord_res;  // GPU group-by stages of ordered mixed groups
+  st.operator_order = e->op_order;
   if (!mixed.empty() || !pstream_segs.empty()) {
     if (cpu_res.empty()) cpu_res.assign(res_ints, 0);
     std::vector<CpuAcc> acc(e->pool->size());
+    if (ordered) {
+      int rc = built.get() == 0 ? 0 : -1;
+      if (!rc) {
+        const hipError_t he = e->d_res_ord.reserve(res_ints * sizeof(int));
+        rc = he != hipSuccess ? hip_fail(e, he, "ordered pipelines: result buffer")
+                              : order_groups(e, r, t, mixed, static_cast<int*>(e->d_res_ord.p), res_ints, acc, st);
+      }
+      if (!rc) {
+        ord_res.assign(res_ints, 0);
+        const hipError_t he = hipMemcpy(ord_res.data(), e->d_res_ord.p, res_ints * sizeof(int), hipMemcpyDeviceToHost);
+        if (he != hipSuccess) rc = hip_fail(e, he, "ordered pipelines: result copy");
+      }
+      if (rc) {
+        feed.finish(false);
+        if (feeder.joinable()) feeder.join();
+        return g.rc ? g.rc : rc;
+      }
+    }
     MixedChunk ch;
     while (feed.pop(ch)) {
       const auto ts0 = Clock::now();
@@ -2452,7 +2826,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
       st.mixed_segments += (int)m.segs.size();
       st.mixed_rows += m.rows;
       st.mixed_survivors += m.survivors;
-      lines += (double)m.survivors * mixed_lines(r, m);
+      if (!ordered) lines += (double)m.survivors * mixed_lines(r, m);
     }
     // calibrate the CPU operators' time per survivor line (moving average)
     if (lines >= (1 << 18) && st.suffix_ms > 0) {
@@ -2473,6 +2847,11 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   st.prefix_segments = g.prefix_segments;
   st.prefix_survivors = pgroup.survivors;
   st.prefix_rows = g.prefix_rows;
+  if (!ord_res.empty()) {
+    if (g.res.empty()) g.res.swap(ord_res);
+    else if (q->kind == MORDRED_KIND_GROUPBY) merge(g.res.data(), ord_res.data(), G);
+    else reinterpret_cast<long long*>(g.res.data())[2] += reinterpret_cast<long long*>(ord_res.data())[2];
+  }
   const auto tm0 = Clock::now();
   if (!cpu_res.empty() && !g.res.empty()) {
     if (q->kind == MORDRED_KIND_GROUPBY) merge(cpu_res.data(), g.res.data(), G);

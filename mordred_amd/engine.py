@@ -14,6 +14,8 @@ import numpy as np
 from . import abi
 
 PLACE_RESIDENCY, PLACE_COST_SPLIT, PLACE_GPU_ONLY, PLACE_CPU_ONLY = 0, 1, 2, 3
+# operator order of partly cached segment groups
+ORDER_GPU_FIRST, ORDER_CPU_FIRST, ORDER_ALTERNATE = 0, 1, 2
 KIND_AGGR, KIND_GROUPBY = 0, 1
 
 
@@ -43,7 +45,7 @@ class RunStats(C.Structure):
                 ("est_mixed_ms_per_segment", C.c_double), ("prefix_segments", C.c_int),
                 ("prefix_rows", C.c_longlong), ("prefix_survivors", C.c_longlong),
                 ("est_prefix_ms_per_segment", C.c_double), ("est_prefix_cpu_ms_per_segment", C.c_double),
-                ("suffix_ms", C.c_double)]
+                ("suffix_ms", C.c_double), ("operator_order", C.c_int), ("order_switches", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -143,6 +145,7 @@ _SIGS = {
     "mordred_engine_segment_minmax": ([C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p], C.c_int),
     "mordred_engine_set_skipping": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_set_prefix_streaming": ([C.c_void_p, C.c_int], C.c_int),
+    "mordred_engine_set_operator_order": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_set_learned_placement": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_placement_model_stats": ([C.c_void_p, C.POINTER(PlacementStats)], C.c_int),
     "mordred_engine_place": ([C.c_void_p, C.POINTER(StarQueryC), C.c_double], C.c_int),
@@ -277,6 +280,11 @@ class Engine:
 
     def set_prefix_streaming(self, on: bool):
         self._check(self.lib.mordred_engine_set_prefix_streaming(self.h, int(on)), "set_prefix_streaming")
+
+    def set_operator_order(self, order: int):
+        """ORDER_GPU_FIRST (default), ORDER_CPU_FIRST (the reference's
+        executeTableFact_v1) or ORDER_ALTERNATE (executeTableFact_v2)."""
+        self._check(self.lib.mordred_engine_set_operator_order(self.h, int(order)), "set_operator_order")
 
     # ---- learned placement
     def set_learned_placement(self, mode: int):

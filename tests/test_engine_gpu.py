@@ -267,3 +267,39 @@ def test_prefix_streaming_cost_split(fresh_engine):
     np.testing.assert_array_equal(res, ref)
     assert st["prefix_segments"] == 0
     assert st["cpu_segments"] + st["stream_segments"] == nseg
+
+
+@pytest.mark.parametrize("order", [E.ORDER_GPU_FIRST, E.ORDER_CPU_FIRST, E.ORDER_ALTERNATE])
+def test_operator_orders(fresh_engine, order):
+    """The reference's operator orders over partly cached segments: the
+    aggregate columns and the last join's key cached everywhere (q11's
+    lo_discount is also its first fact filter's column), the other keys and
+    filter columns on the host.  CPU-first runs the CPU filters, the
+    GPU filters and probes, the CPU probes, then the group-by on the GPU;
+    alternate runs the GPU filters, the CPU filters and probes, then the GPU
+    probes and group-by.  Every order returns the oracle's answer."""
+    q, data, eng, ref = fresh_engine
+    print(f"[gpu] operator order {order} {q.name}", flush=True)
+    _uncache_all(eng, q)
+    nseg = eng.segments(q.fact_columns()[0])
+    cached = {q.joins[-1].fact_key} | {a for a in q.aggr if a}
+    for c in cached:
+        for s in range(nseg):
+            eng.cache_segment(c, s)
+    eng.set_operator_order(order)
+    try:
+        res, st = eng.run(q, E.PLACE_RESIDENCY)
+    finally:
+        eng.set_operator_order(E.ORDER_GPU_FIRST)
+    np.testing.assert_array_equal(res, ref)
+    assert st["operator_order"] == order
+    assert st["mixed_segments"] == nseg and st["gpu_segments"] == 0 and st["cpu_segments"] == 0
+    assert 0 < st["mixed_survivors"] <= data.n
+    if order == E.ORDER_GPU_FIRST:
+        assert st["order_switches"] == 0
+    else:
+        # at least one switch per chunk of 16 segments; with a cached fact
+        # filter (q11: lo_discount) the alternate order switches back
+        chunks = (nseg + 15) // 16
+        back = order == E.ORDER_ALTERNATE and q.fact_filters
+        assert st["order_switches"] >= (2 if back else 1) * chunks
