@@ -526,7 +526,7 @@ def mixed_bench(args):
     threads = min(16, os.cpu_count() or 1)
     bw_pci = pcie_h2d_bandwidth()   # torch initialises the device before the engine
     torch.cuda.synchronize()
-    eng = E.Engine(device=0, cache_segments=2 * nseg + 64 + 2 * 4 * len(q.fact_columns()), threads=threads)
+    eng = E.Engine(device=0, cache_segments=3 * nseg + 64 + 2 * 4 * len(q.fact_columns()), threads=threads)
     eng.add_data(data, q)
     cached = ["lo_suppkey", "lo_partkey"]
     for c in cached:
@@ -547,6 +547,29 @@ def mixed_bench(args):
         runs[name] = {"ms_median": statistics.median(times), "stats": last[1], "res": last[0]}
         log(f"{name}: {statistics.median(times):.1f} ms  mixed {last[1]['mixed_segments']} "
             f"survivors {last[1]['mixed_survivors']} stream {last[1]['stream_segments']} cpu {last[1]['cpu_segments']}")
+    # operator orders (the reference's executeTableFact_v1 / _v2 beside the
+    # GPU-first order), with the keys cached and then also lo_revenue, which
+    # lets the CPU-first and alternate orders run the group-by on the GPU
+    orders = {}
+    for extra in ([], ["lo_revenue"]):
+        for c in extra:
+            for sgm in range(nseg):
+                eng.cache_segment(c, sgm)
+        for oname, order in (("gpu_first", E.ORDER_GPU_FIRST), ("cpu_first_v1", E.ORDER_CPU_FIRST),
+                             ("alternate_v2", E.ORDER_ALTERNATE)):
+            eng.set_operator_order(order)
+            for _ in range(args.warmup):
+                eng.run(q, E.PLACE_RESIDENCY)
+            times, last = [], None
+            for _ in range(args.steps):
+                r, st = eng.run(q, E.PLACE_RESIDENCY)
+                times.append(st["total_ms"])
+                last = (r, st)
+            key = f"{oname}{'+revenue' if extra else ''}"
+            orders[key] = {"ms_median": statistics.median(times), "stats": last[1], "res": last[0]}
+            log(f"order {key}: {statistics.median(times):.1f} ms  switches {last[1]['order_switches']} "
+                f"survivors {last[1]['mixed_survivors']}")
+        eng.set_operator_order(E.ORDER_GPU_FIRST)
     best = min(runs, key=lambda k: runs[k]["ms_median"])
     out = {
         "metric": METRIC + " (partly cached fact table, mixed CPU/GPU operator pipelines)",
@@ -563,6 +586,8 @@ def mixed_bench(args):
         "placements": {k: {"ms": v["ms_median"], "stats": v["stats"]} for k, v in runs.items()},
         "speedup_mixed_vs_stream": runs["gpu_only_stream"]["ms_median"] / runs["mixed_residency"]["ms_median"],
         "speedup_mixed_vs_cpu": runs["cpu_only"]["ms_median"] / runs["mixed_residency"]["ms_median"],
+        "operator_orders": {k: {"ms": v["ms_median"], "switches": v["stats"]["order_switches"],
+                                "survivors": v["stats"]["mixed_survivors"]} for k, v in orders.items()},
     }
     if not args.no_cpu:
         import oracle
@@ -573,7 +598,7 @@ def mixed_bench(args):
         cpu_s = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": n / cpu_s, "unit": "fact rows/s", "cores": threads, "kind": "port",
                                "sample": f"oracle, whole query on all {n} rows, {cpu_s * 1e3:.0f} ms"}
-        ok = {k: bool((ref == v["res"]).all()) for k, v in runs.items()}
+        ok = {k: bool((ref == v["res"]).all()) for k, v in list(runs.items()) + list(orders.items())}
         out["parity"] = all(ok.values())
         out["parity_by_placement"] = ok
     print(json.dumps(out), flush=True)
