@@ -236,9 +236,33 @@ static void* build_compact(Pipe& P, const ProbePlan& plan, int sink, hipStream_t
   return scratch;
 }
 
+// The group index sum_k (v_k - minv_k) * uniq_k is a mixed-radix number when
+// the non-zero strides, in increasing order, each divide the next and the
+// largest divides total_val: digit k then ranges over [0, next stride /
+// uniq_k), and keys whose digits lie in range (the kernel checks each row)
+// decode exactly from the index.
+static void set_key_decode(Pipe& P) {
+  P.kdecode = 0;
+  int idx[4], m = 0;
+  for (int k = 0; k < 4; k++) {
+    P.krange[k] = 0;
+    if (P.uniq[k] < 0) return;
+    if (P.uniq[k] > 0) idx[m++] = k;
+  }
+  if (P.total_val <= 0) return;
+  std::stable_sort(idx, idx + m, [&](int a, int b) { return P.uniq[a] < P.uniq[b]; });
+  for (int i = 0; i < m; i++) {
+    const int next = i + 1 < m ? P.uniq[idx[i + 1]] : P.total_val;
+    if (next % P.uniq[idx[i]] != 0) return;
+    P.krange[idx[i]] = next / P.uniq[idx[i]];
+  }
+  P.kdecode = 1;
+}
+
 // sink < 0: group-by, placed in LDS when the dense array fits.
 static int dispatch(Pipe P, int mode, int sink, hipStream_t stream) {
   if (P.num_tuples == 0) return hipSuccess;
+  if (sink < 0) set_key_decode(P);
   if (sink < 0) {
     const int shape = shape_of(P, mode, GROUPBY_LDS);
     const size_t ring = shape_compacts(mode, GROUPBY_LDS, shape) ? ring_lds_bytes(kLdsBlock, shape) : 0;

@@ -45,6 +45,10 @@ struct Pipe {
   const int* grp[4];
   int minv[4];
   int uniq[4];
+  // kdecode: the group index is a mixed-radix number with digit k in
+  // [0, krange[k]) for uniq[k] != 0, so keys in range decode from the index
+  int krange[4];
+  int kdecode;
   int total_val;
   int* res;
   int* total;
@@ -198,6 +202,18 @@ constexpr int shape_first(int shape) {
   return (shape & 1) ? 0 : (shape & 2) ? 1 : (shape & 4) ? 2 : (shape & 8) ? 3 : -1;
 }
 
+// Whether keys v decode back from their group index: with P.kdecode, every
+// grouping dimension's digit v[k] - minv[k] lies in [0, krange[k]) and every
+// other key is 0.  Such keys are not stored per row; the flush writes them
+// from the index.
+__device__ __forceinline__ bool keys_decode(const Pipe& P, const int v[4]) {
+  bool d = P.kdecode != 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    d = d && (P.uniq[k] != 0 ? (unsigned)v[k] - (unsigned)P.minv[k] < (unsigned)P.krange[k] : v[k] == 0);
+  return d;
+}
+
 template <int BLOCK, int VPT, int MODE, int SINK, int SHAPE = kShapeGeneric>
 __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
   constexpr bool kGen = SHAPE < 0;
@@ -210,12 +226,15 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
   constexpr bool kEager = MODE != OFF;
   __shared__ long long red[BLOCK / kWave];
   __shared__ int scan[BLOCK / kWave + 1];
-  // dynamic LDS: [group sums (u64) | group touch bitmap] for GROUPBY_LDS, then
-  // the packed words of the LDS-staged compact probe table
+  // dynamic LDS: [group sums (u64) | group touch bitmap | bitmap of groups
+  // whose keys this workgroup stored] for GROUPBY_LDS, then the packed words
+  // of the LDS-staged compact probe table
   extern __shared__ unsigned long long lds[];
-  const int gwords = SINK == GROUPBY_LDS ? P.total_val + (P.total_val + 63) / 64 : 0;
+  const int bwords = (P.total_val + 63) / 64;
+  const int gwords = SINK == GROUPBY_LDS ? P.total_val + 2 * bwords : 0;
   unsigned long long* gsum = lds;
   uint32_t* touched = reinterpret_cast<uint32_t*>(lds + (SINK == GROUPBY_LDS ? P.total_val : 0));
+  uint32_t* stored = touched + 2 * bwords;
   uint32_t* ldsw = reinterpret_cast<uint32_t*>(lds + gwords);
   constexpr bool kCompact = shape_compacts(MODE, SINK, SHAPE);
   constexpr int kFirst = shape_first(SHAPE);
@@ -319,10 +338,11 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
                               P.total_val);
         atomicAdd(&gsum[g], (unsigned long long)(long long)(a0 - a1));
         const uint32_t bit = 1u << (g & 31);
-        if (!(atomicOr(&touched[g >> 5], bit) & bit)) {
+        if (!(atomicOr(&touched[g >> 5], bit) & bit) && !keys_decode(P, v)) {
           int2* key = reinterpret_cast<int2*>(P.res + 6 * (size_t)g);
           key[0] = make_int2(v[0], v[1]);
           key[1] = make_int2(v[2], v[3]);
+          atomicOr(&stored[g >> 5], bit);
         }
       }
     }
@@ -619,13 +639,15 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
       for (int j = 0; j < IPT; j++) {
         if (!(sel >> j & 1u)) continue;
         atomicAdd(&gsum[gh[j]], (unsigned long long)(long long)(av[0][j] - av[1][j]));
-        if (first >> j & 1u) {
-          // keys are written by the first item of this workgroup that reaches
-          // the group; the reference stores them for every item, with the
-          // same values
+        const int vj[4] = {kv[0][j], kv[1][j], kv[2][j], kv[3][j]};
+        if ((first >> j & 1u) && !keys_decode(P, vj)) {
+          // keys that do not decode from the index are written by the first
+          // item of this workgroup that reaches the group; the reference
+          // stores them for every item, with the same values
           int2* key = reinterpret_cast<int2*>(P.res + 6 * (size_t)gh[j]);
-          key[0] = make_int2(kv[0][j], kv[1][j]);
-          key[1] = make_int2(kv[2][j], kv[3][j]);
+          key[0] = make_int2(vj[0], vj[1]);
+          key[1] = make_int2(vj[2], vj[3]);
+          atomicOr(&stored[gh[j] >> 5], 1u << (gh[j] & 31));
         }
       }
     }
@@ -636,9 +658,25 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
   }
   if (SINK == GROUPBY_LDS) {
     __syncthreads();
-    for (int h = threadIdx.x; h < P.total_val; h += BLOCK)
-      if (touched[h >> 5] >> (h & 31) & 1u)
-        atomicAdd(reinterpret_cast<unsigned long long*>(P.res + 6 * (size_t)h + 4), gsum[h]);
+    for (int h = threadIdx.x; h < P.total_val; h += BLOCK) {
+      if (!(touched[h >> 5] >> (h & 31) & 1u)) continue;
+      unsigned long long* sum = reinterpret_cast<unsigned long long*>(P.res + 6 * (size_t)h + 4);
+      if (!P.kdecode) {
+        atomicAdd(sum, gsum[h]);
+        continue;
+      }
+      // the workgroup that finds the sum still 0 (the first to add, or one
+      // after contributions that cancelled) stores the keys decoded from the
+      // index, unless this workgroup stored keys that do not decode
+      if (atomicAdd(sum, gsum[h]) == 0ull && !(stored[h >> 5] >> (h & 31) & 1u)) {
+        int kk[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) kk[k] = P.uniq[k] != 0 ? P.minv[k] + (h / P.uniq[k]) % P.krange[k] : 0;
+        int2* key = reinterpret_cast<int2*>(P.res + 6 * (size_t)h);
+        key[0] = make_int2(kk[0], kk[1]);
+        key[1] = make_int2(kk[2], kk[3]);
+      }
+    }
   } else if (SINK == AGGR) {
     const long long s = block_sum<BLOCK>(acc, red);
     if (threadIdx.x == 0 && s != 0)
@@ -751,7 +789,7 @@ inline int balanced_grid(int ntiles, int resident) {
 
 inline size_t group_lds_bytes(int sink, int total_val) {
   if (sink != GROUPBY_LDS) return 0;
-  return ((size_t)total_val + ((size_t)total_val + 63) / 64) * sizeof(unsigned long long);
+  return ((size_t)total_val + 2 * (((size_t)total_val + 63) / 64)) * sizeof(unsigned long long);
 }
 
 // 32-bit words of a compact table of len entries at width w (entries never

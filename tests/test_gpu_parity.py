@@ -525,3 +525,37 @@ def test_lds_staged_tables_and_late_columns(cuda):
         gq.run()
         sync()
         np.testing.assert_array_equal(gq.result(), ref)
+
+
+@pytest.mark.parametrize("case", ["brands_past_range", "strides_not_mixed_radix"])
+def test_group_keys_decoded_or_stored(cuda, case):
+    """Group keys in the LDS group-by sink.  When the strides form a mixed
+    radix (Q2.1: year stride 1, brand stride 7, 7,000 groups), rows whose
+    digits lie in range leave their keys to the flush, which decodes them
+    from the group index; here the brands are moved to 500..1499, so rows of
+    brands 1000-1499 wrap past the index's range and store their keys
+    themselves.  No two key tuples share an index (brands 0-499 are absent),
+    so the result is deterministic.  With 7,001 groups the strides are not a
+    mixed radix and every first touch stores its keys."""
+    import copy
+    from mordred_amd import query as Q
+    q, data = cases.star("q21", N_STAR, seed=77)
+    print(f"[gpu] group keys {case}", flush=True)
+    data.tables["part"] = dict(data.tables["part"])
+    if case == "brands_past_range":
+        data.tables["part"]["p_brand1"] = (data.tables["part"]["p_brand1"] % 1000 + 500).astype(np.int32)
+    else:
+        q = copy.deepcopy(q)
+        q.total_val = 7001
+    cache = Q.load_cache(data, q, shuffle_seed=5)
+    gq = Q.GpuQuery(q, data, cache)
+    gq.run()
+    sync()
+    res = gq.result()
+    ref = oracle_res(q, data)
+    np.testing.assert_array_equal(res, ref)
+    groups = ref.reshape(-1, 6)
+    touched = groups[:, 4:].any(axis=1)
+    if case == "brands_past_range":
+        # both kinds of group are present
+        assert (groups[touched, 2] >= 1000).any() and (groups[touched, 2] < 1000).any()
