@@ -226,7 +226,7 @@ static void* build_compact(Pipe& P, const ProbePlan& plan, int sink, hipStream_t
   prep_stats_kernel<<<dim3(kStatsGroups, nt), kPrepBlock, 0, stream>>>(prep);
   int maxlen = 0;
   for (int t = 0; t < nt; t++) maxlen = std::max(maxlen, prep.len[t]);
-  const int pack_groups = std::max(1, std::min(1024, (maxlen + kPrepBlock - 1) / kPrepBlock));
+  const int pack_groups = std::max(1, std::min(kPackGroups, (maxlen + kPrepBlock - 1) / kPrepBlock));
   prep_pack_kernel<<<dim3(pack_groups, nt), kPrepBlock, 0, stream>>>(prep);
   if (hipGetLastError() != hipSuccess) {
     (void)hipFreeAsync(scratch, stream);

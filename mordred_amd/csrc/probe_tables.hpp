@@ -40,7 +40,14 @@ struct CompactHdr {
 };
 
 constexpr int kPrepMaxTables = 4;
-constexpr int kStatsGroups = 256;   // workgroups of the stats pass per table
+#ifndef MORDRED_STATS_GROUPS
+#define MORDRED_STATS_GROUPS 512
+#endif
+#ifndef MORDRED_PACK_GROUPS
+#define MORDRED_PACK_GROUPS 1024
+#endif
+constexpr int kStatsGroups = MORDRED_STATS_GROUPS;  // workgroups of the stats pass per table
+constexpr int kPackGroups = MORDRED_PACK_GROUPS;    // most workgroups of the pack pass per table
 constexpr int kPrepBlock = 256;
 
 struct Prep {
@@ -53,14 +60,21 @@ struct Prep {
   uint32_t* bits[kPrepMaxTables];   // existence bitmap, or nullptr
 };
 
-__device__ __forceinline__ bool slot_result(const int* ht, int s, int row_probe, int& r) {
-  const int2 v = *reinterpret_cast<const int2*>(ht + 2 * (size_t)s);
+__device__ __forceinline__ bool slot_decode(int2 v, int row_probe, int& r) {
   if (row_probe) {
     r = v.y - 1;
     return v.y != 0;
   }
   r = v.x;
   return (v.x | v.y) != 0;
+}
+
+__device__ __forceinline__ int2 slot_load(const int* ht, int s) {
+  return *reinterpret_cast<const int2*>(ht + 2 * (size_t)s);
+}
+
+__device__ __forceinline__ bool slot_result(const int* ht, int s, int row_probe, int& r) {
+  return slot_decode(slot_load(ht, s), row_probe, r);
 }
 
 // The two pre-pass kernels are defined in exactly one translation unit
@@ -75,7 +89,24 @@ __global__ __launch_bounds__(kPrepBlock) void prep_stats_kernel(Prep P) {
   const int* ht = P.ht[t];
   const int len = P.len[t];
   int mn = 0x7fffffff, mx = (int)0x80000000, any = 0;
-  for (int s = blockIdx.x * kPrepBlock + threadIdx.x; s < len; s += kStatsGroups * kPrepBlock) {
+  constexpr int kStride = kStatsGroups * kPrepBlock;
+  int s = blockIdx.x * kPrepBlock + threadIdx.x;
+  // four independent slot loads in flight per thread, then the remainder
+  for (; s + 3 * kStride < len; s += 4 * kStride) {
+    int2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = slot_load(ht, s + u * kStride);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int r;
+      if (slot_decode(v[u], P.row_probe, r)) {
+        mn = min(mn, r);
+        mx = max(mx, r);
+        any++;
+      }
+    }
+  }
+  for (; s < len; s += kStride) {
     int r;
     if (slot_result(ht, s, P.row_probe, r)) {
       mn = min(mn, r);
@@ -162,22 +193,35 @@ __global__ __launch_bounds__(kPrepBlock) void prep_pack_kernel(Prep P) {
   uint32_t* words = w ? P.words[t] : nullptr;
   const int nb = (len + 31) >> 5;
   const int epw = 1 << shift;
-  for (int base = blockIdx.x * kPrepBlock; base < len; base += gridDim.x * kPrepBlock) {
-    const int s = base + (int)threadIdx.x;
-    int r = 0;
-    const bool hit = s < len && slot_result(ht, s, P.row_probe, r);
-    if (bits) {
-      const unsigned long long b = __ballot(hit);
-      if (lane == 0) {
-        const int wb = s >> 5;
-        if (wb < nb) bits[wb] = (uint32_t)b;
-        if (wb + 1 < nb) bits[wb + 1] = (uint32_t)(b >> 32);
-      }
+  const int stride = gridDim.x * kPrepBlock;
+  // the slots of up to four passes are loaded before any is packed
+  for (int base0 = blockIdx.x * kPrepBlock; base0 < len; base0 += 4 * stride) {
+    int2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int s = base0 + u * stride + (int)threadIdx.x;
+      v[u] = s < len ? slot_load(ht, s) : make_int2(0, 0);
     }
-    if (words) {
-      uint32_t c = hit ? (uint32_t)(r - vmin + 1) << ((s & (epw - 1)) * w) : 0u;
-      for (int d = 1; d < epw; d <<= 1) c |= __shfl_xor(c, d, kWave);
-      if ((s & (epw - 1)) == 0 && s < len) words[s >> shift] = c;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int base = base0 + u * stride;
+      if (base >= len) break;
+      const int s = base + (int)threadIdx.x;
+      int r = 0;
+      const bool hit = s < len && slot_decode(v[u], P.row_probe, r);
+      if (bits) {
+        const unsigned long long b = __ballot(hit);
+        if (lane == 0) {
+          const int wb = s >> 5;
+          if (wb < nb) bits[wb] = (uint32_t)b;
+          if (wb + 1 < nb) bits[wb + 1] = (uint32_t)(b >> 32);
+        }
+      }
+      if (words) {
+        uint32_t c = hit ? (uint32_t)(r - vmin + 1) << ((s & (epw - 1)) * w) : 0u;
+        for (int d = 1; d < epw; d <<= 1) c |= __shfl_xor(c, d, kWave);
+        if ((s & (epw - 1)) == 0 && s < len) words[s >> shift] = c;
+      }
     }
   }
 }
