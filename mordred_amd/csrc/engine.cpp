@@ -713,7 +713,21 @@ struct MixedFeed {
 
 struct CpuTables {
   std::vector<int> ht[4];
+  // the CPU operators probe in this order: the probes commute, so the most
+  // selective table goes first and the others see only the rows it keeps
+  int order[4] = {0, 1, 2, 3};
 };
+
+// Probe order by estimated pass fraction, smallest first (ties: the smaller
+// table first, whose random reads are more likely to hit the cache).
+void set_probe_order(const Resolved& r, const Selectivity& sel, CpuTables& t) {
+  std::stable_sort(t.order, t.order + 4, [&](int a, int b) {
+    const double sa = r.join[a] ? sel.join[a] : 2.0, sb = r.join[b] ? sel.join[b] : 2.0;
+    if (sa != sb) return sa < sb;
+    const int la = r.join[a] ? r.join[a]->dim_len : 0, lb = r.join[b] ? r.join[b]->dim_len : 0;
+    return la < lb;
+  });
+}
 
 void cpu_build(mordred_engine* e, const Resolved& r, CpuTables& t) {
   for (int k = 0; k < 4; k++) {
@@ -794,7 +808,8 @@ void cpu_fact(mordred_engine* e, const Resolved& r, const CpuTables& t, const st
       }
       if (!ok) continue;
       int v[4] = {0, 0, 0, 0};
-      for (int k = 0; k < 4 && ok; k++) {
+      for (int j = 0; j < 4 && ok; j++) {
+        const int k = t.order[j];
         if (!kcol[k]) continue;
         const int h = (kcol[k][row] - mk[k]) % len[k];
         const int lo = hts[k][2 * (size_t)h], hi = hts[k][2 * (size_t)h + 1];
@@ -892,7 +907,8 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
       }
       if (!ok) continue;
       int v[4] = {0, 0, 0, 0};
-      for (int k = 0; k < 4 && ok; k++) {
+      for (int j = 0; j < 4 && ok; j++) {
+        const int k = t.order[j];
         if (!r.join[k]) continue;
         if (m.gpu_probe[k]) {
           if (r.dval[k]) v[k] = r.dval[k]->host[c.dim[k][i]];
@@ -1685,7 +1701,8 @@ void order_cpu_compact(mordred_engine* e, const Resolved& r, const CpuTables& t,
         ok = x >= q.filters[fi].lo && x <= q.filters[fi].hi;
       }
       int drow[4] = {0, 0, 0, 0};
-      for (int k = 0; k < 4 && ok; k++) {
+      for (int o = 0; o < 4 && ok; o++) {
+        const int k = t.order[o];
         if (!st.probe[k]) continue;
         const mordred_join& j = *r.join[k];
         const int h = (r.key[k]->host[row] - j.min_key) % j.dim_len;
@@ -2667,6 +2684,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
       placement == MORDRED_PLACE_COST_SPLIT) {
     cpu_build(e, r, t);
     sel = estimate_selectivity(r, t);
+    set_probe_order(r, sel, t);
     cpu_tables = true;
   }
 
