@@ -120,6 +120,8 @@ typedef struct mordred_run_stats {
   double suffix_ms;                  /* CPU operators over mixed / prefix survivors */
   int operator_order;                /* MORDRED_ORDER_* the mixed groups ran with */
   int order_switches;                /* device switches of ordered mixed pipelines (all chunks) */
+  double cpu_build_ms;               /* the CPU hash-table builds (0 when none ran) */
+  double est_cpu_build_ms;           /* their estimate in the split model (0 before any was measured) */
 } mordred_run_stats;
 
 /* Engine on HIP device `device` with an HBM segment cache of cache_segments
@@ -127,6 +129,9 @@ typedef struct mordred_run_stats {
 mordred_engine* mordred_engine_create(int device, int cache_segments, int threads);
 void mordred_engine_destroy(mordred_engine* e);
 const char* mordred_engine_last_error(const mordred_engine* e);
+/* sizeof of a struct of this header by its name ("mordred_run_stats", ...),
+ * or 0 for an unknown name: bindings check their copies of the layouts. */
+long long mordred_engine_struct_size(const char* name);
 
 /* Copy a column (n int32 values) into the engine's pinned host store.  With
  * gpu_resident set, every segment is also cached in HBM (dimension tables). */

@@ -43,6 +43,7 @@ constexpr int kSegLog = 20;
 constexpr int kSeg = 1 << kSegLog;
 constexpr int kCpuChunk = 1 << 16;   // rows per CPU task
 constexpr int kStreamBatch = 4;      // segments per streamed batch
+constexpr int kMaxPrefixBatch = 16;  // segments per prefix-streamed batch, at most
 constexpr int kMixedChunk = 64;      // segments per GPU call of a mixed pipeline
 constexpr int kSuffixTask = 8192;    // survivors per CPU task of a mixed pipeline
 constexpr double kCacheLine = 64.0;  // CostModel.h:4
@@ -181,6 +182,7 @@ struct mordred_engine {
   bool skipping = true;                 // segment skipping (main.cu:21)
   bool prefix_stream = true;            // COST_SPLIT prefix streaming
   double suffix_ns_per_line = 0;        // measured CPU-operator time per survivor line (0 = none yet)
+  double build_ns_per_row = 0;          // measured CPU hash-table build time per dimension row
   int op_order = MORDRED_ORDER_GPU_FIRST;  // operator order of partly cached segment groups
   hipStream_t s_order = nullptr;        // the calling thread's stream for ordered pipelines
   hipEvent_t ev_built = nullptr;        // recorded after a run's hash-table builds
@@ -575,19 +577,20 @@ double transfer_cost(const Resolved& r, double L, double bw_pci) {
 // mordred_costmodel.h) predicts the whole run's time from the features below
 // and is trained on the measured times of the runs themselves.
 // A segment sent to the GPU side costs t_gpu there and, when it is prefix
-// streamed, t_gcpu of CPU operator time over its survivors.
-double split_analytic_ms(int n, int k, double t_cpu, double t_gpu, double t_gcpu = 0) {
-  return std::max((n - k) * t_cpu + k * t_gcpu, k * t_gpu);
+// streamed, t_gcpu of CPU operator time over its survivors; the CPU side
+// also builds its hash tables first (t_fix).
+double split_analytic_ms(int n, int k, double t_cpu, double t_gpu, double t_gcpu = 0, double t_fix = 0) {
+  return std::max(t_fix + (n - k) * t_cpu + k * t_gcpu, k * t_gpu);
 }
 
-// The k of the two sides finishing together: k*t_gpu = (n-k)*t_cpu + k*t_gcpu.
-int split_balance(int n, double t_cpu, double t_gpu, double t_gcpu = 0) {
+// The k of the two sides finishing together: k*t_gpu = t_fix + (n-k)*t_cpu + k*t_gcpu.
+int split_balance(int n, double t_cpu, double t_gpu, double t_gcpu = 0, double t_fix = 0) {
   if (n <= 0 || t_gcpu >= t_cpu) return 0;
-  return (int)std::min<double>(n, std::max(0.0, n * t_cpu / (t_gpu + t_cpu - t_gcpu) + 0.5));
+  return (int)std::min<double>(n, std::max(0.0, (t_fix + n * t_cpu) / (t_gpu + t_cpu - t_gcpu) + 0.5));
 }
 
 void split_features(const mordred_engine* e, const Resolved& r, const Selectivity& sel, int cached, int n, int k,
-                    double t_cpu, double t_str, double t_gcpu, float* f) {
+                    double t_cpu, double t_str, double t_gcpu, double t_fix, float* f) {
   std::fill(f, f + MORDRED_COST_FEATURES, 0.f);
   const mordred_star_query& q = *r.q;
   int njoins = 0;
@@ -605,7 +608,7 @@ void split_features(const mordred_engine* e, const Resolved& r, const Selectivit
   f[13] = q.kind == MORDRED_KIND_GROUPBY ? 1.f : 0.f;
   f[14] = (float)std::log(std::max(t_cpu, 1e-3));
   f[15] = (float)std::log(std::max(t_str, 1e-3));
-  f[16] = (float)std::log(std::max(split_analytic_ms(n, k, t_cpu, t_str, t_gcpu), 1e-3));
+  f[16] = (float)std::log(std::max(split_analytic_ms(n, k, t_cpu, t_str, t_gcpu, t_fix), 1e-3));
   f[17] = (float)(std::log2((double)std::max(1, e->pool->size())) / 5.0);
   f[18] = (float)std::log(std::max(t_gcpu, 1e-3));
 }
@@ -729,6 +732,15 @@ void set_probe_order(const Resolved& r, const Selectivity& sel, CpuTables& t) {
   });
 }
 
+// The dimension filter of build_CPU (CPUProcessing.cu:1064-1069) with the
+// GPU build's modes: 0 EQ, 1 BETWEEN, 2 EQ-or-EQ.
+inline bool dim_pass(const mordred_join& j, int x) {
+  if (j.filter_mode == 1) return x >= j.c1 && x <= j.c2;
+  if (j.filter_mode == 2) return x == j.c1 || x == j.c2;
+  if (j.filter_mode == 0) return x == j.c1;
+  return true;
+}
+
 void cpu_build(mordred_engine* e, const Resolved& r, CpuTables& t) {
   for (int k = 0; k < 4; k++) {
     if (!r.join[k]) continue;
@@ -743,14 +755,7 @@ void cpu_build(mordred_engine* e, const Resolved& r, CpuTables& t) {
     e->pool->run(tasks, [&](int task, int) {
       const long long b = (long long)task * kCpuChunk, en = std::min(n, b + kCpuChunk);
       for (long long row = b; row < en; row++) {
-        if (fc) {
-          const int x = fc->host[row];
-          bool pass = true;
-          if (j.filter_mode == 1) pass = x >= j.c1 && x <= j.c2;
-          else if (j.filter_mode == 2) pass = x == j.c1 || x == j.c2;
-          else if (j.filter_mode == 0) pass = x == j.c1;
-          if (!pass) continue;
-        }
+        if (fc && !dim_pass(j, fc->host[row])) continue;
         const int h = (key->host[row] - j.min_key) % j.dim_len;
         if (vc) ht[2 * (size_t)h] = vc->host[row];
         ht[2 * (size_t)h + 1] = (int)row + 1;
@@ -1054,15 +1059,19 @@ void order_last_partial(const Resolved& r, std::vector<int>& segs) {
   }
 }
 
-Selectivity estimate_selectivity(const Resolved& r, const CpuTables& t) {
+// Pass fractions without the hash tables, so that a run can plan before its
+// CPU builds: a dimension filter over up to 2^16 dimension rows spread evenly
+// over the table (the fraction of rows the build keeps), a fact filter over
+// the first 2^14 fact rows.
+Selectivity estimate_selectivity(const Resolved& r) {
   Selectivity s;
-  // dimension filters: fraction of dimension rows the build kept
   for (int k = 0; k < 4; k++) {
-    if (!r.join[k]) continue;
-    long long hits = 0;
-    const std::vector<int>& ht = t.ht[k];
-    for (size_t i = 1; i < ht.size(); i += 2) hits += ht[i] != 0;
-    s.join[k] = r.dkey[k]->n > 0 ? (double)hits / (double)r.dkey[k]->n : 1.0;
+    if (!r.join[k] || !r.dfilt[k] || r.dfilt[k]->n <= 0) continue;
+    const Column* fc = r.dfilt[k];
+    const long long step = std::max<long long>(1, fc->n >> 16);
+    long long seen = 0, pass = 0;
+    for (long long row = 0; row < fc->n; row += step, seen++) pass += dim_pass(*r.join[k], fc->host[row]);
+    s.join[k] = (double)pass / (double)seen;
   }
   // fact filters: a sample of the first segment
   const long long m = std::min<long long>(r.n, 1 << 14);
@@ -1270,7 +1279,7 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
   // nc x nseg ints][one segment group per streamed batch]
   const size_t nseg = (size_t)r.nseg, nc = r.fact_cols.size();
   const size_t sg_bytes = align64(nseg * sizeof(short)), tab_bytes = align64(nc * nseg * sizeof(int));
-  const size_t batch_bytes = align64(kStreamBatch * sizeof(short));
+  const size_t batch_bytes = align64(std::max(kStreamBatch, kMaxPrefixBatch) * sizeof(short));
   const size_t max_batches = nseg / kStreamBatch + 2;  // whole and prefix batches
   const size_t scratch = sg_bytes + tab_bytes + max_batches * batch_bytes;
   HIP_OK(e->h_stage.reserve(scratch));
@@ -1311,23 +1320,45 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
     };
     std::vector<int> all(seg_order);
     all.insert(all.end(), pseg.begin(), pseg.end());
+    // columns a prefix batch copies (those of the prefix group's GPU
+    // operators); a set's kStreamBatch x nc slots then hold pbatch segments
+    // of these npc columns
+    std::vector<char> pcopy(nc, 0);
+    std::vector<int> pidx(nc, -1);
+    int npc = 0;
+    if (!pseg.empty()) {
+      const MixedGroup& m = *plan.pgroup;
+      for (size_t c = 0; c < nc; c++) {
+        for (int i = 0; i < 2; i++) pcopy[c] |= m.gpu_filter[i] && r.fact_cols[c] == r.filt[i];
+        for (int k = 0; k < 4; k++) pcopy[c] |= m.gpu_probe[k] && r.fact_cols[c] == r.key[k];
+        if (pcopy[c]) pidx[c] = npc++;
+      }
+    }
+    const int pbatch = npc > 0 ? std::max(1, std::min<int>(kMaxPrefixBatch, kStreamBatch * (int)nc / npc)) : 1;
     std::vector<Batch> batches;
     for (size_t i = 0; i < seg_order.size(); i += kStreamBatch)
       batches.push_back({false, i, std::min<size_t>(kStreamBatch, seg_order.size() - i)});
-    for (size_t i = 0; i < pseg.size(); i += kStreamBatch)
-      batches.push_back({true, seg_order.size() + i, std::min<size_t>(kStreamBatch, pseg.size() - i)});
+    for (size_t i = 0; i < pseg.size(); i += pbatch)
+      batches.push_back({true, seg_order.size() + i, std::min<size_t>(pbatch, pseg.size() - i)});
     for (size_t i = 0; i < 2 * kStreamBatch * nc; i++) {
       loan.slots.push_back(e->free_slots.back());
       e->free_slots.pop_back();
     }
+    // staging slot of segment i of batch bi, column c
+    auto slot_of = [&](size_t bi, size_t i, size_t c) -> int {
+      const size_t set = bi & 1;
+      if (batches[bi].prefix) return loan.slots[set * kStreamBatch * nc + i * npc + pidx[c]];
+      return loan.slots[(set * kStreamBatch + i) * nc + c];
+    };
     int* h_tab = reinterpret_cast<int*>(hs + sg_bytes);
     int* d_tab = reinterpret_cast<int*>(ds + sg_bytes);
     for (size_t c = 0; c < nc; c++) {
       const std::vector<int>& mirror = r.fact_cols[c]->slot;
       for (size_t s = 0; s < nseg; s++) h_tab[c * nseg + s] = std::max(0, mirror[s]);
-      for (size_t bi = 0; bi < batches.size(); bi++)
-        for (size_t i = 0; i < batches[bi].cnt; i++)
-          h_tab[c * nseg + all[batches[bi].first + i]] = loan.slots[((bi & 1) * kStreamBatch + i) * nc + c];
+      for (size_t bi = 0; bi < batches.size(); bi++) {
+        if (batches[bi].prefix && !pcopy[c]) continue;
+        for (size_t i = 0; i < batches[bi].cnt; i++) h_tab[c * nseg + all[batches[bi].first + i]] = slot_of(bi, i, c);
+      }
     }
     for (int b = 0; b < 2; b++) HIP_OK(hipStreamWaitEvent(e->s_copy[b], ev.built, 0));
     HIP_OK(hipMemcpyAsync(d_tab, h_tab, nc * nseg * sizeof(int), hipMemcpyHostToDevice, e->s_copy[0]));
@@ -1338,7 +1369,7 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
     gpu_args(r, ht, d_tab, as);
     // prefix batches: the staged tables of the operators' columns, the columns
     // to copy, and per staging set an output area [row ids | dimension rows x 4
-    // | filtered row ids] of kStreamBatch segments, the count
+    // | filtered row ids] of pbatch segments, the count
     auto staged_of = [&](const Column* c) -> int* {
       for (size_t i = 0; i < nc; i++)
         if (r.fact_cols[i] == c) return d_tab + i * nseg;
@@ -1346,8 +1377,7 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
     };
     int* pf[2] = {nullptr, nullptr};
     int* pk[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::vector<char> pcopy(nc, 0);
-    const size_t pcap = (size_t)kStreamBatch * kSeg;
+    const size_t pcap = (size_t)pbatch * kSeg;
     offsetGPU pout[2] = {};
     int* ptotal[2] = {nullptr, nullptr};
     int* pflo[2] = {nullptr, nullptr};
@@ -1357,10 +1387,6 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
         if (m.gpu_filter[i]) pf[i] = staged_of(r.filt[i]);
       for (int k = 0; k < 4; k++)
         if (m.gpu_probe[k]) pk[k] = staged_of(r.key[k]);
-      for (size_t c = 0; c < nc; c++) {
-        for (int i = 0; i < 2; i++) pcopy[c] |= m.gpu_filter[i] && r.fact_cols[c] == r.filt[i];
-        for (int k = 0; k < 4; k++) pcopy[c] |= m.gpu_probe[k] && r.fact_cols[c] == r.key[k];
-      }
       const size_t per_set = 6 * pcap + 16;
       HIP_OK(e->d_pmix.reserve(2 * per_set * sizeof(int)));
       for (int set = 0; set < 2; set++) {
@@ -1390,7 +1416,7 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
         rows += segment_rows(r, seg);
         for (size_t c = 0; c < nc; c++) {
           if (B.prefix && !pcopy[c]) continue;
-          const int slot = loan.slots[(set * kStreamBatch + i) * nc + c];
+          const int slot = slot_of(bi, i, c);
           HIP_OK(hipMemcpyAsync(e->cache + (size_t)slot * kSeg, r.fact_cols[c]->host + (size_t)seg * kSeg,
                                 segment_rows(r, seg) * sizeof(int), hipMemcpyHostToDevice, s));
         }
@@ -2542,6 +2568,26 @@ int mordred_engine_set_skipping(mordred_engine* e, int on) {
   return 0;
 }
 
+long long mordred_engine_struct_size(const char* name) {
+  if (!name) return 0;
+  const std::string n(name);
+#define MORDRED_SIZE_OF(T) \
+  if (n == #T) return (long long)sizeof(T);
+  MORDRED_SIZE_OF(mordred_join)
+  MORDRED_SIZE_OF(mordred_fact_filter)
+  MORDRED_SIZE_OF(mordred_star_query)
+  MORDRED_SIZE_OF(mordred_run_stats)
+  MORDRED_SIZE_OF(mordred_placement_stats)
+  MORDRED_SIZE_OF(mordred_batch_config)
+  MORDRED_SIZE_OF(mordred_batch_stats)
+  MORDRED_SIZE_OF(mordred_batch_tune_entry)
+  MORDRED_SIZE_OF(mordred_serve_opts)
+  MORDRED_SIZE_OF(mordred_serve_times)
+  MORDRED_SIZE_OF(mordred_serve_stats)
+#undef MORDRED_SIZE_OF
+  return 0;
+}
+
 int mordred_engine_set_operator_order(mordred_engine* e, int order) {
   if (!e) return -1;
   if (order < MORDRED_ORDER_GPU_FIRST || order > MORDRED_ORDER_ALTERNATE) return fail(e, "bad operator order");
@@ -2565,9 +2611,7 @@ int mordred_engine_place(mordred_engine* e, const mordred_star_query* q, double 
   // permute_cost weight of a segment: what moving its pipeline from the CPU to
   // the GPU saves (CostModel.cu:147-181), accumulated over calls like the
   // reference's segment weights (CacheManager.cu:533-587).
-  CpuTables t;
-  cpu_build(e, r, t);
-  const Selectivity sel = estimate_selectivity(r, t);
+  const Selectivity sel = estimate_selectivity(r);
   // segments the query skips weigh nothing and are ranked last (the
   // reference updates segment statistics only for segments it processes,
   // QueryOptimizer.cu:1696-1699)
@@ -2675,22 +2719,23 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   std::thread feeder;
   if (use_gpu) feeder = std::thread([&] { g.rc = gpu_part(e, r, g, res_ints); });
 
-  // ---- CPU hash tables (the CPU operators and the cost model need them)
+  // ---- selectivities (sampled, so the plan does not wait for the CPU
+  // hash tables, which are built once the GPU side has its plan)
   const auto tc0 = Clock::now();
   CpuTables t;
-  Selectivity sel;
+  const Selectivity sel = estimate_selectivity(r);
+  set_probe_order(r, sel, t);
   bool cpu_tables = false;
-  if ((placement != MORDRED_PLACE_GPU_ONLY && (!other.empty() || !mixed.empty())) ||
-      placement == MORDRED_PLACE_COST_SPLIT) {
-    cpu_build(e, r, t);
-    sel = estimate_selectivity(r, t);
-    set_probe_order(r, sel, t);
-    cpu_tables = true;
-  }
 
   // ---- split the non-cached segments between the CPU and the PCIe stream
   std::vector<int> cpu_segs, stream_segs;
   const double t_cpu = cpu_cost(r, sel, kSeg, e->bw_cpu) * 1e3;
+  // the CPU hash-table builds, at the engine's measured time per dimension row
+  long long dim_rows = 0;
+  for (int k = 0; k < 4; k++)
+    if (r.join[k]) dim_rows += r.dkey[k]->n;
+  const double t_fix = e->build_ns_per_row * (double)dim_rows * 1e-6;
+  st.est_cpu_build_ms = t_fix;
   const double t_str = transfer_cost(r, kSeg, e->bw_pci) * 1e3;
   st.est_cpu_ms_per_segment = t_cpu;
   st.est_stream_ms_per_segment = t_str;
@@ -2726,8 +2771,9 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
       st.est_prefix_ms_per_segment = pc.gpu;
       st.est_prefix_cpu_ms_per_segment = pc.cpu;
       const int n = (int)other.size();
-      const int kw = split_balance(n, t_cpu, t_str), kp = split_balance(n, t_cpu, pc.gpu, pc.cpu);
-      prefix_plan = split_analytic_ms(n, kp, t_cpu, pc.gpu, pc.cpu) < split_analytic_ms(n, kw, t_cpu, t_str);
+      const int kw = split_balance(n, t_cpu, t_str, 0, t_fix), kp = split_balance(n, t_cpu, pc.gpu, pc.cpu, t_fix);
+      prefix_plan = split_analytic_ms(n, kp, t_cpu, pc.gpu, pc.cpu, t_fix) <
+                    split_analytic_ms(n, kw, t_cpu, t_str, 0, t_fix);
     }
   }
   // the GPU side's time per segment, and the CPU time it leaves per segment
@@ -2739,7 +2785,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     // both sides run concurrently: send k segments to the GPU side so that
     // both finish together
     const int n = (int)other.size();
-    int k = split_balance(n, t_cpu, t_gpu, t_gcpu);
+    int k = split_balance(n, t_cpu, t_gpu, t_gcpu, t_fix);
     if (e->cm_place && e->place_mode > 0 && n > 0) {
       // candidate plans: k = n*c/8 for c = 0..8, and the analytic k
       std::vector<int> ks;
@@ -2749,7 +2795,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
       ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
       std::vector<float> xs(ks.size() * MORDRED_COST_FEATURES), pred(ks.size());
       for (size_t c = 0; c < ks.size(); c++)
-        split_features(e, r, sel, (int)g.gpu_segs.size(), n, ks[c], t_cpu, t_gpu, t_gcpu,
+        split_features(e, r, sel, (int)g.gpu_segs.size(), n, ks[c], t_cpu, t_gpu, t_gcpu, t_fix,
                        &xs[c * MORDRED_COST_FEATURES]);
       mordred_cost_model_stats cs{};
       // a model error leaves the analytic plan (the GPU half already runs)
@@ -2772,7 +2818,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
           st.plan_model = 2;
         }
       }
-      st.plan_analytic_ms = split_analytic_ms(n, k, t_cpu, t_gpu, t_gcpu);
+      st.plan_analytic_ms = split_analytic_ms(n, k, t_cpu, t_gpu, t_gcpu, t_fix);
       st.plan_streamed = k;
     }
     if (prefix_plan) pstream_segs.assign(other.begin(), other.begin() + k);
@@ -2783,8 +2829,6 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   }
   if ((!stream_segs.empty() || !pstream_segs.empty()) && e->free_slots.size() < staging) {
     // not enough free cache slots for two staging batches: CPU instead
-    if (!cpu_tables) cpu_build(e, r, t);
-    cpu_tables = true;
     cpu_segs.insert(cpu_segs.end(), stream_segs.begin(), stream_segs.end());
     cpu_segs.insert(cpu_segs.end(), pstream_segs.begin(), pstream_segs.end());
     stream_segs.clear();
@@ -2795,6 +2839,18 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   const bool ordered = e->op_order != MORDRED_ORDER_GPU_FIRST && !mixed.empty();
   std::vector<MixedGroup> no_mixed;
   streamed.set_value(SplitPlan{stream_segs, ordered ? &no_mixed : &mixed, pstream_segs, &pgroup});
+
+  // ---- CPU hash tables, for every CPU operator of the run
+  if (!cpu_segs.empty() || !mixed.empty() || !pstream_segs.empty()) {
+    const auto tb0 = Clock::now();
+    cpu_build(e, r, t);
+    cpu_tables = true;
+    st.cpu_build_ms = ms_since(tb0);
+    if (dim_rows >= (1 << 16)) {
+      const double ns = st.cpu_build_ms * 1e6 / (double)dim_rows;
+      e->build_ns_per_row = e->build_ns_per_row > 0 ? 0.7 * e->build_ns_per_row + 0.3 * ns : ns;
+    }
+  }
 
   // ---- CPU: the CPU segment group, concurrently with the GPU
   std::vector<int> cpu_res;
@@ -2886,7 +2942,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   if (placement == MORDRED_PLACE_COST_SPLIT && e->cm_place && e->place_mode > 0 && !other.empty() &&
       st.plan_streamed == st.stream_segments + st.prefix_segments) {
     float f[MORDRED_COST_FEATURES];
-    split_features(e, r, sel, st.gpu_segments, (int)other.size(), st.plan_streamed, t_cpu, t_gpu, t_gcpu, f);
+    split_features(e, r, sel, st.gpu_segments, (int)other.size(), st.plan_streamed, t_cpu, t_gpu, t_gcpu, t_fix, f);
     if (mordred_cost_model_observe(e->cm_place, f, std::max(1e-3, st.total_ms), std::max(1e-3, st.plan_analytic_ms)))
       return fail(e, std::string("placement model: ") + mordred_cost_model_last_error(e->cm_place));
     if (++e->place_since_train >= 8) {

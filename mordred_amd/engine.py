@@ -45,7 +45,8 @@ class RunStats(C.Structure):
                 ("est_mixed_ms_per_segment", C.c_double), ("prefix_segments", C.c_int),
                 ("prefix_rows", C.c_longlong), ("prefix_survivors", C.c_longlong),
                 ("est_prefix_ms_per_segment", C.c_double), ("est_prefix_cpu_ms_per_segment", C.c_double),
-                ("suffix_ms", C.c_double), ("operator_order", C.c_int), ("order_switches", C.c_int)]
+                ("suffix_ms", C.c_double), ("operator_order", C.c_int), ("order_switches", C.c_int),
+                ("cpu_build_ms", C.c_double), ("est_cpu_build_ms", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -146,6 +147,7 @@ _SIGS = {
     "mordred_engine_set_skipping": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_set_prefix_streaming": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_set_operator_order": ([C.c_void_p, C.c_int], C.c_int),
+    "mordred_engine_struct_size": ([C.c_char_p], C.c_longlong),
     "mordred_engine_set_learned_placement": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_placement_model_stats": ([C.c_void_p, C.POINTER(PlacementStats)], C.c_int),
     "mordred_engine_place": ([C.c_void_p, C.POINTER(StarQueryC), C.c_double], C.c_int),

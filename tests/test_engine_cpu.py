@@ -77,3 +77,21 @@ def test_learned_placement_needs_a_gpu(host_engine):
     with pytest.raises(E.EngineError, match="bad learned placement mode"):
         eng.set_learned_placement(7)
     assert eng.placement_model_stats()["samples"] == 0
+
+
+def test_binding_struct_sizes_match_library():
+    """The ctypes copies of the engine header's structs have the library's
+    sizes (a field added out of order on one side would let the library
+    write past the binding's buffer)."""
+    lib = E._lib()
+    pairs = {
+        "mordred_join": E.Join, "mordred_fact_filter": E.FactFilter, "mordred_star_query": E.StarQueryC,
+        "mordred_run_stats": E.RunStats, "mordred_placement_stats": E.PlacementStats,
+        "mordred_batch_config": E.BatchConfig, "mordred_batch_stats": E.BatchStats,
+        "mordred_batch_tune_entry": E.BatchTuneEntry, "mordred_serve_opts": E.ServeOpts,
+        "mordred_serve_times": E.ServeTimes, "mordred_serve_stats": E.ServeStats,
+    }
+    import ctypes
+    for name, cls in pairs.items():
+        assert lib.mordred_engine_struct_size(name.encode()) == ctypes.sizeof(cls), name
+    assert lib.mordred_engine_struct_size(b"no_such_struct") == 0
