@@ -903,6 +903,13 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
         for (int k = 0; k < 4; k++)
           if (m.gpu_probe[k] && r.dval[k]) __builtin_prefetch(r.dval[k]->host + c.dim[k][i + kPrefetch]);
       }
+      if (i + kPrefetch / 2 < en) {
+        // the hash slots of the CPU probes, from keys prefetched above
+        const long long prow = c.lo[i + kPrefetch / 2];
+        for (int k = 0; k < 4; k++)
+          if (r.join[k] && !m.gpu_probe[k])
+            __builtin_prefetch(t.ht[k].data() + 2 * (size_t)((r.key[k]->host[prow] - mk[k]) % len[k]));
+      }
       const long long row = c.lo[i];
       bool ok = true;
       for (int f = 0; f < q.nfilters && ok; f++) {
