@@ -716,9 +716,14 @@ struct MixedFeed {
 
 struct CpuTables {
   std::vector<int> ht[4];
+  // one bit per slot, set when the slot holds a row: a probe that misses
+  // reads only this (len/8 bytes, which stays in a core's cache) and not the
+  // 8-byte slot of a table that may not
+  std::vector<uint64_t> bits[4];
   // the CPU operators probe in this order: the probes commute, so the most
   // selective table goes first and the others see only the rows it keeps
   int order[4] = {0, 1, 2, 3};
+  bool hit(int k, int h) const { return bits[k][(size_t)h >> 6] >> (h & 63) & 1u; }
 };
 
 // Probe order by estimated pass fraction, smallest first (ties: the smaller
@@ -759,6 +764,20 @@ void cpu_build(mordred_engine* e, const Resolved& r, CpuTables& t) {
         const int h = (key->host[row] - j.min_key) % j.dim_len;
         if (vc) ht[2 * (size_t)h] = vc->host[row];
         ht[2 * (size_t)h + 1] = (int)row + 1;
+      }
+    });
+    const size_t nw = ((size_t)j.dim_len + 63) / 64;
+    t.bits[k].assign(nw, 0);
+    uint64_t* bw = t.bits[k].data();
+    const int wtasks = (int)((nw + 4095) / 4096);
+    e->pool->run(wtasks, [&](int task, int) {
+      for (size_t w = (size_t)task * 4096; w < std::min(nw, (size_t)(task + 1) * 4096); w++) {
+        uint64_t b = 0;
+        for (int i = 0; i < 64; i++) {
+          const size_t h = w * 64 + i;
+          if (h < (size_t)j.dim_len && (ht[2 * h] | ht[2 * h + 1]) != 0) b |= 1ull << i;
+        }
+        bw[w] = b;
       }
     });
   }
@@ -817,9 +836,8 @@ void cpu_fact(mordred_engine* e, const Resolved& r, const CpuTables& t, const st
         const int k = t.order[j];
         if (!kcol[k]) continue;
         const int h = (kcol[k][row] - mk[k]) % len[k];
-        const int lo = hts[k][2 * (size_t)h], hi = hts[k][2 * (size_t)h + 1];
-        if ((lo | hi) == 0) ok = false;
-        else v[k] = lo;
+        if (!t.hit(k, h)) ok = false;
+        else v[k] = hts[k][2 * (size_t)h];
       }
       if (!ok) continue;
       const int x1 = a1[row], x2 = a2 ? a2[row] : 0;
