@@ -906,6 +906,12 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
     if (r.join[k] && !m.gpu_probe[k]) pcol[npc++] = r.key[k]->host;
   pcol[npc++] = r.a1->host;
   if (r.a2) pcol[npc++] = r.a2->host;
+  // CPU-probed tables too large for the cores' caches (above 1 MiB of
+  // slots): their slots are prefetched as well; a small table's slot is in
+  // cache already and its prefetch would only add a modulo per survivor
+  int big[4], nbig = 0;
+  for (int k = 0; k < 4; k++)
+    if (r.join[k] && !m.gpu_probe[k] && len[k] >= (1 << 17)) big[nbig++] = k;
   e->pool->run(tasks, [&](int task, int w) {
     CpuAcc& A = acc[w];
     if (q.kind == MORDRED_KIND_GROUPBY && A.sum.empty()) {
@@ -921,12 +927,13 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
         for (int k = 0; k < 4; k++)
           if (m.gpu_probe[k] && r.dval[k]) __builtin_prefetch(r.dval[k]->host + c.dim[k][i + kPrefetch]);
       }
-      if (i + kPrefetch / 2 < en) {
-        // the hash slots of the CPU probes, from keys prefetched above
+      if (nbig && i + kPrefetch / 2 < en) {
+        // the hash slots of the large CPU-probed tables, from keys prefetched above
         const long long prow = c.lo[i + kPrefetch / 2];
-        for (int k = 0; k < 4; k++)
-          if (r.join[k] && !m.gpu_probe[k])
-            __builtin_prefetch(t.ht[k].data() + 2 * (size_t)((r.key[k]->host[prow] - mk[k]) % len[k]));
+        for (int j = 0; j < nbig; j++) {
+          const int k = big[j];
+          __builtin_prefetch(t.ht[k].data() + 2 * (size_t)((r.key[k]->host[prow] - mk[k]) % len[k]));
+        }
       }
       const long long row = c.lo[i];
       bool ok = true;
