@@ -1376,11 +1376,13 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
       loan.slots.push_back(e->free_slots.back());
       e->free_slots.pop_back();
     }
-    // staging slot of segment i of batch bi, column c
+    // staging slot of segment i of batch bi, column c: a column's segments
+    // of one batch take consecutive loaned slots, so that a run of
+    // consecutive segments is one copy when the loaned slots are consecutive
     auto slot_of = [&](size_t bi, size_t i, size_t c) -> int {
       const size_t set = bi & 1;
-      if (batches[bi].prefix) return loan.slots[set * kStreamBatch * nc + i * npc + pidx[c]];
-      return loan.slots[(set * kStreamBatch + i) * nc + c];
+      if (batches[bi].prefix) return loan.slots[set * kStreamBatch * nc + (size_t)pidx[c] * pbatch + i];
+      return loan.slots[set * kStreamBatch * nc + c * kStreamBatch + i];
     };
     int* h_tab = reinterpret_cast<int*>(hs + sg_bytes);
     int* d_tab = reinterpret_cast<int*>(ds + sg_bytes);
@@ -1446,11 +1448,23 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
         const int seg = all[B.first + i];
         h_bsg[i] = (short)seg;
         rows += segment_rows(r, seg);
-        for (size_t c = 0; c < nc; c++) {
-          if (B.prefix && !pcopy[c]) continue;
-          const int slot = slot_of(bi, i, c);
-          HIP_OK(hipMemcpyAsync(e->cache + (size_t)slot * kSeg, r.fact_cols[c]->host + (size_t)seg * kSeg,
-                                segment_rows(r, seg) * sizeof(int), hipMemcpyHostToDevice, s));
+      }
+      // one copy per run of segments that are consecutive both in the host
+      // column and in the staging slots
+      for (size_t c = 0; c < nc; c++) {
+        if (B.prefix && !pcopy[c]) continue;
+        size_t i0 = 0;
+        while (i0 < B.cnt) {
+          size_t i1 = i0 + 1;
+          while (i1 < B.cnt && all[B.first + i1] == all[B.first + i1 - 1] + 1 &&
+                 slot_of(bi, i1, c) == slot_of(bi, i1 - 1, c) + 1)
+            i1++;
+          const int seg0 = all[B.first + i0];
+          long long n = 0;
+          for (size_t i = i0; i < i1; i++) n += segment_rows(r, all[B.first + i]);
+          HIP_OK(hipMemcpyAsync(e->cache + (size_t)slot_of(bi, i0, c) * kSeg, r.fact_cols[c]->host + (size_t)seg0 * kSeg,
+                                (size_t)n * sizeof(int), hipMemcpyHostToDevice, s));
+          i0 = i1;
         }
       }
       HIP_OK(hipMemcpyAsync(d_bsg, h_bsg, B.cnt * sizeof(short), hipMemcpyHostToDevice, s));
