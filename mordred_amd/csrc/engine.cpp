@@ -1836,6 +1836,7 @@ int order_groups(mordred_engine* e, const Resolved& r, const CpuTables& t, std::
       const std::vector<short> sgl(segs.begin(), segs.end());
       const long long rows = group_rows(r, segs);
       HIP_OK(hipMemcpyAsync(B.sg, sgl.data(), sgl.size() * sizeof(short), hipMemcpyHostToDevice, s));
+      HIP_OK(hipStreamSynchronize(s));  // sgl is pageable and goes out of scope with the chunk
       RowList L;
       L.n = rows;
       for (const OrderStage& st : stages) {
