@@ -84,7 +84,11 @@ enum {
   MORDRED_ORDER_CPU_FIRST = 1,
   /* executeTableFact_v2 (QueryProcessing.cu:676-914): GPU filters, CPU
    * filters and probes, GPU probes, then the group-by as above */
-  MORDRED_ORDER_ALTERNATE = 2
+  MORDRED_ORDER_ALTERNATE = 2,
+  /* main.cu:155-161 runs both fact pipelines and keeps the faster: the first
+   * runs of a query with a given residency of its partly cached groups try
+   * each order above once; later runs use the one whose run was fastest */
+  MORDRED_ORDER_AUTO = 3
 };
 int mordred_engine_set_operator_order(mordred_engine* e, int order);
 

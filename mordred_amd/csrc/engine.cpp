@@ -13,6 +13,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <array>
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -184,6 +185,9 @@ struct mordred_engine {
   double suffix_ns_per_line = 0;        // measured CPU-operator time per survivor line (0 = none yet)
   double build_ns_per_row = 0;          // measured CPU hash-table build time per dimension row
   int op_order = MORDRED_ORDER_GPU_FIRST;  // operator order of partly cached segment groups
+  // MORDRED_ORDER_AUTO: per query and residency, the run time of each order
+  // tried (0 = not yet)
+  std::unordered_map<std::string, std::array<double, 3>> order_ms;
   hipStream_t s_order = nullptr;        // the calling thread's stream for ordered pipelines
   hipEvent_t ev_built = nullptr;        // recorded after a run's hash-table builds
   std::unordered_map<long long, double> seg_weight;  // (column id << 32 | segment) -> weight
@@ -1807,7 +1811,7 @@ void order_cpu_compact(mordred_engine* e, const Resolved& r, const CpuTables& t,
 
 // Runs the mixed groups in the engine's operator order.  CPU group-by stages
 // accumulate into acc, GPU ones into d_res (zeroed here).
-int order_groups(mordred_engine* e, const Resolved& r, const CpuTables& t, std::vector<MixedGroup>& mixed,
+int order_groups(mordred_engine* e, const Resolved& r, const CpuTables& t, std::vector<MixedGroup>& mixed, int order,
                  int* d_res, size_t res_ints, std::vector<CpuAcc>& acc, mordred_run_stats& stats) {
   HIP_OK(hipSetDevice(e->device));
   hipStream_t s = e->s_order;
@@ -1827,7 +1831,7 @@ int order_groups(mordred_engine* e, const Resolved& r, const CpuTables& t, std::
   B.sg = reinterpret_cast<short*>(base + 11 * B.cap + 16);
   for (MixedGroup& m : mixed) {
     order_last_partial(r, m.segs);
-    const std::vector<OrderStage> stages = order_stages(r, m, e->op_order);
+    const std::vector<OrderStage> stages = order_stages(r, m, order);
     if (stages.empty() || !stages.back().group) return fail(e, "operator order: no group-by stage");
     m.survivors = 0;
     for (size_t c0 = 0; c0 < m.segs.size(); c0 += kOrderChunk) {
@@ -2637,7 +2641,7 @@ long long mordred_engine_struct_size(const char* name) {
 
 int mordred_engine_set_operator_order(mordred_engine* e, int order) {
   if (!e) return -1;
-  if (order < MORDRED_ORDER_GPU_FIRST || order > MORDRED_ORDER_ALTERNATE) return fail(e, "bad operator order");
+  if (order < MORDRED_ORDER_GPU_FIRST || order > MORDRED_ORDER_AUTO) return fail(e, "bad operator order");
   e->op_order = order;
   return 0;
 }
@@ -2690,6 +2694,38 @@ int mordred_engine_place(mordred_engine* e, const mordred_star_query* q, double 
       for (Column* c : r.fact_cols)
         if ((rc = cache_seg(e, c, s))) return rc;
   return 0;
+}
+
+// Key of a query and the residency patterns of its partly cached groups
+// (MORDRED_ORDER_AUTO keeps one choice per key).
+std::string order_signature(const Resolved& r, const std::vector<MixedGroup>& mixed) {
+  const mordred_star_query& q = *r.q;
+  std::string k;
+  auto add = [&](const char* s) {
+    k += s ? s : "-";
+    k += '|';
+  };
+  auto addi = [&](long long v) {
+    k += std::to_string(v);
+    k += '|';
+  };
+  addi(q.kind), addi(q.total_val), add(q.aggr1), add(q.aggr2);
+  for (int i = 0; i < q.nfilters; i++) add(q.filters[i].col), addi(q.filters[i].lo), addi(q.filters[i].hi);
+  for (int i = 0; i < q.njoins; i++) {
+    const mordred_join& j = q.joins[i];
+    addi(j.slot), add(j.fact_key), add(j.dim_key), add(j.filter_col), addi(j.filter_mode), addi(j.c1), addi(j.c2),
+        add(j.group_col), addi(j.min_val), addi(j.unique_val);
+  }
+  std::vector<int> pats;
+  for (const MixedGroup& m : mixed) {
+    int p = 0;
+    for (int f = 0; f < 2; f++) p |= m.gpu_filter[f] << f;
+    for (int j = 0; j < 4; j++) p |= m.gpu_probe[j] << (2 + j);
+    pats.push_back(p);
+  }
+  std::sort(pats.begin(), pats.end());
+  for (int p : pats) addi(p);
+  return k;
 }
 
 int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int placement, int* res,
@@ -2882,8 +2918,23 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     pstream_segs.clear();
   }
   std::sort(cpu_segs.begin(), cpu_segs.end());
-  // mixed groups in another operator order run on this thread (below)
-  const bool ordered = e->op_order != MORDRED_ORDER_GPU_FIRST && !mixed.empty();
+  // the operator order of the mixed groups; another order than GPU-first
+  // runs on this thread (below)
+  std::string order_key;
+  int order = e->op_order;
+  if (order == MORDRED_ORDER_AUTO) {
+    order = MORDRED_ORDER_GPU_FIRST;
+    if (!mixed.empty()) {
+      order_key = order_signature(r, mixed);
+      const std::array<double, 3>& ms = e->order_ms[order_key];
+      int best = -1;
+      for (int o = 0; o < 3 && best < 0; o++)
+        if (ms[o] == 0) best = o;  // not tried yet
+      if (best < 0) best = (int)(std::min_element(ms.begin(), ms.end()) - ms.begin());
+      order = best;
+    }
+  }
+  const bool ordered = order != MORDRED_ORDER_GPU_FIRST && !mixed.empty();
   std::vector<MixedGroup> no_mixed;
   streamed.set_value(SplitPlan{stream_segs, ordered ? &no_mixed : &mixed, pstream_segs, &pgroup});
 
@@ -2909,7 +2960,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   }
   // ---- mixed groups: the CPU operators over the GPU operators' survivors
   std::vector<int> ord_res;  // GPU group-by stages of ordered mixed groups
-  st.operator_order = e->op_order;
+  st.operator_order = order;
   if (!mixed.empty() || !pstream_segs.empty()) {
     if (cpu_res.empty()) cpu_res.assign(res_ints, 0);
     std::vector<CpuAcc> acc(e->pool->size());
@@ -2918,7 +2969,8 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
       if (!rc) {
         const hipError_t he = e->d_res_ord.reserve(res_ints * sizeof(int));
         rc = he != hipSuccess ? hip_fail(e, he, "ordered pipelines: result buffer")
-                              : order_groups(e, r, t, mixed, static_cast<int*>(e->d_res_ord.p), res_ints, acc, st);
+                              : order_groups(e, r, t, mixed, order, static_cast<int*>(e->d_res_ord.p), res_ints, acc,
+                                             st);
       }
       if (!rc) {
         ord_res.assign(res_ints, 0);
@@ -2997,6 +3049,10 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
       if (mordred_cost_model_train(e->cm_place, 32, 64, 2e-3f, nullptr))
         return fail(e, std::string("placement model: ") + mordred_cost_model_last_error(e->cm_place));
     }
+  }
+  if (!order_key.empty()) {
+    double& slot = e->order_ms[order_key][order];
+    slot = slot > 0 ? std::min(slot, st.total_ms) : std::max(st.total_ms, 1e-6);
   }
   if (stats) *stats = st;
   return 0;

@@ -15,7 +15,7 @@ from . import abi
 
 PLACE_RESIDENCY, PLACE_COST_SPLIT, PLACE_GPU_ONLY, PLACE_CPU_ONLY = 0, 1, 2, 3
 # operator order of partly cached segment groups
-ORDER_GPU_FIRST, ORDER_CPU_FIRST, ORDER_ALTERNATE = 0, 1, 2
+ORDER_GPU_FIRST, ORDER_CPU_FIRST, ORDER_ALTERNATE, ORDER_AUTO = 0, 1, 2, 3
 KIND_AGGR, KIND_GROUPBY = 0, 1
 
 
@@ -285,7 +285,8 @@ class Engine:
 
     def set_operator_order(self, order: int):
         """ORDER_GPU_FIRST (default), ORDER_CPU_FIRST (the reference's
-        executeTableFact_v1) or ORDER_ALTERNATE (executeTableFact_v2)."""
+        executeTableFact_v1), ORDER_ALTERNATE (executeTableFact_v2), or
+        ORDER_AUTO (try each once per query and residency, keep the fastest)."""
         self._check(self.lib.mordred_engine_set_operator_order(self.h, int(order)), "set_operator_order")
 
     # ---- learned placement

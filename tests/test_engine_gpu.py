@@ -303,3 +303,28 @@ def test_operator_orders(fresh_engine, order):
         chunks = (nseg + 15) // 16
         back = order == E.ORDER_ALTERNATE and q.fact_filters
         assert st["order_switches"] >= (2 if back else 1) * chunks
+
+
+def test_operator_order_auto(fresh_engine):
+    """MORDRED_ORDER_AUTO (main.cu:155-161 keeps the faster fact pipeline):
+    the first three runs of a query and residency try GPU-first, CPU-first
+    and alternate once each; the fourth uses one of them; all are exact."""
+    q, data, eng, ref = fresh_engine
+    print(f"[gpu] operator order auto {q.name}", flush=True)
+    _uncache_all(eng, q)
+    nseg = eng.segments(q.fact_columns()[0])
+    for c in {q.joins[-1].fact_key} | {a for a in q.aggr if a}:
+        for s in range(nseg):
+            eng.cache_segment(c, s)
+    eng.set_operator_order(E.ORDER_AUTO)
+    try:
+        seen = []
+        for _ in range(4):
+            res, st = eng.run(q, E.PLACE_RESIDENCY)
+            np.testing.assert_array_equal(res, ref)
+            assert st["mixed_segments"] == nseg
+            seen.append(st["operator_order"])
+    finally:
+        eng.set_operator_order(E.ORDER_GPU_FIRST)
+    assert seen[:3] == [E.ORDER_GPU_FIRST, E.ORDER_CPU_FIRST, E.ORDER_ALTERNATE]
+    assert seen[3] in seen[:3]
