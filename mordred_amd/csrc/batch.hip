@@ -113,6 +113,9 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       Hd.pw = B.pw;
       Hd.nattr = B.nattr;
       Hd.ncmp = B.ncmp;
+      Hd.ivl_src = B.ivl_src;
+      Hd.ivl_n = B.ivl_n;
+      Hd.ivl_off = B.ivl_off;
       for (int a = 0; a < kBatchMaxAttr; a++) {
         Hd.off[a] = B.attr[a].off;
         Hd.bits[a] = B.attr[a].bits;
@@ -292,6 +295,40 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
 #pragma unroll
         for (int j = 0; j < IPT; j++) {
           const unsigned long long m = llut[lut + ((w[j] >> off) & vm)];
+          mlo[j] &= (uint32_t)m;
+          mhi[j] &= (uint32_t)(m >> 32);
+        }
+      }
+      // interval table: the number of breakpoints at or below the value
+      // (binary lifting over a wave-uniform number of steps) picks the mask
+      const int in = __builtin_amdgcn_readfirstlane(Hd.ivl_n);
+      if (in > 0) {
+        const int isrc = __builtin_amdgcn_readfirstlane(Hd.ivl_src);
+        const int ioff = __builtin_amdgcn_readfirstlane(Hd.ivl_off);
+        int v[IPT];
+        if (isrc < 0) {
+#pragma unroll
+          for (int j = 0; j < IPT; j++) v[j] = x[d][j];
+        } else {
+          const int off = __builtin_amdgcn_readfirstlane(Hd.off[isrc]);
+          const uint32_t vm = (1u << __builtin_amdgcn_readfirstlane(Hd.bits[isrc])) - 1u;
+          const int base = __builtin_amdgcn_readfirstlane(Hd.base[isrc]);
+#pragma unroll
+          for (int j = 0; j < IPT; j++) v[j] = base + (int)((w[j] >> off) & vm);
+        }
+        int top_step = 1;
+        while (top_step <= in) top_step <<= 1;
+        int pos[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; j++) pos[j] = 0;
+        for (int st = top_step >> 1; st > 0; st >>= 1) {
+#pragma unroll
+          for (int j = 0; j < IPT; j++)
+            if (pos[j] + st <= in && (int)(uint32_t)llut[ioff + pos[j] + st - 1] <= v[j]) pos[j] += st;
+        }
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+          const unsigned long long m = llut[ioff + in + pos[j]];
           mlo[j] &= (uint32_t)m;
           mhi[j] &= (uint32_t)(m >> 32);
         }

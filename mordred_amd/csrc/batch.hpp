@@ -41,6 +41,7 @@ constexpr int kBatchMaxAttr = 3;    // packed attributes per dimension
 constexpr int kBatchMaxCmp = 64;    // range-compare filters per dimension (one per query)
 constexpr int kBatchMaxLut = 4096;  // 64-bit mask entries staged in LDS (32 KiB)
 constexpr int kBatchLutSpan = 2048; // widest attribute span given a mask table
+constexpr int kBatchIvlMin = 4;     // range filters on one source that get an interval table
 
 enum BatchExprKind { kExprDiff = 0, kExprProduct = 1 };  // a1 - a2 (group-by), a1 * a2 (aggregate)
 enum BatchQueryFlags { kQueryAggr = 1 };
@@ -96,6 +97,10 @@ struct BatchDim {
   int used, len, minkey, nrows;
   int pw;                      // bytes per packed entry: 1, 2 or 4
   int nattr, ncmp, pad;
+  // interval table of the range filters on one source (src as in BatchCmp)
+  // when many queries filter it: ivl_n sorted breakpoints at lut[ivl_off ..]
+  // (low 32 bits), then ivl_n + 1 query masks, one per interval between them
+  int ivl_src, ivl_n, ivl_off, pad2;
   BatchAttr attr[kBatchMaxAttr];
   BatchCmp cmp[kBatchMaxCmp];
 };
@@ -137,6 +142,7 @@ struct BatchHotDim {
   const void* pk;
   unsigned long long uses;
   int len, minkey, pw, nattr, ncmp, pad;
+  int ivl_src, ivl_n, ivl_off, pad2;
   int off[kBatchMaxAttr], bits[kBatchMaxAttr], lut[kBatchMaxAttr], base[kBatchMaxAttr];
 };
 

@@ -1962,6 +1962,7 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
   // dimensions: one packed attribute table each; mask tables only as far as
   // the launch configuration's LDS holds them (the rest use range compares)
   Column* attrs[4][kBatchMaxAttr] = {};
+  std::vector<char> qdead(nq, 0);  // a dimension filter with an empty range
   D.nlut = 0;
   const long long lut_cap = std::min<long long>(
       kBatchMaxLut, std::max<long long>(0, ((long long)cfg.lds_bytes - (long long)sizeof(BatchHot) -
@@ -2052,6 +2053,9 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       }
       D.nlut += A.span;
     }
+    // range filters on the key and wide attributes; an empty range keeps no
+    // dimension row, so its query is dead
+    std::vector<BatchCmp> cands;
     for (const Filt& f : filts) {
       int src = -1;
       if (f.col != key) {
@@ -2059,8 +2063,56 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
         if (B.attr[src].lut >= 0) continue;
       }
       if (f.mode == 2) return fail(e, "EQ-or-EQ filter on a dimension key or wide column in a batch");
+      const int hi = f.mode == 0 ? f.c1 : f.c2;
+      if (f.c1 > hi) {
+        qdead[f.q] = 1;
+        continue;
+      }
+      cands.push_back(BatchCmp{src, f.c1, hi, f.q});
+    }
+    // the source most queries filter gets an interval table when it has at
+    // least kBatchIvlMin filters and the table fits the LDS beside the mask tables
+    B.ivl_src = -1;
+    B.ivl_n = 0;
+    B.ivl_off = 0;
+    int best_src = -2, best_n = 0;
+    for (int src = -1; src < na; src++) {
+      int n = 0;
+      for (const BatchCmp& c : cands) n += c.src == src;
+      if (n > best_n) best_n = n, best_src = src;
+    }
+    if (best_n >= kBatchIvlMin) {
+      std::vector<long long> pts;
+      for (const BatchCmp& c : cands) {
+        if (c.src != best_src) continue;
+        if (c.lo > INT_MIN) pts.push_back(c.lo);
+        if (c.hi < INT_MAX) pts.push_back((long long)c.hi + 1);
+      }
+      std::sort(pts.begin(), pts.end());
+      pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
+      const int n = (int)pts.size();
+      if (D.nlut + 2LL * n + 1 <= lut_cap) {
+        B.ivl_src = best_src;
+        B.ivl_n = n;
+        B.ivl_off = D.nlut;
+        for (int i = 0; i < n; i++) D.lut[D.nlut + i] = (unsigned long long)(uint32_t)(int)pts[i];
+        // interval i covers [pts[i-1], pts[i]) with the ends open at INT_MIN
+        // and past INT_MAX; a query passes it iff it lies inside [lo, hi]
+        for (int i = 0; i <= n; i++) {
+          const long long lo_i = i == 0 ? (long long)INT_MIN : pts[i - 1];
+          const long long hi_i = i == n ? (long long)INT_MAX + 1 : pts[i];  // exclusive
+          unsigned long long m = ~0ull;
+          for (const BatchCmp& c : cands)
+            if (c.src == best_src && !(lo_i >= c.lo && hi_i <= (long long)c.hi + 1)) m &= ~(1ull << c.q);
+          D.lut[D.nlut + n + i] = m;
+        }
+        D.nlut += 2 * n + 1;
+      }
+    }
+    for (const BatchCmp& c : cands) {
+      if (B.ivl_n > 0 && c.src == B.ivl_src) continue;
       if (B.ncmp == kBatchMaxCmp) return fail(e, "too many range filters on one dimension in a batch");
-      B.cmp[B.ncmp++] = BatchCmp{src, f.c1, f.mode == 0 ? f.c1 : f.c2, f.q};
+      B.cmp[B.ncmp++] = c;
     }
     B.used = 1;
     const size_t bytes = (((size_t)B.len * B.pw) + 15) & ~(size_t)15;
@@ -2074,7 +2126,7 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     if (!qp[i]) continue;
     const mordred_star_query& q = *qp[i];
     const Resolved& r = rs[i];
-    bool dead = false;
+    bool dead = qdead[i] != 0;
     for (int f = 0; f < q.nfilters; f++) {
       const int sl = slot_of(r.filt[f]);
       if (sl < 0) return fail(e, "a batch reads more than 12 fact columns");

@@ -120,3 +120,32 @@ def test_hash_collision_is_reported(bengine):
     bad = dataclasses.replace(q, joins=joins)
     with pytest.raises(E.EngineError, match="collision"):
         eng.batch([bad])
+
+
+def test_empty_dimension_range_is_dead(bengine):
+    """A dimension filter BETWEEN c1 AND c2 with c1 > c2 keeps no dimension
+    row (build_GPU2 mode 1): on a key or wide column the batch marks the query
+    dead and returns its empty result; on a narrow attribute the mask table
+    rejects every row."""
+    qs, _, data, eng, refs = bengine
+    q = next(q for q in qs if any(j.filter_col and j.filter_col == j.dim_key for j in q.joins))
+    joins = [dataclasses.replace(j, c1=j.c2 + 1, c2=j.c2) if j.filter_col == j.dim_key else j for j in q.joins]
+    bad = dataclasses.replace(q, joins=joins)
+    print(f"[gpu] empty key range {q.name}", flush=True)
+    outs, st = eng.batch([bad, qs[1]])
+    assert st["dead_queries"] == 1 and not outs[0].any()
+    np.testing.assert_array_equal(outs[1], refs[qs[1].name])
+    np.testing.assert_array_equal(outs[0], OQ.run_query(bad, data, oracle.CONV_GPU))
+
+
+def test_interval_tables_match_range_compares(bengine):
+    """Many key-range filters on one dimension share an interval table when
+    four or more queries filter the same column; with few queries they stay
+    range compares.  Both give each query's own answer."""
+    qs, other, _, eng, refs = bengine
+    print("[gpu] interval tables", flush=True)
+    keyed = [q for q in qs + other if any(j.filter_col and j.filter_col == j.dim_key for j in q.joins)]
+    assert len(keyed) >= 8
+    for sub in (keyed[:3], keyed[:12], keyed):
+        outs, _ = eng.batch(sub[:64])
+        check(sub[:64], outs, refs)
