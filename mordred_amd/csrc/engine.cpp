@@ -2978,6 +2978,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     order = MORDRED_ORDER_GPU_FIRST;
     if (!mixed.empty()) {
       order_key = order_signature(r, mixed);
+      if (e->order_ms.size() >= 4096 && !e->order_ms.count(order_key)) e->order_ms.clear();  // bounded memory
       const std::array<double, 3>& ms = e->order_ms[order_key];
       int best = -1;
       for (int o = 0; o < 3 && best < 0; o++)
