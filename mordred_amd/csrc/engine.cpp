@@ -2898,17 +2898,33 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   std::vector<int> pstream_segs;
   bool prefix_plan = false;
   if (placement == MORDRED_PLACE_COST_SPLIT && e->prefix_stream && !other.empty()) {
-    bool any = false;
-    for (int i = 0; i < r.q->nfilters; i++) any |= pgroup.gpu_filter[i] = true;
-    for (int k = 0; k < 4; k++) any |= pgroup.gpu_probe[k] = r.join[k] && sel.join[k] < 0.5;
-    if (any) {
-      const MixedCost pc = mixed_cost(e, r, sel, pgroup, true);
-      st.est_prefix_ms_per_segment = pc.gpu;
-      st.est_prefix_cpu_ms_per_segment = pc.cpu;
-      const int n = (int)other.size();
-      const int kw = split_balance(n, t_cpu, t_str, 0, t_fix), kp = split_balance(n, t_cpu, pc.gpu, pc.cpu, t_fix);
-      prefix_plan = split_analytic_ms(n, kp, t_cpu, pc.gpu, pc.cpu, t_fix) <
-                    split_analytic_ms(n, kw, t_cpu, t_str, 0, t_fix);
+    // the GPU operator set: every non-empty subset of the fact filters and
+    // probes (at most 6 operators), timed by the split model; the best one
+    // is kept if it beats whole-segment streaming
+    int ops[6], nops = 0;
+    for (int i = 0; i < r.q->nfilters; i++) ops[nops++] = i;
+    for (int k = 0; k < 4; k++)
+      if (r.join[k]) ops[nops++] = 2 + k;
+    const int n = (int)other.size();
+    const int kw = split_balance(n, t_cpu, t_str, 0, t_fix);
+    double best = split_analytic_ms(n, kw, t_cpu, t_str, 0, t_fix);
+    for (int set = 1; set < (1 << nops); set++) {
+      MixedGroup cand;
+      for (int b = 0; b < nops; b++) {
+        if (!(set >> b & 1)) continue;
+        if (ops[b] < 2) cand.gpu_filter[ops[b]] = true;
+        else cand.gpu_probe[ops[b] - 2] = true;
+      }
+      const MixedCost pc = mixed_cost(e, r, sel, cand, true);
+      const int kp = split_balance(n, t_cpu, pc.gpu, pc.cpu, t_fix);
+      const double tp = split_analytic_ms(n, kp, t_cpu, pc.gpu, pc.cpu, t_fix);
+      if (tp < best) {
+        best = tp;
+        pgroup = cand;
+        prefix_plan = true;
+        st.est_prefix_ms_per_segment = pc.gpu;
+        st.est_prefix_cpu_ms_per_segment = pc.cpu;
+      }
     }
   }
   // the GPU side's time per segment, and the CPU time it leaves per segment
