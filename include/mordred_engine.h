@@ -177,9 +177,10 @@ int mordred_engine_store_segment_minmax(mordred_engine* e, const char* name, con
 int mordred_engine_segment_minmax(mordred_engine* e, const char* name, int* mins, int* maxs);
 int mordred_engine_set_skipping(mordred_engine* e, int on);
 /* COST_SPLIT prefix streaming (default on): the GPU side of the split may
- * stream only the columns of the fact filters and selective probes of a
- * non-resident segment and hand the survivors to the CPU operators, when the
- * analytic model prices that below streaming the whole segment. */
+ * stream only the columns of a subset of the fact filters and probes of a
+ * non-resident segment, run those operators, and hand the survivors to the
+ * CPU operators, when the analytic model prices that below streaming the
+ * whole segment (the subset it prices lowest). */
 int mordred_engine_set_prefix_streaming(mordred_engine* e, int on);
 
 /* ---- learned placement -------------------------------------------------

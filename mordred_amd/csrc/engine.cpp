@@ -2889,11 +2889,10 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
     mixed.swap(keep);
     std::sort(other.begin(), other.end());
   }
-  // prefix streaming: of a whole non-resident segment, only the columns of the
-  // fact filters and of the selective probes (those keeping under half of
-  // the dimension) cross PCIe; those operators run on the GPU and the CPU
-  // operators finish the survivors, as in a mixed pipeline
-  // (the survivors' CPU operators run on the CPU side of the split)
+  // prefix streaming: of a whole non-resident segment, only the columns of a
+  // subset of the fact filters and probes cross PCIe; those operators run on
+  // the GPU and the CPU operators finish the survivors, as in a mixed
+  // pipeline (the survivors' CPU operators run on the CPU side of the split)
   MixedGroup pgroup;
   std::vector<int> pstream_segs;
   bool prefix_plan = false;
