@@ -115,7 +115,7 @@ typedef struct mordred_run_stats {
   long long mixed_survivors;        /* rows that left the GPU operators */
   double est_mixed_ms_per_segment;  /* analytic model (COST_SPLIT) */
   /* COST_SPLIT prefix streaming: of these non-resident segments only the
-   * columns of the fact filters and selective probes crossed PCIe; those
+   * columns of the chosen fact filters and probes crossed PCIe; those
    * operators ran on the GPU, the CPU operators finished the survivors */
   int prefix_segments;
   long long prefix_rows, prefix_survivors;
