@@ -727,6 +727,7 @@ struct CpuTables {
   // the CPU operators probe in this order: the probes commute, so the most
   // selective table goes first and the others see only the rows it keeps
   int order[4] = {0, 1, 2, 3};
+  double frac[4] = {1, 1, 1, 1};  // fraction of slots that hold a row
   bool hit(int k, int h) const { return bits[k][(size_t)h >> 6] >> (h & 63) & 1u; }
 };
 
@@ -784,6 +785,9 @@ void cpu_build(mordred_engine* e, const Resolved& r, CpuTables& t) {
         bw[w] = b;
       }
     });
+    long long set = 0;
+    for (size_t w = 0; w < nw; w++) set += __builtin_popcountll(bw[w]);
+    t.frac[k] = j.dim_len > 0 ? (double)set / (double)j.dim_len : 1.0;
   }
 }
 
@@ -792,6 +796,7 @@ struct CpuAcc {
   std::vector<int> key;        // 4 per group, 0 = not seen with a non-zero key
   std::vector<uint8_t> touched;
   long long scalar = 0;
+  std::vector<int> sel, selh;  // cpu_suffix scratch: kept positions, their slots (4 each)
 };
 
 void cpu_fact(mordred_engine* e, const Resolved& r, const CpuTables& t, const std::vector<int>& segs,
@@ -899,20 +904,29 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
     mv[k] = r.join[k] && r.join[k]->group_col ? r.join[k]->min_val : 0;
     uq[k] = r.join[k] && r.join[k]->group_col ? r.join[k]->unique_val : 0;
   }
-  // fact columns read at random rows (prefetched kPrefetch survivors ahead:
-  // each access is a cache and TLB miss, so overlap several per thread)
+  // Two passes per task.  The first runs the CPU filters and tests the CPU
+  // probes' slot bitmaps (small, cached), reading only the filter and key
+  // columns at the survivors' rows (random rows: each a cache and TLB miss,
+  // so they are prefetched kPrefetch rows ahead); it keeps the rows that pass
+  // and their slots.  The second reads, for those rows only, the group values
+  // (hash slots, dimension values) and the aggregate columns, again
+  // prefetched ahead, and accumulates.
   constexpr int kPrefetch = 16;
-  const int* pcol[8];
-  int npc = 0;
+  const int* scol[6];
+  int nsc = 0;
   for (int f = 0; f < q.nfilters; f++)
-    if (!m.gpu_filter[f]) pcol[npc++] = r.filt[f]->host;
+    if (!m.gpu_filter[f]) scol[nsc++] = r.filt[f]->host;
   for (int k = 0; k < 4; k++)
-    if (r.join[k] && !m.gpu_probe[k]) pcol[npc++] = r.key[k]->host;
-  pcol[npc++] = r.a1->host;
-  if (r.a2) pcol[npc++] = r.a2->host;
-  // CPU-probed tables too large for the cores' caches (above 1 MiB of
-  // slots): their slots are prefetched as well; a small table's slot is in
-  // cache already and its prefetch would only add a modulo per survivor
+    if (r.join[k] && !m.gpu_probe[k]) scol[nsc++] = r.key[k]->host;
+  // (when the CPU probes keep most rows, the first pass saves nothing: one
+  // pass then prefetches everything)
+  double keep = 1;
+  for (int k = 0; k < 4; k++)
+    if (r.join[k] && !m.gpu_probe[k]) keep *= t.frac[k] * (double)len[k] / (double)std::max<long long>(1, r.dkey[k]->n);
+  const bool two_pass = keep < 0.5;
+  // single pass: the slots of CPU-probed tables above 1 MiB are prefetched
+  // half the distance ahead, from keys prefetched earlier (a small table's
+  // slot is in cache already)
   int big[4], nbig = 0;
   for (int k = 0; k < 4; k++)
     if (r.join[k] && !m.gpu_probe[k] && len[k] >= (1 << 17)) big[nbig++] = k;
@@ -924,20 +938,62 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
       A.touched.assign(G, 0);
     }
     const long long b = (long long)task * per, en = std::min(total, b + per);
+    A.sel.clear();
+    A.selh.clear();
+    if (!two_pass) {
+      for (long long i = b; i < en; i++) {
+        if (i + kPrefetch < en) {
+          const long long prow = c.lo[i + kPrefetch];
+          for (int j = 0; j < nsc; j++) __builtin_prefetch(scol[j] + prow);
+          __builtin_prefetch(r.a1->host + prow);
+          if (r.a2) __builtin_prefetch(r.a2->host + prow);
+          for (int k = 0; k < 4; k++)
+            if (m.gpu_probe[k] && r.dval[k]) __builtin_prefetch(r.dval[k]->host + c.dim[k][i + kPrefetch]);
+        }
+        if (nbig && i + kPrefetch / 2 < en) {
+          const long long prow = c.lo[i + kPrefetch / 2];
+          for (int j = 0; j < nbig; j++)
+            __builtin_prefetch(t.ht[big[j]].data() +
+                               2 * (size_t)((r.key[big[j]]->host[prow] - mk[big[j]]) % len[big[j]]));
+        }
+        const long long row = c.lo[i];
+        bool ok = true;
+        for (int f = 0; f < q.nfilters && ok; f++) {
+          if (m.gpu_filter[f]) continue;
+          const int x = r.filt[f]->host[row];
+          ok = x >= q.filters[f].lo && x <= q.filters[f].hi;
+        }
+        int v[4] = {0, 0, 0, 0};
+        for (int j = 0; j < 4 && ok; j++) {
+          const int k = t.order[j];
+          if (!r.join[k]) continue;
+          if (m.gpu_probe[k]) {
+            if (r.dval[k]) v[k] = r.dval[k]->host[c.dim[k][i]];
+            continue;
+          }
+          const int hk = (r.key[k]->host[row] - mk[k]) % len[k];
+          ok = t.hit(k, hk);
+          if (ok) v[k] = t.ht[k][2 * (size_t)hk];
+        }
+        if (!ok) continue;
+        const int x1 = r.a1->host[row], x2 = r.a2 ? r.a2->host[row] : 0;
+        if (q.kind == MORDRED_KIND_AGGR) {
+          A.scalar += (long long)(int)((unsigned)x1 * (unsigned)x2);
+          continue;
+        }
+        const int h = ((v[0] - mv[0]) * uq[0] + (v[1] - mv[1]) * uq[1] + (v[2] - mv[2]) * uq[2] +
+                       (v[3] - mv[3]) * uq[3]) % G;
+        A.sum[h] += (long long)(int)((unsigned)x1 - (unsigned)x2);
+        A.touched[h] = 1;
+        for (int k = 0; k < 4; k++)
+          if (v[k] != 0) A.key[4 * (size_t)h + k] = v[k];
+      }
+      return;
+    }
     for (long long i = b; i < en; i++) {
       if (i + kPrefetch < en) {
         const long long prow = c.lo[i + kPrefetch];
-        for (int j = 0; j < npc; j++) __builtin_prefetch(pcol[j] + prow);
-        for (int k = 0; k < 4; k++)
-          if (m.gpu_probe[k] && r.dval[k]) __builtin_prefetch(r.dval[k]->host + c.dim[k][i + kPrefetch]);
-      }
-      if (nbig && i + kPrefetch / 2 < en) {
-        // the hash slots of the large CPU-probed tables, from keys prefetched above
-        const long long prow = c.lo[i + kPrefetch / 2];
-        for (int j = 0; j < nbig; j++) {
-          const int k = big[j];
-          __builtin_prefetch(t.ht[k].data() + 2 * (size_t)((r.key[k]->host[prow] - mk[k]) % len[k]));
-        }
+        for (int j = 0; j < nsc; j++) __builtin_prefetch(scol[j] + prow);
       }
       const long long row = c.lo[i];
       bool ok = true;
@@ -946,21 +1002,41 @@ void cpu_suffix(mordred_engine* e, const Resolved& r, const CpuTables& t, const 
         const int x = r.filt[f]->host[row];
         ok = x >= q.filters[f].lo && x <= q.filters[f].hi;
       }
-      if (!ok) continue;
-      int v[4] = {0, 0, 0, 0};
+      int hs[4] = {0, 0, 0, 0};
       for (int j = 0; j < 4 && ok; j++) {
         const int k = t.order[j];
+        if (!r.join[k] || m.gpu_probe[k]) continue;
+        hs[k] = (r.key[k]->host[row] - mk[k]) % len[k];
+        ok = t.hit(k, hs[k]);
+      }
+      if (!ok) continue;
+      A.sel.push_back((int)(i - b));
+      A.selh.insert(A.selh.end(), hs, hs + 4);
+    }
+    const int ns = (int)A.sel.size();
+    for (int s2 = 0; s2 < ns; s2++) {
+      if (s2 + kPrefetch < ns) {
+        const long long pi = b + A.sel[s2 + kPrefetch];
+        const long long prow = c.lo[pi];
+        __builtin_prefetch(r.a1->host + prow);
+        if (r.a2) __builtin_prefetch(r.a2->host + prow);
+        for (int k = 0; k < 4; k++) {
+          if (!r.join[k] || !r.dval[k]) continue;
+          if (m.gpu_probe[k]) __builtin_prefetch(r.dval[k]->host + c.dim[k][pi]);
+          else __builtin_prefetch(t.ht[k].data() + 2 * (size_t)A.selh[4 * (size_t)(s2 + kPrefetch) + k]);
+        }
+      }
+      const long long i = b + A.sel[s2];
+      const long long row = c.lo[i];
+      int v[4] = {0, 0, 0, 0};
+      for (int k = 0; k < 4; k++) {
         if (!r.join[k]) continue;
         if (m.gpu_probe[k]) {
           if (r.dval[k]) v[k] = r.dval[k]->host[c.dim[k][i]];
-          continue;
+        } else {
+          v[k] = t.ht[k][2 * (size_t)A.selh[4 * (size_t)s2 + k]];  // the value word of a hit slot
         }
-        const int h = (r.key[k]->host[row] - mk[k]) % len[k];
-        const int lo = t.ht[k][2 * (size_t)h], hi = t.ht[k][2 * (size_t)h + 1];
-        if ((lo | hi) == 0) ok = false;
-        else v[k] = lo;
       }
-      if (!ok) continue;
       const int x1 = r.a1->host[row], x2 = r.a2 ? r.a2->host[row] : 0;
       if (q.kind == MORDRED_KIND_AGGR) {
         A.scalar += (long long)(int)((unsigned)x1 * (unsigned)x2);
