@@ -1948,6 +1948,38 @@ int order_groups(mordred_engine* e, const Resolved& r, const CpuTables& t, std::
   return 0;
 }
 
+// Key of a query and the residency patterns of its partly cached groups
+// (MORDRED_ORDER_AUTO keeps one choice per key).
+std::string order_signature(const Resolved& r, const std::vector<MixedGroup>& mixed) {
+  const mordred_star_query& q = *r.q;
+  std::string k;
+  auto add = [&](const char* s) {
+    k += s ? s : "-";
+    k += '|';
+  };
+  auto addi = [&](long long v) {
+    k += std::to_string(v);
+    k += '|';
+  };
+  addi(q.kind), addi(q.total_val), add(q.aggr1), add(q.aggr2);
+  for (int i = 0; i < q.nfilters; i++) add(q.filters[i].col), addi(q.filters[i].lo), addi(q.filters[i].hi);
+  for (int i = 0; i < q.njoins; i++) {
+    const mordred_join& j = q.joins[i];
+    addi(j.slot), add(j.fact_key), add(j.dim_key), add(j.filter_col), addi(j.filter_mode), addi(j.c1), addi(j.c2),
+        add(j.group_col), addi(j.min_val), addi(j.unique_val);
+  }
+  std::vector<int> pats;
+  for (const MixedGroup& m : mixed) {
+    int p = 0;
+    for (int f = 0; f < 2; f++) p |= m.gpu_filter[f] << f;
+    for (int j = 0; j < 4; j++) p |= m.gpu_probe[j] << (2 + j);
+    pats.push_back(p);
+  }
+  std::sort(pats.begin(), pats.end());
+  for (int p : pats) addi(p);
+  return k;
+}
+
 // ------------------------------------------------------------ shared-scan batches
 // Host side of batch.hpp: resolve the queries, lay out the descriptor, the
 // LDS and the result arena, capture / replay the graph.
@@ -2822,38 +2854,6 @@ int mordred_engine_place(mordred_engine* e, const mordred_star_query* q, double 
       for (Column* c : r.fact_cols)
         if ((rc = cache_seg(e, c, s))) return rc;
   return 0;
-}
-
-// Key of a query and the residency patterns of its partly cached groups
-// (MORDRED_ORDER_AUTO keeps one choice per key).
-std::string order_signature(const Resolved& r, const std::vector<MixedGroup>& mixed) {
-  const mordred_star_query& q = *r.q;
-  std::string k;
-  auto add = [&](const char* s) {
-    k += s ? s : "-";
-    k += '|';
-  };
-  auto addi = [&](long long v) {
-    k += std::to_string(v);
-    k += '|';
-  };
-  addi(q.kind), addi(q.total_val), add(q.aggr1), add(q.aggr2);
-  for (int i = 0; i < q.nfilters; i++) add(q.filters[i].col), addi(q.filters[i].lo), addi(q.filters[i].hi);
-  for (int i = 0; i < q.njoins; i++) {
-    const mordred_join& j = q.joins[i];
-    addi(j.slot), add(j.fact_key), add(j.dim_key), add(j.filter_col), addi(j.filter_mode), addi(j.c1), addi(j.c2),
-        add(j.group_col), addi(j.min_val), addi(j.unique_val);
-  }
-  std::vector<int> pats;
-  for (const MixedGroup& m : mixed) {
-    int p = 0;
-    for (int f = 0; f < 2; f++) p |= m.gpu_filter[f] << f;
-    for (int j = 0; j < 4; j++) p |= m.gpu_probe[j] << (2 + j);
-    pats.push_back(p);
-  }
-  std::sort(pats.begin(), pats.end());
-  for (int p : pats) addi(p);
-  return k;
 }
 
 int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int placement, int* res,
