@@ -753,10 +753,10 @@ __global__ __launch_bounds__(BLOCK) void build_kernel(Build B) {
 #define MORDRED_VPT 2
 #endif
 #ifndef MORDRED_AGGR_VPT
-#define MORDRED_AGGR_VPT 2
+#define MORDRED_AGGR_VPT 1
 #endif
 #ifndef MORDRED_AGGR_BLOCK
-#define MORDRED_AGGR_BLOCK 512
+#define MORDRED_AGGR_BLOCK 1024
 #endif
 #ifndef MORDRED_LDS_BLOCK
 #define MORDRED_LDS_BLOCK 1024

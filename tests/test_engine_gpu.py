@@ -243,7 +243,7 @@ def test_mixed_filter_then_probe(cuda):
 def test_prefix_streaming_cost_split(fresh_engine):
     """COST_SPLIT over an uncached table: when the analytic model prices it
     below whole-segment streaming, the GPU side of the split streams only the
-    columns of the fact filters and selective probes, runs those operators,
+    columns of the operator subset it times fastest, runs those operators,
     and the CPU operators finish the survivors."""
     q, data, eng, ref = fresh_engine
     print(f"[gpu] prefix streaming {q.name}", flush=True)
@@ -257,10 +257,12 @@ def test_prefix_streaming_cost_split(fresh_engine):
     # one plan or the other for the GPU side of the split
     assert st["stream_segments"] == 0 or st["prefix_segments"] == 0
     if len(q.joins) > 1:
-        # the part and supplier probes keep a small fraction of the rows
+        # the part and supplier probes keep a small fraction of the rows (the
+        # model picks the subset from its measured per-line and build costs,
+        # so which of them it streams depends on the runs before)
         assert 0 < st["est_prefix_ms_per_segment"] < st["est_stream_ms_per_segment"]
         assert st["prefix_segments"] > 0
-        assert 0 < st["prefix_survivors"] < st["prefix_rows"] // 10
+        assert 0 < st["prefix_survivors"] < st["prefix_rows"] // 2
     eng.set_prefix_streaming(False)
     res, st = eng.run(q, E.PLACE_COST_SPLIT)
     eng.set_prefix_streaming(True)
