@@ -718,8 +718,21 @@ struct MixedFeed {
   }
 };
 
+// A host array allocated without initialisation (cpu_build clears it in
+// parallel).
+struct HostInts {
+  std::unique_ptr<int[]> p;
+  size_t n = 0;
+  void alloc(size_t count) {
+    p.reset(new int[count]);
+    n = count;
+  }
+  int* data() const { return p.get(); }
+  int& operator[](size_t i) const { return p[i]; }
+};
+
 struct CpuTables {
-  std::vector<int> ht[4];
+  HostInts ht[4];
   // one bit per slot, set when the slot holds a row: a probe that misses
   // reads only this (len/8 bytes, which stays in a core's cache) and not the
   // 8-byte slot of a table that may not
@@ -752,42 +765,62 @@ inline bool dim_pass(const mordred_join& j, int x) {
 }
 
 void cpu_build(mordred_engine* e, const Resolved& r, CpuTables& t) {
+  // three passes, each one task list over all the query's tables so that
+  // small tables do not leave workers idle: clear the slots, insert the
+  // passing dimension rows, derive the slot bitmaps
+  struct Task {
+    int k;
+    long long b, en;
+  };
+  constexpr long long kClear = 1 << 20;  // ints per clearing task
+  std::vector<Task> clear, ins, bmp;
   for (int k = 0; k < 4; k++) {
     if (!r.join[k]) continue;
+    const long long len = r.join[k]->dim_len;
+    t.ht[k].alloc(2 * (size_t)len);
+    t.bits[k].assign(((size_t)len + 63) / 64, 0);
+    for (long long b = 0; b < 2 * len; b += kClear) clear.push_back({k, b, std::min(2 * len, b + kClear)});
+    const long long n = r.dkey[k]->n;
+    for (long long b = 0; b < n; b += kCpuChunk) ins.push_back({k, b, std::min(n, b + kCpuChunk)});
+    const long long nw = (long long)t.bits[k].size();
+    for (long long b = 0; b < nw; b += 4096) bmp.push_back({k, b, std::min(nw, b + 4096)});
+  }
+  e->pool->run((int)clear.size(), [&](int i, int) {
+    std::memset(t.ht[clear[i].k].data() + clear[i].b, 0, (size_t)(clear[i].en - clear[i].b) * sizeof(int));
+  });
+  e->pool->run((int)ins.size(), [&](int i, int) {
+    const int k = ins[i].k;
     const mordred_join& j = *r.join[k];
-    t.ht[k].assign(2 * (size_t)j.dim_len, 0);
     int* ht = t.ht[k].data();
     const Column* key = r.dkey[k];
     const Column* fc = r.dfilt[k];
     const Column* vc = r.dval[k];
-    const long long n = key->n;
-    const int tasks = (int)((n + kCpuChunk - 1) / kCpuChunk);
-    e->pool->run(tasks, [&](int task, int) {
-      const long long b = (long long)task * kCpuChunk, en = std::min(n, b + kCpuChunk);
-      for (long long row = b; row < en; row++) {
-        if (fc && !dim_pass(j, fc->host[row])) continue;
-        const int h = (key->host[row] - j.min_key) % j.dim_len;
-        if (vc) ht[2 * (size_t)h] = vc->host[row];
-        ht[2 * (size_t)h + 1] = (int)row + 1;
-      }
-    });
-    const size_t nw = ((size_t)j.dim_len + 63) / 64;
-    t.bits[k].assign(nw, 0);
+    for (long long row = ins[i].b; row < ins[i].en; row++) {
+      if (fc && !dim_pass(j, fc->host[row])) continue;
+      const int h = (key->host[row] - j.min_key) % j.dim_len;
+      if (vc) ht[2 * (size_t)h] = vc->host[row];
+      ht[2 * (size_t)h + 1] = (int)row + 1;
+    }
+  });
+  e->pool->run((int)bmp.size(), [&](int i, int) {
+    const int k = bmp[i].k;
+    const size_t len = (size_t)r.join[k]->dim_len;
+    const int* ht = t.ht[k].data();
     uint64_t* bw = t.bits[k].data();
-    const int wtasks = (int)((nw + 4095) / 4096);
-    e->pool->run(wtasks, [&](int task, int) {
-      for (size_t w = (size_t)task * 4096; w < std::min(nw, (size_t)(task + 1) * 4096); w++) {
-        uint64_t b = 0;
-        for (int i = 0; i < 64; i++) {
-          const size_t h = w * 64 + i;
-          if (h < (size_t)j.dim_len && (ht[2 * h] | ht[2 * h + 1]) != 0) b |= 1ull << i;
-        }
-        bw[w] = b;
+    for (long long w = bmp[i].b; w < bmp[i].en; w++) {
+      uint64_t b = 0;
+      for (int x = 0; x < 64; x++) {
+        const size_t h = (size_t)w * 64 + x;
+        if (h < len && (ht[2 * h] | ht[2 * h + 1]) != 0) b |= 1ull << x;
       }
-    });
+      bw[w] = b;
+    }
+  });
+  for (int k = 0; k < 4; k++) {
+    if (!r.join[k]) continue;
     long long set = 0;
-    for (size_t w = 0; w < nw; w++) set += __builtin_popcountll(bw[w]);
-    t.frac[k] = j.dim_len > 0 ? (double)set / (double)j.dim_len : 1.0;
+    for (uint64_t w : t.bits[k]) set += __builtin_popcountll(w);
+    t.frac[k] = r.join[k]->dim_len > 0 ? (double)set / (double)r.join[k]->dim_len : 1.0;
   }
 }
 
