@@ -353,14 +353,23 @@ static bool groupby_inputs_ok(const Pipe& P) {
 
 static constexpr int kGroupBySink = -1;
 
+#ifndef MORDRED_BUILD_BLOCK
+#define MORDRED_BUILD_BLOCK 256
+#endif
+#ifndef MORDRED_BUILD_VPT
+#define MORDRED_BUILD_VPT 1
+#endif
+constexpr int kBuildBlock = MORDRED_BUILD_BLOCK;
+constexpr int kBuildVpt = MORDRED_BUILD_VPT;
+
 template <int MODE>
 static int run_build(const Build& B, hipStream_t stream) {
   if (B.num_tuples == 0) return hipSuccess;
-  constexpr int TILE = kBlock * 4 * kVpt;
-  const void* f = reinterpret_cast<const void*>(&build_kernel<kBlock, kVpt, MODE>);
+  constexpr int TILE = kBuildBlock * 4 * kBuildVpt;
+  const void* f = reinterpret_cast<const void*>(&build_kernel<kBuildBlock, kBuildVpt, MODE>);
   const int ntiles = (B.num_tuples + TILE - 1) / TILE;
-  const int grid = balanced_grid(ntiles, resident_groups(f, kBlock, 0));
-  build_kernel<kBlock, kVpt, MODE><<<grid, kBlock, 0, stream>>>(B);
+  const int grid = balanced_grid(ntiles, resident_groups(f, kBuildBlock, 0));
+  build_kernel<kBuildBlock, kBuildVpt, MODE><<<grid, kBuildBlock, 0, stream>>>(B);
   return hipGetLastError();
 }
 
