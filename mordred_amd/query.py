@@ -234,15 +234,16 @@ class GpuQuery:
         self.q, self.data, self.cache = query, data, cache
         dev = cache.device
         self.stream = stream
-        # the hash tables share one buffer (16-byte aligned parts), so one fill
-        # clears them all
+        # the hash tables and the result share one buffer (16-byte aligned
+        # parts), so one fill clears them all
         offs, total = {}, 0
         for j in query.joins:
             offs[j.slot] = total
             total += (2 * j.dim_len + 3) // 4 * 4
-        self._htbuf = torch.zeros(max(1, total), dtype=torch.int32, device=dev)
+        nres = max(1, query.total_val) * 6
+        self._htbuf = torch.zeros(total + nres, dtype=torch.int32, device=dev)
         self.hts = {j.slot: self._htbuf[offs[j.slot]:offs[j.slot] + 2 * j.dim_len] for j in query.joins}
-        self.res = torch.zeros(max(1, query.total_val) * 6, dtype=torch.int32, device=dev)
+        self.res = self._htbuf[total:total + nres]
         self.dim_sg = {j.slot: seg_group(len(data.tables[j.table][j.dim_key]), dev) for j in query.joins}
         self.fact_sg = seg_group(data.n, dev)
         p = abi.ProbeArgsGPU()
@@ -285,7 +286,6 @@ class GpuQuery:
 
     def reset(self) -> None:
         self._htbuf.zero_()
-        self.res.zero_()
 
     def build(self) -> None:
         s = self._stream()
