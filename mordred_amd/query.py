@@ -234,16 +234,20 @@ class GpuQuery:
         self.q, self.data, self.cache = query, data, cache
         dev = cache.device
         self.stream = stream
-        # the hash tables and the result share one buffer (16-byte aligned
-        # parts), so one fill clears them all
-        offs, total = {}, 0
+        # the result and the hash tables share one buffer, so one fill clears
+        # them all; each part starts on a 512-byte boundary, as a separate
+        # allocation would (the result first: its atomics then share no cache
+        # line with a hash table, which made the Q1.1 kernel 20% slower)
+        def pad(n: int) -> int:
+            return (n + 127) // 128 * 128
+        nres = max(1, query.total_val) * 6
+        offs, total = {}, pad(nres)
         for j in query.joins:
             offs[j.slot] = total
-            total += (2 * j.dim_len + 3) // 4 * 4
-        nres = max(1, query.total_val) * 6
-        self._htbuf = torch.zeros(total + nres, dtype=torch.int32, device=dev)
+            total += pad(2 * j.dim_len)
+        self._htbuf = torch.zeros(total, dtype=torch.int32, device=dev)
         self.hts = {j.slot: self._htbuf[offs[j.slot]:offs[j.slot] + 2 * j.dim_len] for j in query.joins}
-        self.res = self._htbuf[total:total + nres]
+        self.res = self._htbuf[:nres]
         self.dim_sg = {j.slot: seg_group(len(data.tables[j.table][j.dim_key]), dev) for j in query.joins}
         self.fact_sg = seg_group(data.n, dev)
         p = abi.ProbeArgsGPU()
