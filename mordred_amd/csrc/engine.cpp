@@ -398,6 +398,12 @@ int resolve(mordred_engine* e, const mordred_star_query* q, Resolved& r) {
   for (int i = 0; i < q->njoins; i++) {
     const mordred_join& j = q->joins[i];
     if (j.slot < 1 || j.slot > 4 || r.join[j.slot - 1]) return fail(e, "join slots must be distinct in 1..4");
+    // The aggregate operators probe only table 4 (filter_probe_aggr_GPU2,
+    // GPUProcessing.h:1675-1776; filter_probe_aggr_CPU, CPUProcessing.cu:
+    // 1628-1710): another join would be applied by some placements and not
+    // by others, so such a query has no placement-independent answer.
+    if (q->kind == MORDRED_KIND_AGGR && j.slot != 4)
+      return fail(e, "aggregate queries join only table 4 (the date table)");
     const int k = j.slot - 1;
     r.join[k] = &j;
     if (!(r.key[k] = col(e, j.fact_key)) || !(r.dkey[k] = col(e, j.dim_key)))

@@ -110,11 +110,20 @@ static inline int mode_filter2(const oracle_filterArgs* f, int row, int flag, in
 
 /* ---------------------------------------------------------------- probes */
 
-/* Probe returning the dimension row (slot high word - 1): CPUProcessing.cu:203-207,
- * BlockLibrary.cuh:24-30.  Returns 0 on a miss. */
+/* Probe returning the dimension row (slot high word - 1).  Returns 0 on a miss.
+ * The two code bases test different words of the slot:
+ *   CPU  CPUProcessing.cu:203-207   the whole 64-bit slot != 0
+ *   GPU  BlockLibrary.cuh:24-30     the row word ht[2h+1] != 0 only
+ * They agree whenever every build writes the row word (all builds do). */
 static inline int probe_row(const int* key_col, const int* ht, int len, int min_key, int row,
-                            int* dim_row) {
+                            int* dim_row, int conv) {
   int h = hash_slot(key_col[row], len, min_key);
+  if (conv == ORACLE_CONV_GPU) {
+    int r = ht[2 * h + 1];
+    if (r == 0) return 0;
+    *dim_row = r - 1;
+    return 1;
+  }
   long long s = slot64(ht, h);
   if (s == 0) return 0;
   *dim_row = (int)((unsigned long long)s >> 32) - 1;
@@ -280,7 +289,7 @@ static int probe_rows(const oracle_probeArgs* p, const oracle_offset* in, int po
   for (int k = 0; k < 4; k++) {
     d[k] = conv == ORACLE_CONV_GPU ? 1 : 0;
     if (keys[k] != NULL && hts[k] != NULL) {
-      if (!probe_row(keys[k], hts[k], lens[k], mins[k], row, &d[k])) return 0;
+      if (!probe_row(keys[k], hts[k], lens[k], mins[k], row, &d[k], conv)) return 0;
     } else if (doffs[k] != NULL) {
       d[k] = doffs[k][pos];
     }
@@ -340,7 +349,7 @@ void oracle_filter_probe_CPU(oracle_filterArgs fargs, oracle_probeArgs pargs,
       int d4 = conv == ORACLE_CONV_GPU ? 1 : 0;
       if (!fact_filters(&fargs, row, conv)) continue;
       if (conv == ORACLE_CONV_CPU || (pargs.key_col4 != NULL && pargs.ht4 != NULL))
-        if (!probe_row(pargs.key_col4, pargs.ht4, pargs.dim_len4, pargs.min_key4, row, &d4)) continue;
+        if (!probe_row(pargs.key_col4, pargs.ht4, pargs.dim_len4, pargs.min_key4, row, &d4, conv)) continue;
       buf0[count] = row; buf4[count] = d4;
       count++;
     }
@@ -364,7 +373,7 @@ void oracle_filter_probe_CPU2(oracle_offset in_off, oracle_filterArgs fargs,
       int d4 = conv == ORACLE_CONV_GPU ? 1 : 0;
       if (!fact_filters(&fargs, row, conv)) continue;
       if (conv == ORACLE_CONV_CPU || (pargs.key_col4 != NULL && pargs.ht4 != NULL)) {
-        if (!probe_row(pargs.key_col4, pargs.ht4, pargs.dim_len4, pargs.min_key4, row, &d4)) continue;
+        if (!probe_row(pargs.key_col4, pargs.ht4, pargs.dim_len4, pargs.min_key4, row, &d4, conv)) continue;
       } else if (in_off.h_dim_off4 != NULL) {
         d4 = in_off.h_dim_off4[pos];
       }

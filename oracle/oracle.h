@@ -37,6 +37,9 @@
  *                                              CPUProcessing.cu:36-38)
  *   - build value slot: GPU compare-and-swap (first writer wins), CPU plain
  *     store (last writer wins); identical for unique dimension keys.
+ *   - row probes (dimension row out): GPU tests the row word ht[2h+1] only,
+ *     CPU the whole 64-bit slot            (BlockLibrary.cuh:24-30 vs
+ *                                              CPUProcessing.cu:203-207)
  */
 #ifndef MORDRED_ORACLE_H
 #define MORDRED_ORACLE_H

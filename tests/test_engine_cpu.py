@@ -95,3 +95,26 @@ def test_binding_struct_sizes_match_library():
     for name, cls in pairs.items():
         assert lib.mordred_engine_struct_size(name.encode()) == ctypes.sizeof(cls), name
     assert lib.mordred_engine_struct_size(b"no_such_struct") == 0
+
+
+def test_aggregate_query_joins_only_the_date_table():
+    """The aggregate operators probe only table 4 (filter_probe_aggr_GPU2,
+    filter_probe_aggr_CPU), so an aggregate query with another join would give
+    placement-dependent answers; the engine refuses it (resolve() runs before
+    any placement; tests/test_engine_gpu.py covers the GPU placements).  The
+    date-only query still equals the oracle."""
+    import copy
+    from mordred_amd import query as Q
+    q = Q.q11(1994)
+    data = Q.make_data(q, SEG + 999, seed=8)
+    bad = copy.deepcopy(q)
+    data.tables["supplier"] = __import__("mordred_amd").datagen.make_supplier(500, 8)
+    data.fact["lo_suppkey"] = (np.arange(data.n) % 500 + 1).astype(np.int32)
+    bad.joins = [Q.DimJoin(1, "supplier", "lo_suppkey", "s_suppkey", 500, 0, filter_col="s_region", c1=1, c2=1)] + bad.joins
+    eng = E.Engine(device=-1, threads=2)
+    eng.add_data(data, bad)
+    with pytest.raises(E.EngineError, match="aggregate queries join only table 4"):
+        eng.run(bad, E.PLACE_CPU_ONLY)
+    res, _ = eng.run(q, E.PLACE_CPU_ONLY)
+    assert oracle.res_scalar(res) == oracle.res_scalar(OQ.run_query(q, data, oracle.CONV_CPU))
+    eng.close()

@@ -330,3 +330,27 @@ def test_operator_order_auto(fresh_engine):
         eng.set_operator_order(E.ORDER_GPU_FIRST)
     assert seen[:3] == [E.ORDER_GPU_FIRST, E.ORDER_CPU_FIRST, E.ORDER_ALTERNATE]
     assert seen[3] in seen[:3]
+
+
+def test_aggregate_query_with_other_join_rejected_on_gpu(cuda):
+    """An aggregate query that joins a table other than the date table is
+    refused under every placement (the aggregate operators probe only table 4,
+    so placements would disagree); nothing is launched for it."""
+    import copy
+    from mordred_amd import datagen
+    from mordred_amd import query as Q
+    q = Q.q11(1994)
+    data = Q.make_data(q, SEG + 999, seed=8)
+    data.tables["supplier"] = datagen.make_supplier(500, 8)
+    data.fact["lo_suppkey"] = (np.arange(data.n) % 500 + 1).astype(np.int32)
+    bad = copy.deepcopy(q)
+    bad.joins = [Q.DimJoin(1, "supplier", "lo_suppkey", "s_suppkey", 500, 0, filter_col="s_region", c1=1,
+                           c2=1)] + bad.joins
+    eng = E.Engine(device=0, cache_segments=16, threads=2)
+    eng.add_data(data, bad)
+    for placement in (E.PLACE_RESIDENCY, E.PLACE_COST_SPLIT, E.PLACE_GPU_ONLY, E.PLACE_CPU_ONLY):
+        with pytest.raises(E.EngineError, match="aggregate queries join only table 4"):
+            eng.run(bad, placement)
+    res, _ = eng.run(q, E.PLACE_GPU_ONLY)
+    assert oracle.res_scalar(res) == oracle.res_scalar(OQ.run_query(q, data, oracle.CONV_GPU))
+    eng.close()

@@ -559,3 +559,113 @@ def test_group_keys_decoded_or_stored(cuda, case):
     if case == "brands_past_range":
         # both kinds of group are present
         assert (groups[touched, 2] >= 1000).any() and (groups[touched, 2] < 1000).any()
+
+
+# ------------------------------------- offset-mode kernels with fact filters
+
+def _fact_filter_args(cache, spec):
+    """filterArgsGPU of up to two fact BETWEEN filters [(col or None, lo, hi)]."""
+    from mordred_amd import abi
+    fa = abi.FilterArgsGPU()
+    (c1, l1, h1), (c2, l2, h2) = spec
+    fa.filter_idx1 = cache.idx(c1) if c1 else None
+    fa.filter_idx2 = cache.idx(c2) if c2 else None
+    fa.compare1, fa.compare2, fa.compare3, fa.compare4 = l1, h1, l2, h2
+    fa.mode1 = fa.mode2 = 1
+    return fa
+
+
+@pytest.mark.parametrize("spec", ["both", "first", "second"])
+def test_filter_probe_group_by_GPU3_with_fact_filters(gstar, spec):
+    """filter_probe_group_by_GPU3 (GPUProcessing.h:157-272) with fact filters
+    (:187-197): the offset list of a first probe_GPU2 (table 1 only), then
+    the BETWEEN filters, the remaining probes and the group-by, word for word
+    against oracle_filter_probe_group_by_CPU2 in the GPU convention fed the
+    same lists.  One filter column, the other, and both."""
+    from mordred_amd import abi
+    q, data, cache = gstar
+    if q.kind != "groupby":
+        pytest.skip("group-by only")
+    print(f"[gpu] k10 filters {q.name} {spec}", flush=True)
+    gq = _gpu_query(q, data, cache)
+    n = data.n
+    f1 = ("lo_revenue", 1_000_000, 6_000_000)
+    f2 = ("lo_orderdate", 19930000, 19951231)
+    none = (None, 0, 0)
+    fs = {"both": (f1, f2), "first": (f1, none), "second": (none, f2)}[spec]
+    fa = _fact_filter_args(cache, fs)
+    first = q.joins[0].slot
+    p1, prest = abi.ProbeArgsGPU(), abi.ProbeArgsGPU()
+    for fld in ("key_idx", "ht", "dim_len", "min_key"):
+        setattr(p1, f"{fld}{first}", getattr(gq.pargs, f"{fld}{first}"))
+        for j in q.joins[1:]:
+            setattr(prest, f"{fld}{j.slot}", getattr(gq.pargs, f"{fld}{j.slot}"))
+    outs1 = [_alloc(n) for _ in range(5)]
+    t1 = _alloc(1)
+    o1 = abi.OffsetGPU(*[t.data_ptr() for t in outs1])
+    abi.call("mordred_probe_GPU2", cache.ptr, C.pointer(p1), C.pointer(o1), n, t1.data_ptr(), 0,
+             gq.fact_sg.data_ptr(), stream())
+    sync()
+    m1 = int(t1.item())
+    res = _alloc(q.total_val * 6)
+    abi.call("mordred_filter_probe_group_by_GPU3", cache.ptr, C.pointer(o1), C.pointer(fa), C.pointer(prest),
+             C.pointer(gq.gargs), m1, res.data_ptr(), stream())
+    sync()
+    # the oracle over the same lists
+    hts = OQ.build_tables(q, data, CONV_GPU)
+    f, p, g, keep = OQ.fact_args(q, data, hts)
+    for k in (first,):
+        setattr(p, f"key_col{k}", None)
+        setattr(p, f"ht{k}", None)
+    rf = FilterArgs()
+    rf.filter_col1 = iptr(data.fact[fs[0][0]]) if fs[0][0] else None
+    rf.filter_col2 = iptr(data.fact[fs[1][0]]) if fs[1][0] else None
+    rf.compare1, rf.compare2, rf.compare3, rf.compare4 = fs[0][1], fs[0][2], fs[1][1], fs[1][2]
+    rf.mode1 = rf.mode2 = 1
+    lo = outs1[0].cpu().numpy()[:m1].copy()
+    dims = [t.cpu().numpy()[:m1].copy() for t in outs1[1:]]
+    ref = np.zeros(q.total_val * 6, dtype=np.int32)
+    lib.oracle_filter_probe_group_by_CPU2(Offset(iptr(lo), *[iptr(d) for d in dims]), rf, p, g, m1, iptr(ref), 0,
+                                          CONV_GPU)
+    got = res.cpu().numpy()
+    assert oracle.res_sums(ref).any()
+    np.testing.assert_array_equal(got, ref)
+    # and the filters really cut rows: the unfiltered offset-mode call differs
+    if spec == "both":
+        assert not np.array_equal(got, oracle_res(q, data))
+
+
+def test_filter_probe_GPU3_permuted_list(gstar):
+    """filter_probe_GPU3 (GPUProcessing.h:602-688) over a permuted half of the
+    row ids against oracle_filter_probe_CPU2 (GPU convention); without a date
+    table it passes the incoming dim_off4 through."""
+    from mordred_amd import abi
+    q, data, cache = gstar
+    if q.kind != "aggr":
+        pytest.skip("Q1.x compaction")
+    gq = _gpu_query(q, data, cache)
+    rows = np.random.default_rng(17).permutation(data.n)[: data.n // 2].astype(np.int32)
+    carried = (np.arange(len(rows), dtype=np.int32) * 5 + 3).astype(np.int32)
+    m_in = len(rows)
+    drows, dcar = dev(rows), dev(carried)
+    hts = OQ.build_tables(q, data, CONV_GPU)
+    f, p, g, keep = OQ.fact_args(q, data, hts)
+    for with_table in (True, False):
+        pa = gq.pargs
+        if not with_table:
+            pa = abi.ProbeArgsGPU()
+            p.key_col4, p.ht4 = None, None
+        lo, d4, total = _alloc(m_in), _alloc(m_in), _alloc(1)
+        abi.call("mordred_filter_probe_GPU3", cache.ptr,
+                 C.pointer(abi.OffsetGPU(drows.data_ptr(), None, None, None, dcar.data_ptr())),
+                 abi.ref(gq.fargs), C.pointer(pa), C.pointer(abi.OffsetGPU(lo.data_ptr(), None, None, None,
+                                                                           d4.data_ptr())),
+                 m_in, total.data_ptr(), stream())
+        sync()
+        m = int(total.item())
+        rlo, rd4, rt = np.zeros(m_in, dtype=np.int32), np.zeros(m_in, dtype=np.int32), np.zeros(1, dtype=np.int32)
+        lib.oracle_filter_probe_CPU2(Offset(iptr(rows), None, None, None, iptr(carried)), f, p,
+                                     Offset(iptr(rlo), None, None, None, iptr(rd4)), m_in, iptr(rt), 0, CONV_GPU)
+        assert m == int(rt[0]) > 0, with_table
+        np.testing.assert_array_equal(_sorted_rows([lo.cpu().numpy(), d4.cpu().numpy()], m),
+                                      _sorted_rows([rlo, rd4], m))

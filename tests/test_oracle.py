@@ -365,3 +365,178 @@ def test_build_minmax():
     lib.oracle_build_CPU_minmax2(iptr(rows), FilterArgs(), BuildArgs(iptr(d), None, 10, 0, 10), 2, iptr(ht),
                                  iptr(mn), iptr(mx), 0, CONV_CPU)
     assert (mn[0], mx[0]) == (1, 7)
+
+
+# ------------------------------------------- offset-mode and remaining pins
+#
+# Each check below states the function's meaning with numpy alone: the
+# dimension row of a fact key is found by sorting the dimension's keys (no
+# hashing), filters are numpy masks, group maps come from the pandas
+# statement.  So a wrong hash, probe word, offset position, filter, carried
+# dimension row or aggregate in the oracle fails one of them.
+
+def _dim_rows(dim_keys, fact_keys):
+    """Row of each fact key in the dimension, or -1 (sort + search, no hash)."""
+    order = np.argsort(dim_keys, kind="stable")
+    sk = dim_keys[order]
+    pos = np.minimum(np.searchsorted(sk, fact_keys), len(sk) - 1)
+    return np.where(sk[pos] == fact_keys, order[pos], -1)
+
+
+def _pairs(*cols):
+    a = np.stack([np.asarray(c, dtype=np.int64) for c in cols], axis=1)
+    return a[np.lexsort(a.T[::-1])] if len(a) else a
+
+
+@pytest.mark.parametrize("conv", [CONV_CPU, CONV_GPU])
+def test_filter_probe_CPU2_pinned(conv):
+    """c2 filter_probe_CPU2 (CPUProcessing.cu:88-168): over a permuted row
+    list from start_offset on, the rows passing both BETWEEN filters whose
+    orderdate joins the year-filtered date table, with the date row."""
+    q, data = cases.star("q11", SEG + 77_777, seed=31)
+    hts = OQ.build_tables(q, data, conv)
+    f, p, g, keep = OQ.fact_args(q, data, hts)
+    rows = np.random.default_rng(5).permutation(data.n)[: data.n // 2].astype(np.int32)
+    start = 1000
+    n = len(rows) - start
+    lo, d4 = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+    total = np.zeros(1, dtype=np.int32)
+    lib.oracle_filter_probe_CPU2(Offset(iptr(rows), None, None, None, None), f, p,
+                                 Offset(iptr(lo), None, None, None, iptr(d4)), n, iptr(total), start, conv)
+    m = int(total[0])
+    r = rows[start:]
+    disc, qty = data.fact["lo_discount"][r], data.fact["lo_quantity"][r]
+    date = data.tables["date"]
+    drow = _dim_rows(date["d_datekey"], data.fact["lo_orderdate"][r])
+    ok = (disc >= 1) & (disc <= 3) & (qty >= 0) & (qty <= 24) & (drow >= 0)
+    ok &= date["d_year"][np.maximum(drow, 0)] == 1993
+    assert m == int(ok.sum()) > 0
+    np.testing.assert_array_equal(_pairs(lo[:m], d4[:m]), _pairs(r[ok], drow[ok]))
+    if conv == CONV_GPU:
+        # filter_probe_GPU3 (GPUProcessing.h:602-688) without a date table
+        # passes the incoming dim_off4 through
+        carried = np.arange(len(rows), dtype=np.int32) * 3 + 7
+        p.key_col4, p.ht4 = None, None
+        total[0] = 0
+        lib.oracle_filter_probe_CPU2(Offset(iptr(rows), None, None, None, iptr(carried)), f, p,
+                                     Offset(iptr(lo), None, None, None, iptr(d4)), n, iptr(total), start, conv)
+        okf = (disc >= 1) & (disc <= 3) & (qty >= 0) & (qty <= 24)
+        m = int(total[0])
+        assert m == int(okf.sum())
+        np.testing.assert_array_equal(_pairs(lo[:m], d4[:m]), _pairs(r[okf], carried[start:][okf]))
+
+
+def _q4way_filtered():
+    import copy
+    q, data = cases.star("q4way", SEG + 54_321, seed=41)
+    qf = copy.deepcopy(q)
+    qf.fact_filters = [("lo_revenue", 1_000_000, 6_000_000), ("lo_supplycost", 60_000, 100_000)]
+    return qf, data
+
+
+@pytest.mark.parametrize("conv", [CONV_CPU, CONV_GPU])
+def test_filter_probe_group_by_CPU2_pinned(conv):
+    """c8 filter_probe_group_by_CPU2 (CPUProcessing.cu:868-1032): fact
+    BETWEEN filters + 4 probes + group-by over a permuted row list from
+    start_offset on equals the pandas statement over the same rows; with the
+    supplier table not probed, its group value comes from the carried
+    dimension row (dim_off1) instead."""
+    qf, data = _q4way_filtered()
+    hts = OQ.build_tables(qf, data, conv)
+    f, p, g, keep = OQ.fact_args(qf, data, hts)
+    assert f.filter_col1 and f.filter_col2
+    rows = np.random.default_rng(6).permutation(data.n)[: 2 * data.n // 3].astype(np.int32)
+    start = 777
+    res = np.zeros(qf.total_val * 6, dtype=np.int32)
+    lib.oracle_filter_probe_group_by_CPU2(Offset(iptr(rows), None, None, None, None), f, p, g,
+                                          len(rows) - start, iptr(res), start, conv)
+    expect = semantics.group_map(qf, data, rows[start:])
+    assert len(expect) > 100
+    assert semantics.decode_groups(res) == expect
+    # supplier join done earlier: the list holds only rows that pass it, and
+    # dim_off1 carries each row's supplier row
+    supp = data.tables["supplier"]
+    srow = _dim_rows(supp["s_suppkey"], data.fact["lo_suppkey"][rows])
+    sok = (srow >= 0) & (supp["s_region"][np.maximum(srow, 0)] == 1)
+    rows2 = np.ascontiguousarray(rows[sok])
+    doff1 = np.ascontiguousarray(srow[sok].astype(np.int32))
+    p.key_col1, p.ht1 = None, None
+    res[:] = 0
+    lib.oracle_filter_probe_group_by_CPU2(Offset(iptr(rows2), iptr(doff1), None, None, None), f, p, g,
+                                          len(rows2), iptr(res), 0, conv)
+    assert semantics.decode_groups(res) == semantics.group_map(qf, data, rows2)
+
+
+@pytest.mark.parametrize("conv", [CONV_CPU, CONV_GPU])
+@pytest.mark.parametrize("with_val", [True, False])
+@pytest.mark.parametrize("dim_len,min_key", [(60_000, 1), (50_000, 0)])
+def test_build_CPU2_pinned(conv, with_val, dim_len, min_key):
+    """c10 build_CPU2 (CPUProcessing.cu:1106-1155): from a permuted dimension
+    row list (from start_offset on), slot (key-min) % len holds [value, row+1]
+    for each listed row and 0 elsewhere.  With min_key 0 and len = |dim|
+    (the engine's dense keys) key |dim| wraps onto slot 0."""
+    from mordred_amd import datagen
+    part = datagen.make_part(50_000, 9)
+    keys, vals = part["p_partkey"], part["p_brand1"]
+    rows = np.random.default_rng(7).permutation(50_000)[:20_000].astype(np.int32)
+    start = 123
+    if min_key == 0:
+        rows = rows[rows != 49_999]
+        rows[start] = 49_999      # key 50,000 -> slot 0
+    ht = np.zeros(2 * dim_len, dtype=np.int32)
+    lib.oracle_build_CPU2(iptr(rows), FilterArgs(),
+                          BuildArgs(iptr(keys), iptr(vals) if with_val else None, dim_len, min_key, 0),
+                          len(rows) - start, iptr(ht), start, conv)
+    expect = np.zeros(2 * dim_len, dtype=np.int64)
+    r = rows[start:].astype(np.int64)
+    h = (keys[r].astype(np.int64) - min_key) % dim_len
+    expect[2 * h + 1] = r + 1
+    if with_val:
+        expect[2 * h] = vals[r]
+    np.testing.assert_array_equal(ht, expect)
+    if min_key == 0:
+        assert ht[1] == 50_000
+
+
+def test_probe_aggr_CPU_pinned():
+    """c15 probe_aggr_CPU (CPUProcessing.cu:1491-1558): sum of int64(a1*a2)
+    over the rows of a permuted segment group whose orderdate joins the
+    year-filtered date table; no fact filter."""
+    q, data = cases.star("q11", 2 * SEG + 123_457, seed=43)
+    hts = OQ.build_tables(q, data, CONV_CPU)
+    f, p, g, keep = OQ.fact_args(q, data, hts)
+    res = np.zeros(6, dtype=np.int32)
+    lib.oracle_probe_aggr_CPU(p, g, data.n, iptr(res), 0, sptr(OQ.segment_group(data.n, [1, 0, 2])))
+    date = data.tables["date"]
+    drow = _dim_rows(date["d_datekey"], data.fact["lo_orderdate"])
+    ok = (drow >= 0) & (date["d_year"][np.maximum(drow, 0)] == 1993)
+    a = data.fact["lo_extendedprice"].astype(np.int64) * data.fact["lo_discount"].astype(np.int64)
+    assert oracle.res_scalar(res) == int(a[ok].sum()) > 0
+    # one segment only: rows 2^20 .. 2^21
+    res[:] = 0
+    lib.oracle_probe_aggr_CPU(p, g, SEG, iptr(res), 0, sptr(OQ.segment_group(SEG, [1])))
+    seg1 = np.zeros(data.n, dtype=bool)
+    seg1[SEG:2 * SEG] = True
+    assert oracle.res_scalar(res) == int(a[ok & seg1].sum())
+
+
+def test_row_probe_word_by_convention():
+    """Row probes: the GPU kernels test the row word ht[2h+1] only
+    (BlockProbeGPU, BlockLibrary.cuh:24-30), the CPU path the whole 64-bit
+    slot (CPUProcessing.cu:203-207).  A slot with a value word but no row word
+    hits on the CPU (row -1) and misses on the GPU."""
+    ht = np.zeros(8, dtype=np.int32)
+    ht[0:2] = [5, 0]          # slot 0: value only
+    ht[2:4] = [0, 3]          # slot 1: row word only -> row 2
+    keys = np.array([0, 1, 2], dtype=np.int32)
+    p = ProbeArgs()
+    p.key_col4, p.ht4, p.dim_len4, p.min_key4 = iptr(keys), iptr(ht), 4, 0
+    got = {}
+    for conv in (CONV_CPU, CONV_GPU):
+        lo, d4 = np.zeros(3, dtype=np.int32), np.zeros(3, dtype=np.int32)
+        t = np.zeros(1, dtype=np.int32)
+        lib.oracle_probe_CPU(p, Offset(iptr(lo), None, None, None, iptr(d4)), 3, iptr(t), 0,
+                             sptr(OQ.segment_group(3)), conv)
+        got[conv] = sorted(zip(lo[: t[0]].tolist(), d4[: t[0]].tolist()))
+    assert got[CONV_CPU] == [(0, -1), (1, 2)]
+    assert got[CONV_GPU] == [(1, 2)]
