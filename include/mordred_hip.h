@@ -36,7 +36,8 @@
  *    return value is a hipError_t code: 0 on success, hipErrorInvalidValue (1)
  *    for arguments the kernels cannot honour (checked before any launch),
  *    otherwise the launch error.  Nothing here calls exit().
- *  - Entry points are re-entrant and keep no hidden global state.
+ *  - Entry points are re-entrant and keep no hidden global state, except the
+ *    fixed scratch array of the no-alloc entry point (declared below).
  */
 #ifndef MORDRED_HIP_H
 #define MORDRED_HIP_H
@@ -95,6 +96,21 @@ typedef struct offsetGPU {
 
 /* filter_probe_group_by_GPU2, src/gpudb/GPUProcessing.h:17-155 */
 int mordred_filter_probe_group_by_GPU2(const int* gpuCache, const filterArgsGPU* fargs,
+    const probeArgsGPU* pargs, const groupbyArgsGPU* gargs, int num_tuples, int* res,
+    int start_offset, const short* segment_group, mordred_stream_t stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops and harnesses that forbid
+ * hipMalloc): mordred_filter_probe_group_by_GPU2 with the same arguments and
+ * results, whose compact probe tables (the per-call pre-pass products) live in
+ * a fixed per-GPU scratch array of MORDRED_NOALLOC_SCRATCH_BYTES, held until
+ * the process ends, instead of stream-ordered pool memory.  It calls no
+ * hipMalloc, hipFree or pool allocation.  A query whose compact tables need
+ * more than the array (about 2.1 bytes per slot of each compacted hash table,
+ * plus 8.5 KiB per table) is refused with hipErrorOutOfMemory (2) before
+ * anything is launched.  Every call on a GPU shares that GPU's array, so calls
+ * that overlap on one GPU (different streams or threads) race. */
+#define MORDRED_NOALLOC_SCRATCH_BYTES (16u << 20)
+int mordred_filter_probe_group_by_GPU2_noalloc(const int* gpuCache, const filterArgsGPU* fargs,
     const probeArgsGPU* pargs, const groupbyArgsGPU* gargs, int num_tuples, int* res,
     int start_offset, const short* segment_group, mordred_stream_t stream);
 /* filter_probe_group_by_GPU3, GPUProcessing.h:157-272 */

@@ -63,6 +63,7 @@ _F, _P, _B, _G, _O = (C.POINTER(FilterArgsGPU), C.POINTER(ProbeArgsGPU), C.POINT
 # name -> argtypes (the trailing VP is the stream)
 SIGNATURES = {
     "mordred_filter_probe_group_by_GPU2": [VP, _F, _P, _G, I, VP, I, VP, VP],
+    "mordred_filter_probe_group_by_GPU2_noalloc": [VP, _F, _P, _G, I, VP, I, VP, VP],
     "mordred_filter_probe_group_by_GPU3": [VP, _O, _F, _P, _G, I, VP, VP],
     "mordred_probe_group_by_GPU2": [VP, _P, _G, I, VP, I, VP, VP],
     "mordred_probe_group_by_GPU3": [VP, _O, _P, _G, I, VP, VP],

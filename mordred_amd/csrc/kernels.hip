@@ -179,6 +179,62 @@ static int run_pipe(const Pipe& P, int sink, hipStream_t stream) {
   }
 }
 
+// Byte layout of the compact probe tables of `plan` in one scratch block:
+// per compacted table a 256-byte header, the stats partials, the packed words
+// at the widest width, and (large tables) the existence bitmap.
+struct CompactLayout {
+  size_t off[4];
+  bool bits[4];
+  size_t bytes;
+};
+
+static CompactLayout compact_layout(const Pipe& P, const ProbePlan& plan) {
+  CompactLayout L{};
+  for (int k = 0; k < 4; k++) {
+    if (!(plan.compact >> k & 1)) continue;
+    L.off[k] = L.bytes;
+    L.bits[k] = P.len[k] >= kCompactMinLen;
+    L.bytes += 256 + kStatsGroups * sizeof(int4) + ((compact_words_bytes(P.len[k]) + 255) & ~(size_t)255);
+    if (L.bits[k]) L.bytes += (compact_bits_bytes(P.len[k]) + 255) & ~(size_t)255;
+  }
+  return L;
+}
+
+// Launch the two pre-pass kernels that derive the compact tables of `plan`
+// into `base` (laid out by L) and point the pipe at them.  Returns the launch
+// status; on failure the pipe probes the hash tables themselves.
+static hipError_t launch_prep(Pipe& P, const ProbePlan& plan, const CompactLayout& L, char* base, int sink,
+                              hipStream_t stream) {
+  Prep prep{};
+  prep.row_probe = sink == COMPACT;
+  int nt = 0;
+  for (int k = 0; k < 4; k++) {
+    if (!(plan.compact >> k & 1)) continue;
+    char* t = base + L.off[k];
+    prep.ht[nt] = P.ht[k];
+    prep.len[nt] = P.len[k];
+    prep.hdr[nt] = reinterpret_cast<CompactHdr*>(t);
+    prep.partial[nt] = reinterpret_cast<int4*>(t + 256);
+    prep.words[nt] = reinterpret_cast<uint32_t*>(t + 256 + kStatsGroups * sizeof(int4));
+    prep.bits[nt] = L.bits[k] ? reinterpret_cast<uint32_t*>(t + 256 + kStatsGroups * sizeof(int4) +
+                                                            ((compact_words_bytes(P.len[k]) + 255) & ~(size_t)255))
+                              : nullptr;
+    P.chdr[k] = prep.hdr[nt];
+    P.cwords[k] = prep.words[nt];
+    P.cbits[k] = prep.bits[nt];
+    nt++;
+  }
+  prep_stats_kernel<<<dim3(kStatsGroups, nt), kPrepBlock, 0, stream>>>(prep);
+  int maxlen = 0;
+  for (int t = 0; t < nt; t++) maxlen = std::max(maxlen, prep.len[t]);
+  const int pack_groups = std::max(1, std::min(kPackGroups, (maxlen + kPrepBlock - 1) / kPrepBlock));
+  prep_pack_kernel<<<dim3(pack_groups, nt), kPrepBlock, 0, stream>>>(prep);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    for (int k = 0; k < 4; k++) { P.chdr[k] = nullptr; P.cwords[k] = nullptr; P.cbits[k] = nullptr; }
+  return e;
+}
+
 // Build the compact probe tables of `plan` into pool scratch (stream-ordered).
 // Returns the scratch block (to free after the fact kernel) or nullptr, in
 // which case the fact kernel probes the hash tables directly.
@@ -188,52 +244,50 @@ static void* build_compact(Pipe& P, const ProbePlan& plan, int sink, hipStream_t
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   hipMemPool_t pool = device_pool(dev);
   if (!pool) return nullptr;
-  size_t off[4] = {0, 0, 0, 0}, bytes = 0;
-  bool with_bits[4] = {false, false, false, false};
-  for (int k = 0; k < 4; k++) {
-    if (!(plan.compact >> k & 1)) continue;
-    off[k] = bytes;
-    with_bits[k] = P.len[k] >= kCompactMinLen;
-    bytes += 256 + kStatsGroups * sizeof(int4) + ((compact_words_bytes(P.len[k]) + 255) & ~(size_t)255);
-    if (with_bits[k]) bytes += (compact_bits_bytes(P.len[k]) + 255) & ~(size_t)255;
-  }
+  const CompactLayout L = compact_layout(P, plan);
   void* scratch = nullptr;
-  if (hipMallocFromPoolAsync(&scratch, bytes, pool, stream) != hipSuccess) {
+  if (hipMallocFromPoolAsync(&scratch, L.bytes, pool, stream) != hipSuccess) {
     (void)hipGetLastError();
     return nullptr;
   }
-  Prep prep{};
-  prep.row_probe = sink == COMPACT;
-  int nt = 0, tab_of[4];
-  for (int k = 0; k < 4; k++) {
-    if (!(plan.compact >> k & 1)) continue;
-    char* base = static_cast<char*>(scratch) + off[k];
-    prep.ht[nt] = P.ht[k];
-    prep.len[nt] = P.len[k];
-    prep.hdr[nt] = reinterpret_cast<CompactHdr*>(base);
-    prep.partial[nt] = reinterpret_cast<int4*>(base + 256);
-    prep.words[nt] = reinterpret_cast<uint32_t*>(base + 256 + kStatsGroups * sizeof(int4));
-    prep.bits[nt] = with_bits[k] ? reinterpret_cast<uint32_t*>(
-                                       base + 256 + kStatsGroups * sizeof(int4) +
-                                       ((compact_words_bytes(P.len[k]) + 255) & ~(size_t)255))
-                                 : nullptr;
-    P.chdr[k] = prep.hdr[nt];
-    P.cwords[k] = prep.words[nt];
-    P.cbits[k] = prep.bits[nt];
-    tab_of[nt++] = k;
-  }
-  (void)tab_of;
-  prep_stats_kernel<<<dim3(kStatsGroups, nt), kPrepBlock, 0, stream>>>(prep);
-  int maxlen = 0;
-  for (int t = 0; t < nt; t++) maxlen = std::max(maxlen, prep.len[t]);
-  const int pack_groups = std::max(1, std::min(kPackGroups, (maxlen + kPrepBlock - 1) / kPrepBlock));
-  prep_pack_kernel<<<dim3(pack_groups, nt), kPrepBlock, 0, stream>>>(prep);
-  if (hipGetLastError() != hipSuccess) {
+  if (launch_prep(P, plan, L, static_cast<char*>(scratch), sink, stream) != hipSuccess) {
     (void)hipFreeAsync(scratch, stream);
-    for (int k = 0; k < 4; k++) { P.chdr[k] = nullptr; P.cwords[k] = nullptr; P.cbits[k] = nullptr; }
     return nullptr;
   }
   return scratch;
+}
+
+// ------------------------------------------------- fixed scratch (no-alloc)
+// Device memory of the no-alloc entry point: one array per GPU, of a fixed
+// size, held until the process ends.  In host code `scratch` is a host-side
+// stand-in, not a device address: host code uses only the per-device address
+// that hipGetSymbolAddress returns (taken once per device, with that device
+// current).  Every call on a GPU shares that GPU's array; the pre-pass
+// kernels rewrite every byte a call reads (headers, partials, packed words,
+// bitmaps), so nothing needs clearing between calls, and calls that overlap
+// on one GPU race.
+static __device__ __align__(256) unsigned char scratch[MORDRED_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+
+// Address of the current device's scratch array, or nullptr (err set).
+static char* device_scratch(hipError_t& err) {
+  static std::mutex mu;
+  static void* ptrs[kMaxDevices] = {};
+  int dev = 0;
+  if ((err = hipGetDevice(&dev)) != hipSuccess) return nullptr;
+  if (dev < 0 || dev >= kMaxDevices) {
+    err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(mu);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    if ((err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return nullptr;
+    ptrs[dev] = p;
+  }
+  err = hipSuccess;
+  return static_cast<char*>(ptrs[dev]);
 }
 
 // The group index sum_k (v_k - minv_k) * uniq_k is a mixed-radix number when
@@ -260,7 +314,11 @@ static void set_key_decode(Pipe& P) {
 }
 
 // sink < 0: group-by, placed in LDS when the dense array fits.
-static int dispatch(Pipe P, int mode, int sink, hipStream_t stream) {
+// fixed != nullptr: the compact probe tables go into that block of `cap`
+// bytes (the no-alloc entry point); a plan that needs more is refused with
+// hipErrorOutOfMemory before anything is launched.  Otherwise they go into
+// stream-ordered pool memory, or are skipped if none is available.
+static int dispatch(Pipe P, int mode, int sink, hipStream_t stream, char* fixed = nullptr, size_t cap = 0) {
   if (P.num_tuples == 0) return hipSuccess;
   if (sink < 0) set_key_decode(P);
   if (sink < 0) {
@@ -269,8 +327,21 @@ static int dispatch(Pipe P, int mode, int sink, hipStream_t stream) {
     sink = group_lds_bytes(GROUPBY_LDS, P.total_val) + ring <= kLdsBudgetOneGroup ? GROUPBY_LDS : GROUPBY_GLOBAL;
   }
   const ProbePlan plan = plan_probes(P, mode, sink);
-  void* scratch = build_compact(P, plan, sink, stream);
-  if (scratch && plan.stage) {
+  void* pooled = nullptr;
+  bool compacted = false;
+  if (fixed) {
+    const CompactLayout L = compact_layout(P, plan);
+    if (L.bytes > cap) return hipErrorOutOfMemory;
+    if (plan.compact) {
+      const hipError_t e = launch_prep(P, plan, L, fixed, sink, stream);
+      if (e != hipSuccess) return e;
+      compacted = true;
+    }
+  } else {
+    pooled = build_compact(P, plan, sink, stream);
+    compacted = pooled != nullptr;
+  }
+  if (compacted && plan.stage) {
     P.lds_stage = plan.stage;
     P.lds_words = plan.lds_words;
   }
@@ -278,8 +349,8 @@ static int dispatch(Pipe P, int mode, int sink, hipStream_t stream) {
   if (mode == SEG) rc = run_pipe<SEG>(P, sink, stream);
   else if (mode == OFF) rc = run_pipe<OFF>(P, sink, stream);
   else rc = run_pipe<PLAIN>(P, sink, stream);
-  if (scratch) {
-    const hipError_t e = hipFreeAsync(scratch, stream);
+  if (pooled) {
+    const hipError_t e = hipFreeAsync(pooled, stream);
     if (rc == hipSuccess) rc = e;
   }
   return rc;
@@ -397,6 +468,22 @@ int mordred_filter_probe_group_by_GPU2(const int* gpuCache, const filterArgsGPU*
   put_probe(P, pargs, 0xF);
   put_groupby(P, gargs);
   return dispatch(P, SEG, kGroupBySink, (hipStream_t)stream);
+}
+
+int mordred_filter_probe_group_by_GPU2_noalloc(const int* gpuCache, const filterArgsGPU* fargs,
+    const probeArgsGPU* pargs, const groupbyArgsGPU* gargs, int num_tuples, int* res,
+    int start_offset, const short* segment_group, mordred_stream_t stream) {
+  INVALID_IF(!seg_ok(gpuCache, num_tuples, start_offset, segment_group) || !gargs || !res ||
+             gargs->total_val <= 0);
+  hipError_t err;
+  char* fixed = device_scratch(err);
+  if (!fixed) return err;
+  Pipe P = make_pipe(gpuCache, num_tuples);
+  P.seg_group = segment_group; P.start_offset = start_offset; P.res = res;
+  put_filter_between(P, fargs, F_BETWEEN);
+  put_probe(P, pargs, 0xF);
+  put_groupby(P, gargs);
+  return dispatch(P, SEG, kGroupBySink, (hipStream_t)stream, fixed, MORDRED_NOALLOC_SCRATCH_BYTES);
 }
 
 int mordred_filter_probe_group_by_GPU3(const int* gpuCache, const offsetGPU* offset,
