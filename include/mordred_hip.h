@@ -164,6 +164,24 @@ int mordred_build_GPU2(const int* gpuCache, const filterArgsGPU* fargs, const bu
 int mordred_build_GPU3(const int* gpuCache, const int* dim_off, const filterArgsGPU* fargs,
     const buildArgsGPU* bargs, int num_tuples, int* hash_table, mordred_stream_t stream);
 
+/* build_GPU_minmax, GPUProcessing.h:1030-1113: build_GPU2 plus the min and
+ * max of the keys it inserts, merged with atomicMin / atomicMax into the
+ * caller-initialised DEVICE ints *min_global / *max_global.  Each workgroup
+ * starts from (bargs->val_max, bargs->val_min), as the reference's threads do
+ * (:1050), so those two values take part in the merge whenever num_tuples > 0.
+ * The reference engine never launches it. */
+int mordred_build_GPU_minmax(const int* gpuCache, const filterArgsGPU* fargs, const buildArgsGPU* bargs,
+    int num_tuples, int* hash_table, int* min_global, int* max_global, int start_offset,
+    const short* segment_group, mordred_stream_t stream);
+/* build_GPU_minmax2, GPUProcessing.h:1115-1172: the offset-list twin (rows
+ * from the DEVICE list dim_off, no filter).  The keys are those of the listed
+ * rows, as build_CPU_minmax2 (CPUProcessing.cu:1910-1983) reads them; the
+ * reference's unguarded BlockMinMaxGPU2 overload passes the wrong arguments
+ * (BlockLibrary.cuh:1011-1015), which is why the kernel is dead there. */
+int mordred_build_GPU_minmax2(const int* gpuCache, const int* dim_off, const filterArgsGPU* fargs,
+    const buildArgsGPU* bargs, int num_tuples, int* hash_table, int* min_global, int* max_global,
+    mordred_stream_t stream);
+
 /* ---------------- scalar aggregates ------------------------------------ */
 
 /* aggregationGPU, GPUProcessing.h:1472-1523 */

@@ -76,6 +76,8 @@ SIGNATURES = {
     "mordred_filter_GPU3": [VP, VP, _F, VP, I, VP, VP],
     "mordred_build_GPU2": [VP, _F, _B, I, VP, I, VP, VP],
     "mordred_build_GPU3": [VP, VP, _F, _B, I, VP, VP],
+    "mordred_build_GPU_minmax": [VP, _F, _B, I, VP, VP, VP, I, VP, VP],
+    "mordred_build_GPU_minmax2": [VP, VP, _F, _B, I, VP, VP, VP, VP],
     "mordred_aggregationGPU": [VP, VP, _G, I, VP, VP],
     "mordred_probe_aggr_GPU2": [VP, _P, _G, I, VP, I, VP, VP],
     "mordred_probe_aggr_GPU3": [VP, _O, _P, _G, I, VP, VP],

@@ -662,6 +662,37 @@ int mordred_build_GPU2(const int* gpuCache, const filterArgsGPU* fargs, const bu
   return run_build<SEG>(B, (hipStream_t)stream);
 }
 
+int mordred_build_GPU_minmax(const int* gpuCache, const filterArgsGPU* fargs, const buildArgsGPU* bargs,
+    int num_tuples, int* hash_table, int* min_global, int* max_global, int start_offset,
+    const short* segment_group, mordred_stream_t stream) {
+  INVALID_IF(!seg_ok(gpuCache, num_tuples, start_offset, segment_group) || !bargs ||
+             !bargs->key_idx || !hash_table || bargs->num_slots <= 0 || !min_global || !max_global);
+  Build B{};
+  B.cache = gpuCache; B.seg_group = segment_group; B.start_offset = start_offset;
+  B.num_tuples = num_tuples; B.aligned = aligned16(gpuCache);
+  if (fargs) { B.fcol = fargs->filter_idx1; B.fmode = fargs->mode1; B.c1 = fargs->compare1; B.c2 = fargs->compare2; }
+  B.key = bargs->key_idx; B.val = bargs->val_idx; B.ht = hash_table;
+  B.num_slots = bargs->num_slots; B.key_min = bargs->val_min;
+  B.kmin_out = min_global; B.kmax_out = max_global;
+  B.kmin_init = bargs->val_max; B.kmax_init = bargs->val_min;
+  return run_build<SEG>(B, (hipStream_t)stream);
+}
+
+int mordred_build_GPU_minmax2(const int* gpuCache, const int* dim_off, const filterArgsGPU* fargs,
+    const buildArgsGPU* bargs, int num_tuples, int* hash_table, int* min_global, int* max_global,
+    mordred_stream_t stream) {
+  // build_GPU_minmax2 asserts that no filter is given (GPUProcessing.h:1141).
+  INVALID_IF(num_tuples < 0 || !dim_off || !bargs || !bargs->key_idx || !hash_table ||
+             bargs->num_slots <= 0 || (fargs && fargs->filter_idx1) || !min_global || !max_global);
+  Build B{};
+  B.cache = gpuCache; B.num_tuples = num_tuples; B.in_rows = dim_off; B.aligned = aligned16(dim_off);
+  B.key = bargs->key_idx; B.val = bargs->val_idx; B.ht = hash_table;
+  B.num_slots = bargs->num_slots; B.key_min = bargs->val_min;
+  B.kmin_out = min_global; B.kmax_out = max_global;
+  B.kmin_init = bargs->val_max; B.kmax_init = bargs->val_min;
+  return run_build<OFF>(B, (hipStream_t)stream);
+}
+
 int mordred_build_GPU3(const int* gpuCache, const int* dim_off, const filterArgsGPU* fargs,
     const buildArgsGPU* bargs, int num_tuples, int* hash_table, mordred_stream_t stream) {
   // build_GPU3 asserts that no filter is given (GPUProcessing.h:1017).

@@ -82,6 +82,13 @@ struct Build {
   int* ht;
   int num_slots;
   int key_min;
+  // build_GPU_minmax / build_GPU_minmax2: min / max of the built keys merged
+  // into *kmin_out / *kmax_out with atomics; every workgroup starts from
+  // (kmin_init, kmax_init) = (val_max, val_min) as GPUProcessing.h:1050 does.
+  // nullptr: a plain build.
+  int* kmin_out;
+  int* kmax_out;
+  int kmin_init, kmax_init;
 };
 
 // Contiguous tile pointer of a column: SEG -> cached segment run, PLAIN -> column.
@@ -696,6 +703,7 @@ __global__ __launch_bounds__(BLOCK) void build_kernel(Build B) {
   const unsigned sused = (B.fcol ? 1u : 0u) | 2u | (B.val ? 4u : 0u);
   int tile0, tile1;
   tile_range(ntiles, tile0, tile1);
+  int kmn = B.kmin_init, kmx = B.kmax_init;
   for (int tile = tile0; tile < tile1; tile++) {
     const int start = tile * TILE;
     const int n = min(TILE, B.num_tuples - start);
@@ -738,6 +746,27 @@ __global__ __launch_bounds__(BLOCK) void build_kernel(Build B) {
       const size_t h = (size_t)phash(x[j], B.num_slots, B.key_min);
       if (B.val) atomicCAS(&B.ht[2 * h], 0, v[j]);
       B.ht[2 * h + 1] = row[j] + 1;
+    }
+    if (B.kmin_out) {
+#pragma unroll
+      for (int j = 0; j < IPT; j++)
+        if (sel >> j & 1u) {
+          kmn = min(kmn, x[j]);
+          kmx = max(kmx, x[j]);
+        }
+    }
+  }
+  if (B.kmin_out) {
+    // wave reduction, then one atomic pair per wave (min and max are
+    // associative, so the merge order does not change the result)
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+      kmn = min(kmn, __shfl_xor(kmn, d, kWave));
+      kmx = max(kmx, __shfl_xor(kmx, d, kWave));
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      atomicMin(B.kmin_out, kmn);
+      atomicMax(B.kmax_out, kmx);
     }
   }
 }

@@ -669,3 +669,63 @@ def test_filter_probe_GPU3_permuted_list(gstar):
         assert m == int(rt[0]) > 0, with_table
         np.testing.assert_array_equal(_sorted_rows([lo.cpu().numpy(), d4.cpu().numpy()], m),
                                       _sorted_rows([rlo, rd4], m))
+
+
+@pytest.mark.parametrize("val_max", [0, 1 << 30])
+def test_build_GPU_minmax_and_minmax2(gstar, val_max):
+    """build_GPU_minmax (GPUProcessing.h:1030-1113) and build_GPU_minmax2
+    (:1115-1172): the hash table word for word and the merged key min / max
+    against oracle_build_CPU_minmax / _minmax2 in the GPU convention.  With
+    val_max = 0 (what the reference's callers put there,
+    CPUGPUProcessing.cu:1606-1609) every workgroup's starting minimum 0 wins;
+    with a large val_max the minimum is the smallest built key."""
+    from mordred_amd import abi
+    q, data, cache = gstar
+    j = q.joins[-1] if q.kind == "aggr" else q.joins[1]
+    t = data.tables[j.table]
+    m = len(t[j.dim_key])
+    print(f"[gpu] build minmax {q.name} {j.table} val_max={val_max}", flush=True)
+    bg = abi.BuildArgsGPU(cache.idx(j.dim_key), cache.idx(j.group_col), j.dim_len, j.min_key, val_max)
+    fa = None
+    rf = FilterArgs()
+    if j.filter_col:
+        fa = abi.FilterArgsGPU()
+        fa.filter_idx1, fa.compare1, fa.compare2, fa.mode1 = cache.idx(j.filter_col), j.c1, j.c2, j.filter_mode
+        rf.filter_col1, rf.compare1, rf.compare2, rf.mode1 = iptr(t[j.filter_col]), j.c1, j.c2, j.filter_mode
+    b = BuildArgs(iptr(t[j.dim_key]), iptr(t[j.group_col]) if j.group_col else None, j.dim_len, j.min_key, val_max)
+    big, small = np.int32(2**31 - 1), np.int32(-2**31)
+    ht = _alloc(2 * j.dim_len)
+    mm = dev(np.array([big, small], dtype=np.int32))
+    sg = dev(OQ.segment_group(m))
+    abi.call("mordred_build_GPU_minmax", cache.ptr, abi.ref(fa), C.pointer(bg), m, ht.data_ptr(), mm.data_ptr(),
+             mm.data_ptr() + 4, 0, sg.data_ptr(), stream())
+    sync()
+    ref = np.zeros(2 * j.dim_len, dtype=np.int32)
+    rmn, rmx = np.array([big], dtype=np.int32), np.array([small], dtype=np.int32)
+    lib.oracle_build_CPU_minmax(rf, b, m, iptr(ref), iptr(rmn), iptr(rmx), 0, sptr(OQ.segment_group(m)), CONV_GPU)
+    np.testing.assert_array_equal(ht.cpu().numpy(), ref)
+    got = mm.cpu().numpy()
+    assert (int(got[0]), int(got[1])) == (int(rmn[0]), int(rmx[0]))
+    if val_max:
+        keys = t[j.dim_key]
+        if j.filter_col:
+            x = t[j.filter_col]
+            keys = keys[(x >= j.c1) & (x <= j.c2)]
+        assert (int(got[0]), int(got[1])) == (int(keys.min()), int(keys.max()))
+    # offset-list twin over a permuted half of the dimension rows, no filter
+    rows = np.random.default_rng(23).permutation(m)[: m // 2].astype(np.int32)
+    ht.zero_()
+    mm.copy_(dev(np.array([big, small], dtype=np.int32)))
+    drows = dev(rows)
+    abi.call("mordred_build_GPU_minmax2", cache.ptr, drows.data_ptr(), None, C.pointer(bg), len(rows), ht.data_ptr(),
+             mm.data_ptr(), mm.data_ptr() + 4, stream())
+    sync()
+    ref[:] = 0
+    rmn[0], rmx[0] = big, small
+    lib.oracle_build_CPU_minmax2(iptr(rows), FilterArgs(), b, len(rows), iptr(ref), iptr(rmn), iptr(rmx), 0, CONV_GPU)
+    np.testing.assert_array_equal(ht.cpu().numpy(), ref)
+    got = mm.cpu().numpy()
+    assert (int(got[0]), int(got[1])) == (int(rmn[0]), int(rmx[0]))
+    if val_max:
+        keys = t[j.dim_key][rows]
+        assert (int(got[0]), int(got[1])) == (int(keys.min()), int(keys.max()))
