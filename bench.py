@@ -1,24 +1,36 @@
 #!/usr/bin/env python3
-"""Benchmark of the plain per-query GPU form on one MI355X.
+"""Benchmark of the query-execution path on one MI355X.
 
-Default workload: the 3-way star join + group-by over 120,000,000 fact rows
-(lo_suppkey, lo_partkey, lo_orderdate, lo_revenue; supplier 40,000 rows with
-region = 1, part 1,400,000 rows with category = 1, date 2,556 rows; group by
-(year, brand) into 7 x 1,000 int64 sums).  `--workload q11` runs the
-filter+join+sum over 6,001,215 rows instead.
+Default workload (the metric's configuration): a Poisson stream of requests
+drawn from the seeded 64-query star mix (mordred_amd/mix.py, seed 2024) over a
+1,200,000,000-row fact table (7 int32 fact columns, 33.6 GB, all resident in
+HBM; supplier 40,000, part 1,400,000, customer 3,000,000 and date 2,556 rows),
+served two ways on the same arrivals:
+  served  the request loop: continuous batching into the persistent
+          shared-scan kernel (hipGraph, autotuned tile shape), launch sizes
+          ranked by the learned fp32 cost model once it is trusted, admission
+          by an HBM budget of 200 GB;
+  plain   first come first served, each query alone through its builds and
+          its fused fact kernel.
+A step is 64 consecutive requests of the stream: W untimed warm-up steps, then
+K timed steps (one continuous stream of 64 K requests at --rate q/s, default
+2,000 q/s, which saturates both forms).  `value` is the served form's fact
+rows answered per second (requests/s x 1.2e9); the line also carries both
+forms' q/s and p50/p99 latency, and whether every request's answer was
+identical in both forms.  The roofline is that of the shared-scan launches:
+algorithmic bytes (distinct fact columns x 4 B x rows scanned) over the HIP
+event time of the launches.  The plain per-query form's own kernel measurement
+(the 3-way join + group-by over 120,000,000 rows) is nested under
+`plain_query_kernel`.
 
-A step is one whole query with every column already resident in HBM: zero the
-three hash tables and the result, build the three perfect-hash tables, run the
-fused probe + group-by kernel over the fact table.  W untimed steps, then K
-timed steps between two synchronizations.  The fused kernel's launches are
-bracketed with HIP events on the stream they run on (torch's current stream),
-giving the kernel duration used for the HBM roofline.
+Other workloads: --workload q21 (3-way join + group-by, 120M rows), q11
+(filter+join+sum, 6,001,215 rows), q4way-split (segment cache + CPU/GPU
+split, 600M rows), batch64 (64-query shared scan, 600M rows), serve-sweep
+(offered rates 10..2,000 q/s), skip, q21-mixed.
 
-The answer of the last step is checked bit-exactly against the oracle (the C
-restatement of the reference CPU path) on the same inputs, and the oracle's
-full query is timed on the host cores as the CPU baseline.
-
-Prints one JSON line.
+Answers are checked bit-exactly against the oracle (the C restatement of the
+reference CPU path), which is also timed on the host cores as the CPU
+baseline.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -58,26 +70,56 @@ def workload(name: str, rows: int | None):
     return q, Q.make_data(q, n, seed=2024, S=40_000, P=1_400_000), desc
 
 
-def cpu_baseline(q, data, budget_s: float):
+def oracle_threads() -> int:
+    """Host threads for the oracle: every core this process may run on, capped
+    by OMP_NUM_THREADS when the environment sets the box's CPU share."""
+    n = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
+def host_info() -> dict:
+    """nproc (the affinity mask), the CPU model string and the NUMA node count."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    numa = len([d for d in os.listdir("/sys/devices/system/node") if d.startswith("node") and d[4:].isdigit()]) \
+        if os.path.isdir("/sys/devices/system/node") else None
+    return {"nproc": len(os.sched_getaffinity(0)), "cpu_model": model, "numa_nodes": numa,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(q, data, budget_s: float, min_runs: int = 10):
     """Oracle (C + OpenMP restatement of the reference CPU path): whole query
-    (builds + fact operator) on the same inputs, repeated within budget_s."""
+    (builds + fact operator) on the same inputs, one warm-up run, then at
+    least min_runs timed runs (more while within budget_s); median."""
     import oracle
     from oracle import queries as OQ
-    cores = min(16, os.cpu_count() or 1)
+    cores = oracle_threads()
     oracle.set_threads(cores)
+    res = OQ.run_query(q, data, oracle.CONV_GPU)
     times = []
-    res = None
     t_end = time.perf_counter() + budget_s
-    while True:
+    while len(times) < min_runs or time.perf_counter() < t_end:
         t0 = time.perf_counter()
-        res = OQ.run_query(q, data, oracle.CONV_GPU)
+        OQ.run_query(q, data, oracle.CONV_GPU)
         times.append(time.perf_counter() - t0)
-        if time.perf_counter() > t_end or len(times) >= 5:
+        if len(times) >= 100:
             break
     med = statistics.median(times)
     return res, {"value": data.n / med, "unit": "fact rows/s", "cores": cores, "kind": "port",
-                 "sample": f"whole query on all {data.n} fact rows (3 builds + fused fact operator), "
-                           f"median of {len(times)} runs, {med * 1e3:.1f} ms each"}
+                 "sample": f"whole query on all {data.n} fact rows (builds + fused fact operator), one warm-up, "
+                           f"median of {len(times)} runs, {med * 1e3:.1f} ms each "
+                           f"(min {min(times) * 1e3:.1f}, max {max(times) * 1e3:.1f})",
+                 "host": host_info()}
 
 
 def pcie_h2d_bandwidth(nbytes: int = 1 << 30) -> float:
@@ -108,7 +150,7 @@ def split_bench(args):
     log(f"generated {n} rows x {len(q.fact_columns())} columns in {time.time() - t0:.1f}s")
     nseg = (n + (1 << 20) - 1) >> 20
     cache = int(0.5 * nseg + 1) * len(q.fact_columns()) + 64 + 2 * 4 * len(q.fact_columns())
-    threads = min(16, os.cpu_count() or 1)
+    threads = oracle_threads()
     eng = E.Engine(device=0, cache_segments=cache, threads=threads)
     eng.add_data(data, q)
     eng.place(q, 0.5)
@@ -216,7 +258,7 @@ def batch_bench(args):
     cols = mix.fact_columns(qs)
     log(f"generated {n} rows x {len(cols)} fact columns in {time.time() - t0:.1f}s")
     nseg = (n + (1 << 20) - 1) >> 20
-    threads = min(16, os.cpu_count() or 1)
+    threads = oracle_threads()
     eng = E.Engine(device=0, cache_segments=len(cols) * nseg + 64, threads=threads)
     t0 = time.time()
     eng.add_mix_data(data, qs)
@@ -321,7 +363,7 @@ def serve_bench(args):
     cols = mix.fact_columns(qs)
     log(f"generated {n} rows x {len(cols)} fact columns in {time.time() - t0:.1f}s")
     nseg = (n + (1 << 20) - 1) >> 20
-    threads = min(16, os.cpu_count() or 1)
+    threads = oracle_threads()
     eng = E.Engine(device=0, cache_segments=len(cols) * nseg + 64, threads=threads)
     t0 = time.time()
     eng.add_mix_data(data, qs)
@@ -403,6 +445,144 @@ def serve_bench(args):
         sys.exit(1)
 
 
+def serve_headline(args):
+    """The metric's configuration: one Poisson request stream of the 64-query
+    mix over the 1.2B-row fact table, served (continuous batching into the
+    shared-scan kernel, learned cost model, HBM admission) and plain (each
+    request alone), on the same arrivals.  A step is 64 consecutive requests;
+    W warm-up steps, then K timed steps.  Prints one JSON line."""
+    import gc
+
+    import torch
+    from mordred_amd import engine as E
+    from mordred_amd import mix, serving
+    n = args.rows or 1_200_000_000
+    dims = {"S": 40_000, "P": 1_400_000, "C": 3_000_000}
+    mq = mix.make_mix(64, 2024, dims)
+    qs = [m.query for m in mq]
+    t0 = time.time()
+    data = mix.make_mix_data(qs, n, seed=2024, S=dims["S"], P=dims["P"], Cn=dims["C"])
+    cols = mix.fact_columns(qs)
+    log(f"generated {n} rows x {len(cols)} fact columns in {time.time() - t0:.1f}s")
+    nseg = (n + (1 << 20) - 1) >> 20
+    threads = oracle_threads()
+    eng = E.Engine(device=0, cache_segments=len(cols) * nseg + 64, threads=threads)
+    t0 = time.time()
+    eng.add_mix_data(data, qs)
+    log(f"loaded into the segment cache in {time.time() - t0:.1f}s")
+    best, table = eng.batch_tune(qs, reps=1)
+    log(f"autotuner: batch configuration {best}")
+    budget, target, rate = 200e9, args.target_chunk_ms, args.rate
+    # setup: a 400-request stream trains the learned cost model online
+    warm = serving.request_stream(qs, 400, seed=1)
+    _, _, wst = eng.serve(warm, serving.poisson_arrivals(4000.0, len(warm), seed=1), E.SERVE_CONTINUOUS,
+                          target_chunk_ms=target, hbm_budget_bytes=budget, learned=True)
+    log(f"cost-model training stream: {wst}")
+    nw, nk = 64 * args.warmup, 64 * args.steps
+    reqs = serving.request_stream(qs, nw + nk, seed=7)
+    arr = serving.poisson_arrivals(rate, nw + nk, seed=7)
+    timed, tarr = reqs[nw:], arr[nw:] - arr[nw]
+
+    def run(mode):
+        kw = {} if mode == E.SERVE_PLAIN else dict(target_chunk_ms=target, hbm_budget_bytes=budget, learned=True)
+        if nw:
+            eng.serve(reqs[:nw], arr[:nw], mode, **kw)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        outs, times, st = eng.serve(timed, tarr, mode, **kw)
+        torch.cuda.synchronize()
+        return outs, times, st, time.perf_counter() - t
+
+    co, ct, cst, c_wall = run(E.SERVE_CONTINUOUS)
+    c = serving.latency_summary(ct)
+    log(f"served: {c['throughput_qps']:.1f} q/s p50 {c['p50_ms']:.1f} p99 {c['p99_ms']:.1f} ms, {cst}")
+    po, pt, pst, p_wall = run(E.SERVE_PLAIN)
+    p = serving.latency_summary(pt)
+    log(f"plain: {p['throughput_qps']:.1f} q/s p50 {p['p50_ms']:.1f} p99 {p['p99_ms']:.1f} ms")
+    identical = all(bool((a == b).all()) for a, b in zip(po, co))
+    scan_ms = cst["gpu_ms"] / max(1, cst["launches"])
+    achieved = cst["alg_bytes"] / (cst["gpu_ms"] * 1e-3) / 1e9 if cst["gpu_ms"] > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": c["throughput_qps"] * n,
+        "unit": "fact rows/s answered (served requests/s x 1.2e9 fact rows per request)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": c_wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 columns, int64 sums",
+        "data": "synthetic, seeded SplitMix64 columns (mordred_amd/datagen.py); query mix mordred_amd/mix.py seed "
+                "2024; requests drawn uniformly from the mix, Poisson arrivals (seed 7)",
+        "config": {"workload": "Poisson request stream of the 64-query star mix over 1.2B fact rows: learned + "
+                               "batched + served vs plain per-query GPU form",
+                   "fact_rows": n, "fact_columns": len(cols), "offered_qps": rate,
+                   "requests_timed": nk, "requests_warmup": nw, "step": "64 consecutive requests",
+                   "dims": {"supplier": dims["S"], "part": dims["P"], "customer": dims["C"], "date": 2556},
+                   "hbm_budget_bytes": budget, "target_launch_ms": target,
+                   "launch": {"block": best[0], "items_per_thread": 4 * best[1], "lds_bytes": best[2]}},
+        "served": dict(c, launches=cst["launches"], gpu_ms=cst["gpu_ms"], wall_ms=cst["wall_ms"],
+                       max_active=cst["max_active_seen"], learned_choices=cst["learned_choices"],
+                       analytic_choices=cst["analytic_choices"], learned_trusted=cst["learned_trusted"],
+                       learned_error=cst["learned_error"], analytic_error=cst["analytic_error"]),
+        "plain": dict(p, gpu_ms=pst["gpu_ms"], wall_ms=pst["wall_ms"]),
+        "throughput_vs_plain": c["throughput_qps"] / p["throughput_qps"],
+        "p50_vs_plain": c["p50_ms"] / p["p50_ms"],
+        "p99_vs_plain": c["p99_ms"] / p["p99_ms"],
+        "identical_served_vs_plain": identical,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "batch_kernel (shared scan of the served launches)",
+                     "algorithmic_bytes": cst["alg_bytes"] / max(1, cst["launches"]),
+                     "kernel_ms_avg": scan_ms, "launches": cst["launches"],
+                     "note": "per launch: distinct fact columns of the active queries x 4 B x rows of the "
+                             "launch; time: HIP events around each launch's graph (clear + packed-table "
+                             "builds when the active set changed + the scan), on the engine's stream"},
+    }
+    # the oracle on a sample of the timed requests: 10 distinct queries in
+    # arrival order, one warm-up run, each over all n rows
+    if not args.no_cpu:
+        import oracle
+        from oracle import queries as OQ
+        oracle.set_threads(threads)
+        seen, sample = set(), []
+        for i, q in enumerate(timed):
+            if q.name not in seen:
+                seen.add(q.name)
+                sample.append(i)
+            if len(sample) == 10:
+                break
+        OQ.run_query(timed[sample[0]], data, oracle.CONV_GPU)
+        ok, secs = True, []
+        for i in sample:
+            t = time.perf_counter()
+            ref = OQ.run_query(timed[i], data, oracle.CONV_GPU)
+            secs.append(time.perf_counter() - t)
+            ok = ok and bool((ref == co[i]).all())
+        med = statistics.median(secs)
+        out["cpu_baseline"] = {"value": n / med, "unit": "fact rows/s (one query at a time)", "cores": threads,
+                               "kind": "port", "queries_per_s": 1.0 / med,
+                               "sample": f"oracle on 10 distinct queries of the timed stream ({[timed[i].name for i in sample]}) "
+                                         f"over all {n} rows, one warm-up run, median {med * 1e3:.0f} ms per query "
+                                         f"(min {min(secs) * 1e3:.0f}, max {max(secs) * 1e3:.0f})",
+                               "host": host_info()}
+        out["parity_oracle_sample"] = ok
+    eng.close()
+    del eng, data
+    gc.collect()
+    # the plain per-query form's own kernel: 3-way join + group-by, 120M rows
+    q2, d2, desc2 = workload("q21", None)
+    out["plain_query_kernel"] = plain_query_measure(q2, d2, desc2, 20, 5, args.no_cpu, 5.0)
+    out["parity"] = identical and out.get("parity_oracle_sample", True) and \
+        out["plain_query_kernel"]["parity"] is not False
+    print(json.dumps(out), flush=True)
+    if not out["parity"]:
+        log("PARITY FAILURE")
+        sys.exit(1)
+
+
 def skip_bench(args):
     """SSB Q1.1, Q1.2 and Q1.3 over a 600M-row lineorder sorted by
     lo_orderdate, read from on-disk column files and minmax files, every fact
@@ -427,7 +607,7 @@ def skip_bench(args):
         shutil.rmtree(tmp, ignore_errors=True)
         raise SystemExit(f"skip workload: {need / 1e9:.1f} GB of column files do not fit in {tmp}")
     nseg = (n + (1 << 20) - 1) >> 20
-    threads = min(16, os.cpu_count() or 1)
+    threads = oracle_threads()
     eng = E.Engine(device=0, cache_segments=nseg * len(cols) + 64, threads=threads)
     try:
         t0 = time.time()
@@ -523,7 +703,7 @@ def mixed_bench(args):
     data = Q.make_data(q, n, seed=2024, S=40_000, P=1_400_000)
     log(f"generated {n} rows x {len(q.fact_columns())} columns in {time.time() - t0:.1f}s")
     nseg = (n + (1 << 20) - 1) >> 20
-    threads = min(16, os.cpu_count() or 1)
+    threads = oracle_threads()
     bw_pci = pcie_h2d_bandwidth()   # torch initialises the device before the engine
     torch.cuda.synchronize()
     eng = E.Engine(device=0, cache_segments=3 * nseg + 64 + 2 * 4 * len(q.fact_columns()), threads=threads)
@@ -608,6 +788,73 @@ def mixed_bench(args):
         sys.exit(1)
 
 
+def plain_query_measure(q, data, desc, steps: int, warmup: int, no_cpu: bool = False, cpu_budget: float = 12.0):
+    """One plain per-query step (zero the hash tables and the result, the
+    dimension builds, the fused fact call) repeated `steps` times with the
+    columns resident in HBM.  The fused call is bracketed with HIP events on
+    the stream it runs on (torch's current stream): it holds the compact-table
+    pre-pass (when the plan compacts a table) and the fused kernel; the
+    kernel alone is in the rocprofv3 summaries under profiles/.  Returns the
+    measurement fields of a bench line (value, ms_per_step, config, roofline,
+    cpu_baseline, parity)."""
+    import torch
+    from mordred_amd import abi
+    from mordred_amd import query as Q
+    abi.load()
+    cache = Q.load_cache(data, q)
+    gq = Q.GpuQuery(q, data, cache)
+    torch.cuda.synchronize()
+    log("columns resident in HBM")
+    for _ in range(warmup):
+        gq.run()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(steps):
+        gq.reset()
+        gq.build()
+        ev[i][0].record(stream)
+        gq.probe()
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    call_ms = [a.elapsed_time(b) for a, b in ev]
+    ms_step = wall * 1e3 / steps
+    c_avg = sum(call_ms) / len(call_ms)
+    log(f"{q.name}: step {ms_step:.3f} ms, fused call avg {c_avg:.3f} ms (min {min(call_ms):.3f})")
+    ncols = len(q.fact_columns())
+    alg_bytes = ncols * 4 * data.n
+    achieved = alg_bytes / (c_avg * 1e-3) / 1e9
+    gpu_res = gq.result()
+    out = {
+        "value": data.n / (ms_step * 1e-3),
+        "ms_per_step": ms_step,
+        "config": dict(desc, form="plain per-query GPU (builds + fused fact call per step)"),
+        "kernel_rows_per_s": data.n / (c_avg * 1e-3),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "pipeline_kernel (fused fact probe+group-by)" if q.kind == "groupby"
+                     else "pipeline_kernel (fused fact filter+probe+sum)",
+                     "algorithmic_bytes": alg_bytes, "call_ms_avg": c_avg,
+                     "note": "achieved = algorithmic bytes / fused-call event time (compact-table pre-pass "
+                             "+ fused kernel); the kernel alone and its DRAM traffic (rocprofv3 FETCH_SIZE, "
+                             "WRITE_SIZE) are in profiles/"},
+    }
+    del gq, cache
+    torch.cuda.empty_cache()
+    if not no_cpu:
+        log("oracle: parity check + CPU baseline")
+        ref, base = cpu_baseline(q, data, cpu_budget)
+        out["cpu_baseline"] = base
+        out["parity"] = bool((ref == gpu_res).all())
+    else:
+        out["cpu_baseline"] = None
+        out["parity"] = None
+    return out
+
+
 def copy_probe():
     """A device copy of exactly 1,920,000,000 bytes read + written, repeated 5
     times: the reference kernel for scaling FETCH_SIZE / WRITE_SIZE."""
@@ -626,9 +873,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["q21", "q11", "q4way-split", "batch64", "serve", "skip", "q21-mixed"],
-                    default="q21")
-    ap.add_argument("--rates", default="10,50,200,500,1000,2000", help="serve: offered rates (q/s)")
+    ap.add_argument("--workload", choices=["serve", "q21", "q11", "q4way-split", "batch64", "serve-sweep", "skip",
+                                           "q21-mixed"], default="serve")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serve: offered rate (q/s)")
+    ap.add_argument("--rates", default="10,50,200,500,1000,2000", help="serve-sweep: offered rates (q/s)")
     ap.add_argument("--target-chunk-ms", type=float, default=8.0, help="serve: launch-time target")
     ap.add_argument("--rows", type=int, default=None, help="override fact rows")
     ap.add_argument("--learn-runs", type=int, default=48,
@@ -644,8 +892,8 @@ def main():
         copy_probe()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this path runs on one GPU; multi-GPU runs are not part of this "
-                                   "build (see DESIGN.md for how it would shard)"}))
+        print(json.dumps({"error": "this build runs on one GPU: more GPUs would each scan their own fact "
+                                   "segments and all-reduce the int64 group arrays, which is not built here"}))
         sys.exit(2)
     if args.workload == "q4way-split":
         split_bench(args)
@@ -654,6 +902,9 @@ def main():
         batch_bench(args)
         return
     if args.workload == "serve":
+        serve_headline(args)
+        return
+    if args.workload == "serve-sweep":
         serve_bench(args)
         return
     if args.workload == "skip":
@@ -663,80 +914,14 @@ def main():
         mixed_bench(args)
         return
 
-    import torch
-    from mordred_amd import query as Q
-    from mordred_amd import abi
-
-    abi.load()
     t0 = time.time()
     q, data, desc = workload(args.workload, args.rows)
     log(f"generated {data.n} rows in {time.time() - t0:.1f}s")
-    cache = Q.load_cache(data, q)
-    gq = Q.GpuQuery(q, data, cache)
-    torch.cuda.synchronize()
-    log("columns resident in HBM")
-
-    for _ in range(args.warmup):
-        gq.run()
-    torch.cuda.synchronize()
-
-    stream = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        gq.reset()
-        gq.build()
-        ev[i][0].record(stream)
-        gq.probe()
-        ev[i][1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-    kernel_ms = [a.elapsed_time(b) for a, b in ev]
-    ms_step = wall * 1e3 / args.steps
-    k_avg = sum(kernel_ms) / len(kernel_ms)
-    log(f"step {ms_step:.3f} ms, fused kernel avg {k_avg:.3f} ms (min {min(kernel_ms):.3f})")
-
-    ncols = len(q.fact_columns())
-    alg_bytes = ncols * 4 * data.n
-    achieved = alg_bytes / (k_avg * 1e-3) / 1e9
-    gpu_res = gq.result()
-
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath) and (args.rows is None):
-        with open(tpath) as f:
-            traffic = json.load(f).get(args.workload, {}).get("bytes_per_launch")
-    out = {
-        "metric": METRIC,
-        "value": data.n / (ms_step * 1e-3),
-        "unit": "fact rows/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32 columns, int64 sums",
-        "data": "synthetic, seeded SplitMix64 columns (mordred_amd/datagen.py)",
-        "config": dict(desc, form="plain per-query GPU (builds + fused fact kernel per step)"),
-        "kernel_rows_per_s": data.n / (k_avg * 1e-3),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE of this kernel)",
-                     "kernel": "pipeline_kernel (fused fact probe+group-by)" if q.kind == "groupby"
-                     else "pipeline_kernel (fused fact filter+probe+sum)",
-                     "algorithmic_bytes": alg_bytes, "kernel_ms_avg": k_avg},
-    }
-    if not args.no_cpu:
-        log("oracle: parity check + CPU baseline")
-        ref, base = cpu_baseline(q, data, args.cpu_budget)
-        out["cpu_baseline"] = base
-        out["parity"] = bool((ref == gpu_res).all())
-    else:
-        out["cpu_baseline"] = None
-        out["parity"] = None
+    out = plain_query_measure(q, data, desc, args.steps, args.warmup, args.no_cpu, args.cpu_budget)
+    out = {"metric": METRIC, "value": out.pop("value"), "unit": "fact rows/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": out.pop("ms_per_step"), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int32 columns, int64 sums",
+           "data": "synthetic, seeded SplitMix64 columns (mordred_amd/datagen.py)", **out}
     print(json.dumps(out), flush=True)
     if out["parity"] is False:
         log("PARITY FAILURE: GPU result differs from the oracle")

@@ -255,6 +255,7 @@ typedef struct mordred_batch_stats {
   int dead_queries;          /* empty predicate range: result stays zero */
   int graph_captured;        /* 1 if this call (re)captured the graph */
   mordred_batch_config config;
+  double alg_bytes;          /* distinct fact columns read x 4 B x rows scanned */
 } mordred_batch_stats;
 
 typedef struct mordred_batch_tune_entry {
@@ -324,6 +325,8 @@ typedef struct mordred_serve_stats {
   double analytic_error;
   int learned_trusted;
   double peak_state_bytes;        /* admitted per-query device state, at most */
+  double alg_bytes;               /* continuous: sum over launches of the distinct
+                                     fact columns read x 4 B x rows scanned */
 } mordred_serve_stats;
 
 int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n, const double* arrival_ms,

@@ -2029,6 +2029,7 @@ struct BatchPlan {
   std::vector<long long> res_off;  // ints
   std::vector<int> G;
   int lds_queries = 0, global_queries = 0, dead = 0;
+  int cols_read = 0;  // distinct fact columns the scan reads (algorithmic bytes: 4 per row each)
 };
 
 void column_range(Column* c) {
@@ -2326,7 +2327,11 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     D.pred_begin[c] = p;
   }
   D.ncols = ncols;
-  for (int c = 0; c < ncols; c++) D.col[c] = cols[c] ? cols[c]->d_idx : nullptr;
+  b.cols_read = 0;
+  for (int c = 0; c < ncols; c++) {
+    D.col[c] = cols[c] ? cols[c]->d_idx : nullptr;
+    b.cols_read += cols[c] != nullptr;
+  }
   // group arrays: smallest first into LDS while the budget lasts
   std::vector<int> order(nq);
   for (int i = 0; i < nq; i++) order[i] = i;
@@ -2484,6 +2489,7 @@ int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig&
     st->dead_queries = b.dead;
     st->graph_captured = captured;
     st->config = mordred_batch_config{cfg.block, cfg.vpt, cfg.lds_bytes};
+    st->alg_bytes = 4.0 * b.cols_read * (double)b.d.num_tuples;
   }
   return 0;
 }
@@ -3520,6 +3526,7 @@ int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n,
     float gms = 0;
     HIP_OK(hipEventElapsedTime(&gms, e->ev0, e->ev1));
     st.gpu_ms += gms;
+    st.alg_bytes += 4.0 * b.cols_read * (double)b.d.num_tuples;
     st.launches++;
     changed = false;
     // learn from the measured launch

@@ -73,7 +73,7 @@ class BatchConfig(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("gpu_ms", C.c_double), ("lds_queries", C.c_int),
                 ("global_queries", C.c_int), ("dead_queries", C.c_int), ("graph_captured", C.c_int),
-                ("config", BatchConfig)]
+                ("config", BatchConfig), ("alg_bytes", C.c_double)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "config"}
@@ -103,7 +103,7 @@ class ServeStats(C.Structure):
     _fields_ = [("wall_ms", C.c_double), ("launches", C.c_longlong), ("gpu_ms", C.c_double),
                 ("max_active_seen", C.c_int), ("learned_choices", C.c_longlong), ("analytic_choices", C.c_longlong),
                 ("learned_error", C.c_double), ("analytic_error", C.c_double), ("learned_trusted", C.c_int),
-                ("peak_state_bytes", C.c_double)]
+                ("peak_state_bytes", C.c_double), ("alg_bytes", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
