@@ -44,6 +44,32 @@ __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __rest
     int* res = reinterpret_cast<int*>(Q.res);
     const long long ni = 6 * (long long)Q.total_val;
     for (long long i = t; i < ni; i += stride) res[i] = 0;
+    const int nt = batch_touch_words(Q.total_val, Q.flags);
+    unsigned* tb = D->touch_arena + Q.gtouch;
+    for (long long i = t; i < nt; i += stride) tb[i] = 0;
+  }
+}
+
+// Group keys of every touched group of the queries whose keys decode from the
+// group index: digit d of index h is (h / uniq[d]) % krange[d], the key word
+// minv[d] + digit (0 for a dimension without group value).  blockIdx.y = query.
+__global__ __launch_bounds__(256) void batch_keys_kernel(const BatchDesc* __restrict__ D) {
+  const int q = blockIdx.y;
+  if (q >= D->nq || !(D->all >> q & 1ull)) return;
+  const BatchQuery& Q = D->q[q];
+  if (Q.flags & (kQueryAggr | kQueryKeysInLoop)) return;
+  const BatchKeys K = D->keys[q];
+  const unsigned* tb = D->touch_arena + Q.gtouch;
+  int* res = reinterpret_cast<int*>(Q.res);
+  const int G = Q.total_val;
+  for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < G; h += gridDim.x * blockDim.x) {
+    if (!(tb[h >> 5] >> (h & 31) & 1u)) continue;
+    int kk[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) kk[d] = Q.uniq[d] != 0 ? K.minv[d] + (h / Q.uniq[d]) % K.krange[d] : 0;
+    int2* key = reinterpret_cast<int2*>(res + 6 * (size_t)h);
+    key[0] = make_int2(kk[0], kk[1]);
+    key[1] = make_int2(kk[2], kk[3]);
   }
 }
 
@@ -84,13 +110,14 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   const int nq = D->nq;
   BatchQuery* lq = reinterpret_cast<BatchQuery*>(lds_raw);
   unsigned long long* llut = reinterpret_cast<unsigned long long*>(lds_raw + nq * sizeof(BatchQuery));
-  unsigned long long* lsum = llut + D->nlut;
-  const int nsum = D->lds_sum_words;
+  // group region: per query a touched bitmap, then LDS accumulators (8-byte aligned)
+  unsigned* lgrp = reinterpret_cast<unsigned*>(llut + D->nlut);
+  const int ngrp = D->lds_words;
   {
     const uint4* src = reinterpret_cast<const uint4*>(D->q);
     uint4* dst = reinterpret_cast<uint4*>(lq);
     for (int i = threadIdx.x; i < nq * (int)(sizeof(BatchQuery) / 16); i += BLOCK) dst[i] = src[i];
-    for (int i = threadIdx.x; i < nsum; i += BLOCK) lsum[i] = 0;
+    for (int i = threadIdx.x; i < ngrp; i += BLOCK) lgrp[i] = 0;
     const int np = D->npred;
     for (int i = threadIdx.x; i < np; i += BLOCK) {
       const BatchPred P = D->pred[i];
@@ -410,10 +437,10 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         const int q = __builtin_ctzll(m);
         m &= m - 1;
         const BatchQuery& Q = lq[q];
-        const int h = mod_pos(Q.hbase + (int)__umul24((attrw[0][j] >> Q.shift[0]) & Q.mask[0], (unsigned)Q.uniq[0]) +
-                                  (int)__umul24((attrw[1][j] >> Q.shift[1]) & Q.mask[1], (unsigned)Q.uniq[1]) +
-                                  (int)__umul24((attrw[2][j] >> Q.shift[2]) & Q.mask[2], (unsigned)Q.uniq[2]) +
-                                  (int)__umul24((attrw[3][j] >> Q.shift[3]) & Q.mask[3], (unsigned)Q.uniq[3]),
+        const int h = mod_pos(Q.hbase + (int)__umul24(__builtin_amdgcn_ubfe(attrw[0][j], Q.shift[0], Q.bits[0]), (unsigned)Q.uniq[0]) +
+                                  (int)__umul24(__builtin_amdgcn_ubfe(attrw[1][j], Q.shift[1], Q.bits[1]), (unsigned)Q.uniq[1]) +
+                                  (int)__umul24(__builtin_amdgcn_ubfe(attrw[2][j], Q.shift[2], Q.bits[2]), (unsigned)Q.uniq[2]) +
+                                  (int)__umul24(__builtin_amdgcn_ubfe(attrw[3][j], Q.shift[3], Q.bits[3]), (unsigned)Q.uniq[3]),
                               Q.total_val);
         // an index outside [0, total_val) (group values below min_val) would
         // write outside the query's result; the reference does, here the pair
@@ -425,26 +452,52 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
           if (e == Q.expr) val = ev[e][j];
         const unsigned long long add = (unsigned long long)(long long)val;
         int* res = reinterpret_cast<int*>(Q.res) + 6 * (size_t)h;
-        // The keys are stored whenever the sum the add returns was zero: at
-        // the first pair of the group in this workgroup (LDS) or overall
-        // (global), and again only if a running sum passes through zero.
-        // The reference stores them for every row of the group
-        // (GPUProcessing.h:144-147); they are equal, so any store suffices.
-        // (scoped atomics keep the LDS add a ds_ instruction: one atomicAdd
-        // on either pointer is merged into a slower flat atomic)
+        auto* gsum = (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(res + 4);
+        const int tw = batch_touch_words(Q.total_val, Q.flags);
+        unsigned* acc = lgrp + Q.acc + tw;
+        // (scoped atomics keep the LDS ones ds_ instructions: one atomicAdd
+        // on either pointer would be merged into a slower flat atomic)
+        if (!(Q.flags & kQueryKeysInLoop)) {
+          // no add returns a value here, except the 32-bit LDS one that needs
+          // its carry: the keys are written after the scan from the touched
+          // bitmap, so no pair waits for its atomic
+          if (Q.sink == kSinkLds64) {
+            (void)__hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(acc) + h, add, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+          } else if (Q.sink == kSinkLds32) {
+            const unsigned u = (unsigned)val;
+            const unsigned old = __hip_atomic_fetch_add(acc + h, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            // (int64)val = u - 2^32 [val < 0]; the low word wraps when old + u
+            // carries: the 2^32 that leaves it goes to the result
+            const int delta = (int)(old + u < old) - (int)(val < 0);
+            if (delta)
+              (void)__hip_atomic_fetch_add(gsum, (unsigned long long)(long long)delta << 32, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+          } else {
+            (void)__hip_atomic_fetch_add(gsum, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          if (tw)
+            (void)__hip_atomic_fetch_or(lgrp + Q.acc + (h >> 5), 1u << (h & 31), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+          continue;
+        }
+        // keys that do not decode from the index: stored whenever the sum the
+        // add returns was zero -- at the first pair of the group in this
+        // workgroup (LDS) or overall (global), and again only if a running sum
+        // passes through zero.  The reference stores them for every row of the
+        // group (GPUProcessing.h:144-147); they are equal, so any store suffices.
         unsigned long long old;
-        if (Q.lds >= 0)
-          old = __hip_atomic_fetch_add(&lsum[Q.lds + h], add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (Q.sink == kSinkLds64)
+          old = __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(acc) + h, add, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
         else
-          old = __hip_atomic_fetch_add(
-              (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(res + 4),
-              add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          old = __hip_atomic_fetch_add(gsum, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (!(Q.flags & kQueryAggr) && old == 0ull) {
           int gv[4];
 #pragma unroll
           for (int d = 0; d < 4; d++)
             gv[d] = Q.attr[d] < 0 ? 0
-                                  : H.dim[d].base[Q.attr[d]] + (int)((attrw[d][j] >> Q.shift[d]) & Q.mask[d]);
+                                  : H.dim[d].base[Q.attr[d]] + (int)__builtin_amdgcn_ubfe(attrw[d][j], Q.shift[d], Q.bits[d]);
           int2* key = reinterpret_cast<int2*>(res);
           key[0] = make_int2(gv[0], gv[1]);
           key[1] = make_int2(gv[2], gv[3]);
@@ -465,16 +518,39 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     }
   }
 
-  // flush the LDS group arrays (adding a zero sum changes nothing)
+  // flush: the touched bitmaps into the query's global bitmap (read by the
+  // keys kernel), the LDS sums of the touched groups into the result (adding
+  // a zero sum changes nothing)
   __syncthreads();
+  unsigned* const touch_arena = D->touch_arena;
   for (int q = 0; q < nq; q++) {
-    const int base = lq[q].lds;
-    if (base < 0) continue;
-    const int G = lq[q].total_val;
-    int* res = reinterpret_cast<int*>(lq[q].res);
-    for (int h = threadIdx.x; h < G; h += BLOCK) {
-      const unsigned long long s = lsum[base + h];
-      if (s) atomicAdd(reinterpret_cast<unsigned long long*>(res + 6 * (size_t)h + 4), s);
+    const BatchQuery& Q = lq[q];
+    if (!(all >> q & 1ull)) continue;
+    const int G = Q.total_val, flags = Q.flags, sink = Q.sink;
+    const int tw = batch_touch_words(G, flags);
+    const unsigned* tb = lgrp + Q.acc;
+    const unsigned* acc = tb + tw;
+    auto* res = reinterpret_cast<unsigned long long*>(Q.res);
+    auto add = [&](int h) {
+      const unsigned long long s = sink == kSinkLds64 ? reinterpret_cast<const unsigned long long*>(acc)[h]
+                                                      : (unsigned long long)acc[h];
+      if (s) atomicAdd(res + 3 * (size_t)h + 2, s);
+    };
+    if (tw == 0) {
+      if (sink != kSinkGlobal)
+        for (int h = threadIdx.x; h < G; h += BLOCK) add(h);
+      continue;
+    }
+    for (int w = threadIdx.x; w < tw; w += BLOCK) {
+      unsigned bits = tb[w];
+      if (!bits) continue;
+      (void)__hip_atomic_fetch_or(touch_arena + Q.gtouch + w, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (sink == kSinkGlobal) continue;
+      while (bits) {
+        const int b = __builtin_ctz(bits);
+        bits &= bits - 1u;
+        add(32 * w + b);
+      }
     }
   }
 }
@@ -550,6 +626,11 @@ hipError_t batch_fill(const BatchLaunch& L, const uint4* const* rec, const int* 
 hipError_t batch_records(const BatchLaunch& L, int d, int nrows, hipStream_t s) {
   const int grid = std::max(1, std::min(4096, (nrows + 255) / 256));
   batch_records_kernel<<<grid, 256, 0, s>>>(L, d);
+  return hipGetLastError();
+}
+
+hipError_t batch_keys(const BatchLaunch& L, hipStream_t s) {
+  batch_keys_kernel<<<dim3(32, kBatchMaxQ), 256, 0, s>>>(L.desc);
   return hipGetLastError();
 }
 

@@ -44,30 +44,67 @@ constexpr int kBatchLutSpan = 2048; // widest attribute span given a mask table
 constexpr int kBatchIvlMin = 4;     // range filters on one source that get an interval table
 
 enum BatchExprKind { kExprDiff = 0, kExprProduct = 1 };  // a1 - a2 (group-by), a1 * a2 (aggregate)
-enum BatchQueryFlags { kQueryAggr = 1 };
+enum BatchQueryFlags {
+  kQueryAggr = 1,        // scalar aggregate: no key words
+  kQueryKeysInLoop = 2,  // keys do not decode from the group index: stored by the pair
+                         // whose add returns a zero sum (returning atomics)
+};
+// Where a query's sums accumulate during the scan (per workgroup, in LDS, then
+// added to the result once at the end; or per pair in the result itself).
+enum BatchSink {
+  kSinkLds64 = 0,   // int64 per group in LDS, non-returning adds
+  kSinkLds32 = 1,   // low 32 bits per group in LDS; a carry or borrow out of
+                    // them adds +-2^32 to the result directly (exact mod 2^64)
+  kSinkGlobal = 2,  // one 64-bit atomic per pair on the result
+};
 
 // The fields a (row, query) pair reads sit in the first 48 bytes (three
-// 16-byte LDS reads); attr is read only when the keys of a group are stored.
+// 16-byte LDS reads).
 struct BatchQuery {            // 64 bytes; staged in LDS by the scan kernel
   long long res;               // device address of the query's 6*total_val ints
   int total_val;
-  short lds;                   // u64 offset of its group array in LDS, -1 = global atomics
-  unsigned char expr;          // index into BatchDesc::expr
-  unsigned char flags;         // kQueryAggr: no key words
+  // even 32-bit word offset in the workgroup's LDS group region of the
+  // query's touched bitmap (batch_touch_words words), followed by its
+  // accumulators for an LDS sink
+  int acc;
   // dense group index (GPUProcessing.h:142) as hbase + sum_d field_d * uniq[d]
-  // with field_d = (packed entry >> shift[d]) & mask[d] and hbase = sum over
-  // grouped dimensions of (attribute base - min_val) * unique_val; mask[d] =
-  // uniq[d] = 0 for a dimension without group value (identical in 32-bit
-  // wrap-around).  field_d < 2^16 and uniq[d] < 2^24, so the products are
-  // 24-bit multiplies.
+  // with field_d = (packed entry >> shift[d]) & (2^bits[d] - 1) and hbase =
+  // sum over grouped dimensions of (attribute base - min_val) * unique_val;
+  // bits[d] = uniq[d] = 0 for a dimension without group value (identical in
+  // 32-bit wrap-around).  field_d < 2^16 and uniq[d] < 2^24, so the products
+  // are 24-bit multiplies.
   int hbase;
+  unsigned char expr;          // index into BatchDesc::expr
+  unsigned char flags;         // BatchQueryFlags
+  unsigned char sink;          // BatchSink
+  unsigned char pad;
   unsigned char shift[4];
-  unsigned short mask[4];
+  unsigned char bits[4];
   int uniq[4];
   signed char attr[4];         // per dimension slot: -1 no group value, else its attribute
-  int unused[3];
+  int gtouch;                  // word offset of its touched bitmap in BatchDesc::touch_arena
+  int unused[2];
 };
 static_assert(sizeof(BatchQuery) == 64, "BatchQuery is staged as 4 x 16 bytes");
+
+// Group keys of a query whose keys decode from the group index (a mixed
+// radix: digit d = value - minv[d] in [0, krange[d]) for uniq[d] != 0, key
+// word 0 elsewhere), written after the scan for every touched group.
+struct BatchKeys {
+  int minv[4];
+  int krange[4];
+};
+
+// 32-bit words of a query's touched bitmap (an even count, so that 64-bit
+// accumulators after it stay 8-byte aligned); scalar aggregates and queries
+// that store their keys in the pair loop keep none.
+__host__ __device__ inline int batch_touch_words(int G, int flags) {
+  return (flags & (kQueryAggr | kQueryKeysInLoop)) ? 0 : ((G + 63) >> 6) << 1;
+}
+// 32-bit words of a query's accumulators in LDS.
+__host__ __device__ inline int batch_acc_words(int sink, int G) {
+  return sink == kSinkLds64 ? 2 * G : sink == kSinkLds32 ? G : 0;
+}
 
 struct BatchPred {             // fact BETWEEN predicate of one query
   int col, lo, hi, q;
@@ -114,16 +151,18 @@ struct BatchDesc {
   int rebuild;                 // record tables must be rebuilt (query set changed)
   int pad0;
   unsigned long long reset;    // queries whose result rows are cleared first
-  int lds_sum_words;           // u64 words of LDS group arrays
-  int pad2;
+  int lds_words;               // 32-bit words of the LDS group region
+  int touch_words;             // words of the touch arena
   long long res_ints;          // result arena ints
   int* res_arena;
+  unsigned* touch_arena;       // per query: a touched bitmap of its groups (gtouch)
   const int* col[kBatchMaxCols];
   int pred_begin[kBatchMaxCols + 1];
   BatchPred pred[kBatchMaxPred];
   BatchExpr expr[kBatchMaxExpr];
   BatchDim dim[4];
   BatchQuery q[kBatchMaxQ];
+  BatchKeys keys[kBatchMaxQ];
   int nlut, pad1;
   unsigned long long lut[kBatchMaxLut];  // attribute mask tables
 };
@@ -170,6 +209,8 @@ inline size_t batch_group_lds(int total_val) { return (size_t)total_val * 8; }
 // rec / rec_len: packed tables and their sizes in 16-byte units
 hipError_t batch_fill(const BatchLaunch& L, const uint4* const* rec, const int* rec_len, hipStream_t s);
 hipError_t batch_records(const BatchLaunch& L, int d, int nrows, hipStream_t s);
+// Writes the decoded keys of every touched group (after the scan).
+hipError_t batch_keys(const BatchLaunch& L, hipStream_t s);
 // ncols: fact columns the batch reads (selects the register layout).
 hipError_t batch_scan(const BatchLaunch& L, const BatchConfig& c, int ncols, hipStream_t s);
 bool batch_config_ok(const BatchConfig& c);

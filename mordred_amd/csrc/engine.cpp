@@ -194,7 +194,7 @@ struct mordred_engine {
   DevBuf d_res, d_sg, d_ht[4], d_mix, d_pmix, d_ord, d_res_ord;
   HostBuf h_stage;  // pinned scratch for segment groups / index updates
   // shared-scan batches: record tables, descriptor, result arena, the graph
-  DevBuf d_rec[4], d_desc, d_arena, d_status;
+  DevBuf d_rec[4], d_desc, d_arena, d_status, d_touch;
   HostBuf h_desc, h_arena;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -203,7 +203,7 @@ struct mordred_engine {
   mordred::BatchConfig batch_cfg{512, 1, 64 * 1024};
   bool batch_tuned = false;
   // request loop: persistent slot arenas and the learned cost model
-  DevBuf d_serve_res;
+  DevBuf d_serve_res, d_serve_touch;
   mordred_cost_model* cm = nullptr;
   // learned plan model for MORDRED_PLACE_COST_SPLIT (0 off, 1 use when
   // trusted, 2 also explore), trained on the runs' measured times
@@ -2061,10 +2061,98 @@ bool batch_probes(const mordred_star_query& q, const Resolved& r, int k) {
 struct SlotArenas {
   int* res = nullptr;
   long long res_stride = 0;    // ints per slot
+  unsigned* touch = nullptr;   // touched bitmaps
+  int touch_stride = 0;        // words per slot
 };
 
+struct QueryEst {
+  double sel = 1.0;
+  int G = 1;
+  int njoins = 0;
+  int dims = 0;
+};
+
+double dim_pass_fraction(const Resolved& r, int k) {
+  const Column* f = r.dfilt[k];
+  if (!f) return 1.0;
+  const mordred_join& j = *r.join[k];
+  const long long n = r.dkey[k]->n;
+  const long long step = std::max<long long>(1, n / 4096);
+  long long seen = 0, pass = 0;
+  for (long long row = 0; row < n; row += step, seen++) {
+    const int x = f->host[row];
+    const bool ok = j.filter_mode == 0 ? x == j.c1 : j.filter_mode == 1 ? (x >= j.c1 && x <= j.c2)
+                                                                       : (x == j.c1 || x == j.c2);
+    pass += ok;
+  }
+  return seen ? (double)pass / (double)seen : 1.0;
+}
+
+int estimate_query(mordred_engine* e, const mordred_star_query& q, QueryEst& out) {
+  Resolved r;
+  const int rc = resolve(e, &q, r);
+  if (rc) return rc;
+  double sel = 1.0;
+  for (int k = 0; k < 4; k++) {
+    if (!batch_probes(q, r, k)) continue;
+    sel *= dim_pass_fraction(r, k);
+    out.njoins++;
+    out.dims |= 1 << k;
+  }
+  for (int i = 0; i < q.nfilters; i++) {
+    Column* c = r.filt[i];
+    column_range(c);
+    const double lo = std::max<double>(q.filters[i].lo, c->vmin), hi = std::min<double>(q.filters[i].hi, c->vmax);
+    sel *= hi >= lo ? (hi - lo + 1) / ((double)c->vmax - c->vmin + 1) : 0.0;
+  }
+  out.sel = sel;
+  out.G = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
+  return 0;
+}
+
+// Keys of a group-by query decode from its group index when the joined
+// dimensions with a non-zero stride form a mixed radix (each stride divides
+// the next, the largest divides total_val) and every row's digit value -
+// min_val lies in [0, next stride / stride): the group value's column range
+// for a grouped dimension, 0 for a joined dimension without group value.  A
+// group value with stride 0 (its key word is not in the index) does not
+// decode.  Fills minv / krange for the keys kernel.
+bool keys_decode(const mordred_star_query& q, const Resolved& r, mordred::BatchKeys& K) {
+  int idx[4], m = 0;
+  for (int k = 0; k < 4; k++) {
+    K.minv[k] = 0;
+    K.krange[k] = 0;
+    if (!r.join[k]) continue;
+    const mordred_join& j = *r.join[k];
+    if (j.unique_val < 0) return false;
+    if (j.unique_val == 0) {
+      if (r.dval[k]) return false;
+      continue;
+    }
+    idx[m++] = k;
+  }
+  std::sort(idx, idx + m, [&](int a, int b) { return r.join[a]->unique_val < r.join[b]->unique_val; });
+  for (int i = 0; i < m; i++) {
+    const int k = idx[i];
+    const mordred_join& j = *r.join[k];
+    const long long next = i + 1 < m ? r.join[idx[i + 1]]->unique_val : q.total_val;
+    if (next % j.unique_val != 0) return false;
+    const long long range = next / j.unique_val;
+    long long lo = 0, hi = 0;
+    if (r.dval[k]) {
+      column_range(r.dval[k]);
+      lo = r.dval[k]->vmin;
+      hi = r.dval[k]->vmax;
+    }
+    if (lo - j.min_val < 0 || hi - j.min_val >= range) return false;
+    K.minv[k] = j.min_val;
+    K.krange[k] = (int)range;
+  }
+  return true;
+}
+
 int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, const mordred::BatchConfig& cfg,
-               BatchPlan& b, const SlotArenas* fixed = nullptr) {
+               BatchPlan& b, const SlotArenas* fixed = nullptr, const double* sel = nullptr) {
   using namespace mordred;
   if (nq < 1 || nq > kBatchMaxQ) return fail(e, "a batch holds 1..64 queries");
   if (!batch_config_ok(cfg)) return fail(e, "bad batch configuration");
@@ -2296,25 +2384,34 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     Q.total_val = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
     Q.expr = (unsigned char)ex;
     Q.flags = (unsigned char)(q.kind == MORDRED_KIND_GROUPBY ? 0 : kQueryAggr);
+    Q.gtouch = 0;
     unsigned hbase = 0;
     for (int k = 0; k < 4; k++) {
       Q.attr[k] = -1;
       Q.shift[k] = 0;
-      Q.mask[k] = 0;
+      Q.bits[k] = 0;
       Q.uniq[k] = 0;
-      if (q.kind != MORDRED_KIND_GROUPBY || !r.join[k] || !r.dval[k]) continue;
+      if (q.kind != MORDRED_KIND_GROUPBY || !r.join[k]) continue;
+      const int uv = r.join[k]->unique_val;
+      if (uv < 0 || uv >= (1 << 24)) return fail(e, "batch group index needs unique_val in [0, 2^24)");
+      Q.uniq[k] = uv;
+      if (!r.dval[k]) {
+        // a joined table without group value contributes its probe's value
+        // word, 0 (GPUProcessing.h:142): (0 - min_val) * unique_val
+        hbase += (0u - (unsigned)r.join[k]->min_val) * (unsigned)uv;
+        continue;
+      }
       int a = 0;
       while (a < kBatchMaxAttr && attrs[k][a] != r.dval[k]) a++;
       const BatchAttr& A = D.dim[k].attr[a];
       Q.attr[k] = (signed char)a;
       Q.shift[k] = (unsigned char)A.off;
-      Q.mask[k] = (unsigned short)((1u << A.bits) - 1u);
-      Q.uniq[k] = r.join[k]->unique_val;
-      if (A.bits > 16 || Q.uniq[k] < 0 || Q.uniq[k] >= (1 << 24))
-        return fail(e, "batch group index needs attribute fields of at most 16 bits and unique_val in [0, 2^24)");
-      hbase += ((unsigned)A.base - (unsigned)r.join[k]->min_val) * (unsigned)r.join[k]->unique_val;
+      Q.bits[k] = (unsigned char)A.bits;
+      if (A.bits > 16) return fail(e, "batch group index needs attribute fields of at most 16 bits");
+      hbase += ((unsigned)A.base - (unsigned)r.join[k]->min_val) * (unsigned)uv;
     }
     Q.hbase = (int)hbase;
+    if (q.kind == MORDRED_KIND_GROUPBY && !keys_decode(q, r, D.keys[i])) Q.flags |= kQueryKeysInLoop;
     b.G[i] = Q.total_val;
     if (dead) b.dead++;
     else D.all |= 1ull << i;
@@ -2332,42 +2429,94 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     D.col[c] = cols[c] ? cols[c]->d_idx : nullptr;
     b.cols_read += cols[c] != nullptr;
   }
-  // group arrays: smallest first into LDS while the budget lasts
-  std::vector<int> order(nq);
-  for (int i = 0; i < nq; i++) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b.G[x] < b.G[y]; });
-  size_t lds_used = sizeof(BatchHot) + nq * sizeof(BatchQuery) + (size_t)D.nlut * sizeof(unsigned long long);
-  if (lds_used > (size_t)cfg.lds_bytes) return fail(e, "lds_bytes below the staged descriptor and mask tables");
-  std::vector<char> in_lds(nq, 0);
-  for (int i : order) {
-    if (!qp[i]) continue;
-    const size_t need = batch_group_lds(b.G[i]);
-    if (lds_used + need <= (size_t)cfg.lds_bytes) {
-      in_lds[i] = 1;
-      lds_used += need;
+  // group sinks.  Every query with decodable keys keeps a touched bitmap in
+  // LDS; the rest of the LDS budget holds accumulators, given in the order of
+  // expected pairs per LDS word (selectivity / groups): first as 32-bit
+  // accumulators, then upgraded to 64-bit ones (no returning atomic) while
+  // room is left.  The others add per pair to the result (global atomics).
+  std::vector<double> est_sel(nq, 0.0);
+  for (int i = 0; i < nq; i++) {
+    if (!qp[i] || !(D.all >> i & 1ull)) continue;
+    if (sel) {
+      est_sel[i] = sel[i];
+    } else {
+      QueryEst qe;
+      if (estimate_query(e, *qp[i], qe) == 0) est_sel[i] = qe.sel;
     }
   }
-  long long sum_words = 0, res_ints = 0;
+  const long long fixed_bytes = (long long)sizeof(BatchHot) + nq * (long long)sizeof(BatchQuery) +
+                                (long long)D.nlut * (long long)sizeof(unsigned long long);
+  if (fixed_bytes > cfg.lds_bytes) return fail(e, "lds_bytes below the staged descriptor and mask tables");
+  long long avail = ((long long)cfg.lds_bytes - fixed_bytes) / 4;  // 32-bit words
+  std::vector<int> order;
+  for (int i = 0; i < nq; i++)
+    if (qp[i]) order.push_back(i);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return est_sel[x] * b.G[y] > est_sel[y] * b.G[x];
+  });
+  // touched bitmaps first; if they do not fit, the largest queries store
+  // their keys in the pair loop instead (no bitmap)
+  {
+    long long need = 0;
+    for (int i : order) need += batch_touch_words(b.G[i], D.q[i].flags);
+    std::vector<int> by_g = order;
+    std::stable_sort(by_g.begin(), by_g.end(), [&](int x, int y) { return b.G[x] > b.G[y]; });
+    for (int i : by_g) {
+      if (need <= avail) break;
+      const int tw = batch_touch_words(b.G[i], D.q[i].flags);
+      if (!tw) continue;
+      D.q[i].flags |= kQueryKeysInLoop;
+      need -= tw;
+    }
+    avail -= need;
+  }
+  std::vector<int> sink(nq, kSinkGlobal);
+  auto even = [](long long w) { return (w + 1) & ~1ll; };
+  for (int i : order) {
+    if (!(D.all >> i & 1ull)) continue;
+    // queries that store keys in the pair loop need the 64-bit sum back
+    const int s0 = (D.q[i].flags & kQueryKeysInLoop) ? kSinkLds64 : kSinkLds32;
+    const long long w = even(batch_acc_words(s0, b.G[i]));
+    if (w <= avail) {
+      sink[i] = s0;
+      avail -= w;
+    }
+  }
+  for (int i : order) {
+    if (sink[i] != kSinkLds32) continue;
+    const long long extra = even(batch_acc_words(kSinkLds64, b.G[i])) - even(batch_acc_words(kSinkLds32, b.G[i]));
+    if (extra <= avail) {
+      sink[i] = kSinkLds64;
+      avail -= extra;
+    }
+  }
+  long long lds_words = 0, res_ints = 0, touch_words = 0;
   for (int i = 0; i < nq; i++) {
     BatchQuery& Q = D.q[i];
-    if (!qp[i]) {
-      Q.lds = -1;
-      continue;
-    }
-    if (in_lds[i]) {
-      Q.lds = (short)sum_words;
-      sum_words += b.G[i];
-      b.lds_queries++;
-    } else {
-      Q.lds = -1;
-      b.global_queries++;
-    }
+    Q.sink = (unsigned char)sink[i];
+    Q.acc = (int)lds_words;
+    if (!qp[i]) continue;
+    const int tw = batch_touch_words(b.G[i], Q.flags);
+    lds_words += tw + even(batch_acc_words(sink[i], b.G[i]));
+    if (sink[i] == kSinkGlobal) b.global_queries++;
+    else b.lds_queries++;
     if (fixed && 6 * (long long)b.G[i] > fixed->res_stride)
       return fail(e, "query result larger than its serving slot");
+    if (fixed && tw > fixed->touch_stride) return fail(e, "query groups exceed its serving slot's touched bitmap");
     b.res_off[i] = fixed ? i * fixed->res_stride : res_ints;
+    Q.gtouch = fixed ? i * fixed->touch_stride : (int)touch_words;
     res_ints += 6 * (long long)b.G[i];
+    touch_words += tw;
   }
-  D.lds_sum_words = (int)sum_words;
+  D.lds_words = (int)lds_words;
+  if (fixed) {
+    D.touch_words = nq * fixed->touch_stride;
+    D.touch_arena = fixed->touch;
+  } else {
+    D.touch_words = (int)touch_words;
+    HIP_OK(e->d_touch.reserve(std::max<long long>(16, touch_words * 4)));
+    D.touch_arena = static_cast<unsigned*>(e->d_touch.p);
+  }
   if (fixed) {
     D.res_ints = nq * fixed->res_stride;
     D.res_arena = fixed->res;
@@ -2435,6 +2584,7 @@ int launch_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConf
     for (int d = 0; d < 4 && r1 == hipSuccess; d++)
       if (k.used[d]) r1 = batch_records(k.L, d, k.nrows[d], e->s_main);
     if (r1 == hipSuccess) r1 = batch_scan(k.L, cfg, b.d.ncols, e->s_main);
+    if (r1 == hipSuccess) r1 = batch_keys(k.L, e->s_main);
     hipGraph_t g = nullptr;
     const hipError_t r2 = hipStreamEndCapture(e->s_main, &g);
     if (r1 != hipSuccess || r2 != hipSuccess) {
@@ -2498,51 +2648,6 @@ int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig&
 // Selectivity estimate of one query (the fraction of fact rows it keeps):
 // dimension filters on an evenly spaced sample of the dimension rows, fact
 // BETWEEN filters from the column's value range assuming uniform values.
-struct QueryEst {
-  double sel = 1.0;
-  int G = 1;
-  int njoins = 0;
-  int dims = 0;
-};
-
-double dim_pass_fraction(const Resolved& r, int k) {
-  const Column* f = r.dfilt[k];
-  if (!f) return 1.0;
-  const mordred_join& j = *r.join[k];
-  const long long n = r.dkey[k]->n;
-  const long long step = std::max<long long>(1, n / 4096);
-  long long seen = 0, pass = 0;
-  for (long long row = 0; row < n; row += step, seen++) {
-    const int x = f->host[row];
-    const bool ok = j.filter_mode == 0 ? x == j.c1 : j.filter_mode == 1 ? (x >= j.c1 && x <= j.c2)
-                                                                       : (x == j.c1 || x == j.c2);
-    pass += ok;
-  }
-  return seen ? (double)pass / (double)seen : 1.0;
-}
-
-int estimate_query(mordred_engine* e, const mordred_star_query& q, QueryEst& out) {
-  Resolved r;
-  const int rc = resolve(e, &q, r);
-  if (rc) return rc;
-  double sel = 1.0;
-  for (int k = 0; k < 4; k++) {
-    if (!batch_probes(q, r, k)) continue;
-    sel *= dim_pass_fraction(r, k);
-    out.njoins++;
-    out.dims |= 1 << k;
-  }
-  for (int i = 0; i < q.nfilters; i++) {
-    Column* c = r.filt[i];
-    column_range(c);
-    const double lo = std::max<double>(q.filters[i].lo, c->vmin), hi = std::min<double>(q.filters[i].hi, c->vmax);
-    sel *= hi >= lo ? (hi - lo + 1) / ((double)c->vmax - c->vmin + 1) : 0.0;
-  }
-  out.sel = sel;
-  out.G = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
-  return 0;
-}
-
 // Features of one launch of the shared scan (MORDRED_COST_FEATURES floats).
 void launch_features(const BatchPlan& b, const std::vector<QueryEst>& est, long long rows, int nreset,
                      const mordred::BatchConfig& c, float* f) {
@@ -2558,7 +2663,7 @@ void launch_features(const BatchPlan& b, const std::vector<QueryEst>& est, long 
     s_all += q.sel;
     g_all += q.G;
     gmax = std::max<double>(gmax, q.G);
-    if (D.q[i].lds >= 0) {
+    if (D.q[i].sink != mordred::kSinkGlobal) {
       s_lds += q.sel;
       g_lds += q.G;
     } else {
@@ -3412,6 +3517,9 @@ int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n,
   ar.res_stride = ((6 * (long long)gmax) + 3) & ~3ll;
   HIP_OK(e->d_serve_res.reserve((size_t)nslot * ar.res_stride * sizeof(int)));
   ar.res = static_cast<int*>(e->d_serve_res.p);
+  ar.touch_stride = batch_touch_words(gmax, 0);
+  HIP_OK(e->d_serve_touch.reserve(std::max<size_t>(16, (size_t)nslot * ar.touch_stride * sizeof(unsigned))));
+  ar.touch = static_cast<unsigned*>(e->d_serve_touch.p);
   auto state_bytes = [&](int i) { return 24.0 * est[i].G; };
   if (o->learned && !e->cm) {
     e->cm = mordred_cost_model_create(e->device, 12345);
@@ -3467,12 +3575,16 @@ int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n,
     int limit = nseg;
     for (const Slot& sl : slots)
       if (sl.req >= 0 && sl.start > cur) limit = std::min(limit, sl.start);
-    BatchPlan b;
-    int rc = plan_batch(e, qp.data(), nslot, cfg, b, &ar);
-    if (rc) return rc;
     std::vector<QueryEst> es(nslot);
+    std::vector<double> sel(nslot, 0.0);
     for (int sidx = 0; sidx < nslot; sidx++)
-      if (slots[sidx].req >= 0) es[sidx] = est[slots[sidx].req];
+      if (slots[sidx].req >= 0) {
+        es[sidx] = est[slots[sidx].req];
+        sel[sidx] = es[sidx].sel;
+      }
+    BatchPlan b;
+    int rc = plan_batch(e, qp.data(), nslot, cfg, b, &ar, sel.data());
+    if (rc) return rc;
     b.d.reset = reset;
     b.d.rebuild = changed ? 1 : 0;
     int nreset = 0;
