@@ -1,12 +1,16 @@
-// Shared-scan batch kernels for gfx950 (see batch.hpp for the record table
-// and the batch descriptor).
+// Shared-scan batch kernels for gfx950 (see batch.hpp for the dimension
+// tables and the batch descriptor).
 //
-//   batch_fill_kernel     clears the packed tables of the used dimensions (when
-//                         rebuilt), and the result rows of the queries being
-//                         started;
-//   batch_records_kernel  one pass over a dimension: slot and packed entry
-//                         (the builds of all the batch's queries in one);
-//   batch_kernel          the persistent shared scan of the fact columns.
+//   batch_fill_kernel     clears the dimension tables of the used dimensions
+//                         (when rebuilt), and the result rows and touched
+//                         bitmaps of the queries being started;
+//   batch_records_kernel  one pass over a dimension: per row its slot, the
+//                         mask of the queries whose filter it passes, and its
+//                         packed group attributes (the builds of all the
+//                         batch's queries in one);
+//   batch_kernel          the persistent shared scan of the fact columns;
+//   batch_keys_kernel     the keys of every touched group, decoded from the
+//                         group index.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,23 +21,25 @@
 
 namespace mordred {
 
-
-// Clears the packed tables of the used dimensions when they are rebuilt, and
-// the result rows of the queries being (re)started.
-__global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __restrict__ D, unsigned* status,
-                                                         uint4* r0, uint4* r1, uint4* r2, uint4* r3,
-                                                         int n0, int n1, int n2, int n3) {
+// Clears the dimension tables of the used dimensions when they are rebuilt,
+// and the result rows and touched bitmaps of the queries being (re)started.
+__global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __restrict__ D, unsigned* status) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0) status[0] = 0;
   const uint4 z = make_uint4(0, 0, 0, 0);
   if (D->rebuild) {
-    uint4* recs[4] = {r0, r1, r2, r3};
-    const int lens[4] = {n0, n1, n2, n3};
 #pragma unroll
     for (int d = 0; d < 4; d++) {
-      if (!recs[d] || !D->dim[d].used) continue;
-      for (long long i = t; i < lens[d]; i += stride) recs[d][i] = z;
+      const BatchDim& B = D->dim[d];
+      if (!B.used) continue;
+      uint4* mk = reinterpret_cast<uint4*>(B.mk);
+      const long long nm = ((long long)B.len + 1) / 2;
+      if (mk)
+        for (long long i = t; i < nm; i += stride) mk[i] = z;
+      uint4* pk = reinterpret_cast<uint4*>(B.pk);
+      if (pk)
+        for (long long i = t; i < B.pk_words; i += stride) pk[i] = z;
     }
   }
   const unsigned long long reset = D->reset;
@@ -47,6 +53,59 @@ __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __rest
     const int nt = batch_touch_words(Q.total_val, Q.flags);
     unsigned* tb = D->touch_arena + Q.gtouch;
     for (long long i = t; i < nt; i += stride) tb[i] = 0;
+  }
+}
+
+// One thread per dimension row: its slot, the mask of the queries whose
+// filter the row passes (build_GPU2's modes, GPUProcessing.h:972-980: 0 EQ,
+// 1 BETWEEN, 2 EQ-or-EQ), and its packed group attributes.  The host has
+// checked that no two rows share a slot.
+__global__ __launch_bounds__(256) void batch_records_kernel(BatchLaunch L, int d) {
+  const BatchDim& B = L.desc->dim[d];
+  if (!B.used || !L.desc->rebuild) return;
+  const int nrows = B.nrows, len = B.len, mk = B.minkey, pw = B.pw, na = B.nattr, nfc = B.nfcol, nf = B.nfilt;
+  const int stride = gridDim.x * blockDim.x;
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < nrows; row += stride) {
+    const int key = gather(L.cache, B.key, row);
+    const int h = phash(key, len, mk);
+    if ((unsigned)h >= (unsigned)len) continue;  // key below min: never probed in range
+    if (B.form == kDimAttr) {
+      // every attribute as value - base + 1: an occupied slot is never 0
+      uint32_t w = na ? 0u : 1u;
+      for (int a = 0; a < na; a++) {
+        const BatchAttr A = B.attr[a];
+        const uint32_t v = (uint32_t)(gather(L.cache, A.col, row) - A.base) + 1u;
+        w |= (v & ((1u << A.bits) - 1u)) << A.off;
+      }
+      if (pw == 1) static_cast<uint8_t*>(B.pk)[h] = (uint8_t)w;
+      else if (pw == 2) static_cast<uint16_t*>(B.pk)[h] = (uint16_t)w;
+      else static_cast<uint32_t*>(B.pk)[h] = w;
+      continue;
+    }
+    int fv[kBatchMaxFCol];
+#pragma unroll
+    for (int c = 0; c < kBatchMaxFCol; c++) fv[c] = c < nfc ? gather(L.cache, B.fcol[c], row) : 0;
+    unsigned long long m = B.uses;
+    for (int f = 0; f < nf; f++) {
+      const BatchFilt F = B.filt[f];
+      int x = key;
+#pragma unroll
+      for (int c = 0; c < kBatchMaxFCol; c++)
+        if (F.src == c) x = fv[c];
+      const bool ok = F.mode == 0 ? x == F.c1 : F.mode == 1 ? (x >= F.c1 && x <= F.c2) : (x == F.c1 || x == F.c2);
+      if (!ok) m &= ~(1ull << F.q);
+    }
+    B.mk[h] = m;
+    if (!B.pk) continue;
+    uint32_t w = 0;
+    for (int a = 0; a < na; a++) {
+      const BatchAttr A = B.attr[a];
+      const uint32_t v = (uint32_t)(gather(L.cache, A.col, row) - A.base);
+      w |= (v & ((1u << A.bits) - 1u)) << A.off;
+    }
+    if (pw == 1) static_cast<uint8_t*>(B.pk)[h] = (uint8_t)w;
+    else if (pw == 2) static_cast<uint16_t*>(B.pk)[h] = (uint16_t)w;
+    else static_cast<uint32_t*>(B.pk)[h] = w;
   }
 }
 
@@ -73,33 +132,12 @@ __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchDesc* __rest
   }
 }
 
-// One thread per dimension row: its slot and its packed entry (present bit
-// plus the attributes the batch filters or groups on).  The host has checked
-// that no two rows share a slot.
-__global__ __launch_bounds__(256) void batch_records_kernel(BatchLaunch L, int d) {
-  const BatchDim& B = L.desc->dim[d];
-  if (!B.used || !L.desc->rebuild) return;
-  const int nrows = B.nrows, len = B.len, mk = B.minkey, pw = B.pw, na = B.nattr;
-  const int stride = gridDim.x * blockDim.x;
-  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < nrows; row += stride) {
-    const int h = phash(gather(L.cache, B.key, row), len, mk);
-    if ((unsigned)h >= (unsigned)len) continue;  // key below min: never probed in range
-    uint32_t w = 1u << (8 * pw - 1);
-    for (int a = 0; a < na; a++) {
-      const BatchAttr A = B.attr[a];
-      const uint32_t v = (uint32_t)(gather(L.cache, A.col, row) - A.base);
-      w |= (v & ((1u << A.bits) - 1u)) << A.off;
-    }
-    if (pw == 1) static_cast<uint8_t*>(B.pk)[h] = (uint8_t)w;
-    else if (pw == 2) static_cast<uint16_t*>(B.pk)[h] = (uint16_t)w;
-    else static_cast<uint32_t*>(B.pk)[h] = w;
-  }
-}
-
-// The shared scan.  A workgroup walks one contiguous range of tiles; the
-// columns of the next tile are loaded while the current one is processed.
-// A wave keeps the cache slots of the current segment's columns in one VGPR
-// (lane c = column c) and reloads them only when the segment changes.
+// The shared scan.  A workgroup walks one contiguous range of tiles.  The
+// next tile's columns are loaded once the current tile's masks, group
+// attributes and aggregate values are known (the columns are dead then), so
+// the loads are in flight during the pair work.  A wave keeps the cache slots
+// of the current segment's columns in one VGPR (lane c = column c) and
+// reloads them only when the segment changes.
 template <int BLOCK, int VPT, int MAXC>
 __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   constexpr int IPT = 4 * VPT;
@@ -109,15 +147,18 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   __shared__ BatchHot H;
   const int nq = D->nq;
   BatchQuery* lq = reinterpret_cast<BatchQuery*>(lds_raw);
-  unsigned long long* llut = reinterpret_cast<unsigned long long*>(lds_raw + nq * sizeof(BatchQuery));
-  // group region: per query a touched bitmap, then LDS accumulators (8-byte aligned)
-  unsigned* lgrp = reinterpret_cast<unsigned*>(llut + D->nlut);
+  // interval tables, then the group region: per query a touched bitmap, then
+  // LDS accumulators (8-byte aligned)
+  unsigned* ltab = reinterpret_cast<unsigned*>(lds_raw + nq * sizeof(BatchQuery));
+  const int ntab = (D->ntab + 1) & ~1;
+  unsigned* lgrp = ltab + ntab;
   const int ngrp = D->lds_words;
   {
     const uint4* src = reinterpret_cast<const uint4*>(D->q);
     uint4* dst = reinterpret_cast<uint4*>(lq);
     for (int i = threadIdx.x; i < nq * (int)(sizeof(BatchQuery) / 16); i += BLOCK) dst[i] = src[i];
     for (int i = threadIdx.x; i < ngrp; i += BLOCK) lgrp[i] = 0;
+    for (int i = threadIdx.x; i < ntab; i += BLOCK) ltab[i] = D->tab[i];
     const int np = D->npred;
     for (int i = threadIdx.x; i < np; i += BLOCK) {
       const BatchPred P = D->pred[i];
@@ -129,33 +170,28 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       const BatchExpr E = D->expr[threadIdx.x];
       H.expr[threadIdx.x] = make_int4(E.kind, E.a1, E.a2, 0);
     }
+    if (threadIdx.x < kBatchMaxIvl) H.ivl[threadIdx.x] = D->ivl[threadIdx.x];
     if (threadIdx.x < 4) {
       const int d = threadIdx.x;
       const BatchDim& B = D->dim[d];
       BatchHotDim& Hd = H.dim[d];
+      Hd.mk = B.mk;
       Hd.pk = B.pk;
       Hd.uses = B.uses;
+      Hd.groupers = B.groupers;
       Hd.len = B.len;
       Hd.minkey = B.minkey;
       Hd.pw = B.pw;
+      Hd.form = B.form;
       Hd.nattr = B.nattr;
-      Hd.ncmp = B.ncmp;
-      Hd.ivl_src = B.ivl_src;
-      Hd.ivl_n = B.ivl_n;
-      Hd.ivl_off = B.ivl_off;
       for (int a = 0; a < kBatchMaxAttr; a++) {
+        // attribute form stores value - base + 1
+        Hd.base[a] = B.attr[a].base - (B.form == kDimAttr ? 1 : 0);
         Hd.off[a] = B.attr[a].off;
         Hd.bits[a] = B.attr[a].bits;
         Hd.lut[a] = B.attr[a].lut;
-        Hd.base[a] = B.attr[a].base;
       }
     }
-    for (int i = threadIdx.x; i < 4 * kBatchMaxCmp; i += BLOCK) {
-      const BatchCmp C = D->dim[i / kBatchMaxCmp].cmp[i % kBatchMaxCmp];
-      H.cmp[i / kBatchMaxCmp][i % kBatchMaxCmp] = make_int4(C.src, C.lo, (int)((unsigned)C.hi - (unsigned)C.lo), C.q);
-    }
-    const int nl = D->nlut;
-    for (int i = threadIdx.x; i < nl; i += BLOCK) llut[i] = D->lut[i];
     if (threadIdx.x == 0) {
       int cm = 0, um = 0;
       for (int c = 0; c < D->ncols; c++)
@@ -168,7 +204,8 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       H.nexpr = D->nexpr;
       H.npred = np;
       H.nq = nq;
-      H.nlut = nl;
+      H.ntab = ntab;
+      H.nivl = D->nivl;
     }
   }
   __syncthreads();
@@ -176,6 +213,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   const int colmask = __builtin_amdgcn_readfirstlane(H.colmask);
   const int usedmask = __builtin_amdgcn_readfirstlane(H.usedmask);
   const int nexpr = __builtin_amdgcn_readfirstlane(H.nexpr);
+  const int nivl = __builtin_amdgcn_readfirstlane(H.nivl);
   const unsigned long long all = D->all;
   const int lane = threadIdx.x & (kWave - 1);
 
@@ -231,20 +269,57 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     }
   };
 
-  auto process = [&](int t, const int (&x)[MAXC][IPT]) {
+  unsigned long long alive[IPT];
+  uint32_t attrw[4][IPT];
+  int ev[kBatchMaxExpr][IPT];
+
+  // the alive mask of every item, its group attribute words and the values
+  // of the aggregate expressions
+  auto masks = [&](int t, const int (&x)[MAXC][IPT]) {
     const int pos = t * TILE;
     const int n = min(TILE, num_tuples - pos);
     const uint32_t inr = n == TILE ? (IPT == 32 ? 0xFFFFFFFFu : (1u << IPT) - 1u) : range_mask<BLOCK, VPT>(n);
-    // alive queries of each item, as two 32-bit halves
-    uint32_t alo[IPT], ahi[IPT];
 #pragma unroll
-    for (int j = 0; j < IPT; j++) {
-      alo[j] = (inr >> j & 1u) ? (uint32_t)all : 0u;
-      ahi[j] = (inr >> j & 1u) ? (uint32_t)(all >> 32) : 0u;
-    }
+    for (int j = 0; j < IPT; j++) alive[j] = (inr >> j & 1u) ? all : 0ull;
 
-    // fact BETWEEN predicates, grouped by column (GPUProcessing.h:72-86); the
-    // half a predicate clears is chosen by a wave-uniform branch
+    // fact predicates with an interval table: bucket -> rank at the bucket
+    // start -> at most kmax breakpoints inside it -> the interval's mask
+    for (int v = 0; v < nivl; v++) {
+      const BatchIvl& I = H.ivl[v];
+      const int col = __builtin_amdgcn_readfirstlane(I.col);
+      const int base = __builtin_amdgcn_readfirstlane(I.base);
+      const int shift = __builtin_amdgcn_readfirstlane(I.shift);
+      const unsigned last = (unsigned)__builtin_amdgcn_readfirstlane(I.nb) - 1u;
+      const int kmax = __builtin_amdgcn_readfirstlane(I.kmax);
+      const int nbp = __builtin_amdgcn_readfirstlane(I.n);
+      const unsigned short* rk = reinterpret_cast<const unsigned short*>(ltab + __builtin_amdgcn_readfirstlane(I.rank_off));
+      const int* bp = reinterpret_cast<const int*>(ltab + __builtin_amdgcn_readfirstlane(I.bp_off));
+      const unsigned long long* mt =
+          reinterpret_cast<const unsigned long long*>(ltab + __builtin_amdgcn_readfirstlane(I.mask_off));
+      int xv[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; j++) xv[j] = 0;
+#pragma unroll
+      for (int c = 0; c < MAXC; c++)
+        if (c == col) {
+#pragma unroll
+          for (int j = 0; j < IPT; j++) xv[j] = x[c][j];
+        }
+      int r[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; j++) r[j] = rk[min((unsigned)(xv[j] - base) >> shift, last)];
+      int add[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; j++) add[j] = 0;
+      for (int k = 0; k < kmax; k++) {
+#pragma unroll
+        for (int j = 0; j < IPT; j++) add[j] += xv[j] >= bp[r[j] + k];
+      }
+#pragma unroll
+      for (int j = 0; j < IPT; j++) alive[j] &= mt[min(r[j] + add[j], nbp)];
+    }
+    // the other fact BETWEEN predicates (GPUProcessing.h:72-86), grouped by
+    // column; the half a predicate clears is chosen by a wave-uniform branch
 #pragma unroll
     for (int c = 0; c < MAXC; c++) {
       if (!(colmask >> c & 1)) continue;
@@ -255,151 +330,98 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         const int lo = __builtin_amdgcn_readfirstlane(P.x);
         const unsigned span = (unsigned)__builtin_amdgcn_readfirstlane(P.y);
         const int q = __builtin_amdgcn_readfirstlane(P.z);
-        const uint32_t keep = ~(1u << (q & 31));
-        if (q < 32) {
+        const unsigned long long keep = ~(1ull << q);
 #pragma unroll
-          for (int j = 0; j < IPT; j++)
-            if ((unsigned)(x[c][j] - lo) > span) alo[j] &= keep;
+        for (int j = 0; j < IPT; j++)
+          if ((unsigned)(x[c][j] - lo) > span) alive[j] &= keep;
+      }
+    }
+
+    // one probe per dimension for all queries (every dimension's gathers
+    // issued before any is used; only items some query still needs load):
+    // mask form, the slot's query mask; attribute form, the slot's attribute
+    // word, whose filtered attributes index their mask tables in LDS
+    unsigned long long mv[4][IPT];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      if (!(usedmask >> d & 1)) continue;
+      const BatchHotDim& Hd = H.dim[d];
+      const int len = __builtin_amdgcn_readfirstlane(Hd.len);
+      const int mk = __builtin_amdgcn_readfirstlane(Hd.minkey);
+      const int form = __builtin_amdgcn_readfirstlane(Hd.form);
+      const int pw = __builtin_amdgcn_readfirstlane(Hd.pw);
+      const unsigned long long uses = Hd.uses;
+      const unsigned long long* mkt = Hd.mk;
+      const void* pk = Hd.pk;
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        const int h = phash(x[d][j], len, mk);
+        const bool need = (unsigned)h < (unsigned)len && (alive[j] & uses);
+        if (form == kDimMask) {
+          mv[d][j] = need ? mkt[h] : 0ull;
         } else {
-#pragma unroll
-          for (int j = 0; j < IPT; j++)
-            if ((unsigned)(x[c][j] - lo) > span) ahi[j] &= keep;
+          uint32_t w = 0;
+          if (need)
+            w = pw == 1 ? static_cast<const uint8_t*>(pk)[h]
+                        : pw == 2 ? static_cast<const uint16_t*>(pk)[h] : static_cast<const uint32_t*>(pk)[h];
+          mv[d][j] = w;
         }
       }
     }
-    unsigned long long alive[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; j++) alive[j] = (unsigned long long)ahi[j] << 32 | alo[j];
-
-    // one probe per dimension for all queries: the packed entries of the tile
-    // are gathered before any is used (items no query needs read slot 0);
-    // the filter masks come from the attribute mask tables and range compares
-    uint32_t attrw[4][IPT];
 #pragma unroll
     for (int d = 0; d < 4; d++) {
 #pragma unroll
       for (int j = 0; j < IPT; j++) attrw[d][j] = 0;
       if (!(usedmask >> d & 1)) continue;
       const BatchHotDim& Hd = H.dim[d];
-      const int len = __builtin_amdgcn_readfirstlane(Hd.len);
-      const int mk = __builtin_amdgcn_readfirstlane(Hd.minkey);
-      const int pw = __builtin_amdgcn_readfirstlane(Hd.pw);
       const unsigned long long uses = Hd.uses;
-      const void* pk = Hd.pk;
-      int hs[IPT];
-      bool inb[IPT];
+      if (__builtin_amdgcn_readfirstlane(Hd.form) == kDimMask) {
 #pragma unroll
-      for (int j = 0; j < IPT; j++) {
-        const int h = phash(x[d][j], len, mk);
-        inb[j] = (unsigned)h < (unsigned)len;
-        hs[j] = (inb[j] && (alive[j] & uses)) ? h : 0;
-      }
-      uint32_t w[IPT];
-      if (pw == 1) {
-#pragma unroll
-        for (int j = 0; j < IPT; j++) w[j] = static_cast<const uint8_t*>(pk)[hs[j]];
-      } else if (pw == 2) {
-#pragma unroll
-        for (int j = 0; j < IPT; j++) w[j] = static_cast<const uint16_t*>(pk)[hs[j]];
-      } else {
-#pragma unroll
-        for (int j = 0; j < IPT; j++) w[j] = static_cast<const uint32_t*>(pk)[hs[j]];
-      }
-      const int top = 8 * pw - 1;
-      uint32_t mlo[IPT], mhi[IPT];
-#pragma unroll
-      for (int j = 0; j < IPT; j++) {
-        if (!inb[j]) w[j] = 0;
-        mlo[j] = 0xFFFFFFFFu;
-        mhi[j] = 0xFFFFFFFFu;
+        for (int j = 0; j < IPT; j++) alive[j] &= mv[d][j] | ~uses;
+        continue;
       }
       const int na = __builtin_amdgcn_readfirstlane(Hd.nattr);
       for (int a = 0; a < na; a++) {
         const int lut = __builtin_amdgcn_readfirstlane(Hd.lut[a]);
         if (lut < 0) continue;
-        const int off = __builtin_amdgcn_readfirstlane(Hd.off[a]);
-        const uint32_t vm = (1u << __builtin_amdgcn_readfirstlane(Hd.bits[a])) - 1u;
+        const unsigned off = (unsigned)__builtin_amdgcn_readfirstlane(Hd.off[a]);
+        const unsigned bits = (unsigned)__builtin_amdgcn_readfirstlane(Hd.bits[a]);
+        const unsigned long long* lt = reinterpret_cast<const unsigned long long*>(ltab + lut);
 #pragma unroll
         for (int j = 0; j < IPT; j++) {
-          const unsigned long long m = llut[lut + ((w[j] >> off) & vm)];
-          mlo[j] &= (uint32_t)m;
-          mhi[j] &= (uint32_t)(m >> 32);
-        }
-      }
-      // interval table: the number of breakpoints at or below the value
-      // (binary lifting over a wave-uniform number of steps) picks the mask
-      const int in = __builtin_amdgcn_readfirstlane(Hd.ivl_n);
-      if (in > 0) {
-        const int isrc = __builtin_amdgcn_readfirstlane(Hd.ivl_src);
-        const int ioff = __builtin_amdgcn_readfirstlane(Hd.ivl_off);
-        int v[IPT];
-        if (isrc < 0) {
-#pragma unroll
-          for (int j = 0; j < IPT; j++) v[j] = x[d][j];
-        } else {
-          const int off = __builtin_amdgcn_readfirstlane(Hd.off[isrc]);
-          const uint32_t vm = (1u << __builtin_amdgcn_readfirstlane(Hd.bits[isrc])) - 1u;
-          const int base = __builtin_amdgcn_readfirstlane(Hd.base[isrc]);
-#pragma unroll
-          for (int j = 0; j < IPT; j++) v[j] = base + (int)((w[j] >> off) & vm);
-        }
-        int top_step = 1;
-        while (top_step <= in) top_step <<= 1;
-        int pos[IPT];
-#pragma unroll
-        for (int j = 0; j < IPT; j++) pos[j] = 0;
-        for (int st = top_step >> 1; st > 0; st >>= 1) {
-#pragma unroll
-          for (int j = 0; j < IPT; j++)
-            if (pos[j] + st <= in && (int)(uint32_t)llut[ioff + pos[j] + st - 1] <= v[j]) pos[j] += st;
-        }
-#pragma unroll
-        for (int j = 0; j < IPT; j++) {
-          const unsigned long long m = llut[ioff + in + pos[j]];
-          mlo[j] &= (uint32_t)m;
-          mhi[j] &= (uint32_t)(m >> 32);
-        }
-      }
-      const int nc = __builtin_amdgcn_readfirstlane(Hd.ncmp);
-      for (int c = 0; c < nc; c++) {
-        const int4 C = H.cmp[d][c];
-        const int src = __builtin_amdgcn_readfirstlane(C.x);
-        const int lo = __builtin_amdgcn_readfirstlane(C.y);
-        const unsigned span = (unsigned)__builtin_amdgcn_readfirstlane(C.z);
-        const int q = __builtin_amdgcn_readfirstlane(C.w);
-        int v[IPT];
-        if (src < 0) {
-#pragma unroll
-          for (int j = 0; j < IPT; j++) v[j] = x[d][j];
-        } else {
-          const int off = __builtin_amdgcn_readfirstlane(Hd.off[src]);
-          const uint32_t vm = (1u << __builtin_amdgcn_readfirstlane(Hd.bits[src])) - 1u;
-          const int base = __builtin_amdgcn_readfirstlane(Hd.base[src]);
-#pragma unroll
-          for (int j = 0; j < IPT; j++) v[j] = base + (int)((w[j] >> off) & vm);
-        }
-        const uint32_t keep = ~(1u << (q & 31));
-        if (q < 32) {
-#pragma unroll
-          for (int j = 0; j < IPT; j++)
-            if ((unsigned)(v[j] - lo) > span) mlo[j] &= keep;
-        } else {
-#pragma unroll
-          for (int j = 0; j < IPT; j++)
-            if ((unsigned)(v[j] - lo) > span) mhi[j] &= keep;
+          const uint32_t f = __builtin_amdgcn_ubfe((uint32_t)mv[d][j], off, bits);
+          alive[j] &= lt[f ? f - 1u : 0u];  // (an empty slot clears its users below)
         }
       }
 #pragma unroll
       for (int j = 0; j < IPT; j++) {
-        const bool present = (w[j] >> top) & 1u;
-        const unsigned long long m = present ? ((unsigned long long)mhi[j] << 32 | mlo[j]) : 0ull;
-        alive[j] &= m | ~uses;
-        attrw[d][j] = w[j];
+        const uint32_t w = (uint32_t)mv[d][j];
+        if (!w) alive[j] &= ~uses;
+        attrw[d][j] = w;
+      }
+    }
+    // group attributes of the mask-form dimensions, for the survivors
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      if (!(usedmask >> d & 1)) continue;
+      const BatchHotDim& Hd = H.dim[d];
+      const unsigned long long groupers = Hd.groupers;
+      if (!groupers || __builtin_amdgcn_readfirstlane(Hd.form) != kDimMask) continue;
+      const int pw = __builtin_amdgcn_readfirstlane(Hd.pw);
+      const int len = __builtin_amdgcn_readfirstlane(Hd.len);
+      const int mk = __builtin_amdgcn_readfirstlane(Hd.minkey);
+      const void* pk = Hd.pk;
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        // a survivor for a grouping query passed this dimension: its slot is in range
+        if (!(alive[j] & groupers)) continue;
+        const int h = phash(x[d][j], len, mk);
+        attrw[d][j] = pw == 1 ? static_cast<const uint8_t*>(pk)[h]
+                              : pw == 2 ? static_cast<const uint16_t*>(pk)[h] : static_cast<const uint32_t*>(pk)[h];
       }
     }
 
     // aggregate expressions of the batch (int32 arithmetic, GPUProcessing.h:149, 1763)
-    int ev[kBatchMaxExpr][IPT];
 #pragma unroll
     for (int e = 0; e < kBatchMaxExpr; e++) {
 #pragma unroll
@@ -428,8 +450,10 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         ev[e][j] = kind == kExprProduct ? (int)((unsigned)v1[j] * (unsigned)v2[j])
                                         : (int)((unsigned)v1[j] - (unsigned)v2[j]);
     }
+  };
 
-    // every surviving (row, query) pair
+  // every surviving (row, query) pair
+  auto pairs = [&]() {
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
       unsigned long long m = alive[j];
@@ -507,14 +531,12 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   };
 
   if (t_begin < t_end) {
-    int xa[MAXC][IPT], xb[MAXC][IPT];
-    load_cols(t_begin, xa);
-    for (int t = t_begin; t < t_end; t += 2) {
-      if (t + 1 < t_end) load_cols(t + 1, xb);
-      process(t, xa);
-      if (t + 1 >= t_end) break;
-      if (t + 2 < t_end) load_cols(t + 2, xa);
-      process(t + 1, xb);
+    int x[MAXC][IPT];
+    load_cols(t_begin, x);
+    for (int t = t_begin; t < t_end; t++) {
+      masks(t, x);
+      if (t + 1 < t_end) load_cols(t + 1, x);
+      pairs();
     }
   }
 
@@ -563,18 +585,21 @@ const void* scan_fn() {
 }
 
 // Scan kernel of a configuration: columns held in registers per tile = 8 when
-// the batch reads at most 8 fact columns, else 12.
+// the batch reads at most 8 fact columns, else 12.  With 12 columns the
+// kernel keeps 4 rows per thread and at most 512 threads per workgroup (the
+// register budget of 1,024 threads, or of 8 rows per thread, would spill);
+// the configuration's LDS budget is kept.
 const void* pick(const BatchConfig& c, int ncols) {
   const bool small = ncols <= 8;
+  if (c.vpt != 1 && c.vpt != 2) return nullptr;
+  if (c.block != 256 && c.block != 512 && (c.block != 1024 || c.vpt != 1)) return nullptr;
+  if (!small) return c.block == 256 ? scan_fn<256, 1, 12>() : scan_fn<512, 1, 12>();
   if (c.vpt == 1) {
-    if (c.block == 256) return small ? scan_fn<256, 1, 8>() : scan_fn<256, 1, 12>();
-    if (c.block == 512) return small ? scan_fn<512, 1, 8>() : scan_fn<512, 1, 12>();
-    if (c.block == 1024) return small ? scan_fn<1024, 1, 8>() : scan_fn<1024, 1, 12>();
-  } else if (c.vpt == 2) {
-    if (c.block == 256) return small ? scan_fn<256, 2, 8>() : scan_fn<256, 2, 12>();
-    if (c.block == 512) return small ? scan_fn<512, 2, 8>() : scan_fn<512, 2, 12>();
+    if (c.block == 256) return scan_fn<256, 1, 8>();
+    if (c.block == 512) return scan_fn<512, 1, 8>();
+    return scan_fn<1024, 1, 8>();
   }
-  return nullptr;
+  return c.block == 256 ? scan_fn<256, 2, 8>() : scan_fn<512, 2, 8>();
 }
 
 template <int BLOCK, int VPT, int MAXC>
@@ -587,15 +612,18 @@ hipError_t launch(const BatchLaunch& L, int lds, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int MAXC>
-hipError_t launch_c(const BatchLaunch& L, const BatchConfig& c, hipStream_t s) {
+hipError_t launch_c8(const BatchLaunch& L, const BatchConfig& c, hipStream_t s) {
   if (c.vpt == 1) {
-    if (c.block == 256) return launch<256, 1, MAXC>(L, c.lds_bytes, s);
-    if (c.block == 512) return launch<512, 1, MAXC>(L, c.lds_bytes, s);
-    return launch<1024, 1, MAXC>(L, c.lds_bytes, s);
+    if (c.block == 256) return launch<256, 1, 8>(L, c.lds_bytes, s);
+    if (c.block == 512) return launch<512, 1, 8>(L, c.lds_bytes, s);
+    return launch<1024, 1, 8>(L, c.lds_bytes, s);
   }
-  if (c.block == 256) return launch<256, 2, MAXC>(L, c.lds_bytes, s);
-  return launch<512, 2, MAXC>(L, c.lds_bytes, s);
+  if (c.block == 256) return launch<256, 2, 8>(L, c.lds_bytes, s);
+  return launch<512, 2, 8>(L, c.lds_bytes, s);
+}
+
+hipError_t launch_c12(const BatchLaunch& L, const BatchConfig& c, hipStream_t s) {
+  return c.block == 256 ? launch<256, 1, 12>(L, c.lds_bytes, s) : launch<512, 1, 12>(L, c.lds_bytes, s);
 }
 
 }  // namespace
@@ -613,13 +641,12 @@ hipError_t batch_prepare(const BatchConfig& c, int ncols) {
 
 int batch_resident(const BatchConfig& c, int ncols) {
   const void* f = pick(c, ncols);
-  return f ? resident_groups(f, c.block, c.lds_bytes - sizeof(BatchHot)) : 0;
+  const int block = ncols <= 8 ? c.block : std::min(c.block, 512);
+  return f ? resident_groups(f, block, c.lds_bytes - sizeof(BatchHot)) : 0;
 }
 
-hipError_t batch_fill(const BatchLaunch& L, const uint4* const* rec, const int* rec_len, hipStream_t s) {
-  batch_fill_kernel<<<2048, 256, 0, s>>>(L.desc, L.status, const_cast<uint4*>(rec[0]), const_cast<uint4*>(rec[1]),
-                                         const_cast<uint4*>(rec[2]), const_cast<uint4*>(rec[3]), rec_len[0],
-                                         rec_len[1], rec_len[2], rec_len[3]);
+hipError_t batch_fill(const BatchLaunch& L, hipStream_t s) {
+  batch_fill_kernel<<<2048, 256, 0, s>>>(L.desc, L.status);
   return hipGetLastError();
 }
 
@@ -636,7 +663,7 @@ hipError_t batch_keys(const BatchLaunch& L, hipStream_t s) {
 
 hipError_t batch_scan(const BatchLaunch& L, const BatchConfig& c, int ncols, hipStream_t s) {
   if (!batch_config_ok(c) || ncols > kBatchMaxCols) return hipErrorInvalidValue;
-  return ncols <= 8 ? launch_c<8>(L, c, s) : launch_c<12>(L, c, s);
+  return ncols <= 8 ? launch_c8(L, c, s) : launch_c12(L, c, s);
 }
 
 }  // namespace mordred

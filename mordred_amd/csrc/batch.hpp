@@ -2,29 +2,42 @@
 //
 // One persistent kernel scans the fact columns once for the whole batch.  Per
 // fact row it evaluates every query's fact predicates as bits of a 64-bit
-// "alive" word, probes each joined dimension ONCE through a packed attribute
-// table, and accumulates every (row, query) pair that survives.
+// "alive" word, probes each joined dimension ONCE for all queries, and
+// accumulates every (row, query) pair that survives.
 //
-// Packed table of dimension d (one 1-, 2- or 4-byte entry per hash slot, HASH
-// = (key-min) % len, the slot of the reference's perfect-hash table,
-// crystal/join.cuh:3): the top bit says a dimension row occupies the slot,
-// the other bits hold the row's attributes that the batch filters or groups
-// on (value - base).  The mask of queries whose dimension filter the row
-// passes is rebuilt per fact row:
-//   * a filter on an attribute with a small value span: a 64-bit mask per
-//     attribute value (bit q = query q passes, or does not filter on it),
-//     staged in LDS and AND-ed over the attributes;
-//   * a filter on the dimension key or on a wide attribute: a range compare.
+// Per used dimension the batch builds tables indexed by the perfect-hash slot
+// HASH = (key-min) % len of the reference's hash table (crystal/join.cuh:3),
+// in one of two forms chosen by the table's size (every probe is a random
+// gather, so what matters is how much of the table stays in L2):
+//   * mask form (small dimensions): a query-mask table, 64 bits per slot, bit
+//     q set iff a dimension row occupies the slot and passes query q's
+//     dimension filter (or q does not filter this dimension), 0 for an empty
+//     slot -- the union of the reference builds of all the batch's queries
+//     (build_GPU2, GPUProcessing.h:927-997), evaluated once per dimension
+//     row -- plus, when a query groups on the dimension, a table of the
+//     row's group attributes packed at fixed bit positions;
+//   * attribute form (large dimensions): one table of 1, 2 or 4 bytes per
+//     slot holding every attribute the batch filters or groups on, each as
+//     value - base + 1 at a fixed bit position, so that an empty slot is 0.
+//     A filter on a narrow attribute reads a mask table in LDS (one 64-bit
+//     query mask per value); a filter on the key is evaluated on the fact
+//     row's key (the slot's row has that key) through the key column's
+//     interval table.
 // A fact row survives for query q iff every table q joins holds a row in the
-// probed slot that passes q's filter -- exactly the rows q's own build_GPU2
-// inserts (GPUProcessing.h:927-997) -- and its group value for table d is the
+// probed slot that passes q's filter, and its group value for table d is the
 // attribute q groups on (the value word of q's own hash table), so every
 // query's result equals its plain run bit for bit.  The host checks that keys
-// cannot collide (dimension and fact key ranges within dim_len), which makes
-// the dimension key of a probed row the fact row's key.
+// cannot collide (dimension and fact key ranges within dim_len).
+//
+// The constraints on one fact column -- fact BETWEEN predicates, and key
+// filters of attribute-form dimensions on their fact key -- become an interval
+// table when there are enough of them: their ends split the column's value
+// range into intervals, each with the mask of the queries that accept it; a
+// row finds its interval from a bucket table (rank at the bucket start) plus
+// at most kmax compares with the breakpoints inside the bucket.
 //
 // Everything that changes between batches lives in a device BatchDesc, so
-// one captured hipGraph (fill, packed-table builds, scan) serves every batch
+// one captured hipGraph (fill, table builds, scan, keys) serves every batch
 // of the same dataset and launch configuration.
 #pragma once
 
@@ -37,11 +50,16 @@ constexpr int kBatchMaxQ = 64;
 constexpr int kBatchMaxCols = 12;   // fact columns; slots 0..3 are the dimension keys
 constexpr int kBatchMaxPred = 2 * kBatchMaxQ;
 constexpr int kBatchMaxExpr = 4;
-constexpr int kBatchMaxAttr = 3;    // packed attributes per dimension
-constexpr int kBatchMaxCmp = 64;    // range-compare filters per dimension (one per query)
-constexpr int kBatchMaxLut = 4096;  // 64-bit mask entries staged in LDS (32 KiB)
-constexpr int kBatchLutSpan = 2048; // widest attribute span given a mask table
-constexpr int kBatchIvlMin = 4;     // range filters on one source that get an interval table
+constexpr int kBatchMaxAttr = 3;    // group attributes packed per dimension
+constexpr int kBatchMaxFCol = 4;    // filtered dimension columns other than the key
+constexpr int kBatchMaxFilt = 64;   // dimension filters per dimension (one per query)
+constexpr int kBatchMaxIvl = 8;     // fact columns with an interval table
+constexpr int kBatchIvlMin = 4;     // predicates on one fact column that get an interval table
+constexpr int kBatchMaxBuckets = 512;
+constexpr int kBatchMaxTab = 8192;  // 32-bit words of interval and mask tables (staged in LDS)
+constexpr int kBatchLutSpan = 2048; // widest attribute given a mask table per value
+constexpr long long kBatchMaskFormBytes = 1 << 20;  // dimensions whose mask table stays this small
+enum BatchDimForm { kDimMask = 0, kDimAttr = 1 };
 
 enum BatchExprKind { kExprDiff = 0, kExprProduct = 1 };  // a1 - a2 (group-by), a1 * a2 (aggregate)
 enum BatchQueryFlags {
@@ -68,11 +86,11 @@ struct BatchQuery {            // 64 bytes; staged in LDS by the scan kernel
   // accumulators for an LDS sink
   int acc;
   // dense group index (GPUProcessing.h:142) as hbase + sum_d field_d * uniq[d]
-  // with field_d = (packed entry >> shift[d]) & (2^bits[d] - 1) and hbase =
-  // sum over grouped dimensions of (attribute base - min_val) * unique_val;
-  // bits[d] = uniq[d] = 0 for a dimension without group value (identical in
-  // 32-bit wrap-around).  field_d < 2^16 and uniq[d] < 2^24, so the products
-  // are 24-bit multiplies.
+  // with field_d = (attribute word >> shift[d]) & (2^bits[d] - 1) and hbase =
+  // sum over joined dimensions of (group value base - min_val) * unique_val
+  // (base 0 for a dimension without group value); bits[d] = 0 when the
+  // dimension gives no group value.  field_d < 2^16 and uniq[d] < 2^24, so the
+  // products are 24-bit multiplies (identical in 32-bit wrap-around).
   int hbase;
   unsigned char expr;          // index into BatchDesc::expr
   unsigned char flags;         // BatchQueryFlags
@@ -116,30 +134,48 @@ struct BatchExpr {
 
 struct BatchAttr {             // one packed attribute of a dimension
   const int* col;              // segment table of the dimension column
-  int base;                    // smallest value
-  int off, bits;               // position in the packed entry
-  int lut;                     // offset of its mask table (one entry per value) or -1
-  int span;                    // values base .. base + span - 1
+  int base;                    // smallest value (attribute form: stored as value - base + 1)
+  int off, bits;               // position in the packed word
+  int lut;                     // attribute form, filtered: word offset of its mask table in tab, else -1
+  int pad;
 };
 
-struct BatchCmp {              // range filter of one query, evaluated per row:
-  int src;                     // -1 = the key, else attribute index
-  int lo, hi, q;               // src value in [lo, hi]
+struct BatchFilt {             // dimension filter of one query (build_GPU2's modes)
+  int src;                     // -1: the key column, else an index into BatchDim::fcol
+  int mode;                    // 0 EQ c1, 1 BETWEEN c1 c2, 2 c1 OR c2
+  int c1, c2;
+  int q;
+  int pad[3];
 };
 
 struct BatchDim {
   const int* key;              // segment table of the dimension key column
-  void* pk;                    // packed table, len entries of pw bytes
+  unsigned long long* mk;      // query-mask table, len entries
+  void* pk;                    // group-attribute table, len entries of pw bytes, or nullptr
   unsigned long long uses;     // queries that probe this dimension
+  unsigned long long groupers; // queries that take a group value from it
   int used, len, minkey, nrows;
-  int pw;                      // bytes per packed entry: 1, 2 or 4
-  int nattr, ncmp, pad;
-  // interval table of the range filters on one source (src as in BatchCmp)
-  // when many queries filter it: ivl_n sorted breakpoints at lut[ivl_off ..]
-  // (low 32 bits), then ivl_n + 1 query masks, one per interval between them
-  int ivl_src, ivl_n, ivl_off, pad2;
+  int pw;                      // bytes per attribute entry: 1, 2 or 4
+  int nattr, nfcol, nfilt;
+  int form;                    // BatchDimForm
+  int pad;
+  long long pk_words;          // 16-byte words of the attribute table (cleared by the fill)
   BatchAttr attr[kBatchMaxAttr];
-  BatchCmp cmp[kBatchMaxCmp];
+  const int* fcol[kBatchMaxFCol];
+  BatchFilt filt[kBatchMaxFilt];
+};
+
+// Interval table of the predicates on one fact column, in the LDS table
+// region (32-bit words from BatchDesc::tab): masks (n + 1 u64 at an even
+// offset), bucket ranks (nb u16, two per word), breakpoints (n + kmax ints,
+// padded with INT_MAX).  Bucket t covers values base + [t << shift,
+// (t+1) << shift); its rank is the number of breakpoints below its start.
+struct BatchIvl {
+  int col;                     // fact column slot
+  int base, shift, nb;
+  int n, kmax;
+  int mask_off, rank_off, bp_off;
+  int pad[3];
 };
 
 struct BatchDesc {
@@ -148,8 +184,8 @@ struct BatchDesc {
   // row range of this launch: positions [0, num_tuples) of the segment group
   // starting at seg_group[seg_begin] (a chunk of a circular scan, or all)
   int seg_begin, num_tuples;
-  int rebuild;                 // record tables must be rebuilt (query set changed)
-  int pad0;
+  int rebuild;                 // dimension tables must be rebuilt (query set changed)
+  int nivl;
   unsigned long long reset;    // queries whose result rows are cleared first
   int lds_words;               // 32-bit words of the LDS group region
   int touch_words;             // words of the touch arena
@@ -157,14 +193,15 @@ struct BatchDesc {
   int* res_arena;
   unsigned* touch_arena;       // per query: a touched bitmap of its groups (gtouch)
   const int* col[kBatchMaxCols];
-  int pred_begin[kBatchMaxCols + 1];
+  int pred_begin[kBatchMaxCols + 1];  // generic predicates (not in an interval table), by column
   BatchPred pred[kBatchMaxPred];
   BatchExpr expr[kBatchMaxExpr];
+  BatchIvl ivl[kBatchMaxIvl];
   BatchDim dim[4];
   BatchQuery q[kBatchMaxQ];
   BatchKeys keys[kBatchMaxQ];
-  int nlut, pad1;
-  unsigned long long lut[kBatchMaxLut];  // attribute mask tables
+  int ntab, pad1;
+  unsigned tab[kBatchMaxTab];  // interval tables
 };
 
 struct BatchLaunch {           // kernel arguments, fixed for a captured graph
@@ -178,36 +215,34 @@ struct BatchLaunch {           // kernel arguments, fixed for a captured graph
 // Hot part of the descriptor, staged in LDS once per workgroup so the tile
 // loop reads it with broadcast LDS loads instead of dependent global loads.
 struct BatchHotDim {
+  const unsigned long long* mk;
   const void* pk;
-  unsigned long long uses;
-  int len, minkey, pw, nattr, ncmp, pad;
-  int ivl_src, ivl_n, ivl_off, pad2;
-  int off[kBatchMaxAttr], bits[kBatchMaxAttr], lut[kBatchMaxAttr], base[kBatchMaxAttr];
+  unsigned long long uses, groupers;
+  int len, minkey, pw, form;
+  int nattr, pad;
+  int base[kBatchMaxAttr], pad2;  // group attribute bases (keys stored in the pair loop)
+  int off[kBatchMaxAttr], bits[kBatchMaxAttr], lut[kBatchMaxAttr], pad3[3];
 };
 
 struct BatchHot {
   const int* col[kBatchMaxCols];
   BatchHotDim dim[4];
-  int ncols, colmask, usedmask, nexpr, npred, nq, nlut, pad;
+  int ncols, colmask, usedmask, nexpr, npred, nq, ntab, nivl;
   int pred_begin[kBatchMaxCols + 1];
   int4 expr[kBatchMaxExpr];
   int4 pred[kBatchMaxPred];    // lo, hi - lo (unsigned), query, column
-  int4 cmp[4][kBatchMaxCmp];   // per dimension: src, lo, hi - lo (unsigned), query
+  BatchIvl ivl[kBatchMaxIvl];
 };
 
 struct BatchConfig {
   int block;                   // 256, 512, 1024
   int vpt;                     // 1, 2 (items per thread = 4*vpt)
   int lds_bytes;               // LDS per workgroup: staged descriptor, query
-                               // descriptors and LDS group arrays
+                               // descriptors, interval tables and group region
 };
 
-// Bytes of LDS a query's group array needs.
-inline size_t batch_group_lds(int total_val) { return (size_t)total_val * 8; }
-
 // Host launchers (batch.hip).  Every launch reads the batch from L.desc.
-// rec / rec_len: packed tables and their sizes in 16-byte units
-hipError_t batch_fill(const BatchLaunch& L, const uint4* const* rec, const int* rec_len, hipStream_t s);
+hipError_t batch_fill(const BatchLaunch& L, hipStream_t s);
 hipError_t batch_records(const BatchLaunch& L, int d, int nrows, hipStream_t s);
 // Writes the decoded keys of every touched group (after the scan).
 hipError_t batch_keys(const BatchLaunch& L, hipStream_t s);

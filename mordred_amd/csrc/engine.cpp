@@ -194,7 +194,7 @@ struct mordred_engine {
   DevBuf d_res, d_sg, d_ht[4], d_mix, d_pmix, d_ord, d_res_ord;
   HostBuf h_stage;  // pinned scratch for segment groups / index updates
   // shared-scan batches: record tables, descriptor, result arena, the graph
-  DevBuf d_rec[4], d_desc, d_arena, d_status, d_touch;
+  DevBuf d_rec[4], d_mk[4], d_desc, d_arena, d_status, d_touch;
   HostBuf h_desc, h_arena;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -2024,7 +2024,6 @@ std::string order_signature(const Resolved& r, const std::vector<MixedGroup>& mi
 // LDS and the result arena, capture / replay the graph.
 struct BatchPlan {
   mordred::BatchDesc d;
-  int rec_len[4] = {0, 0, 0, 0};
   long long n = 0;
   std::vector<long long> res_off;  // ints
   std::vector<int> G;
@@ -2195,30 +2194,34 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       cols[k] = r.key[k];
     }
   }
-  // dimensions: one packed attribute table each; mask tables only as far as
-  // the launch configuration's LDS holds them (the rest use range compares)
+  // dimensions: mask form (a query-mask table, and a group-attribute table
+  // when a query groups on the dimension) when the mask table stays small,
+  // else attribute form (one table of every filtered or grouped attribute);
+  // key filters of attribute-form dimensions become constraints on the fact
+  // key column's value (kcons), evaluated by its interval table
+  struct Cons {
+    int q;
+    int nr;
+    long long lo[2], hi[2];  // accepted values: the union of [lo, hi] ranges
+  };
+  std::vector<Cons> kcons[4];
+  auto filter_ranges = [](int mode, int c1, int c2, Cons& c) {
+    if (mode == 0) { c.nr = 1; c.lo[0] = c.hi[0] = c1; }
+    else if (mode == 1) { c.nr = 1; c.lo[0] = c1; c.hi[0] = c2; }
+    else { c.nr = 2; c.lo[0] = c.hi[0] = c1; c.lo[1] = c.hi[1] = c2; }
+  };
   Column* attrs[4][kBatchMaxAttr] = {};
   std::vector<char> qdead(nq, 0);  // a dimension filter with an empty range
-  D.nlut = 0;
-  const long long lut_cap = std::min<long long>(
-      kBatchMaxLut, std::max<long long>(0, ((long long)cfg.lds_bytes - (long long)sizeof(BatchHot) -
-                                            (long long)nq * (long long)sizeof(BatchQuery)) / 8));
+  D.ntab = 0;
   for (int k = 0; k < 4; k++) {
     BatchDim& B = D.dim[k];
     Column* key = nullptr;
-    int na = 0;
-    auto attr_of = [&](Column* c) -> int {
-      for (int a = 0; a < na; a++)
-        if (attrs[k][a] == c) return a;
-      if (na == kBatchMaxAttr) return -1;
-      attrs[k][na] = c;
-      return na++;
-    };
-    struct Filt {
+    struct DFilt {
       int q, mode, c1, c2;
       Column* col;
     };
-    std::vector<Filt> filts;
+    std::vector<DFilt> filts;
+    std::vector<Column*> gcols;
     for (int i = 0; i < nq; i++) {
       if (!qp[i]) continue;
       const Resolved& r = rs[i];
@@ -2240,12 +2243,14 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       B.uses |= 1ull << i;
       if (r.dfilt[k]) {
         if (j.filter_mode < 0 || j.filter_mode > 2) return fail(e, "dimension filter mode must be 0..2");
-        filts.push_back(Filt{i, j.filter_mode, j.c1, j.c2, r.dfilt[k]});
-        if (r.dfilt[k] != key && attr_of(r.dfilt[k]) < 0)
-          return fail(e, "more than 3 filter and group columns on one dimension in a batch");
+        // mode 1 with c1 > c2 keeps no dimension row: the query is dead
+        if (j.filter_mode == 1 && j.c1 > j.c2) qdead[i] = 1;
+        filts.push_back(DFilt{i, j.filter_mode, j.c1, j.c2, r.dfilt[k]});
       }
-      if (qp[i]->kind == MORDRED_KIND_GROUPBY && r.dval[k] && attr_of(r.dval[k]) < 0)
-        return fail(e, "more than 3 filter and group columns on one dimension in a batch");
+      if (qp[i]->kind == MORDRED_KIND_GROUPBY && r.dval[k]) {
+        B.groupers |= 1ull << i;
+        if (std::find(gcols.begin(), gcols.end(), r.dval[k]) == gcols.end()) gcols.push_back(r.dval[k]);
+      }
     }
     if (!key) continue;
     // no two keys may share a slot: the dimension's and the fact's key ranges
@@ -2256,105 +2261,104 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     const long long klo = std::min(key->vmin, fkey->vmin), khi = std::max(key->vmax, fkey->vmax);
     if (khi - klo >= (long long)B.len)
       return fail(e, "dimension key range exceeds dim_len (hash collisions possible): batches need perfect hashing");
-    // packed layout: attributes from bit 0, the present bit on top
+    // form: attribute form for a large table whose non-key filters are all on
+    // attributes narrow enough for a mask table per value
+    std::vector<Column*> fattrs;
+    bool narrow = true;
+    for (const DFilt& f : filts) {
+      if (f.col == key) continue;
+      column_range(f.col);
+      if ((long long)f.col->vmax - f.col->vmin + 1 > kBatchLutSpan) narrow = false;
+      if (std::find(fattrs.begin(), fattrs.end(), f.col) == fattrs.end()) fattrs.push_back(f.col);
+    }
+    std::vector<Column*> all_attrs = gcols;
+    for (Column* c : fattrs)
+      if (std::find(all_attrs.begin(), all_attrs.end(), c) == all_attrs.end()) all_attrs.push_back(c);
+    B.form = ((long long)B.len * 8 > kBatchMaskFormBytes && narrow && (int)all_attrs.size() <= kBatchMaxAttr)
+                 ? kDimAttr : kDimMask;
+    const std::vector<Column*>& packed = B.form == kDimAttr ? all_attrs : gcols;
+    if ((int)packed.size() > kBatchMaxAttr) return fail(e, "more than 3 group columns on one dimension in a batch");
+    // packed attributes from bit 0 (attribute form: value - base + 1, so one
+    // more value than the span)
     int off = 0;
-    for (int a = 0; a < na; a++) {
-      Column* c = attrs[k][a];
+    for (size_t a = 0; a < packed.size(); a++) {
+      Column* c = packed[a];
+      attrs[k][a] = c;
       column_range(c);
-      const long long span = (long long)c->vmax - c->vmin + 1;
+      const long long span = (long long)c->vmax - c->vmin + 1 + (B.form == kDimAttr ? 1 : 0);
       int bits = 1;
       while ((1ll << bits) < span) bits++;
-      B.attr[a] = BatchAttr{c->d_idx, c->vmin, off, bits, -1, (int)std::min<long long>(span, 1 << 30)};
+      B.attr[a] = BatchAttr{c->d_idx, c->vmin, off, bits, -1, 0};
       off += bits;
     }
-    if (off > 31) return fail(e, "packed attributes of one dimension exceed 31 bits");
-    B.nattr = na;
-    B.pw = off + 1 <= 8 ? 1 : off + 1 <= 16 ? 2 : 4;
-    // filters: a mask table per narrow filtered attribute, range compares for
-    // the key and wide attributes
-    auto pass = [](const Filt& f, int x) {
-      return f.mode == 0 ? x == f.c1 : f.mode == 1 ? (x >= f.c1 && x <= f.c2) : (x == f.c1 || x == f.c2);
-    };
-    for (int a = 0; a < na; a++) {
-      BatchAttr& A = B.attr[a];
-      bool filtered = false;
-      for (const Filt& f : filts) filtered |= f.col == attrs[k][a];
-      if (!filtered || A.span > kBatchLutSpan || D.nlut + A.span > lut_cap) continue;
-      A.lut = D.nlut;
-      for (int v = 0; v < A.span; v++) {
-        unsigned long long m = ~0ull;
-        for (const Filt& f : filts)
-          if (f.col == attrs[k][a] && !pass(f, A.base + v)) m &= ~(1ull << f.q);
-        D.lut[D.nlut + v] = m;
-      }
-      D.nlut += A.span;
-    }
-    // range filters on the key and wide attributes; an empty range keeps no
-    // dimension row, so its query is dead
-    std::vector<BatchCmp> cands;
-    for (const Filt& f : filts) {
-      int src = -1;
-      if (f.col != key) {
-        src = attr_of(f.col);
-        if (B.attr[src].lut >= 0) continue;
-      }
-      if (f.mode == 2) return fail(e, "EQ-or-EQ filter on a dimension key or wide column in a batch");
-      const int hi = f.mode == 0 ? f.c1 : f.c2;
-      if (f.c1 > hi) {
-        qdead[f.q] = 1;
-        continue;
-      }
-      cands.push_back(BatchCmp{src, f.c1, hi, f.q});
-    }
-    // the source most queries filter gets an interval table when it has at
-    // least kBatchIvlMin filters and the table fits the LDS beside the mask tables
-    B.ivl_src = -1;
-    B.ivl_n = 0;
-    B.ivl_off = 0;
-    int best_src = -2, best_n = 0;
-    for (int src = -1; src < na; src++) {
-      int n = 0;
-      for (const BatchCmp& c : cands) n += c.src == src;
-      if (n > best_n) best_n = n, best_src = src;
-    }
-    if (best_n >= kBatchIvlMin) {
-      std::vector<long long> pts;
-      for (const BatchCmp& c : cands) {
-        if (c.src != best_src) continue;
-        if (c.lo > INT_MIN) pts.push_back(c.lo);
-        if (c.hi < INT_MAX) pts.push_back((long long)c.hi + 1);
-      }
-      std::sort(pts.begin(), pts.end());
-      pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
-      const int n = (int)pts.size();
-      if (D.nlut + 2LL * n + 1 <= lut_cap) {
-        B.ivl_src = best_src;
-        B.ivl_n = n;
-        B.ivl_off = D.nlut;
-        for (int i = 0; i < n; i++) D.lut[D.nlut + i] = (unsigned long long)(uint32_t)(int)pts[i];
-        // interval i covers [pts[i-1], pts[i]) with the ends open at INT_MIN
-        // and past INT_MAX; a query passes it iff it lies inside [lo, hi]
-        for (int i = 0; i <= n; i++) {
-          const long long lo_i = i == 0 ? (long long)INT_MIN : pts[i - 1];
-          const long long hi_i = i == n ? (long long)INT_MAX + 1 : pts[i];  // exclusive
-          unsigned long long m = ~0ull;
-          for (const BatchCmp& c : cands)
-            if (c.src == best_src && !(lo_i >= c.lo && hi_i <= (long long)c.hi + 1)) m &= ~(1ull << c.q);
-          D.lut[D.nlut + n + i] = m;
+    if (off > 32) return fail(e, "attributes of one dimension exceed 32 bits in a batch");
+    B.nattr = (int)packed.size();
+    B.pw = off <= 8 ? 1 : off <= 16 ? 2 : 4;
+    if (B.form == kDimMask) {
+      // filters evaluated by the table build, per dimension row
+      Column* fcols[kBatchMaxFCol] = {};
+      int nfc = 0;
+      for (const DFilt& f : filts) {
+        int src = -1;
+        if (f.col != key) {
+          src = 0;
+          while (src < nfc && fcols[src] != f.col) src++;
+          if (src == nfc) {
+            if (nfc == kBatchMaxFCol) return fail(e, "more than 4 filtered columns on one dimension in a batch");
+            fcols[nfc++] = f.col;
+          }
         }
-        D.nlut += 2 * n + 1;
+        if (B.nfilt == kBatchMaxFilt) return fail(e, "too many dimension filters in a batch");
+        B.filt[B.nfilt++] = BatchFilt{src, f.mode, f.c1, f.c2, f.q, {0, 0, 0}};
       }
-    }
-    for (const BatchCmp& c : cands) {
-      if (B.ivl_n > 0 && c.src == B.ivl_src) continue;
-      if (B.ncmp == kBatchMaxCmp) return fail(e, "too many range filters on one dimension in a batch");
-      B.cmp[B.ncmp++] = c;
+      B.nfcol = nfc;
+      for (int c = 0; c < nfc; c++) B.fcol[c] = fcols[c]->d_idx;
+      const size_t mbytes = (((size_t)B.len * 8) + 15) & ~(size_t)15;
+      HIP_OK(e->d_mk[k].reserve(mbytes));
+      B.mk = static_cast<unsigned long long*>(e->d_mk[k].p);
+    } else {
+      // a mask table per filtered attribute (bit q cleared for the values q
+      // rejects); key filters go to the fact key column's interval table
+      for (int a = 0; a < B.nattr; a++) {
+        Column* c = attrs[k][a];
+        if (std::find(fattrs.begin(), fattrs.end(), c) == fattrs.end()) continue;
+        const int span = (int)((long long)c->vmax - c->vmin + 1);
+        const int at = (D.ntab + 1) & ~1;
+        if (at + 2 * span > kBatchMaxTab) return fail(e, "attribute mask tables exceed the batch's LDS table space");
+        B.attr[a].lut = at;
+        for (int v = 0; v < span; v++) {
+          unsigned long long m = ~0ull;
+          for (const DFilt& f : filts) {
+            if (f.col != c) continue;
+            const int x = c->vmin + v;
+            const bool ok = f.mode == 0 ? x == f.c1 : f.mode == 1 ? (x >= f.c1 && x <= f.c2) : (x == f.c1 || x == f.c2);
+            if (!ok) m &= ~(1ull << f.q);
+          }
+          D.tab[at + 2 * v] = (unsigned)m;
+          D.tab[at + 2 * v + 1] = (unsigned)(m >> 32);
+        }
+        D.ntab = at + 2 * span;
+      }
+      for (const DFilt& f : filts) {
+        if (f.col != key) continue;
+        Cons c{f.q, 0, {0, 0}, {0, 0}};
+        filter_ranges(f.mode, f.c1, f.c2, c);
+        kcons[k].push_back(c);
+      }
+      B.mk = nullptr;
+      B.nfilt = 0;
+      B.nfcol = 0;
     }
     B.used = 1;
-    const size_t bytes = (((size_t)B.len * B.pw) + 15) & ~(size_t)15;
-    HIP_OK(e->d_rec[k].reserve(bytes));
-    B.pk = e->d_rec[k].p;
-    b.rec_len[k] = (int)(bytes / 16);
+    if (B.nattr || B.form == kDimAttr) {
+      const size_t pbytes = (((size_t)B.len * B.pw) + 15) & ~(size_t)15;
+      HIP_OK(e->d_rec[k].reserve(pbytes));
+      B.pk = e->d_rec[k].p;
+      B.pk_words = (long long)(pbytes / 16);
+    } else {
+      B.pk = nullptr;
+      B.pk_words = 0;
+    }
   }
   // fact predicates, aggregate expressions, query descriptors
   std::vector<BatchPred> preds;
@@ -2408,7 +2412,9 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       Q.shift[k] = (unsigned char)A.off;
       Q.bits[k] = (unsigned char)A.bits;
       if (A.bits > 16) return fail(e, "batch group index needs attribute fields of at most 16 bits");
-      hbase += ((unsigned)A.base - (unsigned)r.join[k]->min_val) * (unsigned)uv;
+      // the field holds value - base (+ 1 in attribute form)
+      const unsigned fbase = (unsigned)A.base - (D.dim[k].form == kDimAttr ? 1u : 0u);
+      hbase += (fbase - (unsigned)r.join[k]->min_val) * (unsigned)uv;
     }
     Q.hbase = (int)hbase;
     if (q.kind == MORDRED_KIND_GROUPBY && !keys_decode(q, r, D.keys[i])) Q.flags |= kQueryKeysInLoop;
@@ -2417,6 +2423,92 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     else D.all |= 1ull << i;
   }
   std::stable_sort(preds.begin(), preds.end(), [](const BatchPred& x, const BatchPred& y) { return x.col < y.col; });
+  // interval tables: a fact column with at least kBatchIvlMin predicates, or
+  // the fact key of an attribute-form dimension with key filters; their
+  // predicates leave the per-predicate compare list
+  D.nivl = 0;
+  for (int c = 0; c < kBatchMaxCols; c++) {
+    std::vector<Cons> on;
+    int npc = 0;
+    for (const BatchPred& p : preds)
+      if (p.col == c) {
+        Cons x{p.q, 1, {p.lo, 0}, {p.hi, 0}};
+        on.push_back(x);
+        npc++;
+      }
+    const bool keyed = c < 4 && !kcons[c].empty();
+    if (keyed) on.insert(on.end(), kcons[c].begin(), kcons[c].end());
+    if (!keyed && npc < kBatchIvlMin) continue;
+    if (D.nivl == kBatchMaxIvl) {
+      if (keyed) return fail(e, "too many interval tables in a batch");
+      continue;
+    }
+    column_range(cols[c]);
+    const long long vmin = cols[c]->vmin, vmax = cols[c]->vmax;
+    // breakpoints that split the column's value range [vmin, vmax]: the
+    // range ends lo and hi + 1 inside (vmin, vmax]
+    std::vector<long long> pts;
+    for (const Cons& x : on)
+      for (int r = 0; r < x.nr; r++)
+        for (long long v : {x.lo[r], x.hi[r] + 1})
+          if (v > vmin && v <= vmax) pts.push_back(v);
+    std::sort(pts.begin(), pts.end());
+    pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
+    const int n = (int)pts.size();
+    int shift = 0;
+    while (((vmax - vmin) >> shift) + 1 > kBatchMaxBuckets) shift++;
+    const int nb = (int)(((vmax - vmin) >> shift) + 1);
+    // rank at each bucket start, and the most breakpoints inside one bucket
+    std::vector<int> rank(nb);
+    int kmax = 0;
+    for (int t = 0, i = 0; t < nb; t++) {
+      const long long start = vmin + ((long long)t << shift), stop = start + (1ll << shift);
+      while (i < n && pts[i] <= start) i++;  // breakpoints at or below the start count for every value here
+      rank[t] = i;
+      int in = 0;
+      while (i + in < n && pts[i + in] < stop) in++;
+      kmax = std::max(kmax, in);
+    }
+    const int mask_words = 2 * (n + 1), rank_words = (nb + 1) / 2, bp_words = n + kmax;
+    const int at = (D.ntab + 1) & ~1;
+    if (at + mask_words + rank_words + bp_words > kBatchMaxTab) {
+      if (keyed) return fail(e, "interval tables exceed the batch's LDS table space");
+      continue;
+    }
+    BatchIvl& I = D.ivl[D.nivl++];
+    I.col = c;
+    I.base = (int)vmin;
+    I.shift = shift;
+    I.nb = nb;
+    I.n = n;
+    I.kmax = kmax;
+    I.mask_off = at;
+    I.rank_off = I.mask_off + mask_words;
+    I.bp_off = I.rank_off + rank_words;
+    D.ntab = I.bp_off + bp_words;
+    // interval r = [pts[r-1], pts[r]), the first from vmin, the last to vmax:
+    // a query accepts it iff it lies inside one of the ranges of each of the
+    // query's constraints on the column
+    for (int r = 0; r <= n; r++) {
+      const long long lo_r = r == 0 ? vmin : pts[r - 1];
+      const long long hi_r = r == n ? vmax + 1 : pts[r];  // exclusive
+      unsigned long long m = ~0ull;
+      for (const Cons& x : on) {
+        bool in = false;
+        for (int k2 = 0; k2 < x.nr; k2++) in |= lo_r >= x.lo[k2] && hi_r <= x.hi[k2] + 1;
+        if (!in) m &= ~(1ull << x.q);
+      }
+      D.tab[I.mask_off + 2 * r] = (unsigned)m;
+      D.tab[I.mask_off + 2 * r + 1] = (unsigned)(m >> 32);
+    }
+    for (int t = 0; t < rank_words; t++) {
+      const unsigned lo16 = (unsigned)rank[2 * t], hi16 = 2 * t + 1 < nb ? (unsigned)rank[2 * t + 1] : 0u;
+      D.tab[I.rank_off + t] = lo16 | hi16 << 16;
+    }
+    for (int i = 0; i < bp_words; i++) D.tab[I.bp_off + i] = i < n ? (unsigned)(int)pts[i] : (unsigned)INT_MAX;
+    preds.erase(std::remove_if(preds.begin(), preds.end(), [&](const BatchPred& p) { return p.col == c; }),
+                preds.end());
+  }
   D.npred = (int)preds.size();
   for (int p = 0; p < D.npred; p++) D.pred[p] = preds[p];
   for (int c = 0, p = 0; c <= kBatchMaxCols; c++) {
@@ -2445,8 +2537,8 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     }
   }
   const long long fixed_bytes = (long long)sizeof(BatchHot) + nq * (long long)sizeof(BatchQuery) +
-                                (long long)D.nlut * (long long)sizeof(unsigned long long);
-  if (fixed_bytes > cfg.lds_bytes) return fail(e, "lds_bytes below the staged descriptor and mask tables");
+                                4LL * ((D.ntab + 1) & ~1);
+  if (fixed_bytes > cfg.lds_bytes) return fail(e, "lds_bytes below the staged descriptor and interval tables");
   long long avail = ((long long)cfg.lds_bytes - fixed_bytes) / 4;  // 32-bit words
   std::vector<int> order;
   for (int i = 0; i < nq; i++)
@@ -2540,9 +2632,7 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
 struct GraphKey {
   mordred::BatchLaunch L;
   mordred::BatchConfig c;
-  const uint4* rec[4];
-  int rec_len[4];   // record slots cleared by the fill
-  int nrows[4];     // dimension rows (record build grids)
+  int nrows[4];     // dimension rows (table build grids)
   int used[4];
   int ncols;        // selects the scan kernel's register layout
 };
@@ -2564,8 +2654,6 @@ int launch_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConf
   k.c = cfg;
   for (int d = 0; d < 4; d++) {
     k.used[d] = b.d.dim[d].used;
-    k.rec[d] = k.used[d] ? static_cast<const uint4*>(b.d.dim[d].pk) : nullptr;
-    k.rec_len[d] = k.used[d] ? b.rec_len[d] : 0;
     k.nrows[d] = k.used[d] ? b.d.dim[d].nrows : 0;
   }
   k.ncols = b.d.ncols <= 8 ? 8 : 12;
@@ -2580,7 +2668,7 @@ int launch_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConf
     HIP_OK(batch_prepare(cfg, b.d.ncols));
     (void)batch_resident(cfg, b.d.ncols);  // occupancy query outside the capture
     HIP_OK(hipStreamBeginCapture(e->s_main, hipStreamCaptureModeRelaxed));
-    hipError_t r1 = batch_fill(k.L, k.rec, k.rec_len, e->s_main);
+    hipError_t r1 = batch_fill(k.L, e->s_main);
     for (int d = 0; d < 4 && r1 == hipSuccess; d++)
       if (k.used[d]) r1 = batch_records(k.L, d, k.nrows[d], e->s_main);
     if (r1 == hipSuccess) r1 = batch_scan(k.L, cfg, b.d.ncols, e->s_main);
