@@ -21,6 +21,34 @@
 
 namespace mordred {
 
+// Loads through a global (address space 1) pointer: the table pointers come
+// from LDS-staged descriptors, so the compiler would otherwise emit flat
+// loads, which wait for every earlier LDS and memory access before issuing.
+template <typename T>
+__device__ __forceinline__ T gload(const void* p, long long i) {
+  return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+
+// A wave-uniform pointer held in scalar registers, so that loads from it take
+// a 32-bit per-lane byte offset (one VGPR per address instead of two).
+__device__ __forceinline__ const void* uniform_ptr(const void* p) {
+  const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return reinterpret_cast<const void*>(((unsigned long long)hi << 32) | lo);
+}
+
+// Global load at a 32-bit byte offset from a uniform base.
+template <typename T>
+__device__ __forceinline__ T gload_at(const void* base, uint32_t byte_off) {
+  return *(const __attribute__((address_space(1))) T*)((const __attribute__((address_space(1))) char*)base + byte_off);
+}
+
+// Attribute-table entry of slot h (pw bytes).
+__device__ __forceinline__ uint32_t gload_pw(const void* pk, int pw, int h) {
+  return pw == 1 ? gload<uint8_t>(pk, h) : pw == 2 ? gload<uint16_t>(pk, h) : gload<uint32_t>(pk, h);
+}
+
 // Clears the dimension tables of the used dimensions when they are rebuilt,
 // and the result rows and touched bitmaps of the queries being (re)started.
 __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __restrict__ D, unsigned* status) {
@@ -34,7 +62,7 @@ __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __rest
       const BatchDim& B = D->dim[d];
       if (!B.used) continue;
       uint4* mk = reinterpret_cast<uint4*>(B.mk);
-      const long long nm = ((long long)B.len + 1) / 2;
+      const long long nm = ((long long)B.len + 2) / 2;  // slots [0, len]
       if (mk)
         for (long long i = t; i < nm; i += stride) mk[i] = z;
       uint4* pk = reinterpret_cast<uint4*>(B.pk);
@@ -146,7 +174,21 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ BatchHot H;
   const int nq = D->nq;
-  BatchQuery* lq = reinterpret_cast<BatchQuery*>(lds_raw);
+  // query descriptors transposed into four arrays of 16-byte chunks (chunk c
+  // of query q at lqc[c * nq + q]): lanes of one wave read the descriptors of
+  // different queries, which a 64-byte stride would put in the same banks
+  uint4* lqc = reinterpret_cast<uint4*>(lds_raw);
+  auto query = [&](int q, bool full) {
+    union {
+      BatchQuery Q;
+      uint4 c[4];
+    } u;
+    u.c[0] = lqc[q];
+    u.c[1] = lqc[nq + q];
+    u.c[2] = lqc[2 * nq + q];
+    u.c[3] = full ? lqc[3 * nq + q] : make_uint4(0, 0, 0, 0);
+    return u.Q;
+  };
   // interval tables, then the group region: per query a touched bitmap, then
   // LDS accumulators (8-byte aligned)
   unsigned* ltab = reinterpret_cast<unsigned*>(lds_raw + nq * sizeof(BatchQuery));
@@ -155,8 +197,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   const int ngrp = D->lds_words;
   {
     const uint4* src = reinterpret_cast<const uint4*>(D->q);
-    uint4* dst = reinterpret_cast<uint4*>(lq);
-    for (int i = threadIdx.x; i < nq * (int)(sizeof(BatchQuery) / 16); i += BLOCK) dst[i] = src[i];
+    for (int i = threadIdx.x; i < nq * 4; i += BLOCK) lqc[(i & 3) * nq + (i >> 2)] = src[i];
     for (int i = threadIdx.x; i < ngrp; i += BLOCK) lgrp[i] = 0;
     for (int i = threadIdx.x; i < ntab; i += BLOCK) ltab[i] = D->tab[i];
     const int np = D->npred;
@@ -232,28 +273,30 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     if (sidx != cur_seg) {
       const int seg = __builtin_amdgcn_readfirstlane((int)seg_group[sidx]);
       const int* tab = lane < MAXC && (colmask >> lane & 1) ? H.col[lane] : nullptr;
-      slotv = tab ? tab[seg] : 0;
+      slotv = tab ? gload<int>(tab, seg) : 0;
       cur_seg = sidx;
     }
     const int off = pos & kSegMask;
     const int n = min(TILE, num_tuples - pos);
-    // every slot into a scalar first, then the loads back to back
+    // every slot into a scalar first, then the loads back to back; a slot the
+    // batch does not read points at the first column it does read (its tile is
+    // loaded anyway; the values are never used), so that no load sits under a
+    // per-column branch, which would make each load wait for the previous one
+    const int c0 = __builtin_ctz((unsigned)colmask | (1u << (MAXC - 1)));
+    const int slot0 = __builtin_amdgcn_readlane(slotv, c0);
     const int* base[MAXC];
 #pragma unroll
-    for (int c = 0; c < MAXC; c++)
-      base[c] = L.cache + ((size_t)(unsigned)__builtin_amdgcn_readlane(slotv, c) << kSegLog) + off;
+    for (int c = 0; c < MAXC; c++) {
+      const int sl = (colmask >> c & 1) ? __builtin_amdgcn_readlane(slotv, c) : slot0;
+      base[c] = L.cache + ((size_t)(unsigned)sl << kSegLog) + off;
+    }
     if (n == TILE) {
 #pragma unroll
       for (int c = 0; c < MAXC; c++) {
-        if (colmask >> c & 1) {
 #pragma unroll
-          for (int v = 0; v < VPT; v++) {
-            const v4i t = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(base[c]) + v * BLOCK + threadIdx.x);
-            x[c][4 * v + 0] = t.x; x[c][4 * v + 1] = t.y; x[c][4 * v + 2] = t.z; x[c][4 * v + 3] = t.w;
-          }
-        } else {
-#pragma unroll
-          for (int j = 0; j < IPT; j++) x[c][j] = 0;
+        for (int v = 0; v < VPT; v++) {
+          const v4i t = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(base[c]) + v * BLOCK + threadIdx.x);
+          x[c][4 * v + 0] = t.x; x[c][4 * v + 1] = t.y; x[c][4 * v + 2] = t.z; x[c][4 * v + 3] = t.w;
         }
       }
     } else {
@@ -337,49 +380,53 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       }
     }
 
-    // one probe per dimension for all queries (every dimension's gathers
-    // issued before any is used; only items some query still needs load):
-    // mask form, the slot's query mask; attribute form, the slot's attribute
-    // word, whose filtered attributes index their mask tables in LDS
-    unsigned long long mv[4][IPT];
-#pragma unroll
-    for (int d = 0; d < 4; d++) {
-      if (!(usedmask >> d & 1)) continue;
+    // one probe per dimension for all queries: mask form, the slot's query
+    // mask; attribute form, the slot's attribute word, whose filtered
+    // attributes index their mask tables in LDS.  Every dimension's gathers
+    // are issued before any is used, and none sits under a branch: a load
+    // behind a branch, or consumed before the next is issued, waits for the
+    // loads before it.  An item no query needs, or whose key has no slot,
+    // reads slot len, which holds zero (no query passes); an unused dimension
+    // reads 8 bytes of the descriptor.  An entry is pw bytes (1, 2 or 4) at
+    // slot x pw, so it never straddles the aligned 8-byte word holding it.
+    const void* const dummy = D;
+    unsigned long long mv[2][IPT];  // two dimensions in flight
+    uint32_t shp[4];                // the entry's byte offset in its word, 3 bits per item
+    auto issue = [&](int d) {
       const BatchHotDim& Hd = H.dim[d];
-      const int len = __builtin_amdgcn_readfirstlane(Hd.len);
+      const bool used = usedmask >> d & 1;
+      const int len = used ? __builtin_amdgcn_readfirstlane(Hd.len) : 0;
       const int mk = __builtin_amdgcn_readfirstlane(Hd.minkey);
-      const int form = __builtin_amdgcn_readfirstlane(Hd.form);
-      const int pw = __builtin_amdgcn_readfirstlane(Hd.pw);
+      const bool mform = __builtin_amdgcn_readfirstlane(Hd.form) == kDimMask;
+      const unsigned stride = mform ? 8u : (unsigned)__builtin_amdgcn_readfirstlane(Hd.pw);
       const unsigned long long uses = Hd.uses;
-      const unsigned long long* mkt = Hd.mk;
-      const void* pk = Hd.pk;
+      const void* tabp = uniform_ptr(!used ? dummy : mform ? static_cast<const void*>(Hd.mk) : Hd.pk);
+      shp[d] = 0;
 #pragma unroll
       for (int j = 0; j < IPT; j++) {
         const int h = phash(x[d][j], len, mk);
-        const bool need = (unsigned)h < (unsigned)len && (alive[j] & uses);
-        if (form == kDimMask) {
-          mv[d][j] = need ? mkt[h] : 0ull;
-        } else {
-          uint32_t w = 0;
-          if (need)
-            w = pw == 1 ? static_cast<const uint8_t*>(pk)[h]
-                        : pw == 2 ? static_cast<const uint16_t*>(pk)[h] : static_cast<const uint32_t*>(pk)[h];
-          mv[d][j] = w;
-        }
+        const unsigned s = (unsigned)h < (unsigned)len && (alive[j] & uses) ? (unsigned)h : (unsigned)len;
+        const uint32_t b = s * stride;  // (a table is far below 4 GB)
+        shp[d] |= (b & 7u) << (3 * j);
+        mv[d & 1][j] = gload_at<unsigned long long>(tabp, b & ~7u);
       }
-    }
-#pragma unroll
-    for (int d = 0; d < 4; d++) {
+    };
+    auto consume = [&](int d) {
 #pragma unroll
       for (int j = 0; j < IPT; j++) attrw[d][j] = 0;
-      if (!(usedmask >> d & 1)) continue;
+      if (!(usedmask >> d & 1)) return;
       const BatchHotDim& Hd = H.dim[d];
       const unsigned long long uses = Hd.uses;
+      const unsigned long long(&m)[IPT] = mv[d & 1];
       if (__builtin_amdgcn_readfirstlane(Hd.form) == kDimMask) {
 #pragma unroll
-        for (int j = 0; j < IPT; j++) alive[j] &= mv[d][j] | ~uses;
-        continue;
+        for (int j = 0; j < IPT; j++) alive[j] &= m[j] | ~uses;
+        return;
       }
+      const unsigned pw = (unsigned)__builtin_amdgcn_readfirstlane(Hd.pw);
+      const uint32_t wm = pw >= 4 ? 0xFFFFFFFFu : (1u << (8 * pw)) - 1u;
+#pragma unroll
+      for (int j = 0; j < IPT; j++) attrw[d][j] = (uint32_t)(m[j] >> (((shp[d] >> (3 * j)) & 7u) * 8u)) & wm;
       const int na = __builtin_amdgcn_readfirstlane(Hd.nattr);
       for (int a = 0; a < na; a++) {
         const int lut = __builtin_amdgcn_readfirstlane(Hd.lut[a]);
@@ -389,36 +436,53 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         const unsigned long long* lt = reinterpret_cast<const unsigned long long*>(ltab + lut);
 #pragma unroll
         for (int j = 0; j < IPT; j++) {
-          const uint32_t f = __builtin_amdgcn_ubfe((uint32_t)mv[d][j], off, bits);
+          const uint32_t f = __builtin_amdgcn_ubfe(attrw[d][j], off, bits);
           alive[j] &= lt[f ? f - 1u : 0u];  // (an empty slot clears its users below)
         }
       }
 #pragma unroll
-      for (int j = 0; j < IPT; j++) {
-        const uint32_t w = (uint32_t)mv[d][j];
-        if (!w) alive[j] &= ~uses;
-        attrw[d][j] = w;
-      }
-    }
-    // group attributes of the mask-form dimensions, for the survivors
+      for (int j = 0; j < IPT; j++)
+        if (!attrw[d][j]) alive[j] &= ~uses;
+    };
+    // the gathers of dimension d+1 are issued before dimension d is used
+    // (the probes see the alive masks from before the other dimensions)
+    issue(0);
 #pragma unroll
     for (int d = 0; d < 4; d++) {
-      if (!(usedmask >> d & 1)) continue;
+      if (d + 1 < 4) issue(d + 1);
+      consume(d);
+    }
+    // group attributes of the mask-form dimensions, for the survivors (each
+    // passed the dimension, so its slot is in range; the others read slot
+    // len), all issued before any is used
+    uint32_t gw[4][IPT];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
       const BatchHotDim& Hd = H.dim[d];
-      const unsigned long long groupers = Hd.groupers;
-      if (!groupers || __builtin_amdgcn_readfirstlane(Hd.form) != kDimMask) continue;
-      const int pw = __builtin_amdgcn_readfirstlane(Hd.pw);
-      const int len = __builtin_amdgcn_readfirstlane(Hd.len);
+      const bool used = usedmask >> d & 1;
+      const unsigned long long gq =
+          used && __builtin_amdgcn_readfirstlane(Hd.form) == kDimMask ? Hd.groupers : 0ull;
+      const int len = gq ? __builtin_amdgcn_readfirstlane(Hd.len) : 0;
       const int mk = __builtin_amdgcn_readfirstlane(Hd.minkey);
-      const void* pk = Hd.pk;
+      const unsigned pw = (unsigned)__builtin_amdgcn_readfirstlane(Hd.pw);
+      const void* gtab = uniform_ptr(gq ? Hd.pk : dummy);
+      shp[d] = 0;
 #pragma unroll
       for (int j = 0; j < IPT; j++) {
-        // a survivor for a grouping query passed this dimension: its slot is in range
-        if (!(alive[j] & groupers)) continue;
-        const int h = phash(x[d][j], len, mk);
-        attrw[d][j] = pw == 1 ? static_cast<const uint8_t*>(pk)[h]
-                              : pw == 2 ? static_cast<const uint16_t*>(pk)[h] : static_cast<const uint32_t*>(pk)[h];
+        const unsigned s = (alive[j] & gq) ? (unsigned)phash(x[d][j], len, mk) : (unsigned)len;
+        const uint32_t b = s * pw;
+        shp[d] |= (b & 3u) << (3 * j);
+        gw[d][j] = gload_at<uint32_t>(gtab, b & ~3u);
       }
+    }
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      const BatchHotDim& Hd = H.dim[d];
+      if (!(usedmask >> d & 1) || !Hd.groupers || __builtin_amdgcn_readfirstlane(Hd.form) != kDimMask) continue;
+      const unsigned pw = (unsigned)__builtin_amdgcn_readfirstlane(Hd.pw);
+      const uint32_t wm = pw >= 4 ? 0xFFFFFFFFu : (1u << (8 * pw)) - 1u;
+#pragma unroll
+      for (int j = 0; j < IPT; j++) attrw[d][j] = (gw[d][j] >> (((shp[d] >> (3 * j)) & 7u) * 8u)) & wm;
     }
 
     // aggregate expressions of the batch (int32 arithmetic, GPUProcessing.h:149, 1763)
@@ -452,54 +516,85 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     }
   };
 
-  // every surviving (row, query) pair
+  // Queries whose pairs need a returning atomic -- 32-bit LDS sums (the
+  // carry) and queries that store their keys in the loop -- are handled in a
+  // loop of their own, so that the main loop over the other pairs never waits
+  // for an atomic (a wave runs every sink path some lane needs).
+  unsigned long long slowq = 0;
+  for (int q = 0; q < nq; q++) {
+    const BatchQuery Q = query(q, false);
+    if ((Q.flags & kQueryKeysInLoop) || Q.sink == kSinkLds32) slowq |= 1ull << q;
+  }
+
+  auto group_of = [&](const BatchQuery& Q, int j) {
+    return mod_pos(Q.hbase + (int)__umul24(__builtin_amdgcn_ubfe(attrw[0][j], Q.shift[0], Q.bits[0]), (unsigned)Q.uniq[0]) +
+                       (int)__umul24(__builtin_amdgcn_ubfe(attrw[1][j], Q.shift[1], Q.bits[1]), (unsigned)Q.uniq[1]) +
+                       (int)__umul24(__builtin_amdgcn_ubfe(attrw[2][j], Q.shift[2], Q.bits[2]), (unsigned)Q.uniq[2]) +
+                       (int)__umul24(__builtin_amdgcn_ubfe(attrw[3][j], Q.shift[3], Q.bits[3]), (unsigned)Q.uniq[3]),
+                   Q.total_val);
+  };
+  auto value_of = [&](const BatchQuery& Q, int j) {
+    int val = 0;
+#pragma unroll
+    for (int e = 0; e < kBatchMaxExpr; e++)
+      if (e == Q.expr) val = ev[e][j];
+    return val;
+  };
+
+  // every surviving (row, query) pair.  An index outside [0, total_val)
+  // (group values below min_val) would write outside the query's result; the
+  // reference does, here the pair is dropped.  (Scoped atomics keep the LDS
+  // ones ds_ instructions: one atomicAdd on either pointer would be merged
+  // into a slower flat atomic.)
   auto pairs = [&]() {
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
-      unsigned long long m = alive[j];
+      // no add returns a value here: the keys are written after the scan from
+      // the touched bitmap
+      unsigned long long m = alive[j] & ~slowq;
       while (m) {
         const int q = __builtin_ctzll(m);
         m &= m - 1;
-        const BatchQuery& Q = lq[q];
-        const int h = mod_pos(Q.hbase + (int)__umul24(__builtin_amdgcn_ubfe(attrw[0][j], Q.shift[0], Q.bits[0]), (unsigned)Q.uniq[0]) +
-                                  (int)__umul24(__builtin_amdgcn_ubfe(attrw[1][j], Q.shift[1], Q.bits[1]), (unsigned)Q.uniq[1]) +
-                                  (int)__umul24(__builtin_amdgcn_ubfe(attrw[2][j], Q.shift[2], Q.bits[2]), (unsigned)Q.uniq[2]) +
-                                  (int)__umul24(__builtin_amdgcn_ubfe(attrw[3][j], Q.shift[3], Q.bits[3]), (unsigned)Q.uniq[3]),
-                              Q.total_val);
-        // an index outside [0, total_val) (group values below min_val) would
-        // write outside the query's result; the reference does, here the pair
-        // is dropped
+        const BatchQuery Q = query(q, false);
+        const int h = group_of(Q, j);
         if ((unsigned)h >= (unsigned)Q.total_val) continue;
-        int val = 0;
-#pragma unroll
-        for (int e = 0; e < kBatchMaxExpr; e++)
-          if (e == Q.expr) val = ev[e][j];
+        const unsigned long long add = (unsigned long long)(long long)value_of(Q, j);
+        const int tw = batch_touch_words(Q.total_val, Q.flags);
+        if (Q.sink == kSinkLds64) {
+          (void)__hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(lgrp + Q.acc + tw) + h, add,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+          auto* gsum = (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(
+              reinterpret_cast<int*>(Q.res) + 6 * (size_t)h + 4);
+          (void)__hip_atomic_fetch_add(gsum, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (tw)
+          (void)__hip_atomic_fetch_or(lgrp + Q.acc + (h >> 5), 1u << (h & 31), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      m = alive[j] & slowq;
+      while (m) {
+        const int q = __builtin_ctzll(m);
+        m &= m - 1;
+        const BatchQuery Q = query(q, false);
+        const int h = group_of(Q, j);
+        if ((unsigned)h >= (unsigned)Q.total_val) continue;
+        const int val = value_of(Q, j);
         const unsigned long long add = (unsigned long long)(long long)val;
         int* res = reinterpret_cast<int*>(Q.res) + 6 * (size_t)h;
         auto* gsum = (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(res + 4);
         const int tw = batch_touch_words(Q.total_val, Q.flags);
         unsigned* acc = lgrp + Q.acc + tw;
-        // (scoped atomics keep the LDS ones ds_ instructions: one atomicAdd
-        // on either pointer would be merged into a slower flat atomic)
         if (!(Q.flags & kQueryKeysInLoop)) {
-          // no add returns a value here, except the 32-bit LDS one that needs
-          // its carry: the keys are written after the scan from the touched
-          // bitmap, so no pair waits for its atomic
-          if (Q.sink == kSinkLds64) {
-            (void)__hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(acc) + h, add, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_WORKGROUP);
-          } else if (Q.sink == kSinkLds32) {
-            const unsigned u = (unsigned)val;
-            const unsigned old = __hip_atomic_fetch_add(acc + h, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            // (int64)val = u - 2^32 [val < 0]; the low word wraps when old + u
-            // carries: the 2^32 that leaves it goes to the result
-            const int delta = (int)(old + u < old) - (int)(val < 0);
-            if (delta)
-              (void)__hip_atomic_fetch_add(gsum, (unsigned long long)(long long)delta << 32, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-          } else {
-            (void)__hip_atomic_fetch_add(gsum, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          }
+          // 32-bit LDS sum: (int64)val = u - 2^32 [val < 0]; the low word
+          // wraps when old + u carries, and the 2^32 that leaves it goes to
+          // the result
+          const unsigned u = (unsigned)val;
+          const unsigned old = __hip_atomic_fetch_add(acc + h, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const int delta = (int)(old + u < old) - (int)(val < 0);
+          if (delta)
+            (void)__hip_atomic_fetch_add(gsum, (unsigned long long)(long long)delta << 32, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
           if (tw)
             (void)__hip_atomic_fetch_or(lgrp + Q.acc + (h >> 5), 1u << (h & 31), __ATOMIC_RELAXED,
                                         __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -517,11 +612,12 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         else
           old = __hip_atomic_fetch_add(gsum, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (!(Q.flags & kQueryAggr) && old == 0ull) {
+          const BatchQuery Qa = query(q, true);
           int gv[4];
 #pragma unroll
           for (int d = 0; d < 4; d++)
-            gv[d] = Q.attr[d] < 0 ? 0
-                                  : H.dim[d].base[Q.attr[d]] + (int)__builtin_amdgcn_ubfe(attrw[d][j], Q.shift[d], Q.bits[d]);
+            gv[d] = Qa.attr[d] < 0 ? 0
+                                   : H.dim[d].base[Qa.attr[d]] + (int)__builtin_amdgcn_ubfe(attrw[d][j], Q.shift[d], Q.bits[d]);
           int2* key = reinterpret_cast<int2*>(res);
           key[0] = make_int2(gv[0], gv[1]);
           key[1] = make_int2(gv[2], gv[3]);
@@ -546,8 +642,8 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   __syncthreads();
   unsigned* const touch_arena = D->touch_arena;
   for (int q = 0; q < nq; q++) {
-    const BatchQuery& Q = lq[q];
     if (!(all >> q & 1ull)) continue;
+    const BatchQuery Q = query(q, true);
     const int G = Q.total_val, flags = Q.flags, sink = Q.sink;
     const int tw = batch_touch_words(G, flags);
     const unsigned* tb = lgrp + Q.acc;

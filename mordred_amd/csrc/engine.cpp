@@ -2313,7 +2313,9 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       }
       B.nfcol = nfc;
       for (int c = 0; c < nfc; c++) B.fcol[c] = fcols[c]->d_idx;
-      const size_t mbytes = (((size_t)B.len * 8) + 15) & ~(size_t)15;
+      // slot len stays zero: the probe of an item no query needs, or whose
+      // key has no slot, reads it
+      const size_t mbytes = (((size_t)(B.len + 1) * 8) + 15) & ~(size_t)15;
       HIP_OK(e->d_mk[k].reserve(mbytes));
       B.mk = static_cast<unsigned long long*>(e->d_mk[k].p);
     } else {
@@ -2351,7 +2353,8 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     }
     B.used = 1;
     if (B.nattr || B.form == kDimAttr) {
-      const size_t pbytes = (((size_t)B.len * B.pw) + 15) & ~(size_t)15;
+      // a zero entry at slot len, and the aligned 8-byte word holding it
+      const size_t pbytes = (((size_t)(B.len + 1) * B.pw + 8) + 15) & ~(size_t)15;
       HIP_OK(e->d_rec[k].reserve(pbytes));
       B.pk = e->d_rec[k].p;
       B.pk_words = (long long)(pbytes / 16);
