@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __rest
       const BatchDim& B = D->dim[d];
       if (!B.used) continue;
       uint4* mk = reinterpret_cast<uint4*>(B.mk);
-      const long long nm = ((long long)B.len + 2) / 2;  // slots [0, len]
+      const long long nm = B.mstride == 2 ? (long long)B.len + 1 : ((long long)B.len + 2) / 2;  // slots [0, len]
       if (mk)
         for (long long i = t; i < nm; i += stride) mk[i] = z;
       uint4* pk = reinterpret_cast<uint4*>(B.pk);
@@ -123,17 +123,17 @@ __global__ __launch_bounds__(256) void batch_records_kernel(BatchLaunch L, int d
       const bool ok = F.mode == 0 ? x == F.c1 : F.mode == 1 ? (x >= F.c1 && x <= F.c2) : (x == F.c1 || x == F.c2);
       if (!ok) m &= ~(1ull << F.q);
     }
-    B.mk[h] = m;
-    if (!B.pk) continue;
+    if (B.mstride == 1) {
+      B.mk[h] = m;
+      continue;
+    }
     uint32_t w = 0;
     for (int a = 0; a < na; a++) {
       const BatchAttr A = B.attr[a];
       const uint32_t v = (uint32_t)(gather(L.cache, A.col, row) - A.base);
       w |= (v & ((1u << A.bits) - 1u)) << A.off;
     }
-    if (pw == 1) static_cast<uint8_t*>(B.pk)[h] = (uint8_t)w;
-    else if (pw == 2) static_cast<uint16_t*>(B.pk)[h] = (uint16_t)w;
-    else static_cast<uint32_t*>(B.pk)[h] = w;
+    reinterpret_cast<uint4*>(B.mk)[h] = make_uint4((unsigned)m, (unsigned)(m >> 32), w, 0u);
   }
 }
 
@@ -189,8 +189,8 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     u.c[3] = full ? lqc[3 * nq + q] : make_uint4(0, 0, 0, 0);
     return u.Q;
   };
-  // interval tables, then the group region: per query a touched bitmap, then
-  // LDS accumulators (8-byte aligned)
+  // interval and mask tables, then the group region: per query a touched
+  // bitmap, then LDS accumulators (8-byte aligned)
   unsigned* ltab = reinterpret_cast<unsigned*>(lds_raw + nq * sizeof(BatchQuery));
   const int ntab = (D->ntab + 1) & ~1;
   unsigned* lgrp = ltab + ntab;
@@ -225,6 +225,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       Hd.pw = B.pw;
       Hd.form = B.form;
       Hd.nattr = B.nattr;
+      Hd.mstride = B.mstride;
       for (int a = 0; a < kBatchMaxAttr; a++) {
         // attribute form stores value - base + 1
         Hd.base[a] = B.attr[a].base - (B.form == kDimAttr ? 1 : 0);
@@ -381,16 +382,19 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     }
 
     // one probe per dimension for all queries: mask form, the slot's query
-    // mask; attribute form, the slot's attribute word, whose filtered
-    // attributes index their mask tables in LDS.  Every dimension's gathers
-    // are issued before any is used, and none sits under a branch: a load
-    // behind a branch, or consumed before the next is issued, waits for the
-    // loads before it.  An item no query needs, or whose key has no slot,
-    // reads slot len, which holds zero (no query passes); an unused dimension
-    // reads 8 bytes of the descriptor.  An entry is pw bytes (1, 2 or 4) at
-    // slot x pw, so it never straddles the aligned 8-byte word holding it.
+    // mask (and, in a 16-byte entry, its group attribute word); attribute
+    // form, the slot's attribute word, whose filtered attributes index their
+    // mask tables in LDS.  The gathers of dimension d+1 are issued before
+    // dimension d is used, and the loads of one dimension sit under no
+    // per-item branch: a load consumed before the next is issued waits for
+    // the loads before it.  An item no query needs, or whose key has no
+    // slot, reads slot len, which holds zero (no query passes); an unused
+    // dimension reads 8 bytes of the descriptor.  An attribute entry is pw
+    // bytes (1, 2 or 4) at slot x pw, so it never straddles the aligned
+    // 8-byte word holding it.
     const void* const dummy = D;
     unsigned long long mv[2][IPT];  // two dimensions in flight
+    uint32_t aw[2][IPT];            // (16-byte mask entries: their attribute word)
     uint32_t shp[4];                // the entry's byte offset in its word, 3 bits per item
     auto issue = [&](int d) {
       const BatchHotDim& Hd = H.dim[d];
@@ -398,17 +402,32 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       const int len = used ? __builtin_amdgcn_readfirstlane(Hd.len) : 0;
       const int mk = __builtin_amdgcn_readfirstlane(Hd.minkey);
       const bool mform = __builtin_amdgcn_readfirstlane(Hd.form) == kDimMask;
-      const unsigned stride = mform ? 8u : (unsigned)__builtin_amdgcn_readfirstlane(Hd.pw);
+      const bool wide = used && mform && __builtin_amdgcn_readfirstlane(Hd.mstride) == 2;
+      const unsigned stride = wide ? 16u : mform ? 8u : (unsigned)__builtin_amdgcn_readfirstlane(Hd.pw);
       const unsigned long long uses = Hd.uses;
       const void* tabp = uniform_ptr(!used ? dummy : mform ? static_cast<const void*>(Hd.mk) : Hd.pk);
+      uint32_t b[IPT];
       shp[d] = 0;
 #pragma unroll
       for (int j = 0; j < IPT; j++) {
         const int h = phash(x[d][j], len, mk);
         const unsigned s = (unsigned)h < (unsigned)len && (alive[j] & uses) ? (unsigned)h : (unsigned)len;
-        const uint32_t b = s * stride;  // (a table is far below 4 GB)
-        shp[d] |= (b & 7u) << (3 * j);
-        mv[d & 1][j] = gload_at<unsigned long long>(tabp, b & ~7u);
+        b[j] = s * stride;  // (a table is far below 4 GB)
+        shp[d] |= (b[j] & 7u) << (3 * j);
+      }
+      if (wide) {
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+          const v4i t = gload_at<v4i>(tabp, b[j]);
+          mv[d & 1][j] = (unsigned long long)(unsigned)t.y << 32 | (unsigned)t.x;
+          aw[d & 1][j] = (uint32_t)t.z;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+          mv[d & 1][j] = gload_at<unsigned long long>(tabp, b[j] & ~7u);
+          aw[d & 1][j] = 0u;
+        }
       }
     };
     auto consume = [&](int d) {
@@ -420,7 +439,10 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       const unsigned long long(&m)[IPT] = mv[d & 1];
       if (__builtin_amdgcn_readfirstlane(Hd.form) == kDimMask) {
 #pragma unroll
-        for (int j = 0; j < IPT; j++) alive[j] &= m[j] | ~uses;
+        for (int j = 0; j < IPT; j++) {
+          alive[j] &= m[j] | ~uses;
+          attrw[d][j] = aw[d & 1][j];
+        }
         return;
       }
       const unsigned pw = (unsigned)__builtin_amdgcn_readfirstlane(Hd.pw);
@@ -444,47 +466,13 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       for (int j = 0; j < IPT; j++)
         if (!attrw[d][j]) alive[j] &= ~uses;
     };
-    // the gathers of dimension d+1 are issued before dimension d is used
-    // (the probes see the alive masks from before the other dimensions)
+    // (the probes see the alive masks from before the previous dimension)
     issue(0);
 #pragma unroll
     for (int d = 0; d < 4; d++) {
       if (d + 1 < 4) issue(d + 1);
       consume(d);
     }
-    // group attributes of the mask-form dimensions, for the survivors (each
-    // passed the dimension, so its slot is in range; the others read slot
-    // len), all issued before any is used
-    uint32_t gw[4][IPT];
-#pragma unroll
-    for (int d = 0; d < 4; d++) {
-      const BatchHotDim& Hd = H.dim[d];
-      const bool used = usedmask >> d & 1;
-      const unsigned long long gq =
-          used && __builtin_amdgcn_readfirstlane(Hd.form) == kDimMask ? Hd.groupers : 0ull;
-      const int len = gq ? __builtin_amdgcn_readfirstlane(Hd.len) : 0;
-      const int mk = __builtin_amdgcn_readfirstlane(Hd.minkey);
-      const unsigned pw = (unsigned)__builtin_amdgcn_readfirstlane(Hd.pw);
-      const void* gtab = uniform_ptr(gq ? Hd.pk : dummy);
-      shp[d] = 0;
-#pragma unroll
-      for (int j = 0; j < IPT; j++) {
-        const unsigned s = (alive[j] & gq) ? (unsigned)phash(x[d][j], len, mk) : (unsigned)len;
-        const uint32_t b = s * pw;
-        shp[d] |= (b & 3u) << (3 * j);
-        gw[d][j] = gload_at<uint32_t>(gtab, b & ~3u);
-      }
-    }
-#pragma unroll
-    for (int d = 0; d < 4; d++) {
-      const BatchHotDim& Hd = H.dim[d];
-      if (!(usedmask >> d & 1) || !Hd.groupers || __builtin_amdgcn_readfirstlane(Hd.form) != kDimMask) continue;
-      const unsigned pw = (unsigned)__builtin_amdgcn_readfirstlane(Hd.pw);
-      const uint32_t wm = pw >= 4 ? 0xFFFFFFFFu : (1u << (8 * pw)) - 1u;
-#pragma unroll
-      for (int j = 0; j < IPT; j++) attrw[d][j] = (gw[d][j] >> (((shp[d] >> (3 * j)) & 7u) * 8u)) & wm;
-    }
-
     // aggregate expressions of the batch (int32 arithmetic, GPUProcessing.h:149, 1763)
 #pragma unroll
     for (int e = 0; e < kBatchMaxExpr; e++) {

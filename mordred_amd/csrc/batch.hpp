@@ -150,15 +150,18 @@ struct BatchFilt {             // dimension filter of one query (build_GPU2's mo
 
 struct BatchDim {
   const int* key;              // segment table of the dimension key column
-  unsigned long long* mk;      // query-mask table, len entries
-  void* pk;                    // group-attribute table, len entries of pw bytes, or nullptr
+  unsigned long long* mk;      // query-mask table, len entries of mstride u64 words
+  void* pk;                    // attribute table (attribute form), len entries of pw bytes, or nullptr
   unsigned long long uses;     // queries that probe this dimension
   unsigned long long groupers; // queries that take a group value from it
   int used, len, minkey, nrows;
   int pw;                      // bytes per attribute entry: 1, 2 or 4
   int nattr, nfcol, nfilt;
   int form;                    // BatchDimForm
-  int pad;
+  // mask form: u64 words per slot of mk; 2 when a query groups on the
+  // dimension (the mask, then the packed group attributes in the next word's
+  // low half: one 16-byte gather reads both)
+  int mstride;
   long long pk_words;          // 16-byte words of the attribute table (cleared by the fill)
   BatchAttr attr[kBatchMaxAttr];
   const int* fcol[kBatchMaxFCol];
@@ -201,7 +204,7 @@ struct BatchDesc {
   BatchQuery q[kBatchMaxQ];
   BatchKeys keys[kBatchMaxQ];
   int ntab, pad1;
-  unsigned tab[kBatchMaxTab];  // interval tables
+  unsigned tab[kBatchMaxTab];  // interval tables and attribute mask tables
 };
 
 struct BatchLaunch {           // kernel arguments, fixed for a captured graph
@@ -221,7 +224,7 @@ struct BatchHotDim {
   int len, minkey, pw, form;
   int nattr, pad;
   int base[kBatchMaxAttr], pad2;  // group attribute bases (keys stored in the pair loop)
-  int off[kBatchMaxAttr], bits[kBatchMaxAttr], lut[kBatchMaxAttr], pad3[3];
+  int off[kBatchMaxAttr], bits[kBatchMaxAttr], lut[kBatchMaxAttr], mstride, pad3[2];
 };
 
 struct BatchHot {

@@ -2194,8 +2194,9 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       cols[k] = r.key[k];
     }
   }
-  // dimensions: mask form (a query-mask table, and a group-attribute table
-  // when a query groups on the dimension) when the mask table stays small,
+  // dimensions: mask form (a query-mask table whose 16-byte entries also
+  // hold the group attributes when a query groups on the dimension) when the
+  // mask table stays small,
   // else attribute form (one table of every filtered or grouped attribute);
   // key filters of attribute-form dimensions become constraints on the fact
   // key column's value (kcons), evaluated by its interval table
@@ -2315,7 +2316,8 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       for (int c = 0; c < nfc; c++) B.fcol[c] = fcols[c]->d_idx;
       // slot len stays zero: the probe of an item no query needs, or whose
       // key has no slot, reads it
-      const size_t mbytes = (((size_t)(B.len + 1) * 8) + 15) & ~(size_t)15;
+      B.mstride = B.nattr ? 2 : 1;
+      const size_t mbytes = (((size_t)(B.len + 1) * 8 * B.mstride) + 15) & ~(size_t)15;
       HIP_OK(e->d_mk[k].reserve(mbytes));
       B.mk = static_cast<unsigned long long*>(e->d_mk[k].p);
     } else {
@@ -2352,7 +2354,7 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       B.nfcol = 0;
     }
     B.used = 1;
-    if (B.nattr || B.form == kDimAttr) {
+    if (B.form == kDimAttr) {
       // a zero entry at slot len, and the aligned 8-byte word holding it
       const size_t pbytes = (((size_t)(B.len + 1) * B.pw + 8) + 15) & ~(size_t)15;
       HIP_OK(e->d_rec[k].reserve(pbytes));
