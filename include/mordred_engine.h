@@ -176,6 +176,23 @@ int mordred_engine_store_segment_minmax(mordred_engine* e, const char* name, con
 /* mins / maxs receive one entry per segment (mordred_engine_column_segments). */
 int mordred_engine_segment_minmax(mordred_engine* e, const char* name, int* mins, int* maxs);
 int mordred_engine_set_skipping(mordred_engine* e, int on);
+/* Dimension table forms a shared-scan batch may choose (a mask of the
+ * MORDRED_BATCH_FORM_* bits; default all).  Mask: per slot the 64-bit mask of
+ * the queries whose filters the row passes; attribute: per slot the packed
+ * attributes; dictionary: per slot a code of the row's attribute tuple,
+ * decoded through a table in LDS.  A dimension takes the dictionary form when
+ * allowed and its rows hold at most 1,024 distinct attribute tuples, else the
+ * attribute form when allowed and applicable (tables above 1 MiB in mask
+ * form, or any table without the MASK bit), else the mask form (always
+ * available).  Every form gives the same results.  No counterpart in the
+ * reference. */
+enum {
+  MORDRED_BATCH_FORM_MASK = 1,
+  MORDRED_BATCH_FORM_ATTR = 2,
+  MORDRED_BATCH_FORM_DICT = 4,
+  MORDRED_BATCH_FORM_ALL = 7
+};
+int mordred_engine_set_batch_forms(mordred_engine* e, int forms);
 /* COST_SPLIT prefix streaming (default on): the GPU side of the split may
  * stream only the columns of a subset of the fact filters and probes of a
  * non-resident segment, run those operators, and hand the survivors to the

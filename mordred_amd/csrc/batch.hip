@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchDesc* __rest
 // checked that no two rows share a slot.
 __global__ __launch_bounds__(256) void batch_records_kernel(BatchLaunch L, int d) {
   const BatchDim& B = L.desc->dim[d];
-  if (!B.used || !L.desc->rebuild) return;
+  if (!B.used || !L.desc->rebuild || B.form == kDimDict) return;  // (dictionary codes: built by the host)
   const int nrows = B.nrows, len = B.len, mk = B.minkey, pw = B.pw, na = B.nattr, nfc = B.nfcol, nf = B.nfilt;
   const int stride = gridDim.x * blockDim.x;
   for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < nrows; row += stride) {
@@ -226,9 +226,11 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       Hd.form = B.form;
       Hd.nattr = B.nattr;
       Hd.mstride = B.mstride;
+      Hd.cb = B.cb;
+      Hd.dict = B.dict_off;
       for (int a = 0; a < kBatchMaxAttr; a++) {
-        // attribute form stores value - base + 1
-        Hd.base[a] = B.attr[a].base - (B.form == kDimAttr ? 1 : 0);
+        // the attribute and dictionary forms store value - base + 1
+        Hd.base[a] = B.attr[a].base - (B.form != kDimMask ? 1 : 0);
         Hd.off[a] = B.attr[a].off;
         Hd.bits[a] = B.attr[a].bits;
         Hd.lut[a] = B.attr[a].lut;
@@ -395,25 +397,45 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     const void* const dummy = D;
     unsigned long long mv[2][IPT];  // two dimensions in flight
     uint32_t aw[2][IPT];            // (16-byte mask entries: their attribute word)
-    uint32_t shp[4];                // the entry's byte offset in its word, 3 bits per item
+    // the slot an item reads: its own, or slot len (zero) when no query
+    // needs it or its key has no slot
+    auto slot_of = [&](int d, int j, int len, int mk, unsigned long long uses) {
+      const int h = phash(x[d][j], len, mk);
+      return (unsigned)h < (unsigned)len && (alive[j] & uses) ? (unsigned)h : (unsigned)len;
+    };
+    // byte offset of the load holding the entry of slot s, and the entry's
+    // bit offset in the loaded 8 bytes: mask form 8 or 16 bytes per slot,
+    // attribute form pw bytes (in the aligned 8-byte word), dictionary form
+    // cb bits (the 8 bytes from the aligned 4-byte word holding its first bit;
+    // cb <= 11 bits, so they hold it)
+    auto entry_at = [](int form, unsigned stride, unsigned s, unsigned& sh) -> uint32_t {
+      if (form == kDimDict) {
+        const uint32_t bit = s * stride;
+        const uint32_t at = (bit >> 3) & ~3u;
+        sh = bit - 8u * at;
+        return at;
+      }
+      const uint32_t b = s * stride;  // (a table is far below 4 GB)
+      sh = (b & 7u) * 8u;
+      return b & ~7u;
+    };
     auto issue = [&](int d) {
       const BatchHotDim& Hd = H.dim[d];
       const bool used = usedmask >> d & 1;
       const int len = used ? __builtin_amdgcn_readfirstlane(Hd.len) : 0;
       const int mk = __builtin_amdgcn_readfirstlane(Hd.minkey);
-      const bool mform = __builtin_amdgcn_readfirstlane(Hd.form) == kDimMask;
-      const bool wide = used && mform && __builtin_amdgcn_readfirstlane(Hd.mstride) == 2;
-      const unsigned stride = wide ? 16u : mform ? 8u : (unsigned)__builtin_amdgcn_readfirstlane(Hd.pw);
+      const int form = __builtin_amdgcn_readfirstlane(Hd.form);
+      const bool wide = used && form == kDimMask && __builtin_amdgcn_readfirstlane(Hd.mstride) == 2;
+      const unsigned stride = wide ? 16u
+                              : form == kDimMask ? 8u
+                              : (unsigned)__builtin_amdgcn_readfirstlane(form == kDimDict ? Hd.cb : Hd.pw);
       const unsigned long long uses = Hd.uses;
-      const void* tabp = uniform_ptr(!used ? dummy : mform ? static_cast<const void*>(Hd.mk) : Hd.pk);
+      const void* tabp = uniform_ptr(!used ? dummy : form == kDimMask ? static_cast<const void*>(Hd.mk) : Hd.pk);
       uint32_t b[IPT];
-      shp[d] = 0;
 #pragma unroll
       for (int j = 0; j < IPT; j++) {
-        const int h = phash(x[d][j], len, mk);
-        const unsigned s = (unsigned)h < (unsigned)len && (alive[j] & uses) ? (unsigned)h : (unsigned)len;
-        b[j] = s * stride;  // (a table is far below 4 GB)
-        shp[d] |= (b[j] & 7u) << (3 * j);
+        unsigned sh;
+        b[j] = wide ? slot_of(d, j, len, mk, uses) * 16u : entry_at(form, stride, slot_of(d, j, len, mk, uses), sh);
       }
       if (wide) {
 #pragma unroll
@@ -425,7 +447,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       } else {
 #pragma unroll
         for (int j = 0; j < IPT; j++) {
-          mv[d & 1][j] = gload_at<unsigned long long>(tabp, b[j] & ~7u);
+          mv[d & 1][j] = gload_at<unsigned long long>(tabp, b[j]);
           aw[d & 1][j] = 0u;
         }
       }
@@ -437,7 +459,8 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       const BatchHotDim& Hd = H.dim[d];
       const unsigned long long uses = Hd.uses;
       const unsigned long long(&m)[IPT] = mv[d & 1];
-      if (__builtin_amdgcn_readfirstlane(Hd.form) == kDimMask) {
+      const int form = __builtin_amdgcn_readfirstlane(Hd.form);
+      if (form == kDimMask) {
 #pragma unroll
         for (int j = 0; j < IPT; j++) {
           alive[j] &= m[j] | ~uses;
@@ -445,10 +468,33 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         }
         return;
       }
+      // the entry's offset, from the slot recomputed with the current alive
+      // masks: an item that lost every query using the dimension since its
+      // load may read another slot's bits, which no query then uses
+      const int len = __builtin_amdgcn_readfirstlane(Hd.len);
+      const int mk = __builtin_amdgcn_readfirstlane(Hd.minkey);
+      if (form == kDimDict) {
+        const unsigned cb = (unsigned)__builtin_amdgcn_readfirstlane(Hd.cb);
+        const uint4* dict = reinterpret_cast<const uint4*>(ltab + __builtin_amdgcn_readfirstlane(Hd.dict));
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+          unsigned sh;
+          (void)entry_at(kDimDict, cb, slot_of(d, j, len, mk, uses), sh);
+          const uint32_t code = (uint32_t)(m[j] >> sh) & ((1u << cb) - 1u);
+          const uint4 t = dict[code];
+          alive[j] &= ((unsigned long long)t.y << 32 | t.x) | ~uses;
+          attrw[d][j] = t.z;
+        }
+        return;
+      }
       const unsigned pw = (unsigned)__builtin_amdgcn_readfirstlane(Hd.pw);
       const uint32_t wm = pw >= 4 ? 0xFFFFFFFFu : (1u << (8 * pw)) - 1u;
 #pragma unroll
-      for (int j = 0; j < IPT; j++) attrw[d][j] = (uint32_t)(m[j] >> (((shp[d] >> (3 * j)) & 7u) * 8u)) & wm;
+      for (int j = 0; j < IPT; j++) {
+        unsigned sh;
+        (void)entry_at(kDimAttr, pw, slot_of(d, j, len, mk, uses), sh);
+        attrw[d][j] = (uint32_t)(m[j] >> sh) & wm;
+      }
       const int na = __builtin_amdgcn_readfirstlane(Hd.nattr);
       for (int a = 0; a < na; a++) {
         const int lut = __builtin_amdgcn_readfirstlane(Hd.lut[a]);

@@ -59,7 +59,13 @@ constexpr int kBatchMaxBuckets = 512;
 constexpr int kBatchMaxTab = 8192;  // 32-bit words of interval and mask tables (staged in LDS)
 constexpr int kBatchLutSpan = 2048; // widest attribute given a mask table per value
 constexpr long long kBatchMaskFormBytes = 1 << 20;  // dimensions whose mask table stays this small
-enum BatchDimForm { kDimMask = 0, kDimAttr = 1 };
+// Dimension table forms: mask (per slot the 64-bit mask of the queries whose
+// filters the row passes), attribute (per slot the packed attributes, pw
+// bytes), dictionary (per slot a cb-bit code of the row's attribute tuple;
+// an LDS dictionary maps a code to its packed attributes and to the mask of
+// the queries whose attribute filters the tuple passes; code 0 = empty slot).
+enum BatchDimForm { kDimMask = 0, kDimAttr = 1, kDimDict = 2 };
+constexpr int kBatchDictMax = 1024;  // distinct attribute tuples of a dictionary (16 LDS bytes each)
 
 enum BatchExprKind { kExprDiff = 0, kExprProduct = 1 };  // a1 - a2 (group-by), a1 * a2 (aggregate)
 enum BatchQueryFlags {
@@ -151,7 +157,8 @@ struct BatchFilt {             // dimension filter of one query (build_GPU2's mo
 struct BatchDim {
   const int* key;              // segment table of the dimension key column
   unsigned long long* mk;      // query-mask table, len entries of mstride u64 words
-  void* pk;                    // attribute table (attribute form), len entries of pw bytes, or nullptr
+  void* pk;                    // attribute table (attribute form, len entries of pw bytes;
+                               // dictionary form, len + 1 codes of cb bits), or nullptr
   unsigned long long uses;     // queries that probe this dimension
   unsigned long long groupers; // queries that take a group value from it
   int used, len, minkey, nrows;
@@ -162,6 +169,8 @@ struct BatchDim {
   // dimension (the mask, then the packed group attributes in the next word's
   // low half: one 16-byte gather reads both)
   int mstride;
+  int cb, dict_off;            // dictionary form: bits per code; LDS table word (a multiple
+                               // of 4) of the dictionary: per code its mask, attributes, 0
   long long pk_words;          // 16-byte words of the attribute table (cleared by the fill)
   BatchAttr attr[kBatchMaxAttr];
   const int* fcol[kBatchMaxFCol];
@@ -222,8 +231,8 @@ struct BatchHotDim {
   const void* pk;
   unsigned long long uses, groupers;
   int len, minkey, pw, form;
-  int nattr, pad;
-  int base[kBatchMaxAttr], pad2;  // group attribute bases (keys stored in the pair loop)
+  int nattr, cb;
+  int base[kBatchMaxAttr], dict;  // group attribute bases (keys stored in the pair loop)
   int off[kBatchMaxAttr], bits[kBatchMaxAttr], lut[kBatchMaxAttr], mstride, pad3[2];
 };
 

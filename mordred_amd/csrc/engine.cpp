@@ -169,6 +169,21 @@ struct HostBuf {
 
 using namespace mordred_host;
 
+// Dictionary of a dimension's attribute tuples (batch dictionary form):
+// per slot (key - min_key) % len the code of its row's tuple (0: no row),
+// cb bits per code, on the device; tuple t (code t + 1) packed as in the
+// attribute form (value - base + 1 per attribute, attrs in order).
+struct DimDict {
+  const Column* key = nullptr;
+  std::vector<Column*> attrs;
+  int len = 0, minkey = 0;
+  bool ok = false;                // false: too many tuples or attributes wider than 32 bits
+  int cb = 0;
+  std::vector<unsigned> tuples;
+  DevBuf codes;
+  unsigned long long last_use = 0;
+};
+
 struct mordred_engine {
   int device = 0;
   int cache_segments = 0;
@@ -196,6 +211,9 @@ struct mordred_engine {
   // shared-scan batches: record tables, descriptor, result arena, the graph
   DevBuf d_rec[4], d_mk[4], d_desc, d_arena, d_status, d_touch;
   HostBuf h_desc, h_arena;
+  int batch_forms = MORDRED_BATCH_FORM_ALL;     // dimension table forms batches may use
+  std::vector<std::unique_ptr<DimDict>> dicts;  // dictionary-form code tables
+  unsigned long long dict_clock = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<char> gkey;
@@ -2046,6 +2064,107 @@ void column_range(Column* c) {
   c->ranged = true;
 }
 
+// Bit layout of packed attributes (attribute and dictionary forms): per
+// attribute value - vmin + 1 in the fewest bits holding its span + 1, from
+// bit 0 in order.  Returns the total bits.
+int attr_layout(const std::vector<Column*>& attrs, int* off, int* bits) {
+  int o = 0;
+  for (size_t a = 0; a < attrs.size(); a++) {
+    column_range(attrs[a]);
+    const long long span = (long long)attrs[a]->vmax - attrs[a]->vmin + 2;
+    int b = 1;
+    while ((1ll << b) < span) b++;
+    off[a] = o;
+    bits[a] = b;
+    o += b;
+  }
+  return o;
+}
+
+// The dictionary of a dimension's attribute tuples over attrs (or a cached
+// one over a superset of them), built on the host from the columns' host
+// copies and kept (at most 16, the least recently used dropped).  A new
+// dictionary covers the attributes of the cached one for the same key too
+// (while at most 3), so a batch stream converges on one per dimension.
+// *out = nullptr when the rows have more than kBatchDictMax distinct tuples.
+int dim_dict(mordred_engine* e, Column* key, const std::vector<Column*>& attrs, int len, int minkey,
+             const DimDict** out) {
+  using namespace mordred;
+  *out = nullptr;
+  DimDict* wider = nullptr;
+  for (auto& d : e->dicts) {
+    if (d->key != key || d->len != len || d->minkey != minkey) continue;
+    bool covers = true;
+    for (Column* c : attrs) covers &= std::find(d->attrs.begin(), d->attrs.end(), c) != d->attrs.end();
+    if (covers) {
+      d->last_use = ++e->dict_clock;
+      if (d->ok) *out = d.get();
+      return 0;
+    }
+    wider = d.get();
+  }
+  auto d = std::make_unique<DimDict>();
+  d->key = key;
+  d->len = len;
+  d->minkey = minkey;
+  d->attrs = attrs;
+  if (wider && wider->ok)
+    for (Column* c : wider->attrs)
+      if (std::find(d->attrs.begin(), d->attrs.end(), c) == d->attrs.end() && d->attrs.size() < (size_t)kBatchMaxAttr)
+        d->attrs.push_back(c);
+  d->last_use = ++e->dict_clock;
+  int off[kBatchMaxAttr], bits[kBatchMaxAttr];
+  const int total = attr_layout(d->attrs, off, bits);
+  const int na = (int)d->attrs.size();
+  std::vector<unsigned> slot_code((size_t)len + 1, 0);
+  std::unordered_map<unsigned, int> code;
+  d->ok = total <= 32;
+  for (long long i = 0; d->ok && i < key->n; i++) {
+    const int v = key->host[i] - minkey;
+    const int sl = v % len;  // the probe's slot (C remainder, as phash)
+    if (sl < 0) continue;
+    unsigned w = 0;
+    for (int a = 0; a < na; a++)
+      w |= (((unsigned)(d->attrs[a]->host[i] - d->attrs[a]->vmin) + 1u) & ((bits[a] == 32 ? 0u : (1u << bits[a])) - 1u))
+           << off[a];
+    auto it = code.find(w);
+    if (it == code.end()) {
+      if ((int)d->tuples.size() == kBatchDictMax) {
+        d->ok = false;
+        break;
+      }
+      it = code.emplace(w, (int)d->tuples.size() + 1).first;
+      d->tuples.push_back(w);
+    }
+    slot_code[sl] = (unsigned)it->second;
+  }
+  if (d->ok) {
+    int cb = 1;
+    while ((1ll << cb) < (long long)d->tuples.size() + 1) cb++;
+    d->cb = cb;
+    // codes at bit slot * cb; 8 bytes of slack for the scan's 8-byte loads
+    const size_t bytes = ((((size_t)len + 1) * cb + 7) / 8 + 8 + 15) & ~(size_t)15;
+    std::vector<unsigned char> buf(bytes, 0);
+    for (size_t sl = 0; sl <= (size_t)len; sl++) {
+      const unsigned c = slot_code[sl];
+      if (!c) continue;
+      const size_t bit = sl * cb;
+      for (int b = 0; b < cb; b++)
+        if (c >> b & 1u) buf[(bit + b) >> 3] |= (unsigned char)(1u << ((bit + b) & 7));
+    }
+    HIP_OK(d->codes.reserve(bytes));
+    HIP_OK(hipMemcpy(d->codes.p, buf.data(), bytes, hipMemcpyHostToDevice));
+  }
+  if (e->dicts.size() >= 16) {
+    auto lru = std::min_element(e->dicts.begin(), e->dicts.end(),
+                                [](const auto& x, const auto& y) { return x->last_use < y->last_use; });
+    e->dicts.erase(lru);
+  }
+  e->dicts.push_back(std::move(d));
+  if (e->dicts.back()->ok) *out = e->dicts.back().get();
+  return 0;
+}
+
 // Whether query i probes table k: every join of a group-by query; only the
 // date table (slot 4) for an aggregate query, like filter_probe_aggr_GPU2
 // (GPUProcessing.h:1675-1776), which probes ht4 alone.
@@ -2150,9 +2269,12 @@ bool keys_decode(const mordred_star_query& q, const Resolved& r, mordred::BatchK
   return true;
 }
 
-int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, const mordred::BatchConfig& cfg,
-               BatchPlan& b, const SlotArenas* fixed = nullptr, const double* sel = nullptr) {
+// forms: the dimension table forms allowed (MORDRED_BATCH_FORM_*).
+// lds_short is set when the plan fails because its LDS tables do not fit.
+int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int nq, const mordred::BatchConfig& cfg,
+                     BatchPlan& b, const SlotArenas* fixed, const double* sel, int forms, bool& lds_short) {
   using namespace mordred;
+  lds_short = false;
   if (nq < 1 || nq > kBatchMaxQ) return fail(e, "a batch holds 1..64 queries");
   if (!batch_config_ok(cfg)) return fail(e, "bad batch configuration");
   if ((size_t)cfg.lds_bytes < sizeof(BatchHot) + nq * sizeof(BatchQuery))
@@ -2275,9 +2397,19 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     std::vector<Column*> all_attrs = gcols;
     for (Column* c : fattrs)
       if (std::find(all_attrs.begin(), all_attrs.end(), c) == all_attrs.end()) all_attrs.push_back(c);
-    B.form = ((long long)B.len * 8 > kBatchMaskFormBytes && narrow && (int)all_attrs.size() <= kBatchMaxAttr)
-                 ? kDimAttr : kDimMask;
-    const std::vector<Column*>& packed = B.form == kDimAttr ? all_attrs : gcols;
+    // dictionary form when the rows hold few distinct attribute tuples
+    const DimDict* dict = nullptr;
+    if ((forms & MORDRED_BATCH_FORM_DICT) && (int)all_attrs.size() <= kBatchMaxAttr) {
+      const int rc = dim_dict(e, key, all_attrs, B.len, B.minkey, &dict);
+      if (rc) return rc;
+      if (dict && ((D.ntab + 3) & ~3) + 4 * ((int)dict->tuples.size() + 1) > kBatchMaxTab) dict = nullptr;
+    }
+    B.form = dict ? kDimDict
+                  : ((forms & MORDRED_BATCH_FORM_ATTR) &&
+                     ((long long)B.len * 8 > kBatchMaskFormBytes || !(forms & MORDRED_BATCH_FORM_MASK)) &&
+                     narrow && (int)all_attrs.size() <= kBatchMaxAttr)
+                        ? kDimAttr : kDimMask;
+    const std::vector<Column*>& packed = B.form == kDimDict ? dict->attrs : B.form == kDimAttr ? all_attrs : gcols;
     if ((int)packed.size() > kBatchMaxAttr) return fail(e, "more than 3 group columns on one dimension in a batch");
     // packed attributes from bit 0 (attribute form: value - base + 1, so one
     // more value than the span)
@@ -2286,7 +2418,7 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       Column* c = packed[a];
       attrs[k][a] = c;
       column_range(c);
-      const long long span = (long long)c->vmax - c->vmin + 1 + (B.form == kDimAttr ? 1 : 0);
+      const long long span = (long long)c->vmax - c->vmin + 1 + (B.form != kDimMask ? 1 : 0);
       int bits = 1;
       while ((1ll << bits) < span) bits++;
       B.attr[a] = BatchAttr{c->d_idx, c->vmin, off, bits, -1, 0};
@@ -2320,6 +2452,41 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       const size_t mbytes = (((size_t)(B.len + 1) * 8 * B.mstride) + 15) & ~(size_t)15;
       HIP_OK(e->d_mk[k].reserve(mbytes));
       B.mk = static_cast<unsigned long long*>(e->d_mk[k].p);
+    } else if (B.form == kDimDict) {
+      // per code 16 bytes: the mask of the queries whose attribute filters
+      // its tuple passes, its packed attributes, 0 (code 0: empty slot)
+      const int nt = (int)dict->tuples.size() + 1;
+      const int at = (D.ntab + 3) & ~3;
+      B.dict_off = at;
+      B.cb = dict->cb;
+      for (int t = 0; t < nt; t++) {
+        const unsigned w = t ? dict->tuples[t - 1] : 0u;
+        unsigned long long m = t ? ~0ull : 0ull;
+        for (const DFilt& f : filts) {
+          if (f.col == key || !t) continue;
+          int a = 0;
+          while (attrs[k][a] != f.col) a++;
+          const unsigned fm = B.attr[a].bits >= 32 ? ~0u : (1u << B.attr[a].bits) - 1u;
+          const int x = (int)((w >> B.attr[a].off) & fm) - 1 + B.attr[a].base;
+          const bool ok = f.mode == 0 ? x == f.c1 : f.mode == 1 ? (x >= f.c1 && x <= f.c2) : (x == f.c1 || x == f.c2);
+          if (!ok) m &= ~(1ull << f.q);
+        }
+        D.tab[at + 4 * t] = (unsigned)m;
+        D.tab[at + 4 * t + 1] = (unsigned)(m >> 32);
+        D.tab[at + 4 * t + 2] = w;
+        D.tab[at + 4 * t + 3] = 0u;
+      }
+      D.ntab = at + 4 * nt;
+      for (const DFilt& f : filts) {
+        if (f.col != key) continue;
+        Cons c{f.q, 0, {0, 0}, {0, 0}};
+        filter_ranges(f.mode, f.c1, f.c2, c);
+        kcons[k].push_back(c);
+      }
+      B.mk = nullptr;
+      B.nfilt = 0;
+      B.nfcol = 0;
+      B.pk = dict->codes.p;
     } else {
       // a mask table per filtered attribute (bit q cleared for the values q
       // rejects); key filters go to the fact key column's interval table
@@ -2328,7 +2495,10 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
         if (std::find(fattrs.begin(), fattrs.end(), c) == fattrs.end()) continue;
         const int span = (int)((long long)c->vmax - c->vmin + 1);
         const int at = (D.ntab + 1) & ~1;
-        if (at + 2 * span > kBatchMaxTab) return fail(e, "attribute mask tables exceed the batch's LDS table space");
+        if (at + 2 * span > kBatchMaxTab) {
+          lds_short = true;
+          return fail(e, "attribute mask tables exceed the batch's LDS table space");
+        }
         B.attr[a].lut = at;
         for (int v = 0; v < span; v++) {
           unsigned long long m = ~0ull;
@@ -2360,6 +2530,8 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       HIP_OK(e->d_rec[k].reserve(pbytes));
       B.pk = e->d_rec[k].p;
       B.pk_words = (long long)(pbytes / 16);
+    } else if (B.form == kDimDict) {
+      B.pk_words = 0;  // (the cached code table, never cleared)
     } else {
       B.pk = nullptr;
       B.pk_words = 0;
@@ -2417,8 +2589,8 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
       Q.shift[k] = (unsigned char)A.off;
       Q.bits[k] = (unsigned char)A.bits;
       if (A.bits > 16) return fail(e, "batch group index needs attribute fields of at most 16 bits");
-      // the field holds value - base (+ 1 in attribute form)
-      const unsigned fbase = (unsigned)A.base - (D.dim[k].form == kDimAttr ? 1u : 0u);
+      // the field holds value - base (+ 1 in the attribute and dictionary forms)
+      const unsigned fbase = (unsigned)A.base - (D.dim[k].form != kDimMask ? 1u : 0u);
       hbase += (fbase - (unsigned)r.join[k]->min_val) * (unsigned)uv;
     }
     Q.hbase = (int)hbase;
@@ -2445,7 +2617,10 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     if (keyed) on.insert(on.end(), kcons[c].begin(), kcons[c].end());
     if (!keyed && npc < kBatchIvlMin) continue;
     if (D.nivl == kBatchMaxIvl) {
-      if (keyed) return fail(e, "too many interval tables in a batch");
+      if (keyed) {
+        lds_short = true;
+        return fail(e, "too many interval tables in a batch");
+      }
       continue;
     }
     column_range(cols[c]);
@@ -2477,7 +2652,10 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
     const int mask_words = 2 * (n + 1), rank_words = (nb + 1) / 2, bp_words = n + kmax;
     const int at = (D.ntab + 1) & ~1;
     if (at + mask_words + rank_words + bp_words > kBatchMaxTab) {
-      if (keyed) return fail(e, "interval tables exceed the batch's LDS table space");
+      if (keyed) {
+        lds_short = true;
+        return fail(e, "interval tables exceed the batch's LDS table space");
+      }
       continue;
     }
     BatchIvl& I = D.ivl[D.nivl++];
@@ -2543,7 +2721,10 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
   }
   const long long fixed_bytes = (long long)sizeof(BatchHot) + nq * (long long)sizeof(BatchQuery) +
                                 4LL * ((D.ntab + 1) & ~1);
-  if (fixed_bytes > cfg.lds_bytes) return fail(e, "lds_bytes below the staged descriptor and interval tables");
+  if (fixed_bytes > cfg.lds_bytes) {
+    lds_short = true;
+    return fail(e, "lds_bytes below the staged descriptor and interval tables");
+  }
   long long avail = ((long long)cfg.lds_bytes - fixed_bytes) / 4;  // 32-bit words
   std::vector<int> order;
   for (int i = 0; i < nq; i++)
@@ -2631,6 +2812,17 @@ int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, c
   D.num_tuples = (int)b.n;
   D.rebuild = 1;
   return 0;
+}
+
+// A plan with the engine's allowed table forms, else (when its LDS tables do
+// not fit the configuration) with the mask form only, whose tables take the
+// least LDS.
+int plan_batch(mordred_engine* e, const mordred_star_query* const* qp, int nq, const mordred::BatchConfig& cfg,
+               BatchPlan& b, const SlotArenas* fixed = nullptr, const double* sel = nullptr) {
+  bool short_ = false;
+  const int rc = plan_batch_forms(e, qp, nq, cfg, b, fixed, sel, e->batch_forms, short_);
+  if (rc == 0 || !short_ || e->batch_forms == MORDRED_BATCH_FORM_MASK) return rc;
+  return plan_batch_forms(e, qp, nq, cfg, b, fixed, sel, MORDRED_BATCH_FORM_MASK, short_);
 }
 
 // Launch arguments and record tables a graph was captured with.
@@ -3019,6 +3211,13 @@ int mordred_engine_segment_minmax(mordred_engine* e, const char* name, int* mins
 int mordred_engine_set_skipping(mordred_engine* e, int on) {
   if (!e) return -1;
   e->skipping = on != 0;
+  return 0;
+}
+
+int mordred_engine_set_batch_forms(mordred_engine* e, int forms) {
+  if (!e) return -1;
+  if (forms & ~MORDRED_BATCH_FORM_ALL) return fail(e, "unknown batch form bits");
+  e->batch_forms = forms;
   return 0;
 }
 

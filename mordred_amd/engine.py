@@ -145,6 +145,7 @@ _SIGS = {
     "mordred_engine_store_segment_minmax": ([C.c_void_p, C.c_char_p, C.c_char_p], C.c_int),
     "mordred_engine_segment_minmax": ([C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p], C.c_int),
     "mordred_engine_set_skipping": ([C.c_void_p, C.c_int], C.c_int),
+    "mordred_engine_set_batch_forms": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_set_prefix_streaming": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_set_operator_order": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_struct_size": ([C.c_char_p], C.c_longlong),
@@ -279,6 +280,12 @@ class Engine:
 
     def set_skipping(self, on: bool):
         self._check(self.lib.mordred_engine_set_skipping(self.h, int(on)), "set_skipping")
+
+    # dimension table forms a batch may use (include/mordred_engine.h)
+    FORM_MASK, FORM_ATTR, FORM_DICT, FORM_ALL = 1, 2, 4, 7
+
+    def set_batch_forms(self, forms: int):
+        self._check(self.lib.mordred_engine_set_batch_forms(self.h, int(forms)), "set_batch_forms")
 
     def set_prefix_streaming(self, on: bool):
         self._check(self.lib.mordred_engine_set_prefix_streaming(self.h, int(on)), "set_prefix_streaming")

@@ -60,6 +60,22 @@ def test_batch_configs_match_oracle(bengine, config):
         assert st["global_queries"] >= 32
 
 
+@pytest.mark.parametrize("forms", [E.Engine.FORM_DICT | E.Engine.FORM_ATTR, E.Engine.FORM_ATTR,
+                                   E.Engine.FORM_MASK, E.Engine.FORM_ALL])
+def test_dimension_forms_match_oracle(bengine, forms):
+    """Dictionary codes, attribute entries (here on every table) and masks."""
+    qs, _, _, eng, refs = bengine
+    print(f"[gpu] batch forms {forms}", flush=True)
+    try:
+        eng.set_batch_forms(forms)
+        outs, st = eng.batch(qs, (1024, 1, 160 << 10))
+        check(qs, outs, refs)
+        outs, st = eng.batch(qs[:9], (512, 1, 64 << 10))
+        check(qs[:9], outs, refs)
+    finally:
+        eng.set_batch_forms(E.Engine.FORM_ALL)
+
+
 def test_each_query_alone_equals_mix(bengine):
     qs, _, _, eng, refs = bengine
     mixed, _ = eng.batch(qs)
