@@ -273,6 +273,8 @@ typedef struct mordred_batch_stats {
   int graph_captured;        /* 1 if this call (re)captured the graph */
   mordred_batch_config config;
   double alg_bytes;          /* distinct fact columns read x 4 B x rows scanned */
+  int lds32_queries;         /* of lds_queries: 32-bit LDS sums (a returning add per row) */
+  int keys_in_loop_queries;  /* keys stored by the scan (a returning add per row) */
 } mordred_batch_stats;
 
 typedef struct mordred_batch_tune_entry {

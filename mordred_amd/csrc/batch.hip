@@ -554,10 +554,11 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   // carry) and queries that store their keys in the loop -- are handled in a
   // loop of their own, so that the main loop over the other pairs never waits
   // for an atomic (a wave runs every sink path some lane needs).
-  unsigned long long slowq = 0;
+  unsigned long long slowq = 0, slow32 = 0;
   for (int q = 0; q < nq; q++) {
     const BatchQuery Q = query(q, false);
     if ((Q.flags & kQueryKeysInLoop) || Q.sink == kSinkLds32) slowq |= 1ull << q;
+    if (!(Q.flags & kQueryKeysInLoop) && Q.sink == kSinkLds32) slow32 |= 1ull << q;
   }
 
   auto group_of = [&](const BatchQuery& Q, int j) {
@@ -606,7 +607,53 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
           (void)__hip_atomic_fetch_or(lgrp + Q.acc + (h >> 5), 1u << (h & 31), __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_WORKGROUP);
       }
-      m = alive[j] & slowq;
+      // 32-bit LDS sums: (int64)val = u - 2^32 [val < 0]; the low word wraps
+      // when old + u carries, and the 2^32 that leaves it goes to the result.
+      // Up to four pairs per lane are prepared (descriptor reads), then their
+      // adds issued back to back and only then their returns read: LDS
+      // operations complete in order, so a wave waits once per four pairs
+      // instead of once per pair.
+      m = alive[j] & slow32;
+      while (m) {
+        unsigned old[4], uadd[4], word[4], touch[4];
+        int qs[4], hs[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          qs[r] = -1;
+          if (!m) continue;
+          const int q = __builtin_ctzll(m);
+          m &= m - 1;
+          const BatchQuery Q = query(q, false);
+          const int h = group_of(Q, j);
+          if ((unsigned)h >= (unsigned)Q.total_val) continue;
+          const int tw = batch_touch_words(Q.total_val, Q.flags);
+          uadd[r] = (unsigned)value_of(Q, j);
+          word[r] = (unsigned)(Q.acc + tw + h);
+          touch[r] = tw ? (unsigned)(Q.acc + (h >> 5)) : ~0u;
+          qs[r] = q;
+          hs[r] = h;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          if (qs[r] < 0) continue;
+          old[r] = __hip_atomic_fetch_add(lgrp + word[r], uadd[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (touch[r] != ~0u)
+            (void)__hip_atomic_fetch_or(lgrp + touch[r], 1u << (hs[r] & 31), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          if (qs[r] < 0) continue;
+          const int delta = (int)(old[r] + uadd[r] < old[r]) - (int)((int)uadd[r] < 0);
+          if (!delta) continue;
+          const BatchQuery Q = query(qs[r], false);
+          auto* gsum = (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(
+              reinterpret_cast<int*>(Q.res) + 6 * (size_t)hs[r] + 4);
+          (void)__hip_atomic_fetch_add(gsum, (unsigned long long)(long long)delta << 32, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      m = alive[j] & slowq & ~slow32;
       while (m) {
         const int q = __builtin_ctzll(m);
         m &= m - 1;
@@ -619,21 +666,6 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         auto* gsum = (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(res + 4);
         const int tw = batch_touch_words(Q.total_val, Q.flags);
         unsigned* acc = lgrp + Q.acc + tw;
-        if (!(Q.flags & kQueryKeysInLoop)) {
-          // 32-bit LDS sum: (int64)val = u - 2^32 [val < 0]; the low word
-          // wraps when old + u carries, and the 2^32 that leaves it goes to
-          // the result
-          const unsigned u = (unsigned)val;
-          const unsigned old = __hip_atomic_fetch_add(acc + h, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          const int delta = (int)(old + u < old) - (int)(val < 0);
-          if (delta)
-            (void)__hip_atomic_fetch_add(gsum, (unsigned long long)(long long)delta << 32, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-          if (tw)
-            (void)__hip_atomic_fetch_or(lgrp + Q.acc + (h >> 5), 1u << (h & 31), __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-          continue;
-        }
         // keys that do not decode from the index: stored whenever the sum the
         // add returns was zero -- at the first pair of the group in this
         // workgroup (LDS) or overall (global), and again only if a running sum

@@ -2045,7 +2045,7 @@ struct BatchPlan {
   long long n = 0;
   std::vector<long long> res_off;  // ints
   std::vector<int> G;
-  int lds_queries = 0, global_queries = 0, dead = 0;
+  int lds_queries = 0, global_queries = 0, dead = 0, lds32_queries = 0, keys_in_loop = 0;
   int cols_read = 0;  // distinct fact columns the scan reads (algorithmic bytes: 4 per row each)
 };
 
@@ -2275,6 +2275,7 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
                      BatchPlan& b, const SlotArenas* fixed, const double* sel, int forms, bool& lds_short) {
   using namespace mordred;
   lds_short = false;
+  b.lds_queries = b.global_queries = b.dead = b.lds32_queries = b.keys_in_loop = 0;  // (a plan may be retried)
   if (nq < 1 || nq > kBatchMaxQ) return fail(e, "a batch holds 1..64 queries");
   if (!batch_config_ok(cfg)) return fail(e, "bad batch configuration");
   if ((size_t)cfg.lds_bytes < sizeof(BatchHot) + nq * sizeof(BatchQuery))
@@ -2729,9 +2730,17 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
   std::vector<int> order;
   for (int i = 0; i < nq; i++)
     if (qp[i]) order.push_back(i);
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    return est_sel[x] * b.G[y] > est_sel[y] * b.G[x];
-  });
+  // Order: the time a query would add on global atomics per LDS word it
+  // takes -- its pairs (selectivity) per group, weighted up for few groups,
+  // whose atomics all land on a few addresses and serialize there.  An
+  // estimate below 1e-3 counts as 1e-3 (estimates of small selectivities are
+  // the least reliable, and a small group array costs little LDS).
+  std::vector<double> prio(nq, 0.0);
+  for (int i = 0; i < nq; i++) {
+    const double g = std::max(1, b.G[i]);
+    prio[i] = std::max(est_sel[i], 1e-3) / g * (1.0 + 64.0 / g);
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return prio[x] > prio[y]; });
   // touched bitmaps first; if they do not fit, the largest queries store
   // their keys in the pair loop instead (no bitmap)
   {
@@ -2778,6 +2787,8 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
     lds_words += tw + even(batch_acc_words(sink[i], b.G[i]));
     if (sink[i] == kSinkGlobal) b.global_queries++;
     else b.lds_queries++;
+    b.lds32_queries += sink[i] == kSinkLds32;
+    b.keys_in_loop += (Q.flags & kQueryKeysInLoop) != 0;
     if (fixed && 6 * (long long)b.G[i] > fixed->res_stride)
       return fail(e, "query result larger than its serving slot");
     if (fixed && tw > fixed->touch_stride) return fail(e, "query groups exceed its serving slot's touched bitmap");
@@ -2925,6 +2936,8 @@ int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig&
     st->graph_captured = captured;
     st->config = mordred_batch_config{cfg.block, cfg.vpt, cfg.lds_bytes};
     st->alg_bytes = 4.0 * b.cols_read * (double)b.d.num_tuples;
+    st->lds32_queries = b.lds32_queries;
+    st->keys_in_loop_queries = b.keys_in_loop;
   }
   return 0;
 }

@@ -73,7 +73,8 @@ class BatchConfig(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("gpu_ms", C.c_double), ("lds_queries", C.c_int),
                 ("global_queries", C.c_int), ("dead_queries", C.c_int), ("graph_captured", C.c_int),
-                ("config", BatchConfig), ("alg_bytes", C.c_double)]
+                ("config", BatchConfig), ("alg_bytes", C.c_double), ("lds32_queries", C.c_int),
+                ("keys_in_loop_queries", C.c_int)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "config"}
