@@ -197,6 +197,7 @@ def split_bench(args):
                "analytic_ms": last[1]["plan_analytic_ms"]}
     log(f"cost_split_learned: {statistics.median(times):.1f} ms  {learned}")
     eng.set_learned_placement(E.LEARNED_OFF)
+    learned_order = learned_order_phase(eng, q, nseg, args.steps, args.learn_runs)
     best = min(runs, key=lambda k: runs[k]["ms_median"])
     out = {
         "metric": METRIC + " (segment cache + CPU/GPU split)",
@@ -217,6 +218,7 @@ def split_bench(args):
                    "placement": best, "cpu_threads": threads},
         "calibrated_bandwidth": {"cpu_Bps": bw_cpu, "pcie_h2d_Bps": bw_pci},
         "learned_placement": learned,
+        "learned_order": learned_order,
         "placements": {k: {"ms": v["ms_median"], "rows_per_s": v["rows_per_s"],
                            "stats": {a: b for a, b in v["stats"].items()}} for k, v in runs.items()},
     }
@@ -852,6 +854,54 @@ def plain_query_measure(q, data, desc, steps: int, warmup: int, no_cpu: bool = F
     else:
         out["cpu_baseline"] = None
         out["parity"] = None
+    return out
+
+
+def learned_order_phase(eng, q, nseg, steps, learn_runs):
+    """Operator order over partly cached segments, chosen by the engine's rule
+    (ORDER_AUTO: each order once, then the fastest measured) and by the
+    learned plan model (the engine's model, trained further here: EXPLORE
+    runs, then USE).  Segments keep only the last join's key and the
+    aggregate columns cached (as many as the cache holds), so they run as
+    mixed pipelines; the others run on the CPU.  Reports the times, the
+    orders and the model's error."""
+    from mordred_amd import engine as E
+    keep = {q.joins[-1].fact_key} | {a for a in q.aggr if a}
+    for s in range(nseg):
+        for c in q.fact_columns():
+            if c not in keep:
+                eng.evict_segment(c, s)
+    # (the cache holds half of the segments' columns)
+    mixed = min(nseg, int(0.5 * nseg * len(q.fact_columns()) / len(keep)))
+    for s in range(mixed):
+        for c in keep:
+            eng.cache_segment(c, s)
+    eng.set_operator_order(E.ORDER_AUTO)
+    try:
+        rule, rule_orders = [], []
+        for _ in range(3 + steps):
+            _, st = eng.run(q, E.PLACE_RESIDENCY)
+            rule.append(st["total_ms"])
+            rule_orders.append(st["operator_order"])
+        eng.set_learned_placement(E.LEARNED_EXPLORE)
+        for _ in range(learn_runs):
+            eng.run(q, E.PLACE_RESIDENCY)
+        eng.set_learned_placement(E.LEARNED_USE)
+        model, model_orders, by = [], [], {0: 0, 1: 0}
+        for _ in range(steps):
+            _, st = eng.run(q, E.PLACE_RESIDENCY)
+            model.append(st["total_ms"])
+            model_orders.append(st["operator_order"])
+            by[st["plan_model"]] = by.get(st["plan_model"], 0) + 1
+        stats = eng.placement_model_stats()
+    finally:
+        eng.set_learned_placement(E.LEARNED_OFF)
+        eng.set_operator_order(E.ORDER_GPU_FIRST)
+    out = {"mixed_segments": mixed, "train_runs": learn_runs,
+           "rule_ms_median_after_trials": statistics.median(rule[3:]), "rule_orders": rule_orders,
+           "model_ms_median": statistics.median(model), "model_orders": model_orders,
+           "plans_by_learned_model": by.get(1, 0), "plans_by_rule": by.get(0, 0), "model": stats}
+    log(f"learned order: {out}")
     return out
 
 

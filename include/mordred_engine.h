@@ -203,16 +203,25 @@ int mordred_engine_set_prefix_streaming(mordred_engine* e, int on);
 /* ---- learned placement -------------------------------------------------
  * No counterpart in the reference, whose optimizer places segments with the
  * analytic model alone (CostModel.cu:283-440, QueryOptimizer.cu:1718-1747).
- * With a mode other than OFF, MORDRED_PLACE_COST_SPLIT ranks candidate plans
- * (stream k = n*c/8 of the n non-resident segments, c = 0..8, plus the
- * analytic k) with an fp32 MLP (mordred_costmodel.h) that predicts the run's
- * time from query and plan features.  It is trained on the measured times of
- * the runs themselves, scored against the analytic model on every new run,
- * and chooses a plan only while it is the more accurate of the two (else the
- * analytic plan runs).  EXPLORE also runs a random candidate one time in
- * four, so that the model sees more than the plans it picks.  Every plan
- * gives the same exact result. */
-enum { MORDRED_LEARNED_OFF = 0, MORDRED_LEARNED_USE = 1, MORDRED_LEARNED_EXPLORE = 2 };
+ * With a mode other than OFF, an fp32 MLP (mordred_costmodel.h) predicts a
+ * run's time from query and plan features and ranks candidate plans:
+ *   - MORDRED_PLACE_COST_SPLIT: stream k = n*c/8 of the n non-resident
+ *     segments (c = 0..8, plus the analytic k);
+ *   - the operator order of partly cached segments (GPU-first, CPU-first,
+ *     alternate; the reference's executeTableFact_v1/v2, main.cu:155-161)
+ *     when the engine's order is MORDRED_ORDER_AUTO, under any placement;
+ *   - the segments mordred_engine_place keeps: a segment weighs what the
+ *     model predicts caching one more segment saves the query, instead of
+ *     the analytic CPU cost (the weights accumulate over calls, as the
+ *     reference's CacheManager.cu:533-587 weights do).
+ * It is trained on the measured times of the runs themselves, scored against
+ * the analytic model on every new run, and chooses only while it is the more
+ * accurate of the two (else the analytic plan, the engine's order rule and
+ * the analytic weights apply).  EXPLORE also runs a random candidate one time
+ * in four, so that the model sees more than the plans it picks; FORCE lets
+ * the model choose as soon as it has been trained once, accurate or not.
+ * Every plan, order and residency gives the same exact result. */
+enum { MORDRED_LEARNED_OFF = 0, MORDRED_LEARNED_USE = 1, MORDRED_LEARNED_EXPLORE = 2, MORDRED_LEARNED_FORCE = 3 };
 
 typedef struct mordred_placement_stats {
   long long samples;      /* runs observed */
@@ -220,6 +229,8 @@ typedef struct mordred_placement_stats {
   double learned_error;   /* EWMA |log error| on fresh runs */
   double analytic_error;
   int trusted;
+  long long order_choices;    /* runs whose operator order the model changed */
+  long long residency_calls;  /* mordred_engine_place calls weighted by the model */
 } mordred_placement_stats;
 
 int mordred_engine_set_learned_placement(mordred_engine* e, int mode);

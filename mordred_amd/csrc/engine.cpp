@@ -228,6 +228,9 @@ struct mordred_engine {
   mordred_cost_model* cm_place = nullptr;
   int place_mode = 0;
   int place_since_train = 0;
+  long long place_trains = 0;
+  long long learned_order_choices = 0;    // runs whose order the model changed
+  long long learned_residency_calls = 0;  // place() calls weighted by the model
   unsigned long long place_rng = 0x2545F4914F6CDD1Dull;
   std::string err;
 };
@@ -617,8 +620,11 @@ int split_balance(int n, double t_cpu, double t_gpu, double t_gcpu = 0, double t
   return (int)std::min<double>(n, std::max(0.0, (t_fix + n * t_cpu) / (t_gpu + t_cpu - t_gcpu) + 0.5));
 }
 
+// order: MORDRED_ORDER_* of the mixed groups; mixed_frac: their share of the
+// segments; mixed_ms: the analytic model's time for them.
 void split_features(const mordred_engine* e, const Resolved& r, const Selectivity& sel, int cached, int n, int k,
-                    double t_cpu, double t_str, double t_gcpu, double t_fix, float* f) {
+                    double t_cpu, double t_str, double t_gcpu, double t_fix, int order, double mixed_frac,
+                    double mixed_ms, float* f) {
   std::fill(f, f + MORDRED_COST_FEATURES, 0.f);
   const mordred_star_query& q = *r.q;
   int njoins = 0;
@@ -639,6 +645,18 @@ void split_features(const mordred_engine* e, const Resolved& r, const Selectivit
   f[16] = (float)std::log(std::max(split_analytic_ms(n, k, t_cpu, t_str, t_gcpu, t_fix), 1e-3));
   f[17] = (float)(std::log2((double)std::max(1, e->pool->size())) / 5.0);
   f[18] = (float)std::log(std::max(t_gcpu, 1e-3));
+  if (mixed_frac > 0 && order >= 0 && order < 3) f[19 + order] = 1.f;
+  f[22] = (float)mixed_frac;
+  f[23] = (float)std::log(std::max(mixed_ms, 1e-3));
+}
+
+// The learned plan model may choose: trusted (more accurate than the analytic
+// model on fresh runs), or, in MORDRED_LEARNED_FORCE, trained at least once.
+bool place_model_ready(mordred_engine* e) {
+  if (!e->cm_place || e->place_mode <= 0) return false;
+  mordred_cost_model_stats cs{};
+  if (mordred_cost_model_get_stats(e->cm_place, &cs) != 0) return false;
+  return cs.trusted || (e->place_mode == MORDRED_LEARNED_FORCE && e->place_trains > 0);
 }
 
 // ------------------------------------------------------------ CPU operators
@@ -3283,16 +3301,50 @@ int mordred_engine_place(mordred_engine* e, const mordred_star_query* q, double 
   // QueryOptimizer.cu:1696-1699)
   std::vector<char> skip;
   if (e->skipping) skip_segments(e, r, skip);
+  // With the learned plan model ready, a segment weighs what the model
+  // predicts caching one more segment saves this query (its COST_SPLIT run
+  // with the analytic split, before and after) instead of the analytic CPU
+  // cost; the weights accumulate over calls either way.
+  double learned_saving = -1;  // ms per full segment
+  if (place_model_ready(e)) {
+    int cached = 0, n = 0;
+    for (int s = 0; s < r.nseg; s++) {
+      if (!skip.empty() && skip[s]) continue;
+      bool all = true;
+      for (Column* c : r.fact_cols) all &= c->slot[s] >= 0;
+      (all ? cached : n)++;
+    }
+    if (n > 0) {
+      const double t_cpu = cpu_cost(r, sel, kSeg, e->bw_cpu) * 1e3;
+      const double t_str = transfer_cost(r, kSeg, e->bw_pci) * 1e3;
+      long long dim_rows = 0;
+      for (int k = 0; k < 4; k++)
+        if (r.join[k]) dim_rows += r.dkey[k]->n;
+      const double t_fix = e->build_ns_per_row * (double)dim_rows * 1e-6;
+      float xs[2 * MORDRED_COST_FEATURES], pred[2];
+      for (int c = 0; c < 2; c++) {
+        const int nn = n - c;
+        split_features(e, r, sel, cached + c, nn, split_balance(nn, t_cpu, t_str, 0, t_fix), t_cpu, t_str, 0, t_fix,
+                       -1, 0.0, 0.0, xs + c * MORDRED_COST_FEATURES);
+      }
+      if (mordred_cost_model_predict(e->cm_place, xs, 2, pred) == 0) {
+        learned_saving = std::max(0.0, std::exp((double)pred[0]) - std::exp((double)pred[1]));
+        e->learned_residency_calls++;
+      }
+    }
+  }
   std::vector<std::pair<double, int>> rank;
   for (int s = 0; s < r.nseg; s++) {
     if (!skip.empty() && skip[s]) {
       rank.emplace_back(1.0, s);
       continue;
     }
-    double w = cpu_cost(r, sel, (double)segment_rows(r, s), e->bw_cpu);
+    const double unit = learned_saving >= 0 ? learned_saving * (double)segment_rows(r, s) / kSeg
+                                            : cpu_cost(r, sel, (double)segment_rows(r, s), e->bw_cpu);
+    double w = unit;
     for (Column* c : r.fact_cols) {
       const long long id = ((long long)(intptr_t)c << 16) ^ s;
-      w += (e->seg_weight[id] += cpu_cost(r, sel, (double)segment_rows(r, s), e->bw_cpu));
+      w += (e->seg_weight[id] += unit);
     }
     rank.emplace_back(-w, s);
   }
@@ -3460,64 +3512,9 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   // the GPU side's time per segment, and the CPU time it leaves per segment
   const double t_gpu = prefix_plan ? st.est_prefix_ms_per_segment : t_str;
   const double t_gcpu = prefix_plan ? st.est_prefix_cpu_ms_per_segment : 0.0;
-  if (placement == MORDRED_PLACE_GPU_ONLY) {
-    stream_segs = other;
-  } else if (placement == MORDRED_PLACE_COST_SPLIT) {
-    // both sides run concurrently: send k segments to the GPU side so that
-    // both finish together
-    const int n = (int)other.size();
-    int k = split_balance(n, t_cpu, t_gpu, t_gcpu, t_fix);
-    if (e->cm_place && e->place_mode > 0 && n > 0) {
-      // candidate plans: k = n*c/8 for c = 0..8, and the analytic k
-      std::vector<int> ks;
-      for (int c = 0; c <= 8; c++) ks.push_back((int)((long long)n * c / 8));
-      ks.push_back(k);
-      std::sort(ks.begin(), ks.end());
-      ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
-      std::vector<float> xs(ks.size() * MORDRED_COST_FEATURES), pred(ks.size());
-      for (size_t c = 0; c < ks.size(); c++)
-        split_features(e, r, sel, (int)g.gpu_segs.size(), n, ks[c], t_cpu, t_gpu, t_gcpu, t_fix,
-                       &xs[c * MORDRED_COST_FEATURES]);
-      mordred_cost_model_stats cs{};
-      // a model error leaves the analytic plan (the GPU half already runs)
-      if (mordred_cost_model_get_stats(e->cm_place, &cs) == 0 && cs.trusted &&
-          mordred_cost_model_predict(e->cm_place, xs.data(), (int)ks.size(), pred.data()) == 0) {
-        size_t best = 0;
-        for (size_t c = 1; c < ks.size(); c++)
-          if (pred[c] < pred[best]) best = c;
-        k = ks[best];
-        st.plan_model = 1;
-        st.plan_predicted_ms = std::exp((double)pred[best]);
-      }
-      if (e->place_mode == 2) {
-        // exploration: one plan in four is drawn at random
-        e->place_rng ^= e->place_rng << 13;
-        e->place_rng ^= e->place_rng >> 7;
-        e->place_rng ^= e->place_rng << 17;
-        if ((e->place_rng & 3) == 0) {
-          k = ks[(size_t)((e->place_rng >> 8) % ks.size())];
-          st.plan_model = 2;
-        }
-      }
-      st.plan_analytic_ms = split_analytic_ms(n, k, t_cpu, t_gpu, t_gcpu, t_fix);
-      st.plan_streamed = k;
-    }
-    if (prefix_plan) pstream_segs.assign(other.begin(), other.begin() + k);
-    else stream_segs.assign(other.begin(), other.begin() + k);
-    cpu_segs.assign(other.begin() + k, other.end());
-  } else {
-    cpu_segs = other;
-  }
-  if ((!stream_segs.empty() || !pstream_segs.empty()) && e->free_slots.size() < staging) {
-    // not enough free cache slots for two staging batches: CPU instead
-    cpu_segs.insert(cpu_segs.end(), stream_segs.begin(), stream_segs.end());
-    cpu_segs.insert(cpu_segs.end(), pstream_segs.begin(), pstream_segs.end());
-    stream_segs.clear();
-    pstream_segs.clear();
-  }
-  std::sort(cpu_segs.begin(), cpu_segs.end());
-  // the operator order of the mixed groups; another order than GPU-first
-  // runs on this thread (below)
+  // the operator order of the mixed groups (the rule: the engine's order, or
+  // under MORDRED_ORDER_AUTO the fastest measured so far for this query and
+  // residency); another order than GPU-first runs on this thread (below)
   std::string order_key;
   int order = e->op_order;
   if (order == MORDRED_ORDER_AUTO) {
@@ -3533,6 +3530,82 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
       order = best;
     }
   }
+  long long mixed_segs = 0;
+  double mixed_ms = 0;  // the analytic model's time of the mixed groups
+  for (const MixedGroup& m : mixed) {
+    mixed_segs += (long long)m.segs.size();
+    mixed_ms += mixed_cost(e, r, sel, m, false).total() * (double)m.segs.size();
+  }
+  const double mixed_frac = r.nseg > 0 ? (double)mixed_segs / r.nseg : 0.0;
+  const int n = (int)other.size();
+  int k = placement == MORDRED_PLACE_GPU_ONLY ? n
+          : placement == MORDRED_PLACE_COST_SPLIT ? split_balance(n, t_cpu, t_gpu, t_gcpu, t_fix) : 0;
+  // the learned plan model: candidate plans are the split k (COST_SPLIT:
+  // k = n*c/8 for c = 0..8, and the analytic k) times the operator order
+  // (all three when the engine's order is AUTO and mixed groups run, else
+  // the rule's); the model's fastest plan runs while it is trusted
+  const bool split_plan = placement == MORDRED_PLACE_COST_SPLIT && n > 0;
+  const bool order_plan = !mixed.empty() && e->op_order == MORDRED_ORDER_AUTO;
+  const bool model_plans = e->cm_place && e->place_mode > 0 && (split_plan || order_plan);
+  if (model_plans) {
+    std::vector<int> ks{k}, os{order};
+    if (split_plan)
+      for (int c = 0; c <= 8; c++) ks.push_back((int)((long long)n * c / 8));
+    if (order_plan) os = {MORDRED_ORDER_GPU_FIRST, MORDRED_ORDER_CPU_FIRST, MORDRED_ORDER_ALTERNATE};
+    std::sort(ks.begin(), ks.end());
+    ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+    std::vector<std::pair<int, int>> plans;
+    for (int kk : ks)
+      for (int o : os) plans.emplace_back(kk, o);
+    std::vector<float> xs(plans.size() * MORDRED_COST_FEATURES), pred(plans.size());
+    for (size_t c = 0; c < plans.size(); c++)
+      split_features(e, r, sel, (int)g.gpu_segs.size(), n, plans[c].first, t_cpu, t_gpu, t_gcpu, t_fix,
+                     plans[c].second, mixed_frac, mixed_ms, &xs[c * MORDRED_COST_FEATURES]);
+    // a model error leaves the rule's plan (the GPU half already runs)
+    if (place_model_ready(e) && mordred_cost_model_predict(e->cm_place, xs.data(), (int)plans.size(), pred.data()) == 0) {
+      size_t best = 0;
+      for (size_t c = 1; c < plans.size(); c++)
+        if (pred[c] < pred[best]) best = c;
+      k = plans[best].first;
+      if (order_plan && plans[best].second != order) e->learned_order_choices++;
+      order = plans[best].second;
+      st.plan_model = 1;
+      st.plan_predicted_ms = std::exp((double)pred[best]);
+    }
+    if (e->place_mode == MORDRED_LEARNED_EXPLORE) {
+      // exploration: one plan in four is drawn at random
+      e->place_rng ^= e->place_rng << 13;
+      e->place_rng ^= e->place_rng >> 7;
+      e->place_rng ^= e->place_rng << 17;
+      if ((e->place_rng & 3) == 0) {
+        const auto& pl = plans[(size_t)((e->place_rng >> 8) % plans.size())];
+        k = pl.first;
+        order = pl.second;
+        st.plan_model = 2;
+      }
+    }
+  }
+  st.plan_analytic_ms = split_analytic_ms(n, k, t_cpu, t_gpu, t_gcpu, t_fix) + mixed_ms;
+  st.plan_streamed = k;
+  if (placement == MORDRED_PLACE_GPU_ONLY) {
+    stream_segs = other;
+  } else if (placement == MORDRED_PLACE_COST_SPLIT) {
+    // both sides run concurrently: k segments go to the GPU side (so that
+    // both finish together, by the analytic model)
+    if (prefix_plan) pstream_segs.assign(other.begin(), other.begin() + k);
+    else stream_segs.assign(other.begin(), other.begin() + k);
+    cpu_segs.assign(other.begin() + k, other.end());
+  } else {
+    cpu_segs = other;
+  }
+  if ((!stream_segs.empty() || !pstream_segs.empty()) && e->free_slots.size() < staging) {
+    // not enough free cache slots for two staging batches: CPU instead
+    cpu_segs.insert(cpu_segs.end(), stream_segs.begin(), stream_segs.end());
+    cpu_segs.insert(cpu_segs.end(), pstream_segs.begin(), pstream_segs.end());
+    stream_segs.clear();
+    pstream_segs.clear();
+  }
+  std::sort(cpu_segs.begin(), cpu_segs.end());
   const bool ordered = order != MORDRED_ORDER_GPU_FIRST && !mixed.empty();
   std::vector<MixedGroup> no_mixed;
   streamed.set_value(SplitPlan{stream_segs, ordered ? &no_mixed : &mixed, pstream_segs, &pgroup});
@@ -3636,17 +3709,18 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   }
   st.merge_ms = ms_since(tm0);
   st.total_ms = ms_since(t0);
-  // learn from the measured run (a plan the model was given)
-  if (placement == MORDRED_PLACE_COST_SPLIT && e->cm_place && e->place_mode > 0 && !other.empty() &&
-      st.plan_streamed == st.stream_segments + st.prefix_segments) {
+  // learn from the measured run (a plan the model was given, as it ran)
+  if (model_plans && (!split_plan || st.plan_streamed == st.stream_segments + st.prefix_segments)) {
     float f[MORDRED_COST_FEATURES];
-    split_features(e, r, sel, st.gpu_segments, (int)other.size(), st.plan_streamed, t_cpu, t_gpu, t_gcpu, t_fix, f);
+    split_features(e, r, sel, st.gpu_segments, n, split_plan ? st.plan_streamed : k, t_cpu, t_gpu, t_gcpu, t_fix,
+                   order, mixed_frac, mixed_ms, f);
     if (mordred_cost_model_observe(e->cm_place, f, std::max(1e-3, st.total_ms), std::max(1e-3, st.plan_analytic_ms)))
       return fail(e, std::string("placement model: ") + mordred_cost_model_last_error(e->cm_place));
     if (++e->place_since_train >= 8) {
       e->place_since_train = 0;
       if (mordred_cost_model_train(e->cm_place, 32, 64, 2e-3f, nullptr))
         return fail(e, std::string("placement model: ") + mordred_cost_model_last_error(e->cm_place));
+      e->place_trains++;
     }
   }
   if (!order_key.empty()) {
@@ -3659,7 +3733,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
 
 int mordred_engine_set_learned_placement(mordred_engine* e, int mode) {
   if (!e) return -1;
-  if (mode < MORDRED_LEARNED_OFF || mode > MORDRED_LEARNED_EXPLORE) return fail(e, "bad learned placement mode");
+  if (mode < MORDRED_LEARNED_OFF || mode > MORDRED_LEARNED_FORCE) return fail(e, "bad learned placement mode");
   if (mode != MORDRED_LEARNED_OFF && e->device < 0) return fail(e, "the learned placement model needs a GPU");
   if (mode != MORDRED_LEARNED_OFF && !e->cm_place) {
     e->cm_place = mordred_cost_model_create(e->device, 777);
@@ -3681,6 +3755,8 @@ int mordred_engine_placement_model_stats(mordred_engine* e, mordred_placement_st
   st->learned_error = cs.ewma_abs_log_error;
   st->analytic_error = cs.ewma_abs_log_error_analytic;
   st->trusted = cs.trusted;
+  st->order_choices = e->learned_order_choices;
+  st->residency_calls = e->learned_residency_calls;
   return 0;
 }
 

@@ -54,13 +54,14 @@ class RunStats(C.Structure):
 
 class PlacementStats(C.Structure):
     _fields_ = [("samples", C.c_longlong), ("scored", C.c_int), ("learned_error", C.c_double),
-                ("analytic_error", C.c_double), ("trusted", C.c_int)]
+                ("analytic_error", C.c_double), ("trusted", C.c_int), ("order_choices", C.c_longlong),
+                ("residency_calls", C.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
-LEARNED_OFF, LEARNED_USE, LEARNED_EXPLORE = 0, 1, 2
+LEARNED_OFF, LEARNED_USE, LEARNED_EXPLORE, LEARNED_FORCE = 0, 1, 2, 3
 
 
 class BatchConfig(C.Structure):
@@ -299,8 +300,10 @@ class Engine:
 
     # ---- learned placement
     def set_learned_placement(self, mode: int):
-        """LEARNED_OFF, LEARNED_USE (plans ranked by the MLP while it is the more
-        accurate model) or LEARNED_EXPLORE (also a random plan one run in four)."""
+        """LEARNED_OFF, LEARNED_USE (plans, operator orders under ORDER_AUTO and
+        place()'s segment weights from the MLP while it is the more accurate
+        model), LEARNED_EXPLORE (also a random plan one run in four) or
+        LEARNED_FORCE (the MLP chooses once trained, accurate or not)."""
         self._check(self.lib.mordred_engine_set_learned_placement(self.h, mode), "set_learned_placement")
 
     def placement_model_stats(self) -> dict:

@@ -332,6 +332,47 @@ def test_operator_order_auto(fresh_engine):
     assert seen[3] in seen[:3]
 
 
+def test_learned_operator_order_and_residency_are_exact(fresh_engine):
+    """The learned plan model also picks the operator order of partly cached
+    segments (engine order AUTO) and weighs the segments place() keeps.
+    Exploration runs random orders; FORCE lets the model choose once it has
+    been trained; every run, and the runs after a model-weighted place(),
+    return the oracle's answer."""
+    q, data, eng, ref = fresh_engine
+    print(f"[gpu] learned order + residency {q.name}", flush=True)
+    _uncache_all(eng, q)
+    nseg = eng.segments(q.fact_columns()[0])
+    for c in {q.joins[-1].fact_key} | {a for a in q.aggr if a}:
+        for s in range(nseg):
+            eng.cache_segment(c, s)
+    eng.set_operator_order(E.ORDER_AUTO)
+    try:
+        eng.set_learned_placement(E.LEARNED_EXPLORE)
+        orders = set()
+        for _ in range(24):
+            res, st = eng.run(q, E.PLACE_RESIDENCY)
+            np.testing.assert_array_equal(res, ref)
+            assert st["mixed_segments"] == nseg
+            orders.add(st["operator_order"])
+        assert len(orders) >= 2
+        before = eng.placement_model_stats()
+        assert before["samples"] >= 24
+        eng.set_learned_placement(E.LEARNED_FORCE)
+        for _ in range(4):
+            res, st = eng.run(q, E.PLACE_RESIDENCY)
+            np.testing.assert_array_equal(res, ref)
+            assert st["plan_model"] == 1 and st["plan_predicted_ms"] > 0
+        eng.place(q, 0.5)
+        after = eng.placement_model_stats()
+        assert after["residency_calls"] == before["residency_calls"] + 1
+        for placement in (E.PLACE_COST_SPLIT, E.PLACE_RESIDENCY):
+            res, st = eng.run(q, placement)
+            np.testing.assert_array_equal(res, ref)
+    finally:
+        eng.set_learned_placement(E.LEARNED_OFF)
+        eng.set_operator_order(E.ORDER_GPU_FIRST)
+
+
 def test_aggregate_query_with_other_join_rejected_on_gpu(cuda):
     """An aggregate query that joins a table other than the date table is
     refused under every placement (the aggregate operators probe only table 4,
