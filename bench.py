@@ -838,7 +838,7 @@ def plain_query_measure(q, data, desc, steps: int, warmup: int, no_cpu: bool = F
                      "frac": achieved / HBM_PEAK_GBS, "frac_kind": "effective (algorithmic bytes)",
                      "traffic": None,
                      "kernel": "pipeline_kernel (fused fact probe+group-by)" if q.kind == "groupby"
-                     else "pipeline_kernel (fused fact filter+probe+sum)",
+                     else "aggr_scan_kernel (fused fact filter+probe+sum, next-tile prefetch)",
                      "algorithmic_bytes": alg_bytes, "call_ms_avg": c_avg,
                      "note": "achieved = algorithmic bytes / fused-call event time (compact-table pre-pass "
                              "+ fused kernel); the kernel alone and its DRAM traffic (rocprofv3 FETCH_SIZE, "

@@ -1,13 +1,13 @@
 // Compile-time specialised segment-mode scalar aggregates (SUM a1*a2): with
 // or without the date probe, with any combination of the two BETWEEN filters
-// (the SSB Q1.x shapes).
-#include "pipeline.hpp"
+// (the SSB Q1.x shapes), run by the prefetching aggregate scan.
+#include "scan_aggr.hpp"
 
 namespace mordred {
 
 template <int S>
 static int go(const Pipe& P, hipStream_t s) {
-  return launch_pipe<kAggrBlock, kAggrVpt, SEG, AGGR, S>(P, s);
+  return launch_aggr_scan<kAggrBlock, kAggrVpt, S>(P, s);
 }
 
 int launch_aggr_spec(const Pipe& P, int shape, hipStream_t stream) {
