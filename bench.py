@@ -24,8 +24,9 @@ event time of the launches.  The plain per-query form's own kernel measurement
 `plain_query_kernel`.
 
 Other workloads: --workload q21 (3-way join + group-by, 120M rows), q11
-(filter+join+sum, 6,001,215 rows), q4way-split (segment cache + CPU/GPU
-split, 600M rows), batch64 (64-query shared scan, 600M rows), serve-sweep
+(filter+join+sum, 6,001,215 rows), q4way (4-way join + group-by over the
+300,941,312 rows that the 600M-row split caches, in one fused call),
+q4way-split (segment cache + CPU/GPU split, 600M rows), batch64 (64-query shared scan, 600M rows), serve-sweep
 (offered rates 10..2,000 q/s), skip, q21-mixed.
 
 Answers are checked bit-exactly against the oracle (the C restatement of the
@@ -61,6 +62,15 @@ def workload(name: str, rows: int | None):
                 "predicates": "d_year=1993, discount 1..3, quantity 0..24",
                 "aggregate": "int64 SUM(extendedprice*discount)"}
         return q, Q.make_data(q, n, seed=2024), desc
+    if name == "q4way":
+        # the GPU share of the 600M-row split: its cached segments in one fused call
+        n = rows or 300_941_312
+        q = Q.q4way(40_000, 1_400_000, 3_000_000)
+        desc = {"workload": "4-way star join + group-by (SSB Q3/Q4 shape), all segments in HBM", "fact_rows": n,
+                "dims": {"supplier": 40_000, "part": 1_400_000, "customer": 3_000_000, "date": 2_556},
+                "predicates": "s_region=1 of 5, p_category=1 of 25",
+                "group_by": "(c_nation, s_nation, d_year) 25x25x7 dense int64 SUM(lo_revenue - lo_supplycost)"}
+        return q, Q.make_data(q, n, seed=2024, S=40_000, P=1_400_000, Cn=3_000_000), desc
     n = rows or 120_000_000
     q = Q.q21(40_000, 1_400_000)
     desc = {"workload": "3-way star join + group-by (SSB Q2.1 shape)", "fact_rows": n,
@@ -946,8 +956,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["serve", "q21", "q11", "q4way-split", "batch64", "serve-sweep", "skip",
-                                           "q21-mixed"], default="serve")
+    ap.add_argument("--workload", choices=["serve", "q21", "q11", "q4way", "q4way-split", "batch64", "serve-sweep",
+                                           "skip", "q21-mixed"], default="serve")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: offered rate (q/s)")
     ap.add_argument("--rates", default="10,50,200,500,1000,2000", help="serve-sweep: offered rates (q/s)")
     ap.add_argument("--target-chunk-ms", type=float, default=8.0, help="serve: launch-time target")

@@ -36,8 +36,15 @@
  *    return value is a hipError_t code: 0 on success, hipErrorInvalidValue (1)
  *    for arguments the kernels cannot honour (checked before any launch),
  *    otherwise the launch error.  Nothing here calls exit().
- *  - Entry points are re-entrant and keep no hidden global state, except the
- *    fixed scratch array of the no-alloc entry point (declared below).
+ *  - Entry points are re-entrant (callable from several threads and streams).
+ *    Process-wide state, all of it internal and thread-safe: per device, a
+ *    stream-ordered memory pool from which a fused call takes the scratch of
+ *    its compact probe tables (allocated and freed in the call's stream
+ *    order; the pool keeps freed blocks for reuse); a cache of occupancy
+ *    queries for launch shapes; the fixed scratch array of the no-alloc entry
+ *    point; and the probe tables a caller registers (declared below).  A
+ *    caller that must own every allocation uses the no-alloc entry point or
+ *    registers its hash tables.
  */
 #ifndef MORDRED_HIP_H
 #define MORDRED_HIP_H
@@ -222,6 +229,29 @@ int mordred_probe_group_by_GPU(const int* key1, const int* key2, const int* key3
     const int* aggr1, const int* aggr2, int min_val1, int min_val2, int min_val3, int min_val4,
     int unique_val1, int unique_val2, int unique_val3, int unique_val4, int total_val,
     int num_tuples, int* res, mordred_stream_t stream);
+
+/* ---------------- registered probe tables ------------------------------ */
+
+/* No counterpart in the reference, whose fused kernels probe each hash table
+ * in place.  A fused call derives, per call, a compact stand-in for each hash
+ * table it probes often enough (an exact packed form of the slots' results,
+ * plus an existence bitmap for large tables).  A caller that probes one table
+ * in many calls -- segment groups, streamed chunks, the stages of one query --
+ * registers it: both compact forms (value probes and row probes) then get
+ * memory now (hipMalloc, about 4.3 bytes per slot plus 17 KiB), are derived
+ * once, by the first later call that compacts the table, and are reused by
+ * every call after it, on any stream (each waits for that derivation).  A
+ * build into the table through any mordred_build_* entry point marks the
+ * forms stale, so the next call derives them again; a caller that changes
+ * the table in any other way (a memset, a copy) calls
+ * mordred_probe_table_invalidate.  Unregister a table only when no call that
+ * probes it is still running, as before freeing the table itself.  Answers
+ * are the same with or without registration.  Registration is per device
+ * (the current one); registering a registered table again with the same
+ * dim_len does nothing, with another dim_len re-sizes it. */
+int mordred_probe_table_register(const int* hash_table, int dim_len);
+int mordred_probe_table_unregister(const int* hash_table);
+int mordred_probe_table_invalidate(const int* hash_table);
 
 /* ---------------- library info ----------------------------------------- */
 

@@ -87,6 +87,9 @@ SIGNATURES = {
     "mordred_filter_probe_aggr_GPU": [VP, VP, I, I, I, I, I, I, VP, VP, I, I, VP, VP, I, VP, VP],
     "mordred_probe_group_by_GPU": ([VP] * 4 + [VP, I] * 4 + [I] * 4 + [VP, VP] + [I] * 4 + [I] * 4
                                    + [I, I, VP, VP]),
+    "mordred_probe_table_register": [VP, I],
+    "mordred_probe_table_unregister": [VP],
+    "mordred_probe_table_invalidate": [VP],
 }
 
 _lib = None

@@ -1441,9 +1441,16 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
   for (int k = 0; k < 4; k++) {
     if (!r.join[k]) continue;
     const mordred_join& j = *r.join[k];
-    HIP_OK(e->d_ht[k].reserve(2 * (size_t)j.dim_len * sizeof(int)));
+    // the engine's hash tables stay registered (mordred_probe_table_register),
+    // so the fused calls of one run derive each compact probe table once;
+    // each run's builds mark them stale
+    const size_t ht_bytes = 2 * (size_t)j.dim_len * sizeof(int);
+    if (e->d_ht[k].p && ht_bytes > e->d_ht[k].bytes)
+      (void)mordred_probe_table_unregister(static_cast<const int*>(e->d_ht[k].p));
+    HIP_OK(e->d_ht[k].reserve(ht_bytes));
     ht[k] = static_cast<int*>(e->d_ht[k].p);
-    HIP_OK(hipMemsetAsync(ht[k], 0, 2 * (size_t)j.dim_len * sizeof(int), e->s_main));
+    HIP_OK((hipError_t)mordred_probe_table_register(ht[k], j.dim_len));
+    HIP_OK(hipMemsetAsync(ht[k], 0, ht_bytes, e->s_main));
     filterArgsGPU bf{};
     if (r.dfilt[k]) {
       bf.filter_idx1 = r.dfilt[k]->d_idx;
@@ -3075,6 +3082,8 @@ void mordred_engine_destroy(mordred_engine* e) {
     return;
   }
   (void)hipSetDevice(e->device);
+  for (auto& h : e->d_ht)
+    if (h.p) (void)mordred_probe_table_unregister(static_cast<const int*>(h.p));
   for (auto& kv : e->cols) free_column(e, kv.second.get());
   e->cols.clear();
   if (e->cm) mordred_cost_model_destroy(e->cm);

@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <mutex>
 #include <unordered_map>
+#include <vector>
 
 #define MORDRED_DEFINE_PREP_KERNELS
 #include "pipeline.hpp"
@@ -179,82 +180,175 @@ static int run_pipe(const Pipe& P, int sink, hipStream_t stream) {
   }
 }
 
-// Byte layout of the compact probe tables of `plan` in one scratch block:
-// per compacted table a 256-byte header, the stats partials, the packed words
-// at the widest width, and (large tables) the existence bitmap.
-struct CompactLayout {
-  size_t off[4];
-  bool bits[4];
-  size_t bytes;
-};
-
-static CompactLayout compact_layout(const Pipe& P, const ProbePlan& plan) {
-  CompactLayout L{};
-  for (int k = 0; k < 4; k++) {
-    if (!(plan.compact >> k & 1)) continue;
-    L.off[k] = L.bytes;
-    L.bits[k] = P.len[k] >= kCompactMinLen;
-    L.bytes += 256 + kStatsGroups * sizeof(int4) + ((compact_words_bytes(P.len[k]) + 255) & ~(size_t)255);
-    if (L.bits[k]) L.bytes += (compact_bits_bytes(P.len[k]) + 255) & ~(size_t)255;
-  }
-  return L;
+// Bytes of one compact probe table of `len` slots: a 256-byte header, the
+// stats partials, the packed words at the widest width, and (large tables)
+// the existence bitmap.
+static size_t compact_table_bytes(int len) {
+  size_t b = 256 + kStatsGroups * sizeof(int4) + ((compact_words_bytes(len) + 255) & ~(size_t)255);
+  if (len >= kCompactMinLen) b += (compact_bits_bytes(len) + 255) & ~(size_t)255;
+  return b;
 }
 
-// Launch the two pre-pass kernels that derive the compact tables of `plan`
-// into `base` (laid out by L) and point the pipe at them.  Returns the launch
-// status; on failure the pipe probes the hash tables themselves.
-static hipError_t launch_prep(Pipe& P, const ProbePlan& plan, const CompactLayout& L, char* base, int sink,
-                              hipStream_t stream) {
-  Prep prep{};
-  prep.row_probe = sink == COMPACT;
-  int nt = 0;
-  for (int k = 0; k < 4; k++) {
-    if (!(plan.compact >> k & 1)) continue;
-    char* t = base + L.off[k];
-    prep.ht[nt] = P.ht[k];
-    prep.len[nt] = P.len[k];
-    prep.hdr[nt] = reinterpret_cast<CompactHdr*>(t);
-    prep.partial[nt] = reinterpret_cast<int4*>(t + 256);
-    prep.words[nt] = reinterpret_cast<uint32_t*>(t + 256 + kStatsGroups * sizeof(int4));
-    prep.bits[nt] = L.bits[k] ? reinterpret_cast<uint32_t*>(t + 256 + kStatsGroups * sizeof(int4) +
-                                                            ((compact_words_bytes(P.len[k]) + 255) & ~(size_t)255))
-                              : nullptr;
-    P.chdr[k] = prep.hdr[nt];
-    P.cwords[k] = prep.words[nt];
-    P.cbits[k] = prep.bits[nt];
-    nt++;
-  }
+// Points slot nt of `prep` at a compact table of `len` slots laid out at t.
+static void prep_table(Prep& prep, int nt, const int* ht, int len, char* t) {
+  prep.ht[nt] = ht;
+  prep.len[nt] = len;
+  prep.hdr[nt] = reinterpret_cast<CompactHdr*>(t);
+  prep.partial[nt] = reinterpret_cast<int4*>(t + 256);
+  char* w = t + 256 + kStatsGroups * sizeof(int4);
+  prep.words[nt] = reinterpret_cast<uint32_t*>(w);
+  prep.bits[nt] = len >= kCompactMinLen
+                      ? reinterpret_cast<uint32_t*>(w + ((compact_words_bytes(len) + 255) & ~(size_t)255))
+                      : nullptr;
+}
+
+// The two pre-pass launches over the nt tables of `prep`.
+static hipError_t launch_prep_kernels(const Prep& prep, int nt, hipStream_t stream) {
   prep_stats_kernel<<<dim3(kStatsGroups, nt), kPrepBlock, 0, stream>>>(prep);
   int maxlen = 0;
   for (int t = 0; t < nt; t++) maxlen = std::max(maxlen, prep.len[t]);
   const int pack_groups = std::max(1, std::min(kPackGroups, (maxlen + kPrepBlock - 1) / kPrepBlock));
   prep_pack_kernel<<<dim3(pack_groups, nt), kPrepBlock, 0, stream>>>(prep);
-  const hipError_t e = hipGetLastError();
+  return hipGetLastError();
+}
+
+// Byte layout of the compact probe tables of the tables in `mask`, one after
+// the other in one scratch block.
+struct CompactLayout {
+  size_t off[4];
+  size_t bytes;
+};
+
+static CompactLayout compact_layout(const Pipe& P, int mask) {
+  CompactLayout L{};
+  for (int k = 0; k < 4; k++) {
+    if (!(mask >> k & 1)) continue;
+    L.off[k] = L.bytes;
+    L.bytes += compact_table_bytes(P.len[k]);
+  }
+  return L;
+}
+
+// Launch the two pre-pass kernels that derive the compact tables of the
+// tables in `mask` into `base` (laid out by L) and point the pipe at them.
+// Returns the launch status; on failure those tables are probed in place.
+static hipError_t launch_prep(Pipe& P, int mask, const CompactLayout& L, char* base, int sink, hipStream_t stream) {
+  Prep prep{};
+  prep.row_probe = sink == COMPACT;
+  int nt = 0;
+  for (int k = 0; k < 4; k++) {
+    if (!(mask >> k & 1)) continue;
+    prep_table(prep, nt, P.ht[k], P.len[k], base + L.off[k]);
+    P.chdr[k] = prep.hdr[nt];
+    P.cwords[k] = prep.words[nt];
+    P.cbits[k] = prep.bits[nt];
+    nt++;
+  }
+  if (nt == 0) return hipSuccess;
+  const hipError_t e = launch_prep_kernels(prep, nt, stream);
   if (e != hipSuccess)
-    for (int k = 0; k < 4; k++) { P.chdr[k] = nullptr; P.cwords[k] = nullptr; P.cbits[k] = nullptr; }
+    for (int k = 0; k < 4; k++)
+      if (mask >> k & 1) { P.chdr[k] = nullptr; P.cwords[k] = nullptr; P.cbits[k] = nullptr; }
   return e;
 }
 
-// Build the compact probe tables of `plan` into pool scratch (stream-ordered).
-// Returns the scratch block (to free after the fact kernel) or nullptr, in
-// which case the fact kernel probes the hash tables directly.
-static void* build_compact(Pipe& P, const ProbePlan& plan, int sink, hipStream_t stream) {
-  if (!plan.compact) return nullptr;
+// Build the compact probe tables of the tables in `mask` into pool scratch
+// (stream-ordered).  Returns the scratch block (to free after the fact
+// kernel) or nullptr, in which case the fact kernel probes those hash tables
+// in place.
+static void* build_compact(Pipe& P, int mask, int sink, hipStream_t stream) {
+  if (!mask) return nullptr;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   hipMemPool_t pool = device_pool(dev);
   if (!pool) return nullptr;
-  const CompactLayout L = compact_layout(P, plan);
+  const CompactLayout L = compact_layout(P, mask);
   void* scratch = nullptr;
   if (hipMallocFromPoolAsync(&scratch, L.bytes, pool, stream) != hipSuccess) {
     (void)hipGetLastError();
     return nullptr;
   }
-  if (launch_prep(P, plan, L, static_cast<char*>(scratch), sink, stream) != hipSuccess) {
+  if (launch_prep(P, mask, L, static_cast<char*>(scratch), sink, stream) != hipSuccess) {
     (void)hipFreeAsync(scratch, stream);
     return nullptr;
   }
   return scratch;
+}
+
+// ------------------------------------------------- registered probe tables
+// A caller that probes one hash table in many fused calls (segment groups,
+// streamed chunks, pipeline stages of one query) registers it.  The compact
+// forms that those calls would each derive (one for value probes, one for
+// row probes) are then derived once, by the first call that compacts the
+// table, into memory allocated at registration, and reused by later calls on
+// any stream (they wait for the derivation's event) until a build into the
+// table (any mordred_build_* entry point) marks them stale.  Whether a call
+// compacts a table at all stays the per-call plan's decision.
+struct RegTable {
+  int dev;
+  const int* ht;
+  int len;
+  char* mem;             // [value-probe form | row-probe form]
+  size_t form_bytes;
+  bool fresh[2];         // derived since the table's last build
+  hipEvent_t ready[2];   // recorded after each form's last derivation
+};
+static std::mutex reg_mu;
+static std::vector<RegTable> reg_tables;
+
+static RegTable* reg_find(int dev, const int* ht) {
+  for (RegTable& t : reg_tables)
+    if (t.dev == dev && t.ht == ht) return &t;
+  return nullptr;
+}
+
+static void reg_free(RegTable& t) {
+  for (int f = 0; f < 2; f++)
+    if (t.ready[f]) (void)hipEventDestroy(t.ready[f]);
+  if (t.mem) (void)hipFree(t.mem);
+}
+
+// A build into `ht` makes its registered forms stale.
+static void reg_mark_stale(const int* ht) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;
+  std::lock_guard<std::mutex> g(reg_mu);
+  if (RegTable* t = reg_find(dev, ht)) t->fresh[0] = t->fresh[1] = false;
+}
+
+// Registered tables among `mask`: point the pipe at their forms, deriving a
+// stale form on `stream` first.  `done` gets the tables served this way.
+// Allocates nothing.
+static hipError_t use_registered(Pipe& P, int mask, int row_probe, hipStream_t stream, int& done) {
+  done = 0;
+  if (!mask) return hipSuccess;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(reg_mu);
+  if (reg_tables.empty()) return hipSuccess;
+  for (int k = 0; k < 4; k++) {
+    if (!(mask >> k & 1)) continue;
+    RegTable* t = reg_find(dev, P.ht[k]);
+    if (!t || t->len != P.len[k]) continue;
+    Prep prep{};
+    prep.row_probe = row_probe;
+    prep_table(prep, 0, t->ht, t->len, t->mem + (size_t)row_probe * t->form_bytes);
+    if (!t->fresh[row_probe]) {
+      // derive under the lock, so that a call on another stream that finds
+      // the form fresh waits for an event already recorded after it
+      if ((e = launch_prep_kernels(prep, 1, stream)) != hipSuccess) return e;
+      if ((e = hipEventRecord(t->ready[row_probe], stream)) != hipSuccess) return e;
+      t->fresh[row_probe] = true;
+    } else if ((e = hipStreamWaitEvent(stream, t->ready[row_probe], 0)) != hipSuccess) {
+      return e;
+    }
+    P.chdr[k] = prep.hdr[0];
+    P.cwords[k] = prep.words[0];
+    P.cbits[k] = prep.bits[0];
+    done |= 1 << k;
+  }
+  return hipSuccess;
 }
 
 // ------------------------------------------------- fixed scratch (no-alloc)
@@ -328,18 +422,24 @@ static int dispatch(Pipe P, int mode, int sink, hipStream_t stream, char* fixed 
   }
   const ProbePlan plan = plan_probes(P, mode, sink);
   void* pooled = nullptr;
-  bool compacted = false;
+  int reg = 0;
   if (fixed) {
-    const CompactLayout L = compact_layout(P, plan);
-    if (L.bytes > cap) return hipErrorOutOfMemory;
-    if (plan.compact) {
-      const hipError_t e = launch_prep(P, plan, L, fixed, sink, stream);
+    // the refusal is decided before anything is launched
+    if (compact_layout(P, plan.compact).bytes > cap) return hipErrorOutOfMemory;
+  }
+  const hipError_t re = use_registered(P, plan.compact, sink == COMPACT, stream, reg);
+  if (re != hipSuccess) return re;
+  const int rest = plan.compact & ~reg;
+  bool compacted = reg != 0;
+  if (fixed) {
+    if (rest) {
+      const hipError_t e = launch_prep(P, rest, compact_layout(P, rest), fixed, sink, stream);
       if (e != hipSuccess) return e;
       compacted = true;
     }
-  } else {
-    pooled = build_compact(P, plan, sink, stream);
-    compacted = pooled != nullptr;
+  } else if (rest) {
+    pooled = build_compact(P, rest, sink, stream);
+    compacted = compacted || pooled != nullptr;
   }
   if (compacted && plan.stage) {
     P.lds_stage = plan.stage;
@@ -436,6 +536,7 @@ constexpr int kBuildVpt = MORDRED_BUILD_VPT;
 template <int MODE>
 static int run_build(const Build& B, hipStream_t stream) {
   if (B.num_tuples == 0) return hipSuccess;
+  reg_mark_stale(B.ht);
   constexpr int TILE = kBuildBlock * 4 * kBuildVpt;
   const void* f = reinterpret_cast<const void*>(&build_kernel<kBuildBlock, kBuildVpt, MODE>);
   const int ntiles = (B.num_tuples + TILE - 1) / TILE;
@@ -456,6 +557,53 @@ using namespace mordred;
 extern "C" {
 
 const char* mordred_version(void) { return "mordred-hip 0.1 (gfx950)"; }
+
+int mordred_probe_table_register(const int* hash_table, int dim_len) {
+  INVALID_IF(!hash_table || dim_len <= 0);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(reg_mu);
+  RegTable* t = reg_find(dev, hash_table);
+  if (t && t->len == dim_len) return hipSuccess;
+  if (t) {
+    reg_free(*t);
+    reg_tables.erase(reg_tables.begin() + (t - reg_tables.data()));
+  }
+  RegTable n{};
+  n.dev = dev;
+  n.ht = hash_table;
+  n.len = dim_len;
+  n.form_bytes = compact_table_bytes(dim_len);
+  if ((e = hipMalloc(&n.mem, 2 * n.form_bytes)) != hipSuccess) return e;
+  for (int f = 0; f < 2; f++) {
+    if ((e = hipEventCreateWithFlags(&n.ready[f], hipEventDisableTiming)) != hipSuccess) {
+      reg_free(n);
+      return e;
+    }
+  }
+  reg_tables.push_back(n);
+  return hipSuccess;
+}
+
+int mordred_probe_table_unregister(const int* hash_table) {
+  INVALID_IF(!hash_table);
+  int dev = 0;
+  const hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(reg_mu);
+  RegTable* t = reg_find(dev, hash_table);
+  if (!t) return hipSuccess;
+  reg_free(*t);
+  reg_tables.erase(reg_tables.begin() + (t - reg_tables.data()));
+  return hipSuccess;
+}
+
+int mordred_probe_table_invalidate(const int* hash_table) {
+  INVALID_IF(!hash_table);
+  reg_mark_stale(hash_table);
+  return hipSuccess;
+}
 
 int mordred_filter_probe_group_by_GPU2(const int* gpuCache, const filterArgsGPU* fargs,
     const probeArgsGPU* pargs, const groupbyArgsGPU* gargs, int num_tuples, int* res,
