@@ -250,17 +250,22 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
   int* ring = kCompact ? reinterpret_cast<int*>(lds + P.ring_off) + wave * (kRing * ring_fields(SHAPE)) : nullptr;
   int ring_head = 0, ring_tail = 0;  // wave-uniform
 
-  // per-table compact probe parameters (workgroup-uniform); cb: probe the
-  // existence bitmap first (a large table where fewer than half the slots hit)
+  // per-table compact probe parameters (workgroup-uniform); cb: test table
+  // k's existence bits per tile, before the ring, when fewer than half its
+  // slots hit: the existence bitmap of a large table, or the packed words
+  // themselves of a 1-bit table (a table without values, e.g. a filtered
+  // part table that is not grouped on), which are that bitmap
   int cw[4], csh[4], cvmin[4];
   bool cb[4];
+  const uint32_t* cbp[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    cw[k] = 0; csh[k] = 0; cvmin[k] = 0; cb[k] = false;
+    cw[k] = 0; csh[k] = 0; cvmin[k] = 0; cb[k] = false; cbp[k] = nullptr;
     if ((kGen || (SHAPE >> k & 1)) && P.chdr[k]) {
       const CompactHdr h = *P.chdr[k];
       cw[k] = h.w; csh[k] = h.shift; cvmin[k] = h.vmin;
-      cb[k] = P.cbits[k] != nullptr && h.w > 1 && h.any < P.len[k] / 2;
+      cbp[k] = h.w == 1 ? P.cwords[k] : P.cbits[k];
+      cb[k] = cbp[k] != nullptr && h.w >= 1 && h.any < P.len[k] / 2;
     }
   }
 
@@ -319,7 +324,11 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
       for (int k = 0; k < 4; k++) {
         if (!(SHAPE >> k & 1) || k == kFirst) continue;
         const int hs = hit ? phash(v[k], P.len[k], P.minkey[k]) : 0;
-        if (cw[k] > 0) {
+        if (cb[k] && cw[k] == 1) {
+          // a 1-bit table: its bits were tested before the ring, and its
+          // result is the constant vmin
+          v[k] = cvmin[k];
+        } else if (cw[k] > 0) {
           // (an existence bitmap, if any, was tested before the ring)
           const int w = cw[k], sh = csh[k];
           uint32_t word = 0;
@@ -459,11 +468,18 @@ __global__ __launch_bounds__(BLOCK) void pipeline_kernel(Pipe P) {
         uint32_t bw[IPT];
         int hb[IPT];
 #pragma unroll
-        for (int j = 0; j < IPT; j++) {
-          hb[j] = (sel >> j & 1u) ? phash(kv[k][j], P.len[k], P.minkey[k]) : 0;
-          // masked: lanes without a survivor issue no address
-          if (sel >> j & 1u) bw[j] = P.cbits[k][hb[j] >> 5];
-          else bw[j] = 0xffffffffu;
+        for (int j = 0; j < IPT; j++) hb[j] = (sel >> j & 1u) ? phash(kv[k][j], P.len[k], P.minkey[k]) : 0;
+        if (stg[k] && cw[k] == 1) {
+          // a 1-bit table staged in LDS
+#pragma unroll
+          for (int j = 0; j < IPT; j++) bw[j] = ldsw[soff[k] + (hb[j] >> 5)];
+        } else {
+#pragma unroll
+          for (int j = 0; j < IPT; j++) {
+            // masked: lanes without a survivor issue no address
+            if (sel >> j & 1u) bw[j] = cbp[k][hb[j] >> 5];
+            else bw[j] = 0xffffffffu;
+          }
         }
 #pragma unroll
         for (int j = 0; j < IPT; j++)
