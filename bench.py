@@ -584,7 +584,7 @@ def serve_headline(args):
     gc.collect()
     # the plain per-query form's own kernel: 3-way join + group-by, 120M rows
     q2, d2, desc2 = workload("q21", None)
-    out["plain_query_kernel"] = plain_query_measure(q2, d2, desc2, 20, 5, args.no_cpu, 5.0)
+    out["plain_query_kernel"] = plain_query_measure(q2, d2, desc2, 20, 5, args.no_cpu, 5.0, "q21")
     out["parity"] = identical and out.get("parity_oracle_sample", True) and \
         out["plain_query_kernel"]["parity"] is not False
     print(json.dumps(out), flush=True)
@@ -799,7 +799,8 @@ def mixed_bench(args):
         sys.exit(1)
 
 
-def plain_query_measure(q, data, desc, steps: int, warmup: int, no_cpu: bool = False, cpu_budget: float = 12.0):
+def plain_query_measure(q, data, desc, steps: int, warmup: int, no_cpu: bool = False, cpu_budget: float = 12.0,
+                        workload_key: str = ""):
     """One plain per-query step (zero the hash tables and the result, the
     dimension builds, the fused fact call) repeated `steps` times with the
     columns resident in HBM.  The fused call is bracketed with HIP events on
@@ -847,13 +848,19 @@ def plain_query_measure(q, data, desc, steps: int, warmup: int, no_cpu: bool = F
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "frac_kind": "effective (algorithmic bytes)",
                      "traffic": None,
-                     "kernel": "pipeline_kernel (fused fact probe+group-by)" if q.kind == "groupby"
+                     "kernel": "pipeline_kernel (fused fact probe+group-by, survivor ring)" if q.kind == "groupby"
                      else "aggr_scan_kernel (fused fact filter+probe+sum, next-tile prefetch)",
                      "algorithmic_bytes": alg_bytes, "call_ms_avg": c_avg,
                      "note": "achieved = algorithmic bytes / fused-call event time (compact-table pre-pass "
                              "+ fused kernel); the kernel alone and its DRAM traffic (rocprofv3 FETCH_SIZE, "
                              "WRITE_SIZE) are in profiles/"},
     }
+    t = _profiled_traffic(workload_key, data.n)
+    if t:
+        out["roofline"]["traffic"] = t["bytes_per_launch"]
+        out["roofline"]["traffic_frac"] = t["bytes_per_launch"] / (c_avg * 1e-3) / 1e9 / HBM_PEAK_GBS
+        out["roofline"]["traffic_source"] = ("rocprofv3 FETCH_SIZE x2 + WRITE_SIZE of this kernel at this size, "
+                                             f"{t['profile']} (measured offline, not in this run)")
     del gq, cache
     torch.cuda.empty_cache()
     if not no_cpu:
@@ -865,6 +872,17 @@ def plain_query_measure(q, data, desc, steps: int, warmup: int, no_cpu: bool = F
         out["cpu_baseline"] = None
         out["parity"] = None
     return out
+
+
+def _profiled_traffic(key, rows):
+    """The workload's DRAM bytes per fused launch from profiles/traffic.json,
+    when it was measured at this many rows."""
+    try:
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")) as f:
+            t = json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+    return t if t and t.get("rows") == rows else None
 
 
 def learned_order_phase(eng, q, nseg, steps, learn_runs):
@@ -1000,7 +1018,7 @@ def main():
     t0 = time.time()
     q, data, desc = workload(args.workload, args.rows)
     log(f"generated {data.n} rows in {time.time() - t0:.1f}s")
-    out = plain_query_measure(q, data, desc, args.steps, args.warmup, args.no_cpu, args.cpu_budget)
+    out = plain_query_measure(q, data, desc, args.steps, args.warmup, args.no_cpu, args.cpu_budget, args.workload)
     out = {"metric": METRIC, "value": out.pop("value"), "unit": "fact rows/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": out.pop("ms_per_step"), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "int32 columns, int64 sums",
