@@ -394,7 +394,7 @@ def serve_bench(args):
         arr = serving.poisson_arrivals(rate, nreq, seed=100 + i)
         po, pt, pst = eng.serve(reqs, arr, E.SERVE_PLAIN)
         co, ct, cst = eng.serve(reqs, arr, E.SERVE_CONTINUOUS, target_chunk_ms=target, hbm_budget_bytes=budget,
-                                learned=True)
+                                learned=True, lone_plain=True)
         same = all(bool((a == b).all()) for a, b in zip(po, co))
         same_all = same_all and same
         p, c = serving.latency_summary(pt), serving.latency_summary(ct)
@@ -402,10 +402,11 @@ def serve_bench(args):
                      "served_launches": cst["launches"], "served_max_active": cst["max_active_seen"],
                      "learned_choices": cst["learned_choices"], "analytic_choices": cst["analytic_choices"],
                      "learned_error": cst["learned_error"], "analytic_error": cst["analytic_error"],
-                     "learned_trusted": cst["learned_trusted"]})
+                     "learned_trusted": cst["learned_trusted"], "served_lone_requests": cst["lone_requests"]})
         log(f"rate {rate:7.0f} q/s: plain {p['throughput_qps']:7.1f} q/s p50 {p['p50_ms']:8.1f} p99 {p['p99_ms']:8.1f} | "
             f"served {c['throughput_qps']:7.1f} q/s p50 {c['p50_ms']:8.1f} p99 {c['p99_ms']:8.1f} | "
-            f"launches {cst['launches']} learned {cst['learned_choices']}/{cst['launches']} identical {same}")
+            f"launches {cst['launches']} lone {cst['lone_requests']} learned {cst['learned_choices']}/{cst['launches']} "
+            f"identical {same}")
     top = rows[-1]
     out = {
         "metric": METRIC + " (request loop, Poisson arrivals)",
@@ -493,7 +494,8 @@ def serve_headline(args):
     timed, tarr = reqs[nw:], arr[nw:] - arr[nw]
 
     def run(mode):
-        kw = {} if mode == E.SERVE_PLAIN else dict(target_chunk_ms=target, hbm_budget_bytes=budget, learned=True)
+        kw = {} if mode == E.SERVE_PLAIN else dict(target_chunk_ms=target, hbm_budget_bytes=budget, learned=True,
+                                                   lone_plain=True)
         if nw:
             eng.serve(reqs[:nw], arr[:nw], mode, **kw)
         torch.cuda.synchronize()
@@ -533,7 +535,8 @@ def serve_headline(args):
                    "hbm_budget_bytes": budget, "target_launch_ms": target,
                    "launch": {"block": best[0], "items_per_thread": 4 * best[1], "lds_bytes": best[2]}},
         "served": dict(c, launches=cst["launches"], gpu_ms=cst["gpu_ms"], wall_ms=cst["wall_ms"],
-                       max_active=cst["max_active_seen"], learned_choices=cst["learned_choices"],
+                       max_active=cst["max_active_seen"], lone_requests=cst["lone_requests"],
+                       learned_choices=cst["learned_choices"],
                        analytic_choices=cst["analytic_choices"], learned_trusted=cst["learned_trusted"],
                        learned_error=cst["learned_error"], analytic_error=cst["analytic_error"]),
         "plain": dict(p, gpu_ms=pst["gpu_ms"], wall_ms=pst["wall_ms"]),

@@ -326,6 +326,11 @@ int mordred_engine_batch_tune(mordred_engine* e, const mordred_star_query* qs, i
  *     from the learned cost model once it is trusted (mordred_costmodel.h),
  *     else from the analytic launch model.  The learned model is trained
  *     online on the measured launch times and persists in the engine.
+ *     With lone_plain set, a request that arrives while no query is active
+ *     and no other request is waiting runs alone through mordred_engine_run
+ *     (its builds and its fused kernel, the same answer): no other query
+ *     could share its scan, and one query's fused kernel finishes sooner than
+ *     a pass of the shared scan.  Requests arriving meanwhile wait for it.
  * The fact columns must be cached in HBM (all segments) for both modes. */
 enum { MORDRED_SERVE_PLAIN = 0, MORDRED_SERVE_CONTINUOUS = 1 };
 
@@ -337,6 +342,7 @@ typedef struct mordred_serve_opts {
   double hbm_budget_bytes;    /* admission budget for per-query device state */
   int learned;                /* 1: use the learned cost model when trusted */
   mordred_batch_config config;  /* block 0: the engine's batch configuration */
+  int lone_plain;             /* continuous: 1 = a lone request on an idle loop runs alone */
 } mordred_serve_opts;
 
 typedef struct mordred_serve_times {
@@ -357,6 +363,8 @@ typedef struct mordred_serve_stats {
   double peak_state_bytes;        /* admitted per-query device state, at most */
   double alg_bytes;               /* continuous: sum over launches of the distinct
                                      fact columns read x 4 B x rows scanned */
+  long long lone_requests;        /* continuous: requests run alone (lone_plain); launches,
+                                     gpu_ms and alg_bytes count the shared-scan launches only */
 } mordred_serve_stats;
 
 int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n, const double* arrival_ms,

@@ -3940,6 +3940,23 @@ int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n,
   float feat[MORDRED_COST_FEATURES];
   long long since_train = 0;
   while (done < n) {
+    // a lone request on an idle loop: alone through its own fused kernel
+    if (o->lone_plain && active == 0 && next < n && arrival_ms[next] <= now_ms() &&
+        (next + 1 >= n || arrival_ms[next + 1] > now_ms())) {
+      const int i = next++;
+      times[i].admit_ms = now_ms();
+      mordred_run_stats rs{};
+      const int rc = mordred_engine_run(e, &qs[i], MORDRED_PLACE_RESIDENCY, res[i], &rs);
+      if (rc) return rc;
+      if (rs.cpu_rows) return fail(e, "serve: fact columns must be cached in HBM");
+      times[i].done_ms = now_ms();
+      times[i].launches = 1;
+      st.lone_requests++;  // (launches, gpu_ms and alg_bytes count shared-scan launches only)
+      st.max_active_seen = std::max(st.max_active_seen, 1);
+      done++;
+      HIP_OK(hipSetDevice(e->device));
+      continue;
+    }
     // admission: free slot, arrived request, budget
     unsigned long long reset = 0;
     for (int sidx = 0; sidx < nslot && next < n && arrival_ms[next] <= now_ms(); sidx++) {

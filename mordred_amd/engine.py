@@ -93,7 +93,7 @@ SERVE_PLAIN, SERVE_CONTINUOUS = 0, 1
 class ServeOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("max_active", C.c_int), ("chunk_segments", C.c_int),
                 ("target_chunk_ms", C.c_double), ("hbm_budget_bytes", C.c_double), ("learned", C.c_int),
-                ("config", BatchConfig)]
+                ("config", BatchConfig), ("lone_plain", C.c_int)]
 
 
 class ServeTimes(C.Structure):
@@ -105,7 +105,7 @@ class ServeStats(C.Structure):
     _fields_ = [("wall_ms", C.c_double), ("launches", C.c_longlong), ("gpu_ms", C.c_double),
                 ("max_active_seen", C.c_int), ("learned_choices", C.c_longlong), ("analytic_choices", C.c_longlong),
                 ("learned_error", C.c_double), ("analytic_error", C.c_double), ("learned_trusted", C.c_int),
-                ("peak_state_bytes", C.c_double), ("alg_bytes", C.c_double)]
+                ("peak_state_bytes", C.c_double), ("alg_bytes", C.c_double), ("lone_requests", C.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -351,7 +351,8 @@ class Engine:
 
     # ---- request loop
     def serve(self, queries, arrivals_ms, mode=SERVE_CONTINUOUS, max_active: int = 64, chunk_segments: int = 0,
-              target_chunk_ms: float = 8.0, hbm_budget_bytes: float = 200e9, learned: bool = True, config=None):
+              target_chunk_ms: float = 8.0, hbm_budget_bytes: float = 200e9, learned: bool = True, config=None,
+              lone_plain: bool = False):
         """Replay requests (query i arriving arrivals_ms[i] after the start) in
         real time.  Returns (results, per-request times, stats)."""
         n = len(queries)
@@ -360,7 +361,7 @@ class Engine:
         ptrs = (C.c_void_p * max(1, n))(*[o.ctypes.data for o in outs])
         arrivals = (C.c_double * max(1, n))(*[float(a) for a in arrivals_ms])
         o = ServeOpts(mode, max_active, chunk_segments, target_chunk_ms, hbm_budget_bytes, int(learned),
-                      BatchConfig(*(config or (0, 0, 0))))
+                      BatchConfig(*(config or (0, 0, 0))), int(lone_plain))
         times = (ServeTimes * max(1, n))()
         st = ServeStats()
         self._check(self.lib.mordred_engine_serve(self.h, arr, n, arrivals, ptrs, C.byref(o), times, C.byref(st)),

@@ -98,3 +98,23 @@ def test_learned_model_is_trained_online(sengine):
     check(reqs, outs, refs)
     assert st["launches"] >= 20
     assert st["learned_error"] >= 0 and st["analytic_error"] >= 0
+
+
+def test_lone_requests_run_alone_with_the_same_answers(sengine):
+    """lone_plain: a request arriving at an idle loop with nobody waiting runs
+    alone through its own fused kernel; sparse arrivals (every request alone)
+    and a burst after them (batched) both give the oracle's answers."""
+    qs, data, eng, refs = sengine
+    reqs = serving.request_stream(qs, 40, seed=12)
+    arr = np.concatenate([np.arange(20) * 50.0, np.full(20, 1100.0)])
+    print("[gpu] serve lone_plain: 20 sparse requests, then a burst of 20", flush=True)
+    outs, times, st = eng.serve(reqs, arr, E.SERVE_CONTINUOUS, chunk_segments=2, lone_plain=True)
+    check(reqs, outs, refs)
+    assert st["lone_requests"] >= 20
+    assert st["max_active_seen"] > 1                       # the burst shared the scan
+    for t in times[:20]:
+        assert t["launches"] == 1
+    # off: every request goes through the shared scan
+    outs, times, st = eng.serve(reqs[:5], arr[:5], E.SERVE_CONTINUOUS, chunk_segments=2)
+    check(reqs[:5], outs, refs)
+    assert st["lone_requests"] == 0
