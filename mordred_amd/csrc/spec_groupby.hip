@@ -1,6 +1,7 @@
 // Compile-time specialised segment-mode group-by kernels (LDS group array):
-// one instantiation per set of probed tables, without fact filters or with
-// both BETWEEN filters (the fused shapes of the SSB Q2-Q4 families).
+// one instantiation per set of probed tables, without fact filters, with the
+// first BETWEEN filter only, or with both (the fused shapes of the SSB Q2-Q4
+// families and of star queries with one fact predicate).
 #include "pipeline.hpp"
 
 namespace mordred {
@@ -20,6 +21,7 @@ int launch_groupby_lds_spec(const Pipe& P, int shape, hipStream_t stream) {
   MORDRED_CASE(F | 9) MORDRED_CASE(F | 10) MORDRED_CASE(F | 11) MORDRED_CASE(F | 12)  \
   MORDRED_CASE(F | 13) MORDRED_CASE(F | 14) MORDRED_CASE(F | 15)
     MORDRED_CASES(0)
+    MORDRED_CASES(0x10)
     MORDRED_CASES(0x30)
 #undef MORDRED_CASES
 #undef MORDRED_CASE

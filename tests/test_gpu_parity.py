@@ -507,6 +507,25 @@ def test_compact_tables_probe_rows(big_part):
     np.testing.assert_array_equal(_sorted_rows([t.cpu().numpy() for t in outs], m), _sorted_rows(ro, m))
 
 
+@pytest.mark.parametrize("kind", ["q21", "q4way"])
+def test_one_fact_filter_group_by(cuda, kind):
+    """The fused group-by with only the first fact filter (a revenue range):
+    its own specialised shape, over a long enough scan that the dimension
+    tables are compacted and the survivors go through the ring."""
+    import dataclasses
+    from mordred_amd import query as Q
+    base, data = cases.star(kind, 5 * SEG + 777, seed=55)
+    q = dataclasses.replace(base, name=base.name + "_f1", fact_filters=[("lo_revenue", 1_000_000, 6_000_000)])
+    cache = Q.load_cache(data, q, shuffle_seed=17)
+    print(f"[gpu] one fact filter: {q.name}", flush=True)
+    gq = Q.GpuQuery(q, data, cache)
+    gq.run()
+    sync()
+    res = gq.result()
+    np.testing.assert_array_equal(res, oracle_res(q, data))
+    assert semantics.decode_groups(res) == semantics.group_map(q, data)
+
+
 def test_lds_staged_tables_and_late_columns(cuda):
     """A fact scan long enough (>= 256 rows per date slot) that the date table
     is compacted and staged in LDS next to the supplier table, with the part
