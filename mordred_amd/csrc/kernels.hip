@@ -173,7 +173,11 @@ static int run_pipe(const Pipe& P, int sink, hipStream_t stream) {
     if (rc != -1) return rc;
   }
   switch (sink) {
-    case GROUPBY_LDS: return launch_pipe<kLdsBlock, kVpt, MODE, GROUPBY_LDS>(P, stream);
+    // offset-mode group-by gathers every column by row: 4 rows per thread
+    // keep its registers within the 1024-thread budget (8 spill to scratch)
+    case GROUPBY_LDS:
+      if constexpr (MODE == OFF) return launch_pipe<kLdsBlock, 1, MODE, GROUPBY_LDS>(P, stream);
+      else return launch_pipe<kLdsBlock, kVpt, MODE, GROUPBY_LDS>(P, stream);
     case GROUPBY_GLOBAL: return launch_pipe<kBlock, kVpt, MODE, GROUPBY_GLOBAL>(P, stream);
     case AGGR: return launch_pipe<kAggrBlock, kAggrVpt, MODE, AGGR>(P, stream);
     default: return launch_pipe<kBlock, kVpt, MODE, COMPACT>(P, stream);
