@@ -32,41 +32,47 @@ constexpr int kOffB3 = kOffW3 + kH;
 constexpr int kParams = kOffB3 + 1;  // 6337
 constexpr int kBlock = 256;
 
-// Forward pass, one thread per sample; parameters staged in LDS.  Writes the
-// prediction and, when h1/h2 are given, the post-ReLU activations.
+// Forward pass, one wavefront per sample: lane o computes hidden unit o of
+// both layers (kH = 64 = the wave), the inputs and the first layer's outputs
+// reach every lane by shuffles, the output is a wave reduction.  The lane's
+// column of W1 sits in registers; W2 is read row by row (each row one
+// coalesced 256-byte load, L2-resident).  A few samples -- a launch-size or
+// placement decision -- finish in one short dependent chain instead of one
+// thread walking all 6,337 parameters.  Writes the prediction and, when
+// H1/H2 are given, the post-ReLU activations.
+static_assert(kH == 64, "one hidden unit per lane of a wave64");
 __global__ __launch_bounds__(kBlock) void mlp_forward_kernel(const float* __restrict__ P, const float* __restrict__ X,
                                                              int n, float* __restrict__ Y, float* __restrict__ H1,
                                                              float* __restrict__ H2) {
-  __shared__ float w[kParams];
-  for (int i = threadIdx.x; i < kParams; i += kBlock) w[i] = P[i];
-  __syncthreads();
-  const int b = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & (kH - 1);
+  const int nw = gridDim.x * (kBlock / kH);
+  int b = (int)(blockIdx.x * kBlock + threadIdx.x) / kH;
   if (b >= n) return;
-  float x[kIn];
+  float w1[kIn];
 #pragma unroll
-  for (int i = 0; i < kIn; i++) x[i] = X[(size_t)b * kIn + i];
-  float h1[kH];
-#pragma unroll 4
-  for (int o = 0; o < kH; o++) {
-    float a = w[kOffB1 + o];
+  for (int i = 0; i < kIn; i++) w1[i] = P[kOffW1 + i * kH + lane];
+  const float b1 = P[kOffB1 + lane], b2 = P[kOffB2 + lane], w3 = P[kOffW3 + lane], b3 = P[kOffB3];
+  for (; b < n; b += nw) {
+    const float xv = lane < kIn ? X[(size_t)b * kIn + lane] : 0.f;
+    float a = b1;
 #pragma unroll
-    for (int i = 0; i < kIn; i++) a = fmaf(x[i], w[kOffW1 + i * kH + o], a);
-    h1[o] = fmaxf(a, 0.f);
+    for (int i = 0; i < kIn; i++) a = fmaf(__shfl(xv, i, kH), w1[i], a);
+    const float h1 = fmaxf(a, 0.f);
+    float c = b2;
+#pragma unroll 16
+    for (int i = 0; i < kH; i++) c = fmaf(__shfl(h1, i, kH), P[kOffW2 + i * kH + lane], c);
+    const float h2 = fmaxf(c, 0.f);
+    float y = h2 * w3;
+#pragma unroll
+    for (int d = kH / 2; d > 0; d >>= 1) y += __shfl_xor(y, d, kH);
+    if (H1) H1[(size_t)b * kH + lane] = h1;
+    if (H2) H2[(size_t)b * kH + lane] = h2;
+    if (lane == 0) Y[b] = y + b3;
   }
-  float y = w[kOffB3];
-  for (int o = 0; o < kH; o++) {
-    float a = w[kOffB2 + o];
-#pragma unroll
-    for (int i = 0; i < kH; i++) a = fmaf(h1[i], w[kOffW2 + i * kH + o], a);
-    a = fmaxf(a, 0.f);
-    if (H2) H2[(size_t)b * kH + o] = a;
-    y = fmaf(a, w[kOffW3 + o], y);
-  }
-  if (H1)
-#pragma unroll
-    for (int o = 0; o < kH; o++) H1[(size_t)b * kH + o] = h1[o];
-  Y[b] = y;
 }
+
+// Workgroups of the forward pass for n samples (one wave per sample).
+static int forward_grid(int n) { return std::max(1, (n + kBlock / kH - 1) / (kBlock / kH)); }
 
 // Backward deltas, one thread per sample: d3 = 2(y-t)/n, D2 = d3 W3 * [h2>0],
 // D1 = (D2 W2^T) * [h1>0].
@@ -207,7 +213,7 @@ int forward(mordred_cost_model* m, const float* x_host, int n, float* y_host) {
   float* dY = dX + (size_t)n * kIn;
   std::memcpy(m->h_stage, x_host, (size_t)n * kIn * sizeof(float));
   CM_OK(hipMemcpyAsync(dX, m->h_stage, (size_t)n * kIn * sizeof(float), hipMemcpyHostToDevice, m->s));
-  mlp_forward_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, m->s>>>(m->d_p, dX, n, dY, nullptr, nullptr);
+  mlp_forward_kernel<<<forward_grid(n), kBlock, 0, m->s>>>(m->d_p, dX, n, dY, nullptr, nullptr);
   CM_OK(hipGetLastError());
   CM_OK(hipMemcpyAsync(m->h_stage, dY, n * sizeof(float), hipMemcpyDeviceToHost, m->s));
   CM_OK(hipStreamSynchronize(m->s));
@@ -234,7 +240,7 @@ int gradients(mordred_cost_model* m, const float* x_host, const float* t_host, i
   std::memcpy(m->h_stage + (size_t)n * kIn, t_host, n * sizeof(float));
   CM_OK(hipMemcpyAsync(dX, m->h_stage, (size_t)n * (kIn + 1) * sizeof(float), hipMemcpyHostToDevice, m->s));
   const int grid = (n + kBlock - 1) / kBlock;
-  mlp_forward_kernel<<<grid, kBlock, 0, m->s>>>(m->d_p, dX, n, dY, H1, H2);
+  mlp_forward_kernel<<<forward_grid(n), kBlock, 0, m->s>>>(m->d_p, dX, n, dY, H1, H2);
   mlp_delta_kernel<<<grid, kBlock, 0, m->s>>>(m->d_p, dY, dT, H1, H2, n, D3, D2, D1);
   mlp_grad_kernel<<<(kParams + kBlock - 1) / kBlock, kBlock, 0, m->s>>>(dX, H1, H2, D1, D2, D3, n, m->d_g);
   CM_OK(hipGetLastError());
