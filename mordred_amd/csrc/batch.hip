@@ -177,7 +177,9 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   // query descriptors transposed into four arrays of 16-byte chunks (chunk c
   // of query q at lqc[c * nq + q]): lanes of one wave read the descriptors of
   // different queries, which a 64-byte stride would put in the same banks
-  uint4* lqc = reinterpret_cast<uint4*>(lds_raw);
+  // (the waves' pair lists come first in the dynamic region)
+  unsigned char* const lds_q = lds_raw + batch_pair_bytes(BLOCK);
+  uint4* lqc = reinterpret_cast<uint4*>(lds_q);
   auto query = [&](int q, bool full) {
     union {
       BatchQuery Q;
@@ -191,13 +193,14 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   };
   // interval and mask tables, then the group region: per query a touched
   // bitmap, then LDS accumulators (8-byte aligned)
-  unsigned* ltab = reinterpret_cast<unsigned*>(lds_raw + nq * sizeof(BatchQuery));
+  unsigned* ltab = reinterpret_cast<unsigned*>(lds_q + nq * sizeof(BatchQuery));
   const int ntab = (D->ntab + 1) & ~1;
   unsigned* lgrp = ltab + ntab;
   const int ngrp = D->lds_words;
   {
     const uint4* src = reinterpret_cast<const uint4*>(D->q);
     for (int i = threadIdx.x; i < nq * 4; i += BLOCK) lqc[(i & 3) * nq + (i >> 2)] = src[i];
+    if ((threadIdx.x & (kWave - 1)) == 0) reinterpret_cast<unsigned*>(lds_raw)[(threadIdx.x >> 6) * (1 + kBatchPairCap / 2)] = 0u;
     for (int i = threadIdx.x; i < ngrp; i += BLOCK) lgrp[i] = 0;
     for (int i = threadIdx.x; i < ntab; i += BLOCK) ltab[i] = D->tab[i];
     const int np = D->npred;
@@ -550,30 +553,30 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     }
   };
 
-  // Queries whose pairs need a returning atomic -- 32-bit LDS sums (the
-  // carry) and queries that store their keys in the loop -- are handled in a
-  // loop of their own, so that the main loop over the other pairs never waits
-  // for an atomic (a wave runs every sink path some lane needs).
-  unsigned long long slowq = 0, slow32 = 0;
-  for (int q = 0; q < nq; q++) {
-    const BatchQuery Q = query(q, false);
-    if ((Q.flags & kQueryKeysInLoop) || Q.sink == kSinkLds32) slowq |= 1ull << q;
-    if (!(Q.flags & kQueryKeysInLoop) && Q.sink == kSinkLds32) slow32 |= 1ull << q;
-  }
+  // Queries that store their keys in the pair loop (their add must return
+  // the previous sum, with the row's group values at hand) keep a per-lane
+  // loop of their own; every other pair goes through the wave's pair list.
+  unsigned long long keysq = 0;
+  for (int q = 0; q < nq; q++)
+    if (query(q, false).flags & kQueryKeysInLoop) keysq |= 1ull << q;
 
-  auto group_of = [&](const BatchQuery& Q, int j) {
-    return mod_pos(Q.hbase + (int)__umul24(__builtin_amdgcn_ubfe(attrw[0][j], Q.shift[0], Q.bits[0]), (unsigned)Q.uniq[0]) +
-                       (int)__umul24(__builtin_amdgcn_ubfe(attrw[1][j], Q.shift[1], Q.bits[1]), (unsigned)Q.uniq[1]) +
-                       (int)__umul24(__builtin_amdgcn_ubfe(attrw[2][j], Q.shift[2], Q.bits[2]), (unsigned)Q.uniq[2]) +
-                       (int)__umul24(__builtin_amdgcn_ubfe(attrw[3][j], Q.shift[3], Q.bits[3]), (unsigned)Q.uniq[3]),
+  auto group_of = [&](const BatchQuery& Q, const uint32_t (&a)[4]) {
+    return mod_pos(Q.hbase + (int)__umul24(__builtin_amdgcn_ubfe(a[0], Q.shift[0], Q.bits[0]), (unsigned)Q.uniq[0]) +
+                       (int)__umul24(__builtin_amdgcn_ubfe(a[1], Q.shift[1], Q.bits[1]), (unsigned)Q.uniq[1]) +
+                       (int)__umul24(__builtin_amdgcn_ubfe(a[2], Q.shift[2], Q.bits[2]), (unsigned)Q.uniq[2]) +
+                       (int)__umul24(__builtin_amdgcn_ubfe(a[3], Q.shift[3], Q.bits[3]), (unsigned)Q.uniq[3]),
                    Q.total_val);
   };
-  auto value_of = [&](const BatchQuery& Q, int j) {
+  auto value_of = [&](const BatchQuery& Q, const int (&v)[kBatchMaxExpr]) {
     int val = 0;
 #pragma unroll
     for (int e = 0; e < kBatchMaxExpr; e++)
-      if (e == Q.expr) val = ev[e][j];
+      if (e == Q.expr) val = v[e];
     return val;
+  };
+  auto global_sum = [&](const BatchQuery& Q, int h) {
+    return (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(
+        reinterpret_cast<int*>(Q.res) + 6 * (size_t)h + 4);
   };
 
   // every surviving (row, query) pair.  An index outside [0, total_val)
@@ -581,86 +584,95 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   // reference does, here the pair is dropped.  (Scoped atomics keep the LDS
   // ones ds_ instructions: one atomicAdd on either pointer would be merged
   // into a slower flat atomic.)
+  //
+  // Per row slot j the lanes list their pairs in the wave's pair list (each
+  // lane's entries at an offset from one returning add on the wave's
+  // counter), then every lane aggregates one listed pair per step: a wave
+  // takes ceil(pairs / 64) steps instead of the largest per-lane pair count.
+  // The row's attribute words and expression values come from the pair's
+  // source lane by ds_bpermute, which every lane of the wave executes (a
+  // disabled source lane would read as zero).  No add returns a value except
+  // the 32-bit LDS sums' (the carry): keys are written after the scan from
+  // the touched bitmaps.
+  unsigned* const lpn = reinterpret_cast<unsigned*>(lds_raw) + (threadIdx.x >> 6) * (1 + kBatchPairCap / 2);
+  unsigned short* const lpl = reinterpret_cast<unsigned short*>(lpn + 1);
   auto pairs = [&]() {
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
-      // no add returns a value here: the keys are written after the scan from
-      // the touched bitmap
-      unsigned long long m = alive[j] & ~slowq;
-      while (m) {
-        const int q = __builtin_ctzll(m);
-        m &= m - 1;
-        const BatchQuery Q = query(q, false);
-        const int h = group_of(Q, j);
-        if ((unsigned)h >= (unsigned)Q.total_val) continue;
-        const unsigned long long add = (unsigned long long)(long long)value_of(Q, j);
-        const int tw = batch_touch_words(Q.total_val, Q.flags);
-        if (Q.sink == kSinkLds64) {
-          (void)__hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(lgrp + Q.acc + tw) + h, add,
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else {
-          auto* gsum = (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(
-              reinterpret_cast<int*>(Q.res) + 6 * (size_t)h + 4);
-          (void)__hip_atomic_fetch_add(gsum, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (tw)
-          (void)__hip_atomic_fetch_or(lgrp + Q.acc + (h >> 5), 1u << (h & 31), __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-      // 32-bit LDS sums: (int64)val = u - 2^32 [val < 0]; the low word wraps
-      // when old + u carries, and the 2^32 that leaves it goes to the result.
-      // Up to four pairs per lane are prepared (descriptor reads), then their
-      // adds issued back to back and only then their returns read: LDS
-      // operations complete in order, so a wave waits once per four pairs
-      // instead of once per pair.
-      m = alive[j] & slow32;
-      while (m) {
-        unsigned old[4], uadd[4], word[4], touch[4];
-        int qs[4], hs[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          qs[r] = -1;
-          if (!m) continue;
+      unsigned long long m = alive[j] & ~keysq;
+      while (__ballot(m != 0ull)) {
+        // list: this lane's pairs at [p, p + popcount); entries past the
+        // list's end stay in m for the next round
+        unsigned p = 0;
+        if (m) p = __hip_atomic_fetch_add(lpn, (unsigned)__popcll(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        for (; m && p < (unsigned)kBatchPairCap; p++) {
           const int q = __builtin_ctzll(m);
           m &= m - 1;
-          const BatchQuery Q = query(q, false);
-          const int h = group_of(Q, j);
-          if ((unsigned)h >= (unsigned)Q.total_val) continue;
-          const int tw = batch_touch_words(Q.total_val, Q.flags);
-          uadd[r] = (unsigned)value_of(Q, j);
-          word[r] = (unsigned)(Q.acc + tw + h);
-          touch[r] = tw ? (unsigned)(Q.acc + (h >> 5)) : ~0u;
-          qs[r] = q;
-          hs[r] = h;
+          lpl[p] = (unsigned short)(lane << 6 | q);
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int n = min(__builtin_amdgcn_readfirstlane(
+                              (int)__hip_atomic_load(lpn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT)),
+                          kBatchPairCap);
+        for (int b = 0; b < n; b += kWave) {
+          const int k = b + lane;
+          const unsigned ent = k < n ? (unsigned)lpl[k] : 0u;
+          const int src = (int)(ent >> 6) << 2;
+          uint32_t a[4];
+          int v[kBatchMaxExpr];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          if (qs[r] < 0) continue;
-          old[r] = __hip_atomic_fetch_add(lgrp + word[r], uadd[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          if (touch[r] != ~0u)
-            (void)__hip_atomic_fetch_or(lgrp + touch[r], 1u << (hs[r] & 31), __ATOMIC_RELAXED,
+          for (int d = 0; d < 4; d++) a[d] = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)attrw[d][j]);
+#pragma unroll
+          for (int e = 0; e < kBatchMaxExpr; e++) v[e] = e < nexpr ? __builtin_amdgcn_ds_bpermute(src, ev[e][j]) : 0;
+          if (k >= n) continue;
+          const BatchQuery Q = query((int)(ent & 63u), false);
+          const int h = group_of(Q, a);
+          if ((unsigned)h >= (unsigned)Q.total_val) continue;
+          const int val = value_of(Q, v);
+          const int tw = batch_touch_words(Q.total_val, Q.flags);
+          if (Q.sink == kSinkLds64) {
+            (void)__hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(lgrp + Q.acc + tw) + h,
+                                         (unsigned long long)(long long)val, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+          } else if (Q.sink == kSinkLds32) {
+            // 32-bit LDS sums: (int64)val = u - 2^32 [val < 0]; the low word
+            // wraps when old + u carries, and the 2^32 that leaves it goes to
+            // the result
+            const unsigned u = (unsigned)val;
+            const unsigned old = __hip_atomic_fetch_add(lgrp + Q.acc + tw + h, u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            const int delta = (int)(old + u < old) - (int)(val < 0);
+            if (delta)
+              (void)__hip_atomic_fetch_add(global_sum(Q, h), (unsigned long long)(long long)delta << 32,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          } else {
+            (void)__hip_atomic_fetch_add(global_sum(Q, h), (unsigned long long)(long long)val, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+          }
+          if (tw)
+            (void)__hip_atomic_fetch_or(lgrp + Q.acc + (h >> 5), 1u << (h & 31), __ATOMIC_RELAXED,
                                         __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          if (qs[r] < 0) continue;
-          const int delta = (int)(old[r] + uadd[r] < old[r]) - (int)((int)uadd[r] < 0);
-          if (!delta) continue;
-          const BatchQuery Q = query(qs[r], false);
-          auto* gsum = (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(
-              reinterpret_cast<int*>(Q.res) + 6 * (size_t)hs[r] + 4);
-          (void)__hip_atomic_fetch_add(gsum, (unsigned long long)(long long)delta << 32, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-        }
+        // (every lane has read the list and the counter: LDS operations of a
+        // wave complete in order)
+        __hip_atomic_store(lpn, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       }
-      m = alive[j] & slowq & ~slow32;
+      uint32_t a[4];
+      int v[kBatchMaxExpr];
+#pragma unroll
+      for (int d = 0; d < 4; d++) a[d] = attrw[d][j];
+#pragma unroll
+      for (int e = 0; e < kBatchMaxExpr; e++) v[e] = ev[e][j];
+      m = alive[j] & keysq;
       while (m) {
         const int q = __builtin_ctzll(m);
         m &= m - 1;
         const BatchQuery Q = query(q, false);
-        const int h = group_of(Q, j);
+        const int h = group_of(Q, a);
         if ((unsigned)h >= (unsigned)Q.total_val) continue;
-        const int val = value_of(Q, j);
+        const int val = value_of(Q, v);
         const unsigned long long add = (unsigned long long)(long long)val;
         int* res = reinterpret_cast<int*>(Q.res) + 6 * (size_t)h;
         auto* gsum = (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(res + 4);
@@ -683,7 +695,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
 #pragma unroll
           for (int d = 0; d < 4; d++)
             gv[d] = Qa.attr[d] < 0 ? 0
-                                   : H.dim[d].base[Qa.attr[d]] + (int)__builtin_amdgcn_ubfe(attrw[d][j], Q.shift[d], Q.bits[d]);
+                                   : H.dim[d].base[Qa.attr[d]] + (int)__builtin_amdgcn_ubfe(a[d], Q.shift[d], Q.bits[d]);
           int2* key = reinterpret_cast<int2*>(res);
           key[0] = make_int2(gv[0], gv[1]);
           key[1] = make_int2(gv[2], gv[3]);
@@ -791,7 +803,8 @@ hipError_t launch_c12(const BatchLaunch& L, const BatchConfig& c, hipStream_t s)
 }  // namespace
 
 bool batch_config_ok(const BatchConfig& c) {
-  return pick(c, 8) != nullptr && c.lds_bytes >= (int)sizeof(BatchHot) && c.lds_bytes <= 160 * 1024;
+  return pick(c, 8) != nullptr && c.lds_bytes >= (int)sizeof(BatchHot) + batch_pair_bytes(c.block) &&
+         c.lds_bytes <= 160 * 1024;
 }
 
 hipError_t batch_prepare(const BatchConfig& c, int ncols) {

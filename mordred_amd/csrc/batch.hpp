@@ -130,6 +130,16 @@ __host__ __device__ inline int batch_acc_words(int sink, int G) {
   return sink == kSinkLds64 ? 2 * G : sink == kSinkLds32 ? G : 0;
 }
 
+// The scan lists the (row, query) pairs of one row slot per wave before it
+// aggregates them, so that the lanes of a wave share the pairs evenly: per
+// wave a counter word and kBatchPairCap 16-bit entries (source lane << 6 |
+// query) at the start of the dynamic LDS region.  A slot with more pairs is
+// listed in rounds.
+constexpr int kBatchPairCap = 256;
+__host__ __device__ inline int batch_pair_bytes(int block) {
+  return (block / 64) * (4 + 2 * kBatchPairCap);  // a multiple of 16
+}
+
 struct BatchPred {             // fact BETWEEN predicate of one query
   int col, lo, hi, q;
 };

@@ -2303,8 +2303,8 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
   b.lds_queries = b.global_queries = b.dead = b.lds32_queries = b.keys_in_loop = 0;  // (a plan may be retried)
   if (nq < 1 || nq > kBatchMaxQ) return fail(e, "a batch holds 1..64 queries");
   if (!batch_config_ok(cfg)) return fail(e, "bad batch configuration");
-  if ((size_t)cfg.lds_bytes < sizeof(BatchHot) + nq * sizeof(BatchQuery))
-    return fail(e, "lds_bytes below the staged descriptor plus 64 bytes per query");
+  if ((size_t)cfg.lds_bytes < sizeof(BatchHot) + batch_pair_bytes(cfg.block) + nq * sizeof(BatchQuery))
+    return fail(e, "lds_bytes below the staged descriptor, the pair lists and 64 bytes per query");
   std::vector<Resolved> rs(nq);
   int first = -1;
   for (int i = 0; i < nq; i++) {
@@ -2745,8 +2745,8 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
       if (estimate_query(e, *qp[i], qe) == 0) est_sel[i] = qe.sel;
     }
   }
-  const long long fixed_bytes = (long long)sizeof(BatchHot) + nq * (long long)sizeof(BatchQuery) +
-                                4LL * ((D.ntab + 1) & ~1);
+  const long long fixed_bytes = (long long)sizeof(BatchHot) + batch_pair_bytes(cfg.block) +
+                                nq * (long long)sizeof(BatchQuery) + 4LL * ((D.ntab + 1) & ~1);
   if (fixed_bytes > cfg.lds_bytes) {
     lds_short = true;
     return fail(e, "lds_bytes below the staged descriptor and interval tables");
