@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "batch.hpp"
 #include "pipeline.hpp"
@@ -175,32 +176,31 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   __shared__ BatchHot H;
   const int nq = D->nq;
   // query descriptors transposed into four arrays of 16-byte chunks (chunk c
-  // of query q at lqc[c * nq + q]): lanes of one wave read the descriptors of
-  // different queries, which a 64-byte stride would put in the same banks
-  // (the waves' pair lists come first in the dynamic region)
-  unsigned char* const lds_q = lds_raw + batch_pair_bytes(BLOCK);
-  uint4* lqc = reinterpret_cast<uint4*>(lds_q);
+  // of query q at lqc[c * kBatchMaxQ + q]): lanes of one wave read the
+  // descriptors of different queries, which a 64-byte stride would put in the
+  // same banks; the fixed stride makes the chunk offsets immediates
+  uint4* lqc = reinterpret_cast<uint4*>(lds_raw);
   auto query = [&](int q, bool full) {
     union {
       BatchQuery Q;
       uint4 c[4];
     } u;
     u.c[0] = lqc[q];
-    u.c[1] = lqc[nq + q];
-    u.c[2] = lqc[2 * nq + q];
-    u.c[3] = full ? lqc[3 * nq + q] : make_uint4(0, 0, 0, 0);
+    u.c[1] = lqc[kBatchMaxQ + q];
+    u.c[2] = lqc[2 * kBatchMaxQ + q];
+    u.c[3] = full ? lqc[3 * kBatchMaxQ + q] : make_uint4(0, 0, 0, 0);
     return u.Q;
   };
-  // interval and mask tables, then the group region: per query a touched
-  // bitmap, then LDS accumulators (8-byte aligned)
-  unsigned* ltab = reinterpret_cast<unsigned*>(lds_q + nq * sizeof(BatchQuery));
+  // interval and mask tables, then the group region (per query a touched
+  // bitmap, then LDS accumulators, 8-byte aligned; BatchQuery holds their
+  // offsets from lds_raw), and at the end of the region the waves' pair lists
+  unsigned* ltab = reinterpret_cast<unsigned*>(lds_raw + kBatchMaxQ * sizeof(BatchQuery));
   const int ntab = (D->ntab + 1) & ~1;
   unsigned* lgrp = ltab + ntab;
   const int ngrp = D->lds_words;
   {
     const uint4* src = reinterpret_cast<const uint4*>(D->q);
-    for (int i = threadIdx.x; i < nq * 4; i += BLOCK) lqc[(i & 3) * nq + (i >> 2)] = src[i];
-    if ((threadIdx.x & (kWave - 1)) == 0) reinterpret_cast<unsigned*>(lds_raw)[(threadIdx.x >> 6) * (1 + kBatchPairCap / 2)] = 0u;
+    for (int i = threadIdx.x; i < nq * 4; i += BLOCK) lqc[(i & 3) * kBatchMaxQ + (i >> 2)] = src[i];
     for (int i = threadIdx.x; i < ngrp; i += BLOCK) lgrp[i] = 0;
     for (int i = threadIdx.x; i < ntab; i += BLOCK) ltab[i] = D->tab[i];
     const int np = D->npred;
@@ -253,6 +253,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       H.nq = nq;
       H.ntab = ntab;
       H.nivl = D->nivl;
+      H.attr16 = D->attr16;
     }
   }
   __syncthreads();
@@ -261,6 +262,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   const int usedmask = __builtin_amdgcn_readfirstlane(H.usedmask);
   const int nexpr = __builtin_amdgcn_readfirstlane(H.nexpr);
   const int nivl = __builtin_amdgcn_readfirstlane(H.nivl);
+  const bool attr16 = __builtin_amdgcn_readfirstlane(H.attr16) != 0;
   const unsigned long long all = D->all;
   const int lane = threadIdx.x & (kWave - 1);
 
@@ -522,6 +524,15 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       if (d + 1 < 4) issue(d + 1);
       consume(d);
     }
+    // 16-bit attribute words: dimensions 1 and 3 into the high halves of 0
+    // and 2 (one ds_bpermute per two dimensions in the pair loop)
+    if (attr16) {
+#pragma unroll
+      for (int j = 0; j < IPT; j++) {
+        attrw[0][j] |= attrw[1][j] << 16;
+        attrw[2][j] |= attrw[3][j] << 16;
+      }
+    }
     // aggregate expressions of the batch (int32 arithmetic, GPUProcessing.h:149, 1763)
 #pragma unroll
     for (int e = 0; e < kBatchMaxExpr; e++) {
@@ -558,25 +569,30 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   // loop of their own; every other pair goes through the wave's pair list.
   unsigned long long keysq = 0;
   for (int q = 0; q < nq; q++)
-    if (query(q, false).flags & kQueryKeysInLoop) keysq |= 1ull << q;
+    if (query(q, true).flags & kQueryKeysInLoop) keysq |= 1ull << q;
 
   auto group_of = [&](const BatchQuery& Q, const uint32_t (&a)[4]) {
-    return mod_pos(Q.hbase + (int)__umul24(__builtin_amdgcn_ubfe(a[0], Q.shift[0], Q.bits[0]), (unsigned)Q.uniq[0]) +
-                       (int)__umul24(__builtin_amdgcn_ubfe(a[1], Q.shift[1], Q.bits[1]), (unsigned)Q.uniq[1]) +
-                       (int)__umul24(__builtin_amdgcn_ubfe(a[2], Q.shift[2], Q.bits[2]), (unsigned)Q.uniq[2]) +
-                       (int)__umul24(__builtin_amdgcn_ubfe(a[3], Q.shift[3], Q.bits[3]), (unsigned)Q.uniq[3]),
+    // field d's shift and width are bytes d of two words; v_bfe_u32 reads only
+    // the low 5 bits of its offset and width operands, so byte d needs no mask
+    uint32_t S, B;
+    __builtin_memcpy(&S, Q.shift, 4);
+    __builtin_memcpy(&B, Q.bits, 4);
+    return mod_pos(Q.hbase + (int)__umul24(__builtin_amdgcn_ubfe(a[0], S, B), (unsigned)Q.uniq[0]) +
+                       (int)__umul24(__builtin_amdgcn_ubfe(a[1], S >> 8, B >> 8), (unsigned)Q.uniq[1]) +
+                       (int)__umul24(__builtin_amdgcn_ubfe(a[2], S >> 16, B >> 16), (unsigned)Q.uniq[2]) +
+                       (int)__umul24(__builtin_amdgcn_ubfe(a[3], S >> 24, B >> 24), (unsigned)Q.uniq[3]),
                    Q.total_val);
   };
-  auto value_of = [&](const BatchQuery& Q, const int (&v)[kBatchMaxExpr]) {
-    int val = 0;
-#pragma unroll
-    for (int e = 0; e < kBatchMaxExpr; e++)
-      if (e == Q.expr) val = v[e];
+  // (four named values, not an array: a select chain over an array indexed
+  // by the query's expression becomes a dynamic index into scratch memory)
+  static_assert(kBatchMaxExpr == 4, "value_of selects among four expressions");
+  auto value_of = [&](const BatchQuery& Q, int v0, int v1, int v2, int v3) {
+    const unsigned x = Q.expr;
+    int val = v0;
+    val = x == 1u ? v1 : val;
+    val = x == 2u ? v2 : val;
+    val = x == 3u ? v3 : val;
     return val;
-  };
-  auto global_sum = [&](const BatchQuery& Q, int h) {
-    return (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(
-        reinterpret_cast<int*>(Q.res) + 6 * (size_t)h + 4);
   };
 
   // every surviving (row, query) pair.  An index outside [0, total_val)
@@ -594,9 +610,17 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   // disabled source lane would read as zero).  No add returns a value except
   // the 32-bit LDS sums' (the carry): keys are written after the scan from
   // the touched bitmaps.
-  unsigned* const lpn = reinterpret_cast<unsigned*>(lds_raw) + (threadIdx.x >> 6) * (1 + kBatchPairCap / 2);
+  // (the wave's pair list: a counter word, then kBatchPairCap entries, at the
+  // end of the dynamic LDS; the counter is zero between rounds)
+  unsigned* const lpn = reinterpret_cast<unsigned*>(lds_raw + D->lds_dyn - batch_pair_bytes(BLOCK)) +
+                        (threadIdx.x >> 6) * (1 + kBatchPairCap / 2);
   unsigned short* const lpl = reinterpret_cast<unsigned short*>(lpn + 1);
-  auto pairs = [&]() {
+  if (lane == 0) *lpn = 0u;
+  // (A16: the batch's 16-bit attribute words are packed two per register, a
+  // compile-time choice so that no copy of a ds_bpermute result sits between
+  // the step's LDS reads)
+  auto pairs = [&](auto A16) {
+    constexpr bool kA16 = decltype(A16)::value;
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
       unsigned long long m = alive[j] & ~keysq;
@@ -608,7 +632,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         for (; m && p < (unsigned)kBatchPairCap; p++) {
           const int q = __builtin_ctzll(m);
           m &= m - 1;
-          lpl[p] = (unsigned short)(lane << 6 | q);
+          lpl[p] = (unsigned short)(q << 8 | lane << 2);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -617,67 +641,88 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
                               (int)__hip_atomic_load(lpn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT)),
                           kBatchPairCap);
         for (int b = 0; b < n; b += kWave) {
-          const int k = b + lane;
-          const unsigned ent = k < n ? (unsigned)lpl[k] : 0u;
-          const int src = (int)(ent >> 6) << 2;
+          // (entries at and past n are stale or unwritten: read, never used)
+          const unsigned ent = lpl[b + lane];
+          const int src = (int)(ent & 0xFCu);
           uint32_t a[4];
-          int v[kBatchMaxExpr];
-#pragma unroll
-          for (int d = 0; d < 4; d++) a[d] = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)attrw[d][j]);
-#pragma unroll
-          for (int e = 0; e < kBatchMaxExpr; e++) v[e] = e < nexpr ? __builtin_amdgcn_ds_bpermute(src, ev[e][j]) : 0;
-          if (k >= n) continue;
-          const BatchQuery Q = query((int)(ent & 63u), false);
-          const int h = group_of(Q, a);
-          if ((unsigned)h >= (unsigned)Q.total_val) continue;
-          const int val = value_of(Q, v);
-          const int tw = batch_touch_words(Q.total_val, Q.flags);
-          if (Q.sink == kSinkLds64) {
-            (void)__hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(lgrp + Q.acc + tw) + h,
+          a[0] = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)attrw[0][j]);
+          a[2] = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)attrw[2][j]);
+          if constexpr (kA16) {
+            a[1] = a[0];
+            a[3] = a[2];
+          } else {
+            a[1] = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)attrw[1][j]);
+            a[3] = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)attrw[3][j]);
+          }
+          const int v0 = __builtin_amdgcn_ds_bpermute(src, ev[0][j]);
+          const int v1 = nexpr > 1 ? __builtin_amdgcn_ds_bpermute(src, ev[1][j]) : 0;
+          const int v2 = nexpr > 2 ? __builtin_amdgcn_ds_bpermute(src, ev[2][j]) : 0;
+          const int v3 = nexpr > 3 ? __builtin_amdgcn_ds_bpermute(src, ev[3][j]) : 0;
+          if (b + lane >= n) continue;
+          // the descriptor's first three chunks, each one 16-byte LDS read:
+          // c0 = res, total_val, acc; c1 = hbase, expr | sink << 8 | touch <<
+          // 16, shift bytes, bits bytes; c2 = uniq
+          const int q = (int)(ent >> 8);
+          const uint4 c0 = lqc[q], c1 = lqc[kBatchMaxQ + q], c2 = lqc[2 * kBatchMaxQ + q];
+          const int total = (int)c0.z;
+          const int h = mod_pos((int)c1.x + (int)__umul24(__builtin_amdgcn_ubfe(a[0], c1.z, c1.w), c2.x) +
+                                    (int)__umul24(__builtin_amdgcn_ubfe(a[1], c1.z >> 8, c1.w >> 8), c2.y) +
+                                    (int)__umul24(__builtin_amdgcn_ubfe(a[2], c1.z >> 16, c1.w >> 16), c2.z) +
+                                    (int)__umul24(__builtin_amdgcn_ubfe(a[3], c1.z >> 24, c1.w >> 24), c2.w),
+                                total);
+          if ((unsigned)h >= (unsigned)total) continue;
+          const unsigned expr = c1.y & 0xFFu, sink = (c1.y >> 8) & 0xFFu, touch = c1.y >> 16;
+          int val = v0;
+          val = expr == 1u ? v1 : val;
+          val = expr == 2u ? v2 : val;
+          val = expr == 3u ? v3 : val;
+          unsigned char* const acc = lds_raw + c0.w;
+          auto* const gsum = (__attribute__((address_space(1))) unsigned long long*)(
+              reinterpret_cast<int*>((unsigned long long)c0.y << 32 | c0.x) + 6 * (size_t)h + 4);
+          if (sink == kSinkLds64) {
+            (void)__hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(acc) + h,
                                          (unsigned long long)(long long)val, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_WORKGROUP);
-          } else if (Q.sink == kSinkLds32) {
+          } else if (sink == kSinkLds32) {
             // 32-bit LDS sums: (int64)val = u - 2^32 [val < 0]; the low word
             // wraps when old + u carries, and the 2^32 that leaves it goes to
             // the result
             const unsigned u = (unsigned)val;
-            const unsigned old = __hip_atomic_fetch_add(lgrp + Q.acc + tw + h, u, __ATOMIC_RELAXED,
+            const unsigned old = __hip_atomic_fetch_add(reinterpret_cast<unsigned*>(acc) + h, u, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
             const int delta = (int)(old + u < old) - (int)(val < 0);
             if (delta)
-              (void)__hip_atomic_fetch_add(global_sum(Q, h), (unsigned long long)(long long)delta << 32,
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              (void)__hip_atomic_fetch_add(gsum, (unsigned long long)(long long)delta << 32, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
           } else {
-            (void)__hip_atomic_fetch_add(global_sum(Q, h), (unsigned long long)(long long)val, __ATOMIC_RELAXED,
+            (void)__hip_atomic_fetch_add(gsum, (unsigned long long)(long long)val, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
           }
-          if (tw)
-            (void)__hip_atomic_fetch_or(lgrp + Q.acc + (h >> 5), 1u << (h & 31), __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (touch != kBatchNoTouch)
+            (void)__hip_atomic_fetch_or(reinterpret_cast<unsigned*>(lds_raw) + touch + (h >> 5), 1u << (h & 31),
+                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
         // (every lane has read the list and the counter: LDS operations of a
         // wave complete in order)
         __hip_atomic_store(lpn, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       }
       uint32_t a[4];
-      int v[kBatchMaxExpr];
-#pragma unroll
-      for (int d = 0; d < 4; d++) a[d] = attrw[d][j];
-#pragma unroll
-      for (int e = 0; e < kBatchMaxExpr; e++) v[e] = ev[e][j];
+      a[0] = attrw[0][j];
+      a[1] = kA16 ? attrw[0][j] : attrw[1][j];
+      a[2] = attrw[2][j];
+      a[3] = kA16 ? attrw[2][j] : attrw[3][j];
       m = alive[j] & keysq;
       while (m) {
         const int q = __builtin_ctzll(m);
         m &= m - 1;
-        const BatchQuery Q = query(q, false);
+        const BatchQuery Q = query(q, true);
         const int h = group_of(Q, a);
         if ((unsigned)h >= (unsigned)Q.total_val) continue;
-        const int val = value_of(Q, v);
+        const int val = value_of(Q, ev[0][j], ev[1][j], ev[2][j], ev[3][j]);
         const unsigned long long add = (unsigned long long)(long long)val;
         int* res = reinterpret_cast<int*>(Q.res) + 6 * (size_t)h;
         auto* gsum = (__attribute__((address_space(1))) unsigned long long*)reinterpret_cast<unsigned long long*>(res + 4);
-        const int tw = batch_touch_words(Q.total_val, Q.flags);
-        unsigned* acc = lgrp + Q.acc + tw;
+        unsigned char* const acc = lds_raw + Q.acc;
         // keys that do not decode from the index: stored whenever the sum the
         // add returns was zero -- at the first pair of the group in this
         // workgroup (LDS) or overall (global), and again only if a running sum
@@ -690,12 +735,11 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         else
           old = __hip_atomic_fetch_add(gsum, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (!(Q.flags & kQueryAggr) && old == 0ull) {
-          const BatchQuery Qa = query(q, true);
           int gv[4];
 #pragma unroll
           for (int d = 0; d < 4; d++)
-            gv[d] = Qa.attr[d] < 0 ? 0
-                                   : H.dim[d].base[Qa.attr[d]] + (int)__builtin_amdgcn_ubfe(a[d], Q.shift[d], Q.bits[d]);
+            gv[d] = Q.attr[d] < 0 ? 0
+                                  : H.dim[d].base[Q.attr[d]] + (int)__builtin_amdgcn_ubfe(a[d], Q.shift[d], Q.bits[d]);
           int2* key = reinterpret_cast<int2*>(res);
           key[0] = make_int2(gv[0], gv[1]);
           key[1] = make_int2(gv[2], gv[3]);
@@ -710,7 +754,8 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     for (int t = t_begin; t < t_end; t++) {
       masks(t, x);
       if (t + 1 < t_end) load_cols(t + 1, x);
-      pairs();
+      if (attr16) pairs(std::true_type{});
+      else pairs(std::false_type{});
     }
   }
 
@@ -723,9 +768,9 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     if (!(all >> q & 1ull)) continue;
     const BatchQuery Q = query(q, true);
     const int G = Q.total_val, flags = Q.flags, sink = Q.sink;
-    const int tw = batch_touch_words(G, flags);
-    const unsigned* tb = lgrp + Q.acc;
-    const unsigned* acc = tb + tw;
+    const int tw = batch_touch_words(G, flags);  // (Q.touch is kBatchNoTouch iff tw == 0)
+    const unsigned* tb = reinterpret_cast<const unsigned*>(lds_raw) + (tw ? Q.touch : 0);
+    const unsigned* acc = reinterpret_cast<const unsigned*>(lds_raw + Q.acc);
     auto* res = reinterpret_cast<unsigned long long*>(Q.res);
     auto add = [&](int h) {
       const unsigned long long s = sink == kSinkLds64 ? reinterpret_cast<const unsigned long long*>(acc)[h]

@@ -83,14 +83,13 @@ enum BatchSink {
 };
 
 // The fields a (row, query) pair reads sit in the first 48 bytes (three
-// 16-byte LDS reads).
+// 16-byte LDS reads).  LDS offsets count from the start of the scan kernel's
+// dynamic LDS, so that the pair loop's addresses are the offset plus an
+// index.
 struct BatchQuery {            // 64 bytes; staged in LDS by the scan kernel
   long long res;               // device address of the query's 6*total_val ints
   int total_val;
-  // even 32-bit word offset in the workgroup's LDS group region of the
-  // query's touched bitmap (batch_touch_words words), followed by its
-  // accumulators for an LDS sink
-  int acc;
+  int acc;                     // byte offset of the accumulators (LDS sinks, 8-byte aligned)
   // dense group index (GPUProcessing.h:142) as hbase + sum_d field_d * uniq[d]
   // with field_d = (attribute word >> shift[d]) & (2^bits[d] - 1) and hbase =
   // sum over joined dimensions of (group value base - min_val) * unique_val
@@ -99,16 +98,19 @@ struct BatchQuery {            // 64 bytes; staged in LDS by the scan kernel
   // products are 24-bit multiplies (identical in 32-bit wrap-around).
   int hbase;
   unsigned char expr;          // index into BatchDesc::expr
-  unsigned char flags;         // BatchQueryFlags
   unsigned char sink;          // BatchSink
-  unsigned char pad;
+  // 32-bit word offset of the touched bitmap (batch_touch_words words), or
+  // kBatchNoTouch for a query that keeps none
+  unsigned short touch;
   unsigned char shift[4];
   unsigned char bits[4];
   int uniq[4];
   signed char attr[4];         // per dimension slot: -1 no group value, else its attribute
   int gtouch;                  // word offset of its touched bitmap in BatchDesc::touch_arena
-  int unused[2];
+  int flags;                   // BatchQueryFlags
+  int unused;
 };
+constexpr unsigned short kBatchNoTouch = 0xFFFF;
 static_assert(sizeof(BatchQuery) == 64, "BatchQuery is staged as 4 x 16 bytes");
 
 // Group keys of a query whose keys decode from the group index (a mixed
@@ -132,9 +134,10 @@ __host__ __device__ inline int batch_acc_words(int sink, int G) {
 
 // The scan lists the (row, query) pairs of one row slot per wave before it
 // aggregates them, so that the lanes of a wave share the pairs evenly: per
-// wave a counter word and kBatchPairCap 16-bit entries (source lane << 6 |
-// query) at the start of the dynamic LDS region.  A slot with more pairs is
-// listed in rounds.
+// wave a counter word and kBatchPairCap 16-bit entries (query << 8 | source
+// lane << 2) at the end of the dynamic LDS region.  A slot with more pairs is
+// listed in rounds.  The query descriptors are staged at its start, four
+// 16-byte chunks per query at a stride of kBatchMaxQ chunks.
 constexpr int kBatchPairCap = 256;
 __host__ __device__ inline int batch_pair_bytes(int block) {
   return (block / 64) * (4 + 2 * kBatchPairCap);  // a multiple of 16
@@ -222,7 +225,12 @@ struct BatchDesc {
   BatchDim dim[4];
   BatchQuery q[kBatchMaxQ];
   BatchKeys keys[kBatchMaxQ];
-  int ntab, pad1;
+  int ntab;
+  int lds_dyn;                 // dynamic LDS bytes of the scan launch (its pair lists end there)
+  // every dimension's attribute word fits in 16 bits: the scan packs dimensions
+  // 1 and 3 into the high halves of 0 and 2 (BatchQuery::shift of 1 and 3
+  // count from bit 16)
+  int attr16;
   unsigned tab[kBatchMaxTab];  // interval tables and attribute mask tables
 };
 
@@ -249,12 +257,14 @@ struct BatchHotDim {
 struct BatchHot {
   const int* col[kBatchMaxCols];
   BatchHotDim dim[4];
-  int ncols, colmask, usedmask, nexpr, npred, nq, ntab, nivl;
+  int ncols, colmask, usedmask, nexpr, npred, nq, ntab, nivl, attr16, pad[3];
   int pred_begin[kBatchMaxCols + 1];
   int4 expr[kBatchMaxExpr];
   int4 pred[kBatchMaxPred];    // lo, hi - lo (unsigned), query, column
   BatchIvl ivl[kBatchMaxIvl];
 };
+
+static_assert(sizeof(BatchHot) % 16 == 0, "the scan's dynamic LDS (and its pair lists) stay 16-byte aligned");
 
 struct BatchConfig {
   int block;                   // 256, 512, 1024

@@ -2303,8 +2303,8 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
   b.lds_queries = b.global_queries = b.dead = b.lds32_queries = b.keys_in_loop = 0;  // (a plan may be retried)
   if (nq < 1 || nq > kBatchMaxQ) return fail(e, "a batch holds 1..64 queries");
   if (!batch_config_ok(cfg)) return fail(e, "bad batch configuration");
-  if ((size_t)cfg.lds_bytes < sizeof(BatchHot) + batch_pair_bytes(cfg.block) + nq * sizeof(BatchQuery))
-    return fail(e, "lds_bytes below the staged descriptor, the pair lists and 64 bytes per query");
+  if ((size_t)cfg.lds_bytes < sizeof(BatchHot) + batch_pair_bytes(cfg.block) + kBatchMaxQ * sizeof(BatchQuery))
+    return fail(e, "lds_bytes below the staged descriptor, the pair lists and the query descriptors");
   std::vector<Resolved> rs(nq);
   int first = -1;
   for (int i = 0; i < nq; i++) {
@@ -2362,6 +2362,7 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
   Column* attrs[4][kBatchMaxAttr] = {};
   std::vector<char> qdead(nq, 0);  // a dimension filter with an empty range
   D.ntab = 0;
+  D.attr16 = 1;
   for (int k = 0; k < 4; k++) {
     BatchDim& B = D.dim[k];
     Column* key = nullptr;
@@ -2451,6 +2452,7 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
       off += bits;
     }
     if (off > 32) return fail(e, "attributes of one dimension exceed 32 bits in a batch");
+    if (off > 16) D.attr16 = 0;
     B.nattr = (int)packed.size();
     B.pw = off <= 8 ? 1 : off <= 16 ? 2 : 4;
     if (B.form == kDimMask) {
@@ -2612,7 +2614,9 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
       while (a < kBatchMaxAttr && attrs[k][a] != r.dval[k]) a++;
       const BatchAttr& A = D.dim[k].attr[a];
       Q.attr[k] = (signed char)a;
-      Q.shift[k] = (unsigned char)A.off;
+      // (16-bit attribute words: dimensions 1 and 3 ride in the high halves of
+      // 0 and 2 during the scan's pair loop)
+      Q.shift[k] = (unsigned char)(A.off + (D.attr16 && (k & 1) ? 16 : 0));
       Q.bits[k] = (unsigned char)A.bits;
       if (A.bits > 16) return fail(e, "batch group index needs attribute fields of at most 16 bits");
       // the field holds value - base (+ 1 in the attribute and dictionary forms)
@@ -2746,7 +2750,7 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
     }
   }
   const long long fixed_bytes = (long long)sizeof(BatchHot) + batch_pair_bytes(cfg.block) +
-                                nq * (long long)sizeof(BatchQuery) + 4LL * ((D.ntab + 1) & ~1);
+                                kBatchMaxQ * (long long)sizeof(BatchQuery) + 4LL * ((D.ntab + 1) & ~1);
   if (fixed_bytes > cfg.lds_bytes) {
     lds_short = true;
     return fail(e, "lds_bytes below the staged descriptor and interval tables");
@@ -2802,13 +2806,19 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
       avail -= extra;
     }
   }
+  // the group region follows the query descriptors and the tables in the
+  // scan's dynamic LDS; per query its touched bitmap, then its accumulators
+  const long long grp0 = kBatchMaxQ * (long long)sizeof(BatchQuery) / 4 + ((D.ntab + 1) & ~1);
   long long lds_words = 0, res_ints = 0, touch_words = 0;
   for (int i = 0; i < nq; i++) {
     BatchQuery& Q = D.q[i];
     Q.sink = (unsigned char)sink[i];
-    Q.acc = (int)lds_words;
+    Q.touch = kBatchNoTouch;
+    Q.acc = (int)(4 * (grp0 + lds_words));
     if (!qp[i]) continue;
     const int tw = batch_touch_words(b.G[i], Q.flags);
+    if (tw) Q.touch = (unsigned short)(grp0 + lds_words);
+    Q.acc = (int)(4 * (grp0 + lds_words + tw));
     lds_words += tw + even(batch_acc_words(sink[i], b.G[i]));
     if (sink[i] == kSinkGlobal) b.global_queries++;
     else b.lds_queries++;
@@ -2823,6 +2833,9 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
     touch_words += tw;
   }
   D.lds_words = (int)lds_words;
+  D.lds_dyn = cfg.lds_bytes - (int)sizeof(BatchHot);
+  if (4 * (grp0 + lds_words) + batch_pair_bytes(cfg.block) > D.lds_dyn || grp0 + lds_words >= kBatchNoTouch)
+    return fail(e, "batch LDS plan exceeds the launch's LDS");
   if (fixed) {
     D.touch_words = nq * fixed->touch_stride;
     D.touch_arena = fixed->touch;

@@ -117,6 +117,27 @@ def test_autotuner_configs_agree(bengine):
     check(qs, outs, refs)
 
 
+def test_wide_attribute_words(bengine):
+    """A query grouping on d_datekey (16 bits of the date table's attribute
+    word, with d_year and d_yearmonthnum beside it): the batch keeps its
+    attribute words unpacked, one per dimension, in the pair loop."""
+    qs, _, data, eng, refs = bengine
+    from mordred_amd.query import DimJoin, StarQuery
+    from mordred_amd import datagen
+    span = int(data.tables["date"]["d_datekey"].max()) - datagen.DATE_MIN_KEY + 1
+    date = DimJoin(4, "date", "lo_orderdate", "d_datekey", datagen.DATE_HT_LEN, datagen.DATE_MIN_KEY,
+                   filter_col="d_year", c1=1994, c2=1995, group_col="d_datekey",
+                   min_val=datagen.DATE_MIN_KEY, unique_val=1)
+    supp = DimJoin(1, "supplier", "lo_suppkey", "s_suppkey", SMALL["S"], 0, filter_col="s_region", c1=2, c2=2)
+    wide = StarQuery("wide_datekey", [supp, date], [], ("lo_revenue", None), "groupby", span)
+    batch = [wide] + [q for q in qs if q.join(4) is not None][:8]
+    print(f"[gpu] batch with a d_datekey group ({span} groups)", flush=True)
+    outs, st = eng.batch(batch, (1024, 1, 160 << 10))
+    np.testing.assert_array_equal(outs[0], OQ.run_query(wide, data, oracle.CONV_GPU), err_msg=wide.name)
+    assert (outs[0][4::6] != 0).any()
+    check(batch[1:], outs[1:], refs)
+
+
 def test_dead_query_and_rejections(bengine):
     qs, _, _, eng, refs = bengine
     q = dataclasses.replace(qs[0], fact_filters=[("lo_revenue", 900_000, 100_000)])
