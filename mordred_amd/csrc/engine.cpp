@@ -2183,6 +2183,8 @@ int dim_dict(mordred_engine* e, Column* key, const std::vector<Column*>& attrs, 
   if (e->dicts.size() >= 16) {
     auto lru = std::min_element(e->dicts.begin(), e->dicts.end(),
                                 [](const auto& x, const auto& y) { return x->last_use < y->last_use; });
+    // (a launch in flight may read its codes)
+    HIP_OK(hipStreamSynchronize(e->s_main));
     e->dicts.erase(lru);
   }
   e->dicts.push_back(std::move(d));
@@ -2478,6 +2480,9 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
       // key has no slot, reads it
       B.mstride = B.nattr ? 2 : 1;
       const size_t mbytes = (((size_t)(B.len + 1) * 8 * B.mstride) + 15) & ~(size_t)15;
+      // (growing frees the buffer: the request loop may plan while a launch
+      // that reads it is in flight)
+      if (mbytes > e->d_mk[k].bytes) HIP_OK(hipStreamSynchronize(e->s_main));
       HIP_OK(e->d_mk[k].reserve(mbytes));
       B.mk = static_cast<unsigned long long*>(e->d_mk[k].p);
     } else if (B.form == kDimDict) {
@@ -2555,6 +2560,7 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
     if (B.form == kDimAttr) {
       // a zero entry at slot len, and the aligned 8-byte word holding it
       const size_t pbytes = (((size_t)(B.len + 1) * B.pw + 8) + 15) & ~(size_t)15;
+      if (pbytes > e->d_rec[k].bytes) HIP_OK(hipStreamSynchronize(e->s_main));
       HIP_OK(e->d_rec[k].reserve(pbytes));
       B.pk = e->d_rec[k].p;
       B.pk_words = (long long)(pbytes / 16);
@@ -3952,9 +3958,73 @@ int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n,
   const int sizes[] = {8, 16, 32, 64, 128, 256, 512, 1024};
   float feat[MORDRED_COST_FEATURES];
   long long since_train = 0;
+  auto rows_of = [&](int from, int to) {
+    return std::min<long long>((long long)to << kSegLog, nrows) - ((long long)from << kSegLog);
+  };
+  // One launch of the shared scan.  The loop plans launch k+1 while launch k
+  // runs: which queries launch k completes is known when it is planned (each
+  // query sees every segment once, so it finishes when its covered segments
+  // reach nseg), their slots are free for the next plan, and their result
+  // rows are copied to pinned staging right behind launch k on the stream,
+  // before launch k+1's fill kernel can reset a reused slot.
+  struct Launch {
+    bool on = false;
+    BatchPlan b;
+    std::vector<QueryEst> es;
+    int nreset = 0;
+    std::vector<std::pair<int, int>> retire;  // (slot, request) that this launch completes
+  };
+  // pinned staging: the status word, then one result area per slot
+  HIP_OK(e->h_arena.reserve(64 + (size_t)nslot * ar.res_stride * sizeof(int)));
+  char* const stage = static_cast<char*>(e->h_arena.p);
+  auto stage_of = [&](int sidx) { return stage + 64 + (size_t)sidx * ar.res_stride * sizeof(int); };
+  auto result_ints = [&](int req) {
+    return 6 * (size_t)(qs[req].kind == MORDRED_KIND_GROUPBY ? qs[req].total_val : 1);
+  };
+  auto launch = [&](Launch& L) -> int {
+    int rc = launch_batch(e, L.b, cfg, nullptr);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(stage, e->d_status.p, sizeof(unsigned), hipMemcpyDeviceToHost, e->s_main));
+    for (const auto& [sidx, req] : L.retire)
+      HIP_OK(hipMemcpyAsync(stage_of(sidx), ar.res + sidx * ar.res_stride, result_ints(req) * sizeof(int),
+                            hipMemcpyDeviceToHost, e->s_main));
+    return 0;
+  };
+  auto complete = [&](Launch& L) -> int {
+    HIP_OK(hipStreamSynchronize(e->s_main));
+    L.on = false;
+    unsigned collisions = 0;
+    std::memcpy(&collisions, stage, sizeof collisions);
+    int rc = batch_status(e, collisions);
+    if (rc) return rc;
+    float gms = 0;
+    HIP_OK(hipEventElapsedTime(&gms, e->ev0, e->ev1));
+    st.gpu_ms += gms;
+    st.alg_bytes += 4.0 * L.b.cols_read * (double)L.b.d.num_tuples;
+    st.launches++;
+    // learn from the measured launch
+    if (e->cm) {
+      launch_features(L.b, L.es, L.b.d.num_tuples, L.nreset, cfg, feat);
+      rc = mordred_cost_model_observe(e->cm, feat, std::max(1e-3, (double)gms), analytic_launch_ms(feat, L.b.d.ncols));
+      if (rc) return fail(e, std::string("cost model: ") + mordred_cost_model_last_error(e->cm));
+      if (++since_train >= 16) {
+        since_train = 0;
+        rc = mordred_cost_model_train(e->cm, 24, 128, 2e-3f, nullptr);
+        if (rc) return fail(e, std::string("cost model: ") + mordred_cost_model_last_error(e->cm));
+      }
+      HIP_OK(hipSetDevice(e->device));
+    }
+    for (const auto& [sidx, req] : L.retire) {
+      std::memcpy(res[req], stage_of(sidx), result_ints(req) * sizeof(int));
+      times[req].done_ms = now_ms();
+      done++;
+    }
+    return 0;
+  };
+  Launch fly;  // the launch on the GPU
   while (done < n) {
     // a lone request on an idle loop: alone through its own fused kernel
-    if (o->lone_plain && active == 0 && next < n && arrival_ms[next] <= now_ms() &&
+    if (!fly.on && o->lone_plain && active == 0 && next < n && arrival_ms[next] <= now_ms() &&
         (next + 1 >= n || arrival_ms[next + 1] > now_ms())) {
       const int i = next++;
       times[i].admit_ms = now_ms();
@@ -3985,113 +4055,94 @@ int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n,
       next++;
       changed = true;
     }
-    if (active == 0) {
-      if (next < n) wait_until(arrival_ms[next]);
-      continue;
-    }
-    st.max_active_seen = std::max(st.max_active_seen, active);
-    // launch range: from cur, never past the table end or an active query's
-    // starting segment (each query sees every segment exactly once)
-    int limit = nseg;
-    for (const Slot& sl : slots)
-      if (sl.req >= 0 && sl.start > cur) limit = std::min(limit, sl.start);
-    std::vector<QueryEst> es(nslot);
-    std::vector<double> sel(nslot, 0.0);
-    for (int sidx = 0; sidx < nslot; sidx++)
-      if (slots[sidx].req >= 0) {
-        es[sidx] = est[slots[sidx].req];
-        sel[sidx] = es[sidx].sel;
-      }
-    BatchPlan b;
-    int rc = plan_batch(e, qp.data(), nslot, cfg, b, &ar, sel.data());
-    if (rc) return rc;
-    b.d.reset = reset;
-    b.d.rebuild = changed ? 1 : 0;
-    int nreset = 0;
-    for (int sidx = 0; sidx < nslot; sidx++) nreset += reset >> sidx & 1ull;
-    auto rows_of = [&](int from, int to) {
-      return std::min<long long>((long long)to << kSegLog, nrows) - ((long long)from << kSegLog);
-    };
-    // launch size: the largest candidate predicted within the target
-    int span = limit - cur;
-    bool used_learned = false;
-    if (o->chunk_segments > 0) {
-      span = std::min(span, o->chunk_segments);
-    } else {
-      mordred_cost_model_stats cs{};
-      const bool trusted = o->learned && e->cm && mordred_cost_model_get_stats(e->cm, &cs) == 0 && cs.trusted;
-      std::vector<float> xs;
-      std::vector<int> cand;
-      for (int sz : sizes) {
-        const int sp = std::min(sz, limit - cur);
-        if (!cand.empty() && sp == cand.back()) break;
-        cand.push_back(sp);
-        launch_features(b, es, rows_of(cur, cur + sp), nreset, cfg, feat);
-        xs.insert(xs.end(), feat, feat + MORDRED_COST_FEATURES);
-      }
-      std::vector<float> pred(cand.size());
-      if (trusted) {
-        rc = mordred_cost_model_predict(e->cm, xs.data(), (int)cand.size(), pred.data());
-        if (rc) return fail(e, std::string("cost model: ") + mordred_cost_model_last_error(e->cm));
-        for (float& p : pred) p = std::exp(p);
-        used_learned = true;
+    Launch nx;
+    if (active > 0) {
+      st.max_active_seen = std::max(st.max_active_seen, active);
+      // launch range: from cur, never past the table end or an active query's
+      // starting segment (each query sees every segment exactly once)
+      int limit = nseg;
+      for (const Slot& sl : slots)
+        if (sl.req >= 0 && sl.start > cur) limit = std::min(limit, sl.start);
+      nx.es.assign(nslot, QueryEst{});
+      std::vector<double> sel(nslot, 0.0);
+      for (int sidx = 0; sidx < nslot; sidx++)
+        if (slots[sidx].req >= 0) {
+          nx.es[sidx] = est[slots[sidx].req];
+          sel[sidx] = nx.es[sidx].sel;
+        }
+      BatchPlan& b = nx.b;
+      int rc = plan_batch(e, qp.data(), nslot, cfg, b, &ar, sel.data());
+      if (rc) return rc;
+      b.d.reset = reset;
+      b.d.rebuild = changed ? 1 : 0;
+      for (int sidx = 0; sidx < nslot; sidx++) nx.nreset += reset >> sidx & 1ull;
+      // launch size: the largest candidate predicted within the target
+      int span = limit - cur;
+      bool used_learned = false;
+      if (o->chunk_segments > 0) {
+        span = std::min(span, o->chunk_segments);
       } else {
+        mordred_cost_model_stats cs{};
+        const bool trusted = o->learned && e->cm && mordred_cost_model_get_stats(e->cm, &cs) == 0 && cs.trusted;
+        std::vector<float> xs;
+        std::vector<int> cand;
+        for (int sz : sizes) {
+          const int sp = std::min(sz, limit - cur);
+          if (!cand.empty() && sp == cand.back()) break;
+          cand.push_back(sp);
+          launch_features(b, nx.es, rows_of(cur, cur + sp), nx.nreset, cfg, feat);
+          xs.insert(xs.end(), feat, feat + MORDRED_COST_FEATURES);
+        }
+        std::vector<float> pred(cand.size());
+        if (trusted) {
+          rc = mordred_cost_model_predict(e->cm, xs.data(), (int)cand.size(), pred.data());
+          if (rc) return fail(e, std::string("cost model: ") + mordred_cost_model_last_error(e->cm));
+          HIP_OK(hipSetDevice(e->device));
+          for (float& p : pred) p = std::exp(p);
+          used_learned = true;
+        } else {
+          for (size_t c = 0; c < cand.size(); c++)
+            pred[c] = (float)analytic_launch_ms(&xs[c * MORDRED_COST_FEATURES], b.d.ncols);
+        }
+        span = cand[0];
         for (size_t c = 0; c < cand.size(); c++)
-          pred[c] = (float)analytic_launch_ms(&xs[c * MORDRED_COST_FEATURES], b.d.ncols);
+          if (pred[c] <= o->target_chunk_ms) span = cand[c];
+        (used_learned ? st.learned_choices : st.analytic_choices)++;
       }
-      span = cand[0];
-      for (size_t c = 0; c < cand.size(); c++)
-        if (pred[c] <= o->target_chunk_ms) span = cand[c];
-      (used_learned ? st.learned_choices : st.analytic_choices)++;
-    }
-    const int end = cur + span;
-    b.d.seg_begin = cur;
-    b.d.num_tuples = (int)rows_of(cur, end);
-    rc = launch_batch(e, b, cfg, nullptr);
-    if (rc) return rc;
-    HIP_OK(e->h_arena.reserve(64));
-    HIP_OK(hipMemcpyAsync(e->h_arena.p, e->d_status.p, sizeof(unsigned), hipMemcpyDeviceToHost, e->s_main));
-    HIP_OK(hipStreamSynchronize(e->s_main));
-    unsigned collisions = 0;
-    std::memcpy(&collisions, e->h_arena.p, sizeof collisions);
-    if ((rc = batch_status(e, collisions))) return rc;
-    float gms = 0;
-    HIP_OK(hipEventElapsedTime(&gms, e->ev0, e->ev1));
-    st.gpu_ms += gms;
-    st.alg_bytes += 4.0 * b.cols_read * (double)b.d.num_tuples;
-    st.launches++;
-    changed = false;
-    // learn from the measured launch
-    if (e->cm) {
-      launch_features(b, es, b.d.num_tuples, nreset, cfg, feat);
-      rc = mordred_cost_model_observe(e->cm, feat, std::max(1e-3, (double)gms), analytic_launch_ms(feat, b.d.ncols));
-      if (rc) return fail(e, std::string("cost model: ") + mordred_cost_model_last_error(e->cm));
-      if (++since_train >= 16) {
-        since_train = 0;
-        rc = mordred_cost_model_train(e->cm, 24, 128, 2e-3f, nullptr);
-        if (rc) return fail(e, std::string("cost model: ") + mordred_cost_model_last_error(e->cm));
+      const int end = cur + span;
+      b.d.seg_begin = cur;
+      b.d.num_tuples = (int)rows_of(cur, end);
+      changed = false;
+      // account the launch now; queries that will have seen every segment
+      // leave their slots to the next plan
+      for (int sidx = 0; sidx < nslot; sidx++) {
+        Slot& sl = slots[sidx];
+        if (sl.req < 0) continue;
+        sl.covered += span;
+        times[sl.req].launches++;
+        if (sl.covered < nseg) continue;
+        nx.retire.emplace_back(sidx, sl.req);
+        state -= state_bytes(sl.req);
+        sl = Slot{};
+        qp[sidx] = nullptr;
+        active--;
+        changed = true;
       }
+      cur = end >= nseg ? 0 : end;
+      nx.on = true;
     }
-    // account the launch; retire queries that have seen every segment
-    for (int sidx = 0; sidx < nslot; sidx++) {
-      Slot& sl = slots[sidx];
-      if (sl.req < 0) continue;
-      sl.covered += span;
-      times[sl.req].launches++;
-      if (sl.covered < nseg) continue;
-      const mordred_star_query& q = qs[sl.req];
-      const int G = q.kind == MORDRED_KIND_GROUPBY ? q.total_val : 1;
-      HIP_OK(hipMemcpy(res[sl.req], ar.res + sidx * ar.res_stride, 6 * (size_t)G * sizeof(int),
-                       hipMemcpyDeviceToHost));
-      times[sl.req].done_ms = now_ms();
-      state -= state_bytes(sl.req);
-      sl = Slot{};
-      qp[sidx] = nullptr;
-      active--;
-      done++;
-      changed = true;
+    if (fly.on) {
+      const int rc = complete(fly);
+      if (rc) return rc;
     }
-    cur = end >= nseg ? 0 : end;
+    if (nx.on) {
+      const int rc = launch(nx);
+      if (rc) return rc;
+      fly = std::move(nx);
+      fly.on = true;
+    } else if (!fly.on && active == 0 && next < n) {
+      wait_until(arrival_ms[next]);
+    }
   }
   st.wall_ms = now_ms();
   if (e->cm) {
