@@ -321,6 +321,10 @@ int mordred_engine_batch_tune(mordred_engine* e, const mordred_star_query* qs, i
  *     group) fit the HBM budget, and finishes once the scan has covered every
  *     segment exactly once for it (launches never cross an active query's
  *     starting segment), so its result equals its own run bit for bit.
+ *     The loop plans the next launch (admission, plan, launch size) while the
+ *     current one runs: the queries a launch completes are known when it is
+ *     planned, so their slots go to the next plan, and their results are
+ *     copied out behind the launch before the next one can reuse a slot.
  *     Launch sizes (segments per launch) are chosen per launch so that the
  *     predicted launch time stays within target_chunk_ms; predictions come
  *     from the learned cost model once it is trusted (mordred_costmodel.h),
