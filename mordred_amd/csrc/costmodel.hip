@@ -74,61 +74,86 @@ __global__ __launch_bounds__(kBlock) void mlp_forward_kernel(const float* __rest
 // Workgroups of the forward pass for n samples (one wave per sample).
 static int forward_grid(int n) { return std::max(1, (n + kBlock / kH - 1) / (kBlock / kH)); }
 
-// Backward deltas, one thread per sample: d3 = 2(y-t)/n, D2 = d3 W3 * [h2>0],
-// D1 = (D2 W2^T) * [h1>0].
+// Backward deltas, one wavefront per sample: lane o holds d2[o] = dL/dz2[o],
+// lane i sums d1[i] = sum_o d2[o] W2[i][o] over shuffles of d2, reading W2
+// transposed from LDS (consecutive lanes, consecutive words).
 __global__ __launch_bounds__(kBlock) void mlp_delta_kernel(const float* __restrict__ P, const float* __restrict__ Y,
                                                            const float* __restrict__ T, const float* __restrict__ H1,
                                                            const float* __restrict__ H2, int n, float* __restrict__ D3,
                                                            float* __restrict__ D2, float* __restrict__ D1) {
-  __shared__ float w[kH * kH + kH];
-  for (int i = threadIdx.x; i < kH * kH; i += kBlock) w[i] = P[kOffW2 + i];
-  for (int i = threadIdx.x; i < kH; i += kBlock) w[kH * kH + i] = P[kOffW3 + i];
+  __shared__ float w2t[kH * kH];  // w2t[o * kH + i] = W2[i][o]
+  for (int k = threadIdx.x; k < kH * kH; k += kBlock) w2t[(k % kH) * kH + k / kH] = P[kOffW2 + k];
   __syncthreads();
-  const int b = blockIdx.x * kBlock + threadIdx.x;
-  if (b >= n) return;
-  const float d3 = 2.f * (Y[b] - T[b]) / (float)n;
-  D3[b] = d3;
-  float d2[kH];
-#pragma unroll
-  for (int o = 0; o < kH; o++) {
-    d2[o] = H2[(size_t)b * kH + o] > 0.f ? d3 * w[kH * kH + o] : 0.f;
-    D2[(size_t)b * kH + o] = d2[o];
-  }
-  for (int i = 0; i < kH; i++) {
+  const int lane = threadIdx.x & (kH - 1);
+  const float w3 = P[kOffW3 + lane];
+  const int nw = gridDim.x * (kBlock / kH);
+  // (b is the same for every lane of a wave: the shuffles see the whole wave)
+  for (int b = (int)(blockIdx.x * kBlock + threadIdx.x) / kH; b < n; b += nw) {
+    const float d3 = 2.f * (Y[b] - T[b]) / (float)n;
+    const float d2 = H2[(size_t)b * kH + lane] > 0.f ? d3 * w3 : 0.f;
+    D2[(size_t)b * kH + lane] = d2;
     float a = 0.f;
-#pragma unroll
-    for (int o = 0; o < kH; o++) a = fmaf(d2[o], w[i * kH + o], a);
-    D1[(size_t)b * kH + i] = H1[(size_t)b * kH + i] > 0.f ? a : 0.f;
+#pragma unroll 16
+    for (int o = 0; o < kH; o++) a = fmaf(__shfl(d2, o, kH), w2t[o * kH + lane], a);
+    D1[(size_t)b * kH + lane] = H1[(size_t)b * kH + lane] > 0.f ? a : 0.f;
+    if (lane == 0) D3[b] = d3;
   }
 }
 
-// Parameter gradients, one thread per parameter, summed over the samples.
+// Parameter gradients, one workgroup per parameter row: W1 rows (kIn), b1, W2
+// rows (kH), b2, W3, b3 -- kGradRows rows of at most kH parameters, lane o
+// holding column o.  A row's gradient is sum_b a_b * d_b[o] over the samples;
+// the four waves take every fourth sample, then add their partial sums in LDS.
+constexpr int kGradRows = kIn + 1 + kH + 1 + 1 + 1;
 __global__ __launch_bounds__(kBlock) void mlp_grad_kernel(const float* __restrict__ X, const float* __restrict__ H1,
                                                           const float* __restrict__ H2, const float* __restrict__ D1,
                                                           const float* __restrict__ D2, const float* __restrict__ D3,
                                                           int n, float* __restrict__ G) {
-  const int p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= kParams) return;
-  float g = 0.f;
-  if (p < kOffB1) {
-    const int i = p / kH, o = p % kH;
-    for (int b = 0; b < n; b++) g = fmaf(X[(size_t)b * kIn + i], D1[(size_t)b * kH + o], g);
-  } else if (p < kOffW2) {
-    const int o = p - kOffB1;
-    for (int b = 0; b < n; b++) g += D1[(size_t)b * kH + o];
-  } else if (p < kOffB2) {
-    const int i = (p - kOffW2) / kH, o = (p - kOffW2) % kH;
-    for (int b = 0; b < n; b++) g = fmaf(H1[(size_t)b * kH + i], D2[(size_t)b * kH + o], g);
-  } else if (p < kOffW3) {
-    const int o = p - kOffB2;
-    for (int b = 0; b < n; b++) g += D2[(size_t)b * kH + o];
-  } else if (p < kOffB3) {
-    const int o = p - kOffW3;
-    for (int b = 0; b < n; b++) g = fmaf(H2[(size_t)b * kH + o], D3[b], g);
+  constexpr int kWaves = kBlock / kH;
+  __shared__ float part[kWaves][kH];
+  const int row = blockIdx.x, lane = threadIdx.x & (kH - 1), wave = threadIdx.x / kH;
+  // a_b: the row's input activation of sample b (1 for a bias row); d: the
+  // deltas (or, for W3, the activations H2 against the scalar delta D3)
+  const float* A = nullptr;
+  int a_stride = 0, a_col = 0;
+  const float* Dm = nullptr;
+  int out = 0;
+  if (row < kIn) {
+    A = X, a_stride = kIn, a_col = row, Dm = D1, out = kOffW1 + row * kH;
+  } else if (row == kIn) {
+    Dm = D1, out = kOffB1;
+  } else if (row < kIn + 1 + kH) {
+    A = H1, a_stride = kH, a_col = row - kIn - 1, Dm = D2, out = kOffW2 + (row - kIn - 1) * kH;
+  } else if (row == kIn + 1 + kH) {
+    Dm = D2, out = kOffB2;
+  } else if (row == kIn + 2 + kH) {
+    A = D3, a_stride = 1, a_col = 0, Dm = H2, out = kOffW3;
   } else {
-    for (int b = 0; b < n; b++) g += D3[b];
+    out = kOffB3;  // b3: sum_b D3[b], lane 0
   }
-  G[p] = g;
+  float g0 = 0.f, g1 = 0.f;
+  int b = wave;
+  for (; b + kWaves < n; b += 2 * kWaves) {
+    const float a0 = A ? A[(size_t)b * a_stride + a_col] : 1.f;
+    const float a1 = A ? A[(size_t)(b + kWaves) * a_stride + a_col] : 1.f;
+    const float d0 = Dm ? Dm[(size_t)b * kH + lane] : (lane == 0 ? D3[b] : 0.f);
+    const float d1 = Dm ? Dm[(size_t)(b + kWaves) * kH + lane] : (lane == 0 ? D3[b + kWaves] : 0.f);
+    g0 = fmaf(a0, d0, g0);
+    g1 = fmaf(a1, d1, g1);
+  }
+  if (b < n) g0 = fmaf(A ? A[(size_t)b * a_stride + a_col] : 1.f, Dm ? Dm[(size_t)b * kH + lane] : (lane == 0 ? D3[b] : 0.f), g0);
+  part[wave][lane] = g0 + g1;
+  __syncthreads();
+  if (wave == 0) {
+    float g = 0.f;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) g += part[w][lane];
+    if (out == kOffB3) {
+      if (lane == 0) G[out] = g;
+    } else {
+      G[out + lane] = g;
+    }
+  }
 }
 
 // Adam (Kingma & Ba, 2015), bias-corrected, one thread per parameter.
@@ -221,41 +246,54 @@ int forward(mordred_cost_model* m, const float* x_host, int n, float* y_host) {
   return 0;
 }
 
-// gradients of the mean squared loss over n samples (X, T on host); leaves
-// them in d_g and the loss in *loss
-int gradients(mordred_cost_model* m, const float* x_host, const float* t_host, int n, float* loss) {
-  const size_t need = (size_t)n * (kIn + 2 + 5 * kH + 1);
-  int rc = reserve_work(m, need);
-  if (rc) return rc;
-  if ((rc = reserve_stage(m, (size_t)n * (kIn + 1)))) return rc;
-  float* dX = m->d_work;
-  float* dT = dX + (size_t)n * kIn;
-  float* dY = dT + n;
+// Forward, deltas and parameter gradients of the mean squared loss over n
+// samples already on the device (dX: n x kIn, dT: n) into d_g; the
+// activations and deltas go to the work area's first n * (5 kH + 2) floats,
+// the prediction to *dY_out.
+int gradients_dev(mordred_cost_model* m, const float* dX, const float* dT, int n, float** dY_out) {
+  float* dY = m->d_work;
   float* H1 = dY + n;
   float* H2 = H1 + (size_t)n * kH;
   float* D1 = H2 + (size_t)n * kH;
   float* D2 = D1 + (size_t)n * kH;
   float* D3 = D2 + (size_t)n * kH;
+  mlp_forward_kernel<<<forward_grid(n), kBlock, 0, m->s>>>(m->d_p, dX, n, dY, H1, H2);
+  mlp_delta_kernel<<<forward_grid(n), kBlock, 0, m->s>>>(m->d_p, dY, dT, H1, H2, n, D3, D2, D1);
+  mlp_grad_kernel<<<kGradRows, kBlock, 0, m->s>>>(dX, H1, H2, D1, D2, D3, n, m->d_g);
+  CM_OK(hipGetLastError());
+  if (dY_out) *dY_out = dY;
+  return 0;
+}
+size_t activation_floats(int n) { return (size_t)n * (4 * kH + 2); }
+
+// mean squared loss of the n predictions at dY against t_host (synchronizes)
+int loss_of(mordred_cost_model* m, const float* dY, const float* t_host, int n, float* loss) {
+  CM_OK(hipMemcpyAsync(m->h_stage, dY, n * sizeof(float), hipMemcpyDeviceToHost, m->s));
+  CM_OK(hipStreamSynchronize(m->s));
+  double l = 0;
+  for (int b = 0; b < n; b++) {
+    const double d = (double)m->h_stage[b] - (double)t_host[b];
+    l += d * d;
+  }
+  *loss = (float)(l / n);
+  return 0;
+}
+
+// gradients of the mean squared loss over n samples (X, T on host); leaves
+// them in d_g and the loss in *loss
+int gradients(mordred_cost_model* m, const float* x_host, const float* t_host, int n, float* loss) {
+  const size_t act = activation_floats(n);
+  int rc = reserve_work(m, act + (size_t)n * (kIn + 1));
+  if (rc) return rc;
+  if ((rc = reserve_stage(m, (size_t)n * (kIn + 1)))) return rc;
+  float* dX = m->d_work + act;
+  float* dT = dX + (size_t)n * kIn;
   std::memcpy(m->h_stage, x_host, (size_t)n * kIn * sizeof(float));
   std::memcpy(m->h_stage + (size_t)n * kIn, t_host, n * sizeof(float));
   CM_OK(hipMemcpyAsync(dX, m->h_stage, (size_t)n * (kIn + 1) * sizeof(float), hipMemcpyHostToDevice, m->s));
-  const int grid = (n + kBlock - 1) / kBlock;
-  mlp_forward_kernel<<<forward_grid(n), kBlock, 0, m->s>>>(m->d_p, dX, n, dY, H1, H2);
-  mlp_delta_kernel<<<grid, kBlock, 0, m->s>>>(m->d_p, dY, dT, H1, H2, n, D3, D2, D1);
-  mlp_grad_kernel<<<(kParams + kBlock - 1) / kBlock, kBlock, 0, m->s>>>(dX, H1, H2, D1, D2, D3, n, m->d_g);
-  CM_OK(hipGetLastError());
-  if (loss) {
-    std::vector<float> y(n);
-    CM_OK(hipMemcpyAsync(m->h_stage, dY, n * sizeof(float), hipMemcpyDeviceToHost, m->s));
-    CM_OK(hipStreamSynchronize(m->s));
-    double l = 0;
-    for (int b = 0; b < n; b++) {
-      const double d = (double)m->h_stage[b] - (double)t_host[b];
-      l += d * d;
-    }
-    *loss = (float)(l / n);
-  }
-  return 0;
+  float* dY = nullptr;
+  if ((rc = gradients_dev(m, dX, dT, n, &dY))) return rc;
+  return loss ? loss_of(m, dY, t_host, n, loss) : 0;
 }
 
 int adam_step(mordred_cost_model* m, float lr) {
@@ -390,20 +428,40 @@ int mordred_cost_model_observe(mordred_cost_model* m, const float* x, double mea
 int mordred_cost_model_train(mordred_cost_model* m, int steps, int batch, float lr, float* last_loss) {
   if (!m || steps < 0 || batch <= 0 || !(lr > 0)) return -1;
   const int have = (int)m->ts.size();
-  if (have == 0) return 0;
+  if (have == 0 || steps == 0) return 0;
+  CM_OK(hipSetDevice(m->device));
   const int n = std::min(batch, have);
-  std::vector<float> x((size_t)n * kIn), t(n);
+  // every step's minibatch staged and copied at once; the steps run back to
+  // back on the model's stream with one synchronization at the end (each
+  // step's kernels follow the previous step's Adam update on the stream)
+  const size_t per = (size_t)n * (kIn + 1), act = activation_floats(n);
+  int rc = reserve_work(m, act + per * steps);
+  if (rc) return rc;
+  if ((rc = reserve_stage(m, std::max(per * steps, (size_t)n)))) return rc;
   std::uniform_int_distribution<int> pick(0, have - 1);
-  float loss = 0;
+  std::vector<float> t_last(n);
   for (int s = 0; s < steps; s++) {
+    float* hx = m->h_stage + per * s;
+    float* ht = hx + (size_t)n * kIn;
     for (int b = 0; b < n; b++) {
       const int i = n == have ? b : pick(m->rng);
-      std::memcpy(&x[(size_t)b * kIn], &m->xs[(size_t)i * kIn], kIn * sizeof(float));
-      t[b] = m->ts[i];
+      std::memcpy(&hx[(size_t)b * kIn], &m->xs[(size_t)i * kIn], kIn * sizeof(float));
+      ht[b] = m->ts[i];
     }
-    const int rc = mordred_cost_model_step(m, x.data(), t.data(), n, lr, s + 1 == steps ? &loss : nullptr);
-    if (rc) return rc;
+    if (s + 1 == steps) std::memcpy(t_last.data(), ht, n * sizeof(float));
   }
+  float* dIn = m->d_work + act;
+  CM_OK(hipMemcpyAsync(dIn, m->h_stage, per * steps * sizeof(float), hipMemcpyHostToDevice, m->s));
+  float* dY = nullptr;
+  for (int s = 0; s < steps; s++) {
+    const float* dX = dIn + per * s;
+    if ((rc = gradients_dev(m, dX, dX + (size_t)n * kIn, n, &dY))) return rc;
+    if ((rc = adam_step(m, lr))) return rc;
+  }
+  // (the staging buffer is free again once the copy above has run: the loss
+  // copy below is ordered after it on the stream)
+  float loss = 0;
+  if ((rc = loss_of(m, dY, t_last.data(), n, &loss))) return rc;
   if (last_loss) *last_loss = loss;
   return 0;
 }

@@ -96,3 +96,23 @@ def test_online_learning_and_trust(cuda):
     assert st["ewma_abs_log_error"] < 0.25 < st["ewma_abs_log_error_analytic"]
     assert st["trusted"]
     m.close()
+
+
+def test_train_equals_successive_steps(cuda):
+    """train() stages every minibatch at once and runs its steps back to back
+    with one synchronization; with a replay buffer no larger than the batch
+    each step takes every sample in order, so k steps of train() must equal
+    k calls of step() on those samples."""
+    rng = np.random.default_rng(11)
+    xs = rng.uniform(0, 1, (50, CM.FEATURES)).astype(np.float32)
+    ms = np.exp(rng.uniform(-1, 2, 50)).astype(np.float64)
+    a, b = CM.CostModel(0, seed=12), CM.CostModel(0, seed=12)
+    for x, t in zip(xs, ms):
+        a.observe(x, float(t), analytic_ms=float(t))
+    loss = a.train(3, batch=128, lr=2e-3)
+    for _ in range(3):
+        b.step(xs, np.log(ms).astype(np.float32), 2e-3)
+    np.testing.assert_allclose(a.params(), b.params(), rtol=0, atol=1e-6)
+    assert np.isfinite(loss)
+    a.close()
+    b.close()
