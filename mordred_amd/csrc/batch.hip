@@ -402,6 +402,9 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     const void* const dummy = D;
     unsigned long long mv[2][IPT];  // two dimensions in flight
     uint32_t aw[2][IPT];            // (16-byte mask entries: their attribute word)
+    // the entry's bit offset in the loaded 8 bytes (dictionary and attribute
+    // forms), one byte per item
+    uint32_t shp[2][(IPT + 3) / 4];
     // the slot an item reads: its own, or slot len (zero) when no query
     // needs it or its key has no slot
     auto slot_of = [&](int d, int j, int len, int mk, unsigned long long uses) {
@@ -438,9 +441,12 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       const void* tabp = uniform_ptr(!used ? dummy : form == kDimMask ? static_cast<const void*>(Hd.mk) : Hd.pk);
       uint32_t b[IPT];
 #pragma unroll
+      for (int w = 0; w < (IPT + 3) / 4; w++) shp[d & 1][w] = 0u;
+#pragma unroll
       for (int j = 0; j < IPT; j++) {
-        unsigned sh;
+        unsigned sh = 0;
         b[j] = wide ? slot_of(d, j, len, mk, uses) * 16u : entry_at(form, stride, slot_of(d, j, len, mk, uses), sh);
+        shp[d & 1][j >> 2] |= sh << (8 * (j & 3));
       }
       if (wide) {
 #pragma unroll
@@ -473,18 +479,15 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
         }
         return;
       }
-      // the entry's offset, from the slot recomputed with the current alive
-      // masks: an item that lost every query using the dimension since its
-      // load may read another slot's bits, which no query then uses
-      const int len = __builtin_amdgcn_readfirstlane(Hd.len);
-      const int mk = __builtin_amdgcn_readfirstlane(Hd.minkey);
+      // the entry's bit offset as issued (an item that has lost every query
+      // using the dimension since then reads bits no query uses)
+      auto sh_of = [&](int j) { return (shp[d & 1][j >> 2] >> (8 * (j & 3))) & 0xFFu; };
       if (form == kDimDict) {
         const unsigned cb = (unsigned)__builtin_amdgcn_readfirstlane(Hd.cb);
         const uint4* dict = reinterpret_cast<const uint4*>(ltab + __builtin_amdgcn_readfirstlane(Hd.dict));
 #pragma unroll
         for (int j = 0; j < IPT; j++) {
-          unsigned sh;
-          (void)entry_at(kDimDict, cb, slot_of(d, j, len, mk, uses), sh);
+          const unsigned sh = sh_of(j);
           const uint32_t code = (uint32_t)(m[j] >> sh) & ((1u << cb) - 1u);
           const uint4 t = dict[code];
           alive[j] &= ((unsigned long long)t.y << 32 | t.x) | ~uses;
@@ -495,11 +498,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       const unsigned pw = (unsigned)__builtin_amdgcn_readfirstlane(Hd.pw);
       const uint32_t wm = pw >= 4 ? 0xFFFFFFFFu : (1u << (8 * pw)) - 1u;
 #pragma unroll
-      for (int j = 0; j < IPT; j++) {
-        unsigned sh;
-        (void)entry_at(kDimAttr, pw, slot_of(d, j, len, mk, uses), sh);
-        attrw[d][j] = (uint32_t)(m[j] >> sh) & wm;
-      }
+      for (int j = 0; j < IPT; j++) attrw[d][j] = (uint32_t)(m[j] >> sh_of(j)) & wm;
       const int na = __builtin_amdgcn_readfirstlane(Hd.nattr);
       for (int a = 0; a < na; a++) {
         const int lut = __builtin_amdgcn_readfirstlane(Hd.lut[a]);
