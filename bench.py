@@ -516,10 +516,19 @@ def serve_headline(args):
     co, ct, cst, c_wall = run(E.SERVE_CONTINUOUS)
     c = serving.latency_summary(ct)
     log(f"served: {c['throughput_qps']:.1f} q/s p50 {c['p50_ms']:.1f} p99 {c['p99_ms']:.1f} ms, {cst}")
+    # the same stream with identical concurrent queries scanned separately
+    # (the stream repeats queries of the mix; sharing their sums is part of the
+    # served form, reported here without it too)
+    eng.set_batch_sharing(False)
+    uo, ut, ust, _ = run(E.SERVE_CONTINUOUS)
+    eng.set_batch_sharing(True)
+    u = serving.latency_summary(ut)
+    log(f"served, no sharing: {u['throughput_qps']:.1f} q/s p50 {u['p50_ms']:.1f} p99 {u['p99_ms']:.1f} ms")
     po, pt, pst, p_wall = run(E.SERVE_PLAIN)
     p = serving.latency_summary(pt)
     log(f"plain: {p['throughput_qps']:.1f} q/s p50 {p['p50_ms']:.1f} p99 {p['p99_ms']:.1f} ms")
     identical = all(bool((a == b).all()) for a, b in zip(po, co))
+    identical_unshared = all(bool((a == b).all()) for a, b in zip(po, uo))
     scan_ms = cst["gpu_ms"] / max(1, cst["launches"])
     achieved = cst["alg_bytes"] / (cst["gpu_ms"] * 1e-3) / 1e9 if cst["gpu_ms"] > 0 else 0.0
     out = {
@@ -547,7 +556,14 @@ def serve_headline(args):
                        max_active=cst["max_active_seen"], lone_requests=cst["lone_requests"],
                        learned_choices=cst["learned_choices"],
                        analytic_choices=cst["analytic_choices"], learned_trusted=cst["learned_trusted"],
-                       learned_error=cst["learned_error"], analytic_error=cst["analytic_error"]),
+                       learned_error=cst["learned_error"], analytic_error=cst["analytic_error"],
+                       shared_query_launches=cst["shared_queries"]),
+        "served_without_sharing": dict(u, launches=ust["launches"], gpu_ms=ust["gpu_ms"],
+                                       identical_to_plain=identical_unshared,
+                                       throughput_vs_plain=u["throughput_qps"] / p["throughput_qps"],
+                                       p99_vs_plain=u["p99_ms"] / p["p99_ms"],
+                                       note="the same stream, identical concurrent queries each scanned "
+                                            "(mordred_engine_set_batch_sharing off)"),
         "plain": dict(p, gpu_ms=pst["gpu_ms"], wall_ms=pst["wall_ms"]),
         "throughput_vs_plain": c["throughput_qps"] / p["throughput_qps"],
         "p50_vs_plain": c["p50_ms"] / p["p50_ms"],

@@ -193,6 +193,15 @@ enum {
   MORDRED_BATCH_FORM_ALL = 7
 };
 int mordred_engine_set_batch_forms(mordred_engine* e, int forms);
+/* Identical queries in one shared-scan batch or launch (default on): when two
+ * active queries are the same query (every field equal) and the first gets
+ * LDS accumulators, the second does not scan; the rows of the launch count
+ * alike for both, so the first's sums, touched groups and keys are added into
+ * the second's result rows as well at the end of the launch.  Each query
+ * keeps its own result rows and coverage, so the answers are unchanged; a
+ * request stream that repeats queries scans each repeated query once per
+ * launch.  No counterpart in the reference. */
+int mordred_engine_set_batch_sharing(mordred_engine* e, int on);
 /* COST_SPLIT prefix streaming (default on): the GPU side of the split may
  * stream only the columns of a subset of the fact filters and probes of a
  * non-resident segment, run those operators, and hand the survivors to the
@@ -286,6 +295,7 @@ typedef struct mordred_batch_stats {
   double alg_bytes;          /* distinct fact columns read x 4 B x rows scanned */
   int lds32_queries;         /* of lds_queries: 32-bit LDS sums (a returning add per row) */
   int keys_in_loop_queries;  /* keys stored by the scan (a returning add per row) */
+  int shared_queries;        /* identical to another query of the batch: took its sums */
 } mordred_batch_stats;
 
 typedef struct mordred_batch_tune_entry {
@@ -369,6 +379,8 @@ typedef struct mordred_serve_stats {
                                      fact columns read x 4 B x rows scanned */
   long long lone_requests;        /* continuous: requests run alone (lone_plain); launches,
                                      gpu_ms and alg_bytes count the shared-scan launches only */
+  long long shared_queries;       /* continuous: sum over launches of the active queries that
+                                     took an identical query's sums (mordred_engine_set_batch_sharing) */
 } mordred_serve_stats;
 
 int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n, const double* arrival_ms,

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void batch_records_kernel(BatchLaunch L, int d
 // minv[d] + digit (0 for a dimension without group value).  blockIdx.y = query.
 __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchDesc* __restrict__ D) {
   const int q = blockIdx.y;
-  if (q >= D->nq || !(D->all >> q & 1ull)) return;
+  if (q >= D->nq || !((D->all | D->aliases) >> q & 1ull)) return;
   const BatchQuery& Q = D->q[q];
   if (Q.flags & (kQueryAggr | kQueryKeysInLoop)) return;
   const BatchKeys K = D->keys[q];
@@ -690,9 +690,18 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
             const unsigned old = __hip_atomic_fetch_add(reinterpret_cast<unsigned*>(acc) + h, u, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
             const int delta = (int)(old + u < old) - (int)(val < 0);
-            if (delta)
-              (void)__hip_atomic_fetch_add(gsum, (unsigned long long)(long long)delta << 32, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+            if (delta) {
+              const unsigned long long carry = (unsigned long long)(long long)delta << 32;
+              (void)__hip_atomic_fetch_add(gsum, carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              // identical queries sharing these sums take the carry too (their
+              // low words come from the same LDS sums at the flush)
+              for (int al = (int)lqc[3 * kBatchMaxQ + q].w; al >= 0; al = (int)lqc[3 * kBatchMaxQ + al].w) {
+                const uint4 a0 = lqc[al];
+                auto* const asum = (__attribute__((address_space(1))) unsigned long long*)(
+                    reinterpret_cast<int*>((unsigned long long)a0.y << 32 | a0.x) + 6 * (size_t)h + 4);
+                (void)__hip_atomic_fetch_add(asum, carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              }
+            }
           } else {
             (void)__hip_atomic_fetch_add(gsum, (unsigned long long)(long long)val, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
@@ -771,10 +780,18 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     const unsigned* tb = reinterpret_cast<const unsigned*>(lds_raw) + (tw ? Q.touch : 0);
     const unsigned* acc = reinterpret_cast<const unsigned*>(lds_raw + Q.acc);
     auto* res = reinterpret_cast<unsigned long long*>(Q.res);
+    // (identical queries chained from Q.alias take the same sums and touched
+    // groups: Q's LDS sums count this launch's rows, which are theirs too)
     auto add = [&](int h) {
       const unsigned long long s = sink == kSinkLds64 ? reinterpret_cast<const unsigned long long*>(acc)[h]
                                                       : (unsigned long long)acc[h];
-      if (s) atomicAdd(res + 3 * (size_t)h + 2, s);
+      if (!s) return;
+      atomicAdd(res + 3 * (size_t)h + 2, s);
+      for (int a = Q.alias; a >= 0;) {
+        const BatchQuery A = query(a, true);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.res) + 3 * (size_t)h + 2, s);
+        a = A.alias;
+      }
     };
     if (tw == 0) {
       if (sink != kSinkGlobal)
@@ -785,6 +802,11 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
       unsigned bits = tb[w];
       if (!bits) continue;
       (void)__hip_atomic_fetch_or(touch_arena + Q.gtouch + w, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int a = Q.alias; a >= 0;) {
+        const BatchQuery A = query(a, true);
+        (void)__hip_atomic_fetch_or(touch_arena + A.gtouch + w, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a = A.alias;
+      }
       if (sink == kSinkGlobal) continue;
       while (bits) {
         const int b = __builtin_ctz(bits);

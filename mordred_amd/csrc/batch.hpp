@@ -42,6 +42,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace mordred {
@@ -108,10 +109,13 @@ struct BatchQuery {            // 64 bytes; staged in LDS by the scan kernel
   signed char attr[4];         // per dimension slot: -1 no group value, else its attribute
   int gtouch;                  // word offset of its touched bitmap in BatchDesc::touch_arena
   int flags;                   // BatchQueryFlags
-  int unused;
+  // the next query of the batch that is identical to this one and takes this
+  // one's LDS sums, touched groups and keys instead of scanning (-1: none)
+  int alias;
 };
 constexpr unsigned short kBatchNoTouch = 0xFFFF;
 static_assert(sizeof(BatchQuery) == 64, "BatchQuery is staged as 4 x 16 bytes");
+static_assert(offsetof(BatchQuery, alias) == 60, "the scan reads alias as the last word of chunk 3");
 
 // Group keys of a query whose keys decode from the group index (a mixed
 // radix: digit d = value - minv[d] in [0, krange[d]) for uniq[d] != 0, key
@@ -212,6 +216,7 @@ struct BatchDesc {
   int rebuild;                 // dimension tables must be rebuilt (query set changed)
   int nivl;
   unsigned long long reset;    // queries whose result rows are cleared first
+  unsigned long long aliases;  // queries that take an identical query's sums (not in `all`)
   int lds_words;               // 32-bit words of the LDS group region
   int touch_words;             // words of the touch arena
   long long res_ints;          // result arena ints

@@ -212,6 +212,7 @@ struct mordred_engine {
   DevBuf d_rec[4], d_mk[4], d_desc, d_arena, d_status, d_touch;
   HostBuf h_desc, h_arena;
   int batch_forms = MORDRED_BATCH_FORM_ALL;     // dimension table forms batches may use
+  bool batch_share = true;                      // identical queries of a batch share sums
   std::vector<std::unique_ptr<DimDict>> dicts;  // dictionary-form code tables
   unsigned long long dict_clock = 0;
   hipGraph_t graph = nullptr;
@@ -2071,6 +2072,7 @@ struct BatchPlan {
   std::vector<long long> res_off;  // ints
   std::vector<int> G;
   int lds_queries = 0, global_queries = 0, dead = 0, lds32_queries = 0, keys_in_loop = 0;
+  int shared = 0;  // identical queries that take another's sums
   int cols_read = 0;  // distinct fact columns the scan reads (algorithmic bytes: 4 per row each)
 };
 
@@ -2298,6 +2300,28 @@ bool keys_decode(const mordred_star_query& q, const Resolved& r, mordred::BatchK
 
 // forms: the dimension table forms allowed (MORDRED_BATCH_FORM_*).
 // lds_short is set when the plan fails because its LDS tables do not fit.
+// Whether two star queries are the same query (every field; column names by
+// value).
+bool same_name(const char* a, const char* b) { return a == b || (a && b && std::strcmp(a, b) == 0); }
+bool same_query(const mordred_star_query& a, const mordred_star_query& b) {
+  if (a.njoins != b.njoins || a.nfilters != b.nfilters || a.kind != b.kind || a.total_val != b.total_val ||
+      !same_name(a.aggr1, b.aggr1) || !same_name(a.aggr2, b.aggr2))
+    return false;
+  for (int k = 0; k < a.njoins; k++) {
+    const mordred_join &x = a.joins[k], &y = b.joins[k];
+    if (x.slot != y.slot || x.dim_len != y.dim_len || x.min_key != y.min_key || x.filter_mode != y.filter_mode ||
+        x.c1 != y.c1 || x.c2 != y.c2 || x.min_val != y.min_val || x.unique_val != y.unique_val ||
+        !same_name(x.fact_key, y.fact_key) || !same_name(x.dim_key, y.dim_key) ||
+        !same_name(x.filter_col, y.filter_col) || !same_name(x.group_col, y.group_col))
+      return false;
+  }
+  for (int f = 0; f < a.nfilters; f++)
+    if (a.filters[f].lo != b.filters[f].lo || a.filters[f].hi != b.filters[f].hi ||
+        !same_name(a.filters[f].col, b.filters[f].col))
+      return false;
+  return true;
+}
+
 int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int nq, const mordred::BatchConfig& cfg,
                      BatchPlan& b, const SlotArenas* fixed, const double* sel, int forms, bool& lds_short) {
   using namespace mordred;
@@ -2770,10 +2794,25 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
   // whose atomics all land on a few addresses and serialize there.  An
   // estimate below 1e-3 counts as 1e-3 (estimates of small selectivities are
   // the least reliable, and a small group array costs little LDS).
+  // identical live queries (a request stream repeats queries): the first
+  // is the primary; the others can share its LDS sums (below), so the
+  // primary's sums are worth as much LDS as all of theirs
+  std::vector<int> primary(nq, -1), copies(nq, 1);
+  if (e->batch_share)
+    for (int i = 0; i < nq; i++) {
+      if (!qp[i] || !(D.all >> i & 1ull) || (D.q[i].flags & kQueryKeysInLoop)) continue;
+      for (int j = 0; j < i; j++)
+        if (primary[j] < 0 && qp[j] && (D.all >> j & 1ull) && !(D.q[j].flags & kQueryKeysInLoop) &&
+            same_query(*qp[i], *qp[j])) {
+          primary[i] = j;
+          copies[j]++;
+          break;
+        }
+    }
   std::vector<double> prio(nq, 0.0);
   for (int i = 0; i < nq; i++) {
     const double g = std::max(1, b.G[i]);
-    prio[i] = std::max(est_sel[i], 1e-3) / g * (1.0 + 64.0 / g);
+    prio[i] = primary[i] >= 0 ? 0.0 : copies[i] * std::max(est_sel[i], 1e-3) / g * (1.0 + 64.0 / g);
   }
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return prio[x] > prio[y]; });
   // touched bitmaps first; if they do not fit, the largest queries store
@@ -2804,6 +2843,22 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
       avail -= w;
     }
   }
+  // identical queries whose primary got LDS accumulators take its sums
+  // instead of scanning: the rows of one launch count alike for both
+  D.aliases = 0;
+  for (int i = 0; i < nq; i++) D.q[i].alias = -1;
+  for (int i = 0; i < nq; i++) {
+    const int p = primary[i];
+    if (p < 0 || sink[p] == kSinkGlobal || ((D.q[p].flags | D.q[i].flags) & kQueryKeysInLoop)) continue;
+    D.all &= ~(1ull << i);
+    D.aliases |= 1ull << i;
+    sink[i] = kSinkGlobal;  // (no accumulators of its own)
+    // append to the primary's chain
+    int t = p;
+    while (D.q[t].alias >= 0) t = D.q[t].alias;
+    D.q[t].alias = i;
+    b.shared++;
+  }
   for (int i : order) {
     if (sink[i] != kSinkLds32) continue;
     const long long extra = even(batch_acc_words(kSinkLds64, b.G[i])) - even(batch_acc_words(kSinkLds32, b.G[i]));
@@ -2826,8 +2881,13 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
     if (tw) Q.touch = (unsigned short)(grp0 + lds_words);
     Q.acc = (int)(4 * (grp0 + lds_words + tw));
     lds_words += tw + even(batch_acc_words(sink[i], b.G[i]));
-    if (sink[i] == kSinkGlobal) b.global_queries++;
-    else b.lds_queries++;
+    if (D.aliases >> i & 1ull) {
+      // (its touched bitmap and accumulators stay unused: the primary's serve it)
+    } else if (sink[i] == kSinkGlobal) {
+      b.global_queries++;
+    } else {
+      b.lds_queries++;
+    }
     b.lds32_queries += sink[i] == kSinkLds32;
     b.keys_in_loop += (Q.flags & kQueryKeysInLoop) != 0;
     if (fixed && 6 * (long long)b.G[i] > fixed->res_stride)
@@ -2982,6 +3042,7 @@ int run_batch(mordred_engine* e, const BatchPlan& b, const mordred::BatchConfig&
     st->alg_bytes = 4.0 * b.cols_read * (double)b.d.num_tuples;
     st->lds32_queries = b.lds32_queries;
     st->keys_in_loop_queries = b.keys_in_loop;
+    st->shared_queries = b.shared;
   }
   return 0;
 }
@@ -3270,6 +3331,12 @@ int mordred_engine_segment_minmax(mordred_engine* e, const char* name, int* mins
 int mordred_engine_set_skipping(mordred_engine* e, int on) {
   if (!e) return -1;
   e->skipping = on != 0;
+  return 0;
+}
+
+int mordred_engine_set_batch_sharing(mordred_engine* e, int on) {
+  if (!e) return -1;
+  e->batch_share = on != 0;
   return 0;
 }
 
@@ -4002,6 +4069,7 @@ int mordred_engine_serve(mordred_engine* e, const mordred_star_query* qs, int n,
     st.gpu_ms += gms;
     st.alg_bytes += 4.0 * L.b.cols_read * (double)L.b.d.num_tuples;
     st.launches++;
+    st.shared_queries += L.b.shared;
     // learn from the measured launch
     if (e->cm) {
       launch_features(L.b, L.es, L.b.d.num_tuples, L.nreset, cfg, feat);

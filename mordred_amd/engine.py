@@ -75,7 +75,7 @@ class BatchStats(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("gpu_ms", C.c_double), ("lds_queries", C.c_int),
                 ("global_queries", C.c_int), ("dead_queries", C.c_int), ("graph_captured", C.c_int),
                 ("config", BatchConfig), ("alg_bytes", C.c_double), ("lds32_queries", C.c_int),
-                ("keys_in_loop_queries", C.c_int)]
+                ("keys_in_loop_queries", C.c_int), ("shared_queries", C.c_int)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "config"}
@@ -105,7 +105,8 @@ class ServeStats(C.Structure):
     _fields_ = [("wall_ms", C.c_double), ("launches", C.c_longlong), ("gpu_ms", C.c_double),
                 ("max_active_seen", C.c_int), ("learned_choices", C.c_longlong), ("analytic_choices", C.c_longlong),
                 ("learned_error", C.c_double), ("analytic_error", C.c_double), ("learned_trusted", C.c_int),
-                ("peak_state_bytes", C.c_double), ("alg_bytes", C.c_double), ("lone_requests", C.c_longlong)]
+                ("peak_state_bytes", C.c_double), ("alg_bytes", C.c_double), ("lone_requests", C.c_longlong),
+                ("shared_queries", C.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -148,6 +149,7 @@ _SIGS = {
     "mordred_engine_segment_minmax": ([C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p], C.c_int),
     "mordred_engine_set_skipping": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_set_batch_forms": ([C.c_void_p, C.c_int], C.c_int),
+    "mordred_engine_set_batch_sharing": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_set_prefix_streaming": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_set_operator_order": ([C.c_void_p, C.c_int], C.c_int),
     "mordred_engine_struct_size": ([C.c_char_p], C.c_longlong),
@@ -288,6 +290,9 @@ class Engine:
 
     def set_batch_forms(self, forms: int):
         self._check(self.lib.mordred_engine_set_batch_forms(self.h, int(forms)), "set_batch_forms")
+
+    def set_batch_sharing(self, on: bool):
+        self._check(self.lib.mordred_engine_set_batch_sharing(self.h, int(bool(on))), "set_batch_sharing")
 
     def set_prefix_streaming(self, on: bool):
         self._check(self.lib.mordred_engine_set_prefix_streaming(self.h, int(on)), "set_prefix_streaming")

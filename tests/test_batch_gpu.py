@@ -138,6 +138,26 @@ def test_wide_attribute_words(bengine):
     check(batch[1:], outs[1:], refs)
 
 
+def test_identical_queries_share_sums(bengine):
+    """Repeated queries in one batch: every copy after the first takes the
+    first's sums (when those sit in LDS) instead of scanning; every copy's
+    result must still equal the oracle, with sharing on and off."""
+    qs, _, _, eng, refs = bengine
+    big = max(qs, key=lambda q: q.total_val)
+    batch = [qs[3], big, qs[5], qs[3], qs[7], big, qs[3], qs[5]] + qs[10:20]
+    print(f"[gpu] batch with repeated queries (largest {big.name}, {big.total_val} groups)", flush=True)
+    try:
+        outs, st = eng.batch(batch, (1024, 1, 160 << 10))
+        check(batch, outs, refs)
+        assert st["shared_queries"] >= 3
+        eng.set_batch_sharing(False)
+        outs2, st2 = eng.batch(batch, (1024, 1, 160 << 10))
+        assert st2["shared_queries"] == 0
+        check(batch, outs2, refs)
+    finally:
+        eng.set_batch_sharing(True)
+
+
 def test_dead_query_and_rejections(bengine):
     qs, _, _, eng, refs = bengine
     q = dataclasses.replace(qs[0], fact_filters=[("lo_revenue", 900_000, 100_000)])
