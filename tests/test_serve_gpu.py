@@ -118,3 +118,25 @@ def test_lone_requests_run_alone_with_the_same_answers(sengine):
     outs, times, st = eng.serve(reqs[:5], arr[:5], E.SERVE_CONTINUOUS, chunk_segments=2)
     check(reqs[:5], outs, refs)
     assert st["lone_requests"] == 0
+
+
+@pytest.mark.parametrize("share", [True, False])
+def test_repeated_queries_share_and_stay_exact(sengine, share):
+    """A stream of four distinct queries, each repeated many times, arriving
+    while the loop runs: identical active queries share their per-launch sums
+    (one scans, the copies take its sums), with different starting segments;
+    every answer must equal the oracle, with sharing on and off."""
+    qs, data, eng, refs = sengine
+    big = max(qs, key=lambda q: q.total_val)
+    four = [big, qs[1], qs[2], qs[3]]
+    reqs = [four[i % 4] for i in range(120)]
+    arr = serving.poisson_arrivals(30_000.0, len(reqs), seed=3)
+    print(f"[gpu] serve repeated queries, sharing {share}", flush=True)
+    try:
+        eng.set_batch_sharing(share)
+        outs, times, st = eng.serve(reqs, arr, E.SERVE_CONTINUOUS, chunk_segments=2)
+    finally:
+        eng.set_batch_sharing(True)
+    check(reqs, outs, refs)
+    assert st["max_active_seen"] > 4
+    assert (st["shared_queries"] > 0) == share
