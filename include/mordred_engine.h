@@ -126,6 +126,10 @@ typedef struct mordred_run_stats {
   int order_switches;                /* device switches of ordered mixed pipelines (all chunks) */
   double cpu_build_ms;               /* the CPU hash-table builds (0 when none ran) */
   double est_cpu_build_ms;           /* their estimate in the split model (0 before any was measured) */
+  /* bytes of the prefix columns that crossed PCIe bit-packed (values - column
+   * minimum in the fewest bits, up to 24; unpacked on the device); a wider
+   * column crosses as 4-byte values and adds nothing here */
+  long long prefix_packed_bytes;
 } mordred_run_stats;
 
 /* Engine on HIP device `device` with an HBM segment cache of cache_segments

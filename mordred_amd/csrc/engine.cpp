@@ -37,6 +37,10 @@
 #include "../../include/mordred_engine.h"
 #include "../../include/mordred_hip.h"
 #include "batch.hpp"
+#include "stream_pack.hpp"
+
+using mordred::kPackMaxBits;
+using mordred::packed_words;
 
 namespace mordred_host {
 
@@ -45,6 +49,8 @@ constexpr int kSeg = 1 << kSegLog;
 constexpr int kCpuChunk = 1 << 16;   // rows per CPU task
 constexpr int kStreamBatch = 4;      // segments per streamed batch
 constexpr int kMaxPrefixBatch = 16;  // segments per prefix-streamed batch, at most
+static_assert(kMaxPrefixBatch <= mordred::kPackMaxSegs && kSegLog == mordred::kPackSegLog,
+              "a prefix batch is unpacked in one launch of segments of the engine's size");
 constexpr int kMixedChunk = 64;      // segments per GPU call of a mixed pipeline
 constexpr int kSuffixTask = 8192;    // survivors per CPU task of a mixed pipeline
 constexpr double kCacheLine = 64.0;  // CostModel.h:4
@@ -128,6 +134,11 @@ struct Column {
   std::vector<int> smin, smax;
   std::vector<int> emin, emax;  // exact per-segment min / max of the values
   int clustered = -1;           // -1 unknown, else 1 if segments cover narrow ranges
+  // the packed PCIe copy of prefix-streamed segments (stream_pack.hpp):
+  // pinned, segment s at packed + s * packed_words(pbits), made on first use
+  uint32_t* packed = nullptr;
+  int pbits = 0;
+  std::vector<char> packed_seg;
 };
 
 // Device buffer that only grows.
@@ -207,6 +218,7 @@ struct mordred_engine {
   hipEvent_t ev_built = nullptr;        // recorded after a run's hash-table builds
   std::unordered_map<long long, double> seg_weight;  // (column id << 32 | segment) -> weight
   DevBuf d_res, d_sg, d_ht[4], d_mix, d_pmix, d_ord, d_res_ord;
+  DevBuf d_pack[2];  // packed prefix segments per staging set (stream_pack.hpp)
   HostBuf h_stage;  // pinned scratch for segment groups / index updates
   // shared-scan batches: record tables, descriptor, result arena, the graph
   DevBuf d_rec[4], d_mk[4], d_desc, d_arena, d_status, d_touch;
@@ -294,9 +306,12 @@ void free_column(mordred_engine* e, Column* c) {
   } else {
     if (c->host) (void)hipHostFree(c->host);
     if (c->d_idx) (void)hipFree(c->d_idx);
+    if (c->packed) (void)hipHostFree(c->packed);
   }
   c->host = nullptr;
   c->d_idx = nullptr;
+  c->packed = nullptr;
+  c->packed_seg.clear();
 }
 
 // Fill a column's host store segment by segment on the worker pool, from a
@@ -699,22 +714,82 @@ struct MixedCost {
   double gpu = 0, cpu = 0;
   double total() const { return gpu + cpu; }
 };
+void column_range(Column* c);
+// Bits a value of the column takes in the packed PCIe stream (values - vmin,
+// stream_pack.hpp), or 0 when it crosses unpacked.
+int stream_bits(Column* c) {
+  column_range(c);
+  const long long span = (long long)c->vmax - c->vmin;
+  int b = 1;
+  while (b < 32 && (1ll << b) <= span) b++;
+  return b <= kPackMaxBits ? b : 0;
+}
+double stream_value_bytes(const mordred_engine* e, Column* c) {
+  const int b = e->device >= 0 ? stream_bits(c) : 0;
+  return b ? b / 8.0 : 4.0;
+}
+
+// The packed PCIe copy of segments `segs` of column c, made once per segment
+// on the engine's workers.  The column crosses unpacked (c->pbits stays 0)
+// when its values need more than kPackMaxBits bits or no pinned buffer can be
+// had.
+void pack_segments(mordred_engine* e, Column* c, const std::vector<int>& segs) {
+  if (e->device < 0 || segs.empty()) return;
+  const int bits = stream_bits(c);
+  if (!bits) return;
+  const long long pw = packed_words(bits);
+  if (!c->packed) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, (size_t)c->nseg * (size_t)pw * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+    c->packed = static_cast<uint32_t*>(p);
+    c->pbits = bits;
+    c->packed_seg.assign(c->nseg, 0);
+  }
+  std::vector<int> todo;
+  for (int seg : segs)
+    if (!c->packed_seg[seg]) todo.push_back(seg);
+  e->pool->run((int)todo.size(), [&](int t, int) {
+    const int seg = todo[t];
+    uint32_t* w = c->packed + (size_t)seg * (size_t)pw;
+    const long long n = std::min<long long>(kSeg, c->n - (long long)seg * kSeg);
+    const int* v = c->host + (size_t)seg * kSeg;
+    unsigned long long acc = 0;
+    int have = 0;
+    long long o = 0;
+    for (long long j = 0; j < n; j++) {
+      acc |= (unsigned long long)(uint32_t)(v[j] - c->vmin) << have;
+      have += bits;
+      while (have >= 32) {
+        w[o++] = (uint32_t)acc;
+        acc >>= 32;
+        have -= 32;
+      }
+    }
+    if (have > 0) w[o++] = (uint32_t)acc;
+    while (o < pw) w[o++] = 0u;
+  });
+  for (int seg : todo) c->packed_seg[seg] = 1;
+}
+
 MixedCost mixed_cost(const mordred_engine* e, const Resolved& r, const Selectivity& sel, const MixedGroup& m,
                      bool prefix) {
-  double surv = 1;
-  int outs = 1, cols = 0;
+  double surv = 1, col_bytes = 0;  // (the prefix columns' bytes per row over PCIe)
+  int outs = 1;
   for (int i = 0; i < r.q->nfilters; i++)
-    if (m.gpu_filter[i]) surv *= sel.filt[i], cols++;
+    if (m.gpu_filter[i]) surv *= sel.filt[i], col_bytes += stream_value_bytes(e, r.filt[i]);
   for (int k = 0; k < 4; k++) {
     if (!r.join[k] || !m.gpu_probe[k]) continue;
     surv *= sel.join[k];
-    cols++;
+    col_bytes += stream_value_bytes(e, r.key[k]);
     outs += r.dval[k] != nullptr;
   }
   const double L = (double)kSeg * surv;
   const double per_line = e->suffix_ns_per_line > 0 ? e->suffix_ns_per_line * 1e-9 : kCacheLine / e->bw_cpu;
   MixedCost c;
-  c.gpu = (L * 4.0 * outs + (prefix ? (double)kSeg * 4.0 * cols : 0.0)) / e->bw_pci * 1e3;
+  c.gpu = (L * 4.0 * outs + (prefix ? (double)kSeg * col_bytes : 0.0)) / e->bw_pci * 1e3;
   c.cpu = L * mixed_lines(r, m) * per_line * 1e3;
   return c;
 }
@@ -1297,6 +1372,7 @@ struct GpuPart {
   long long gpu_rows = 0, stream_rows = 0;
   int gpu_segments = 0, stream_segments = 0, prefix_segments = 0;
   long long prefix_rows = 0;
+  long long packed_bytes = 0;  // prefix columns' bytes that crossed PCIe packed
   // the hash tables are built (e->ev_built recorded): 0, or -1 if the GPU
   // side failed first
   std::promise<int> built;
@@ -1596,6 +1672,15 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
         ptotal[set] = base + 6 * pcap;
       }
     }
+    // device area of each staging set for the packed prefix columns: per
+    // prefix column pbatch segments of the widest packed segment
+    long long pack_stride = 0;
+    for (size_t c = 0; c < nc; c++)
+      if (pcopy[c] && r.fact_cols[c]->pbits > 0)
+        pack_stride = std::max(pack_stride, (long long)pbatch * packed_words(r.fact_cols[c]->pbits));
+    if (pack_stride > 0)
+      for (int set = 0; set < 2; set++)
+        HIP_OK(e->d_pack[set].reserve((size_t)npc * (size_t)pack_stride * sizeof(uint32_t)));
     char* hb = hs + sg_bytes + tab_bytes;
     char* db = ds + sg_bytes + tab_bytes;
     long long psurv = 0;
@@ -1613,9 +1698,38 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
         rows += segment_rows(r, seg);
       }
       // one copy per run of segments that are consecutive both in the host
-      // column and in the staging slots
+      // column and in the staging slots; a prefix batch's packed columns
+      // cross packed and are unpacked into the staging slots
       for (size_t c = 0; c < nc; c++) {
         if (B.prefix && !pcopy[c]) continue;
+        Column* const col = r.fact_cols[c];
+        bool packed = B.prefix && col->pbits > 0;
+        for (size_t i = 0; packed && i < B.cnt; i++) packed = col->packed_seg[all[B.first + i]] != 0;
+        if (packed) {
+          const long long pw = packed_words(col->pbits);
+          uint32_t* dpk = static_cast<uint32_t*>(e->d_pack[set].p) + (size_t)pidx[c] * (size_t)pack_stride;
+          size_t i0 = 0;
+          while (i0 < B.cnt) {
+            size_t i1 = i0 + 1;
+            while (i1 < B.cnt && all[B.first + i1] == all[B.first + i1 - 1] + 1) i1++;
+            HIP_OK(hipMemcpyAsync(dpk + i0 * (size_t)pw, col->packed + (size_t)all[B.first + i0] * (size_t)pw,
+                                  (i1 - i0) * (size_t)pw * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+            i0 = i1;
+          }
+          mordred::UnpackBatch u{};
+          u.packed = dpk;
+          u.cache = e->cache;
+          u.bits = col->pbits;
+          u.base = col->vmin;
+          u.nseg = (int)B.cnt;
+          for (size_t i = 0; i < B.cnt; i++) {
+            u.slot[i] = slot_of(bi, i, c);
+            u.rows[i] = (int)segment_rows(r, all[B.first + i]);
+          }
+          HIP_OK(mordred::unpack_segments(u, s));
+          g.packed_bytes += (long long)B.cnt * pw * (long long)sizeof(uint32_t);
+          continue;
+        }
         size_t i0 = 0;
         while (i0 < B.cnt) {
           size_t i1 = i0 + 1;
@@ -3703,6 +3817,14 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   std::sort(cpu_segs.begin(), cpu_segs.end());
   const bool ordered = order != MORDRED_ORDER_GPU_FIRST && !mixed.empty();
   std::vector<MixedGroup> no_mixed;
+  // the prefix-streamed columns cross PCIe packed: their segments are packed
+  // once, here, before the GPU side (which waits for this plan) streams them
+  if (!pstream_segs.empty()) {
+    for (int i = 0; i < r.q->nfilters; i++)
+      if (pgroup.gpu_filter[i]) pack_segments(e, r.filt[i], pstream_segs);
+    for (int k = 0; k < 4; k++)
+      if (r.join[k] && pgroup.gpu_probe[k]) pack_segments(e, r.key[k], pstream_segs);
+  }
   streamed.set_value(SplitPlan{stream_segs, ordered ? &no_mixed : &mixed, pstream_segs, &pgroup});
 
   // ---- CPU hash tables, for every CPU operator of the run
@@ -3786,6 +3908,7 @@ int mordred_engine_run(mordred_engine* e, const mordred_star_query* q, int place
   st.stream_segments = g.stream_segments;
   st.prefix_segments = g.prefix_segments;
   st.prefix_survivors = pgroup.survivors;
+  st.prefix_packed_bytes = g.packed_bytes;
   st.prefix_rows = g.prefix_rows;
   if (!ord_res.empty()) {
     if (g.res.empty()) g.res.swap(ord_res);

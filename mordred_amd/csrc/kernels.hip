@@ -33,6 +33,7 @@
 
 #define MORDRED_DEFINE_PREP_KERNELS
 #include "pipeline.hpp"
+#include "stream_pack.hpp"
 
 namespace mordred {
 
@@ -546,6 +547,32 @@ static int run_build(const Build& B, hipStream_t stream) {
   const int ntiles = (B.num_tuples + TILE - 1) / TILE;
   const int grid = balanced_grid(ntiles, resident_groups(f, kBuildBlock, 0));
   build_kernel<kBuildBlock, kBuildVpt, MODE><<<grid, kBuildBlock, 0, stream>>>(B);
+  return hipGetLastError();
+}
+
+// Bit-packed segments into their staging slots: one thread per value; a
+// field spans at most two words (bits <= 24), the second always in bounds
+// (each packed segment keeps a slack word).
+__global__ __launch_bounds__(256) void unpack_kernel(UnpackBatch u) {
+  const int i = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= u.nseg || j >= u.rows[i]) return;
+  const uint32_t* w = u.packed + (size_t)i * (size_t)packed_words(u.bits);
+  const unsigned long long bit = (unsigned long long)j * (unsigned)u.bits;
+  const unsigned long long pair = (unsigned long long)w[bit >> 5] | (unsigned long long)w[(bit >> 5) + 1] << 32;
+  const uint32_t v = (uint32_t)(pair >> (bit & 31)) & ((1u << u.bits) - 1u);
+  u.cache[((size_t)(unsigned)u.slot[i] << kPackSegLog) + j] = (int)((uint32_t)u.base + v);
+}
+
+hipError_t unpack_segments(const UnpackBatch& u, hipStream_t s) {
+  if (u.nseg <= 0 || u.nseg > kPackMaxSegs || u.bits <= 0 || u.bits > kPackMaxBits) return hipErrorInvalidValue;
+  int rmax = 0;
+  for (int i = 0; i < u.nseg; i++) {
+    if (u.rows[i] < 0 || u.rows[i] > (1 << kPackSegLog) || u.slot[i] < 0) return hipErrorInvalidValue;
+    rmax = std::max(rmax, u.rows[i]);
+  }
+  if (rmax == 0) return hipSuccess;
+  unpack_kernel<<<dim3((rmax + 255) / 256, u.nseg), 256, 0, s>>>(u);
   return hipGetLastError();
 }
 

@@ -46,7 +46,8 @@ class RunStats(C.Structure):
                 ("prefix_rows", C.c_longlong), ("prefix_survivors", C.c_longlong),
                 ("est_prefix_ms_per_segment", C.c_double), ("est_prefix_cpu_ms_per_segment", C.c_double),
                 ("suffix_ms", C.c_double), ("operator_order", C.c_int), ("order_switches", C.c_int),
-                ("cpu_build_ms", C.c_double), ("est_cpu_build_ms", C.c_double)]
+                ("cpu_build_ms", C.c_double), ("est_cpu_build_ms", C.c_double),
+                ("prefix_packed_bytes", C.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

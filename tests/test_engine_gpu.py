@@ -263,11 +263,14 @@ def test_prefix_streaming_cost_split(fresh_engine):
         assert 0 < st["est_prefix_ms_per_segment"] < st["est_stream_ms_per_segment"]
         assert st["prefix_segments"] > 0
         assert 0 < st["prefix_survivors"] < st["prefix_rows"] // 2
+        # the streamed key columns fit in 24 bits: they crossed packed, in
+        # fewer bytes than their 4-byte values
+        assert 0 < st["prefix_packed_bytes"] < 4 * st["prefix_rows"] * len(q.fact_columns())
     eng.set_prefix_streaming(False)
     res, st = eng.run(q, E.PLACE_COST_SPLIT)
     eng.set_prefix_streaming(True)
     np.testing.assert_array_equal(res, ref)
-    assert st["prefix_segments"] == 0
+    assert st["prefix_segments"] == 0 and st["prefix_packed_bytes"] == 0
     assert st["cpu_segments"] + st["stream_segments"] == nseg
 
 
