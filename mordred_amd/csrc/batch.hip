@@ -179,15 +179,27 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   // of query q at lqc[c * kBatchMaxQ + q]): lanes of one wave read the
   // descriptors of different queries, which a 64-byte stride would put in the
   // same banks; the fixed stride makes the chunk offsets immediates
+  // With uniq16 the chunks are repacked so that the pair step reads two:
+  // chunk 0 = total_val, acc, hbase, expr | sink << 8 | touch << 16; chunk 1 =
+  // shift bytes, bits bytes, uniq[0] | uniq[1] << 16, uniq[2] | uniq[3] << 16;
+  // chunk 2 = res; chunk 3 as in BatchQuery.
   uint4* lqc = reinterpret_cast<uint4*>(lds_raw);
+  const bool lq16 = D->uniq16 != 0;
   auto query = [&](int q, bool full) {
     union {
       BatchQuery Q;
       uint4 c[4];
     } u;
-    u.c[0] = lqc[q];
-    u.c[1] = lqc[kBatchMaxQ + q];
-    u.c[2] = lqc[2 * kBatchMaxQ + q];
+    if (lq16) {
+      const uint4 a = lqc[q], b = lqc[kBatchMaxQ + q], r = lqc[2 * kBatchMaxQ + q];
+      u.c[0] = make_uint4(r.x, r.y, a.x, a.y);
+      u.c[1] = make_uint4(a.z, a.w, b.x, b.y);
+      u.c[2] = make_uint4(b.z & 0xFFFFu, b.z >> 16, b.w & 0xFFFFu, b.w >> 16);
+    } else {
+      u.c[0] = lqc[q];
+      u.c[1] = lqc[kBatchMaxQ + q];
+      u.c[2] = lqc[2 * kBatchMaxQ + q];
+    }
     u.c[3] = full ? lqc[3 * kBatchMaxQ + q] : make_uint4(0, 0, 0, 0);
     return u.Q;
   };
@@ -200,7 +212,17 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   const int ngrp = D->lds_words;
   {
     const uint4* src = reinterpret_cast<const uint4*>(D->q);
-    for (int i = threadIdx.x; i < nq * 4; i += BLOCK) lqc[(i & 3) * kBatchMaxQ + (i >> 2)] = src[i];
+    for (int i = threadIdx.x; i < nq * 4; i += BLOCK) {
+      const int c = i & 3;
+      uint4 v = src[i];
+      if (lq16 && c < 3) {
+        const uint4 w0 = src[i - c], w1 = src[i - c + 1], w2 = src[i - c + 2];
+        v = c == 0   ? make_uint4(w0.z, w0.w, w1.x, w1.y)
+            : c == 1 ? make_uint4(w1.z, w1.w, w2.x | w2.y << 16, w2.z | w2.w << 16)
+                     : make_uint4(w0.x, w0.y, 0u, 0u);
+      }
+      lqc[c * kBatchMaxQ + (i >> 2)] = v;
+    }
     for (int i = threadIdx.x; i < ngrp; i += BLOCK) lgrp[i] = 0;
     for (int i = threadIdx.x; i < ntab; i += BLOCK) ltab[i] = D->tab[i];
     const int np = D->npred;
@@ -618,8 +640,14 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
   // (A16: the batch's 16-bit attribute words are packed two per register, a
   // compile-time choice so that no copy of a ds_bpermute result sits between
   // the step's LDS reads)
-  auto pairs = [&](auto A16) {
+  auto pairs = [&](auto A16, auto U16) {
     constexpr bool kA16 = decltype(A16)::value;
+    constexpr bool kU16 = decltype(U16)::value;
+    // the device address of query q's result words
+    auto res_of = [&](int q) {
+      const uint4 r = lqc[(kU16 ? 2 * kBatchMaxQ : 0) + q];
+      return reinterpret_cast<int*>((unsigned long long)r.y << 32 | r.x);
+    };
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
       unsigned long long m = alive[j] & ~keysq;
@@ -658,26 +686,36 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
           const int v2 = nexpr > 2 ? __builtin_amdgcn_ds_bpermute(src, ev[2][j]) : 0;
           const int v3 = nexpr > 3 ? __builtin_amdgcn_ds_bpermute(src, ev[3][j]) : 0;
           if (b + lane >= n) continue;
-          // the descriptor's first three chunks, each one 16-byte LDS read:
-          // c0 = res, total_val, acc; c1 = hbase, expr | sink << 8 | touch <<
-          // 16, shift bytes, bits bytes; c2 = uniq
+          // the descriptor's operands: two 16-byte LDS reads with uniq16
+          // (uniq as 16-bit halves), else three (c0 = res, total_val, acc;
+          // c1 = hbase, expr | sink << 8 | touch << 16, shift bytes, bits
+          // bytes; c2 = uniq)
           const int q = (int)(ent >> 8);
-          const uint4 c0 = lqc[q], c1 = lqc[kBatchMaxQ + q], c2 = lqc[2 * kBatchMaxQ + q];
-          const int total = (int)c0.z;
-          const int h = mod_pos((int)c1.x + (int)__umul24(__builtin_amdgcn_ubfe(a[0], c1.z, c1.w), c2.x) +
-                                    (int)__umul24(__builtin_amdgcn_ubfe(a[1], c1.z >> 8, c1.w >> 8), c2.y) +
-                                    (int)__umul24(__builtin_amdgcn_ubfe(a[2], c1.z >> 16, c1.w >> 16), c2.z) +
-                                    (int)__umul24(__builtin_amdgcn_ubfe(a[3], c1.z >> 24, c1.w >> 24), c2.w),
-                                total);
-          if ((unsigned)h >= (unsigned)total) continue;
-          const unsigned expr = c1.y & 0xFFu, sink = (c1.y >> 8) & 0xFFu, touch = c1.y >> 16;
+          unsigned total, accb, hb, ess, sh, bt, u0, u1, u2, u3;
+          if constexpr (kU16) {
+            const uint4 c0 = lqc[q], c1 = lqc[kBatchMaxQ + q];
+            total = c0.x, accb = c0.y, hb = c0.z, ess = c0.w, sh = c1.x, bt = c1.y;
+            u0 = c1.z & 0xFFFFu, u1 = c1.z >> 16, u2 = c1.w & 0xFFFFu, u3 = c1.w >> 16;
+          } else {
+            const uint4 c0 = lqc[q], c1 = lqc[kBatchMaxQ + q], c2 = lqc[2 * kBatchMaxQ + q];
+            total = c0.z, accb = c0.w, hb = c1.x, ess = c1.y, sh = c1.z, bt = c1.w;
+            u0 = c2.x, u1 = c2.y, u2 = c2.z, u3 = c2.w;
+          }
+          const int h = mod_pos((int)hb + (int)__umul24(__builtin_amdgcn_ubfe(a[0], sh, bt), u0) +
+                                    (int)__umul24(__builtin_amdgcn_ubfe(a[1], sh >> 8, bt >> 8), u1) +
+                                    (int)__umul24(__builtin_amdgcn_ubfe(a[2], sh >> 16, bt >> 16), u2) +
+                                    (int)__umul24(__builtin_amdgcn_ubfe(a[3], sh >> 24, bt >> 24), u3),
+                                (int)total);
+          if ((unsigned)h >= total) continue;
+          const unsigned expr = ess & 0xFFu, sink = (ess >> 8) & 0xFFu, touch = ess >> 16;
           int val = v0;
           val = expr == 1u ? v1 : val;
           val = expr == 2u ? v2 : val;
           val = expr == 3u ? v3 : val;
-          unsigned char* const acc = lds_raw + c0.w;
-          auto* const gsum = (__attribute__((address_space(1))) unsigned long long*)(
-              reinterpret_cast<int*>((unsigned long long)c0.y << 32 | c0.x) + 6 * (size_t)h + 4);
+          unsigned char* const acc = lds_raw + accb;
+          auto gsum_of = [&](int qq) {
+            return (__attribute__((address_space(1))) unsigned long long*)(res_of(qq) + 6 * (size_t)h + 4);
+          };
           if (sink == kSinkLds64) {
             (void)__hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(acc) + h,
                                          (unsigned long long)(long long)val, __ATOMIC_RELAXED,
@@ -692,18 +730,14 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
             const int delta = (int)(old + u < old) - (int)(val < 0);
             if (delta) {
               const unsigned long long carry = (unsigned long long)(long long)delta << 32;
-              (void)__hip_atomic_fetch_add(gsum, carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              (void)__hip_atomic_fetch_add(gsum_of(q), carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               // identical queries sharing these sums take the carry too (their
               // low words come from the same LDS sums at the flush)
-              for (int al = (int)lqc[3 * kBatchMaxQ + q].w; al >= 0; al = (int)lqc[3 * kBatchMaxQ + al].w) {
-                const uint4 a0 = lqc[al];
-                auto* const asum = (__attribute__((address_space(1))) unsigned long long*)(
-                    reinterpret_cast<int*>((unsigned long long)a0.y << 32 | a0.x) + 6 * (size_t)h + 4);
-                (void)__hip_atomic_fetch_add(asum, carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              }
+              for (int al = (int)lqc[3 * kBatchMaxQ + q].w; al >= 0; al = (int)lqc[3 * kBatchMaxQ + al].w)
+                (void)__hip_atomic_fetch_add(gsum_of(al), carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
           } else {
-            (void)__hip_atomic_fetch_add(gsum, (unsigned long long)(long long)val, __ATOMIC_RELAXED,
+            (void)__hip_atomic_fetch_add(gsum_of(q), (unsigned long long)(long long)val, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
           }
           if (touch != kBatchNoTouch)
@@ -762,8 +796,13 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchLaunch L) {
     for (int t = t_begin; t < t_end; t++) {
       masks(t, x);
       if (t + 1 < t_end) load_cols(t + 1, x);
-      if (attr16) pairs(std::true_type{});
-      else pairs(std::false_type{});
+      if (attr16) {
+        if (lq16) pairs(std::true_type{}, std::true_type{});
+        else pairs(std::true_type{}, std::false_type{});
+      } else {
+        if (lq16) pairs(std::false_type{}, std::true_type{});
+        else pairs(std::false_type{}, std::false_type{});
+      }
     }
   }
 

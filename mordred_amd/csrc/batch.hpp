@@ -236,6 +236,10 @@ struct BatchDesc {
   // 1 and 3 into the high halves of 0 and 2 (BatchQuery::shift of 1 and 3
   // count from bit 16)
   int attr16;
+  // every query's uniq[d] is below 2^16: the scan stages each descriptor with
+  // the pair step's operands in its first two 16-byte chunks (uniq as 16-bit
+  // halves) and the result address in the third
+  int uniq16;
   unsigned tab[kBatchMaxTab];  // interval tables and attribute mask tables
 };
 

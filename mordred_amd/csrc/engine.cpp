@@ -2503,6 +2503,7 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
   std::vector<char> qdead(nq, 0);  // a dimension filter with an empty range
   D.ntab = 0;
   D.attr16 = 1;
+  D.uniq16 = 1;
   for (int k = 0; k < 4; k++) {
     BatchDim& B = D.dim[k];
     Column* key = nullptr;
@@ -2748,6 +2749,7 @@ int plan_batch_forms(mordred_engine* e, const mordred_star_query* const* qp, int
       const int uv = r.join[k]->unique_val;
       if (uv < 0 || uv >= (1 << 24)) return fail(e, "batch group index needs unique_val in [0, 2^24)");
       Q.uniq[k] = uv;
+      if (uv >= (1 << 16)) D.uniq16 = 0;
       if (!r.dval[k]) {
         // a joined table without group value contributes its probe's value
         // word, 0 (GPUProcessing.h:142): (0 - min_val) * unique_val
