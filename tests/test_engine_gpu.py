@@ -274,6 +274,38 @@ def test_prefix_streaming_cost_split(fresh_engine):
     assert st["cpu_segments"] + st["stream_segments"] == nseg
 
 
+def test_prefix_streaming_with_a_wide_key_column(cuda):
+    """A prefix column whose values span more than 2^24 crosses PCIe as 4-byte
+    values, beside the columns that cross bit-packed.  lo_partkey is widened
+    by multiples of the part table's length: the probe's slot, (key - min) %
+    len, is unchanged, so the answer is the original data's (checked on the
+    oracle first)."""
+    q, data = cases.star("q4way", 5 * SEG + 77_777, seed=9)
+    ref = OQ.run_query(q, data, oracle.CONV_GPU)
+    j = next(j for j in q.joins if j.fact_key == "lo_partkey")
+    rng = np.random.default_rng(5)
+    wide = data.fact["lo_partkey"].astype(np.int64) + j.dim_len * rng.integers(0, 1 << 10, data.n)
+    assert wide.max() - wide.min() >= 1 << 24 and wide.max() < 2**31
+    data.fact["lo_partkey"] = wide.astype(np.int32)
+    np.testing.assert_array_equal(OQ.run_query(q, data, oracle.CONV_GPU), ref)
+    print("[gpu] prefix streaming with a wide lo_partkey", flush=True)
+    eng = E.Engine(device=0, cache_segments=256, threads=8)
+    try:
+        eng.add_data(data, q)
+        _uncache_all(eng, q)
+        for _ in range(3):  # later runs plan with the measured CPU costs
+            res, st = eng.run(q, E.PLACE_COST_SPLIT)
+            np.testing.assert_array_equal(res, ref)
+            print(f"[gpu]   prefix {st['prefix_segments']} segments, {st['prefix_packed_bytes']} packed bytes",
+                  flush=True)
+            assert st["cpu_rows"] + st["stream_rows"] + st["prefix_rows"] == data.n
+            # the packed columns (supplier and customer keys, date) take
+            # fewer than 4 bytes per row each
+            assert st["prefix_packed_bytes"] < 4 * st["prefix_rows"] * (len(q.fact_columns()) - 1) + 1
+    finally:
+        eng.close()
+
+
 @pytest.mark.parametrize("order", [E.ORDER_GPU_FIRST, E.ORDER_CPU_FIRST, E.ORDER_ALTERNATE])
 def test_operator_orders(fresh_engine, order):
     """The reference's operator orders over partly cached segments: the
