@@ -1609,8 +1609,15 @@ int gpu_part(mordred_engine* e, const Resolved& r, GpuPart& g, size_t res_ints) 
     std::vector<Batch> batches;
     for (size_t i = 0; i < seg_order.size(); i += kStreamBatch)
       batches.push_back({false, i, std::min<size_t>(kStreamBatch, seg_order.size() - i)});
-    for (size_t i = 0; i < pseg.size(); i += pbatch)
-      batches.push_back({true, seg_order.size() + i, std::min<size_t>(pbatch, pseg.size() - i)});
+    // prefix batches taper at the end (half of what is left, once that is
+    // below two full batches): the last batch's kernels, survivor copy and
+    // CPU operators, which nothing overlaps, then cover one segment
+    for (size_t i = 0; i < pseg.size();) {
+      const size_t left = pseg.size() - i;
+      const size_t cnt = left >= 2 * (size_t)pbatch ? (size_t)pbatch : std::max<size_t>(1, (left + 1) / 2);
+      batches.push_back({true, seg_order.size() + i, cnt});
+      i += cnt;
+    }
     for (size_t i = 0; i < 2 * kStreamBatch * nc; i++) {
       loan.slots.push_back(e->free_slots.back());
       e->free_slots.pop_back();
